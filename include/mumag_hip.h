@@ -1,0 +1,179 @@
+/*
+ * mumag_hip.h -- C ABI of libmumag_hip.so, the MI355X (gfx950) implementation of the
+ * mammos-mumag per-field-step hot path.
+ *
+ * Conventions
+ *  - Every entry point returns int: 0 = ok, < 0 = error (see MM_E_*), > 0 = warning
+ *    (MM_W_MAXITER: an iteration cap was reached; the result is still returned, as the
+ *    reference does, since it discards jax CG's info: scripts/solvers.py:326-336).
+ *  - Vectors are caller-owned DEVICE pointers (e.g. torch.Tensor.data_ptr() on cuda:0)
+ *    unless the name ends in _host.  Nodal 3-vectors are interleaved f64[3N]
+ *    ([3i+a] = component a of node i), scalars f64[N], in mesh node order.
+ *  - The context owns the operators and all workspace; no entry point allocates after
+ *    mm_ctx_create.  One HIP stream per context; calls return after the stream is
+ *    synchronised whenever a scalar is returned to the host.  A context is not
+ *    thread-safe.
+ *  - mm_last_error() returns a message for the last failure on the calling thread.
+ */
+#ifndef MUMAG_HIP_H
+#define MUMAG_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MM_OK 0
+#define MM_E_ARG (-1)      /* invalid argument (null pointer, bad size, bad option) */
+#define MM_E_HIP (-2)      /* HIP runtime failure */
+#define MM_E_MESH (-3)     /* degenerate mesh (zero-volume element, no magnetic element) */
+#define MM_E_STATE (-4)    /* call not valid in this context state */
+#define MM_W_MAXITER 1     /* iteration cap reached */
+
+#define MM_PREC_FP64 0     /* every operator and vector in FP64 */
+#define MM_PREC_MIXED 1    /* FP32 operators/fields, FP64 dots, energies, outer PCG residual */
+
+typedef struct mm_ctx mm_ctx;
+
+/* Mesh and materials.  Replaces the escript set-up of scripts/mapping.py:171-236
+ * (Materials: scripts/materials.py:12-41; operators: scripts/matrix.py:9-74,
+ * scripts/trafo.py:4-45; meas: scripts/escript_tools.py:13-17). */
+typedef struct {
+    int32_t n_nodes;          /* N */
+    int32_t n_elems;          /* NE, all Tet4 elements (magnet and air) */
+    const double *coords;     /* host f64[N*3], nm */
+    const int32_t *tets;      /* host i32[NE*4], 0-based node indices */
+    const int32_t *elem_mat;  /* host i32[NE], index into the material table */
+    int32_t n_mats;
+    const double *mat_Js;     /* host f64[n_mats], T   (Js <= 0: non-magnetic) */
+    const double *mat_A;      /* host f64[n_mats], J/m */
+    const double *mat_K1;     /* host f64[n_mats], J/m^3 */
+    const double *mat_u;      /* host f64[n_mats*3], easy axis (unit) */
+    const uint8_t *dirichlet; /* host u8[N]: 1 = u fixed to 0 (identity row/col in K) */
+    const uint8_t *shell;     /* host u8[NE] or NULL: 1 = Imhoff shell tensor element */
+    double shell_R;           /* inner radius of the shell transform (if shell != NULL) */
+    double shell_Rinf;        /* outer radius */
+    double length_scale;      /* metres per coordinate unit; the reference fixes 1e-9 */
+    /* Optional structured Kuhn-grid description (all zero = unstructured). When set, the
+     * mesh MUST be the one produced by the structured generator: nodes ordered
+     * x-fastest on an (nx+1)(ny+1)(nz+1) rectilinear grid, 6 tets per voxel. */
+    int32_t grid_n[3];        /* voxels per axis of the whole grid */
+    const double *grid_x;     /* host f64[nx+1], f64[ny+1], f64[nz+1] node coordinates */
+    const double *grid_y;
+    const double *grid_z;
+} mm_mesh_desc;
+
+typedef struct {
+    int32_t precision;        /* MM_PREC_FP64 or MM_PREC_MIXED */
+    int32_t device;           /* HIP device ordinal */
+    double drop_tol;          /* drop |K_ij| < drop_tol * max(|K_ii|,|K_jj|); 0 keeps all */
+    int32_t pcg_batch;        /* PCG iterations launched between host checks (0 = default) */
+    int32_t use_structured;   /* 1: use the Kuhn-grid stencil kernels when grid_n is set */
+} mm_options;
+
+typedef struct {
+    int64_t n_nodes, n_elems, n_mag_elems, n_mag_nodes;
+    int64_t k_nnz, k_sell_entries;   /* stored K entries, SELL-64 padded entries */
+    double volume;                   /* V, nm^3 (scripts/materials.py:36-37) */
+    int32_t structured;              /* 1 if the stencil kernels are active */
+    int32_t precision;
+    int64_t device_bytes;            /* device memory held by the context */
+} mm_info;
+
+/* Minimiser parameters: scripts/tools.py:136-192 (read_params) and loop.py:58-65. */
+typedef struct {
+    double tol_fun;           /* NCG tolerance (minimizers.py:164-166) */
+    double tol_u;             /* PCG tolerance of the potential solve */
+    int32_t precond_iter;     /* truncated-CG iterations of M_inv (loop.py:40-55) */
+    int32_t hmag_on;          /* 0: no magnetostatics (energies.py:89-94) */
+    int32_t iter_max;         /* NCG cap; the reference hard-codes 100000 (minimizers.py:142) */
+} mm_min_params;
+
+typedef struct {
+    int64_t ncg_iters;        /* NCG iterations (the reference's cg_iter, loop.py:394) */
+    int64_t fcalls;           /* total_eg evaluations (the reference's stats) */
+    int64_t pcg_iters;        /* Poisson PCG iterations */
+    int64_t pcg_calls;
+    int64_t tcg_iters;        /* truncated-CG iterations inside M_inv */
+    int64_t c_applies;        /* exchange/anisotropy operator applications */
+    int64_t backtracks;       /* line-search halvings */
+} mm_stats;
+
+const char *mm_last_error(void);
+
+/* Build the operators on the device.  Replaces escript2arrays (scripts/mapping.py:171-236). */
+int mm_ctx_create(const mm_mesh_desc *mesh, const mm_options *opts, mm_ctx **out);
+int mm_ctx_destroy(mm_ctx *ctx);
+int mm_ctx_info(mm_ctx *ctx, mm_info *out);
+/* Copies meas (f64[N], T nm^3) and the exchange diagonal D (f64[3N]) to host arrays. */
+int mm_get_meas_host(mm_ctx *ctx, double *meas_host);
+int mm_get_diag_host(mm_ctx *ctx, double *dex_host, double *diagK_host);
+
+/* Zeeman energy and gradient: scripts/energies.py:7-21 (external_g / external_eg). */
+int mm_external_eg(mm_ctx *ctx, const double *m, double hext, const double hdir[3],
+                   double *E_host, double *g);
+/* Exchange+anisotropy energy and gradient g = C m, E = m.g/2: energies.py:22-33. */
+int mm_exani_eg(mm_ctx *ctx, const double *m, double *E_host, double *g);
+/* Magnetostatic energy/gradient/potential: energies.py:36-59 (hmag_g / hmag_eg).
+ * u_inout: warm start in, potential out.  pcg_iters_host may be NULL. */
+int mm_hmag_eg(mm_ctx *ctx, const double *m, double *u_inout, double tol,
+               double *E_host, double *g, int32_t *pcg_iters_host);
+/* total_eg: energies.py:67-99.  Writes E, the projected gradient P(m,g) = m x (g x m)
+ * (energies.py:61-65) and the full gradient g_full; u_inout is the warm start/result. */
+int mm_total_eg(mm_ctx *ctx, const double *m, double hext, const double hdir[3],
+                double tol_u, int32_t hmag_on, double *u_inout, double *E_host,
+                double *g_proj, double *g_full, int32_t *pcg_iters_host);
+
+/* Jacobi PCG on K with jax.scipy.sparse.linalg.cg semantics (solvers.py:326-336):
+ * stop when r.r <= max(tol^2 b.b, atol^2) or after maxiter iterations. */
+int mm_pcg(mm_ctx *ctx, const double *b, double *x_inout, double tol, double atol,
+           int32_t maxiter, int32_t *iters_host);
+
+/* M_inv: truncated diagonally preconditioned CG (solvers.py:112-236) on
+ * H v = m_ref x ((C v) x m_ref) - (m_ref . g_full)_i v_i (loop.py:315-330), called with
+ * -g.  Writes z (the NCG then uses d = -z). */
+int mm_precond(mm_ctx *ctx, const double *m_ref, const double *g_full, const double *g,
+               int32_t max_iter, double *z, int32_t *iters_host);
+
+/* Retraction m_out = normalize(x + alpha d) per node, zero vectors kept
+ * (jax_tools.py:4-18 update_m).  m_out may alias x. */
+int mm_update_m(mm_ctx *ctx, const double *x, double alpha, const double *d, double *m_out);
+
+/* Reductions over f64[n] device vectors. */
+int mm_dot(mm_ctx *ctx, const double *a, const double *b, int64_t n, double *out_host);
+int mm_norm_inf(mm_ctx *ctx, const double *a, int64_t n, double *out_host);
+
+/* M.h = sum_i (h . m_i) meas_i / V in Tesla (loop.py:300-313 compute_mh). */
+int mm_compute_mh(mm_ctx *ctx, const double *m, const double hdir[3], double *mh_host);
+
+/* One field step: hestenes_stiefel_ncg (minimizers.py:139-190) with line_search
+ * (92-137) on total_eg and M_inv, i.e. loop.py:58-65 minimize.  m_inout is the
+ * iterate; u_inout and gF_inout are the reference's alt_args (potential and full
+ * gradient) carried across calls; m_ref is the preconditioner's fixed magnetisation
+ * (the initial state, loop.py:325-328).  Writes the final energy. */
+int mm_minimize(mm_ctx *ctx, double *m_inout, double hext, const double hdir[3],
+                const mm_min_params *params, const double *m_ref, double *u_inout,
+                double *gF_inout, double *E_host, mm_stats *stats_inout);
+
+/* Low-level operator applications (device vectors), for tests and callers that drive
+ * their own iteration:  y = C v (3N);  y = K x (N);  b = Dx mx + Dy my + Dz mz (N);
+ * g = interleave(Gx u, Gy u, Gz u) (3N). */
+int mm_apply_C(mm_ctx *ctx, const double *v, double *y);
+int mm_apply_K(mm_ctx *ctx, const double *x, double *y);
+int mm_divergence(mm_ctx *ctx, const double *m, double *b);
+int mm_gradient(mm_ctx *ctx, const double *u, double *g);
+
+/* Event-timed statistics of the dominant kernels since the last reset. */
+typedef struct {
+    int64_t spmv_launches;    /* K SpMV launches inside PCG */
+    double spmv_ms;           /* summed device time of those launches (HIP events) */
+    int64_t spmv_bytes;       /* algorithmic bytes of one K SpMV launch */
+} mm_kernel_timing;
+int mm_timing_enable(mm_ctx *ctx, int32_t on);
+int mm_timing_get(mm_ctx *ctx, mm_kernel_timing *out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* MUMAG_HIP_H */

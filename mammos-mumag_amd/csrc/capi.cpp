@@ -1,0 +1,652 @@
+// C ABI of libmumag_hip.so: context creation, the energy/gradient evaluation,
+// the Jacobi PCG, the truncated-CG preconditioner and the nonlinear-CG minimiser.
+//
+// The minimiser is a host-side control loop over device kernels; each iteration
+// returns only the few scalars the reference's control flow branches on.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cfloat>
+#include <cmath>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/mumag_hip.h"
+#include "assemble.h"
+#include "common.h"
+
+using namespace mm;
+
+namespace {
+thread_local std::string g_err;
+
+int fail(int code, const std::string &msg) {
+    g_err = msg;
+    return code;
+}
+}  // namespace
+
+#define HIPC(x)                                                                              \
+    do {                                                                                     \
+        hipError_t e_ = (x);                                                                 \
+        if (e_ != hipSuccess)                                                                \
+            return fail(MM_E_HIP, std::string(#x) + ": " + hipGetErrorString(e_));         \
+    } while (0)
+
+struct mm_ctx {
+    int device = 0;
+    hipStream_t s = nullptr;
+    int precision = MM_PREC_FP64;
+    int N = 0, NE = 0, NEm = 0, Nm = 0;
+    double V = 0.0;
+    int pcg_batch = 16;
+    int64_t k_nnz = 0, k_entries = 0;
+    size_t bytes = 0;
+    std::vector<void *> allocs;
+
+    double *meas = nullptr, *Dex = nullptr, *invDK = nullptr, *diagK = nullptr;
+    SellDev<double> K;
+    ElemDev E;
+    MatCoef *mats = nullptr;
+    int n_mats = 0;
+    DevScalars *dsc = nullptr;
+    DevScalars *hsc = nullptr;  // pinned host mirror
+    Reducer red;
+
+    // workspace (N)
+    double *b = nullptr, *r = nullptr, *z = nullptr, *p = nullptr, *q = nullptr;
+    // workspace (3N)
+    double *gC = nullptr, *gd = nullptr;
+    double *tp = nullptr, *tr = nullptr, *tz = nullptr, *td = nullptr, *tCd = nullptr, *tHd = nullptr;
+    double *X = nullptr, *X0 = nullptr, *X1 = nullptr, *G = nullptr, *G1 = nullptr, *Dd = nullptr,
+           *Z1 = nullptr, *gtmp = nullptr;
+
+    // timing of the K SpMV inside PCG
+    bool timing = false;
+    std::vector<hipEvent_t> evpool;
+    int64_t spmv_launches = 0;
+    double spmv_ms = 0.0;
+    int64_t spmv_bytes = 0;
+
+    Launch L() const {
+        Launch l;
+        l.s = s;
+        l.red = red;
+        l.dsc = dsc;
+        l.mats = mats;
+        l.n_mats = n_mats;
+        return l;
+    }
+
+    template <typename T>
+    int alloc(T **ptr, size_t count) {
+        void *p_ = nullptr;
+        hipError_t e = hipMalloc(&p_, std::max<size_t>(count, 1) * sizeof(T));
+        if (e != hipSuccess) return fail(MM_E_HIP, std::string("hipMalloc: ") + hipGetErrorString(e));
+        allocs.push_back(p_);
+        bytes += count * sizeof(T);
+        *ptr = (T *)p_;
+        return 0;
+    }
+    template <typename T>
+    int upload(T **ptr, const std::vector<T> &v) {
+        int rc = alloc(ptr, v.size());
+        if (rc) return rc;
+        if (!v.empty()) HIPC(hipMemcpy(*ptr, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice));
+        return 0;
+    }
+    ~mm_ctx() {
+        if (s) (void)hipStreamSynchronize(s);
+        for (void *a : allocs) (void)hipFree(a);
+        if (hsc) (void)hipHostFree(hsc);
+        for (hipEvent_t e : evpool) (void)hipEventDestroy(e);
+        if (s) (void)hipStreamDestroy(s);
+    }
+};
+
+namespace {
+
+int sync_scalars(mm_ctx *c) {
+    HIPC(hipMemcpyAsync(c->hsc, c->dsc, sizeof(DevScalars), hipMemcpyDeviceToHost, c->s));
+    HIPC(hipStreamSynchronize(c->s));
+    return 0;
+}
+
+// jax.scipy.sparse.linalg.cg semantics (see kernels.hip); x is updated in place.
+int pcg(mm_ctx *c, const double *b, double *x, double tol, double atol, int maxiter, int *iters) {
+    Launch L = c->L();
+    const int n = c->N;
+    launch_sell_spmv(L, c->K, x, c->q, nullptr, nullptr);
+    launch_pcg_init(L, n, b, c->q, c->r, c->z, c->p, c->invDK, tol, atol, maxiter);
+    int done = 0;
+    const int B = c->pcg_batch;
+    int launched = 0;
+    const int pool = (int)c->evpool.size() / 2;
+    while (!done) {
+        for (int k = 0; k < B; ++k) {
+            const bool tm = c->timing && launched < pool;
+            if (tm) HIPC(hipEventRecord(c->evpool[2 * launched], c->s));
+            launch_sell_spmv(L, c->K, c->p, c->q, &c->dsc->pcg_pq, &c->dsc->pcg_done);
+            if (tm) HIPC(hipEventRecord(c->evpool[2 * launched + 1], c->s));
+            ++launched;
+            launch_pcg_update(L, n, x, c->r, c->z, c->p, c->q, c->invDK);
+            launch_pcg_pupdate(L, n, c->p, c->z);
+        }
+        int rc = sync_scalars(c);
+        if (rc) return rc;
+        done = c->hsc->pcg_done;
+    }
+    if (c->timing) {
+        // launches past convergence are no-ops: count the first pcg_iter only
+        const int used = std::min(c->hsc->pcg_iter, pool);
+        for (int j = 0; j < used; ++j) {
+            float ms = 0.f;
+            HIPC(hipEventElapsedTime(&ms, c->evpool[2 * j], c->evpool[2 * j + 1]));
+            c->spmv_ms += ms;
+            c->spmv_launches++;
+        }
+    }
+    if (iters) *iters = c->hsc->pcg_iter;
+    if (c->hsc->pcg_iter >= maxiter && c->hsc->pcg_rr > c->hsc->pcg_bound) return MM_W_MAXITER;
+    return 0;
+}
+
+// E, g_proj, g_full for magnetisation m (energies.py:67-99).
+int total_eg(mm_ctx *c, const double *m, double hext, const double *hdir, double tol_u,
+             int hmag_on, double *u, double *E, double *gproj, double *gfull, int *pcg_it) {
+    Launch L = c->L();
+    const int n = c->N;
+    HIPC(hipMemsetAsync(c->gC, 0, 3 * (size_t)n * sizeof(double), c->s));
+    if (hmag_on > 0) HIPC(hipMemsetAsync(c->b, 0, (size_t)n * sizeof(double), c->s));
+    launch_ebe_exani(L, c->E, m, c->gC, hmag_on > 0 ? c->b : nullptr, nullptr);
+    int it = 0, rc = 0;
+    if (hmag_on > 0) {
+        rc = pcg(c, c->b, u, tol_u, 0.0, 10000, &it);
+        if (rc < 0) return rc;
+        HIPC(hipMemsetAsync(c->gd, 0, 3 * (size_t)n * sizeof(double), c->s));
+        launch_ebe_grad(L, c->E, u, c->gd, 1.0 / c->V);
+    }
+    const double zc0 = -hext * hdir[0] / c->V, zc1 = -hext * hdir[1] / c->V,
+                 zc2 = -hext * hdir[2] / c->V;
+    launch_finish_eg(L, n, m, c->meas, zc0, zc1, zc2, c->gC, c->gd, gfull, gproj, &c->dsc->energy,
+                     hmag_on > 0);
+    int rc2 = sync_scalars(c);
+    if (rc2) return rc2;
+    *E = c->hsc->energy;
+    if (pcg_it) *pcg_it = it;
+    return rc;
+}
+
+// M_inv (loop.py:315-330): truncated CG on H with grad_fk = -g.  No host sync.
+int precond(mm_ctx *c, const double *m_ref, const double *gF, const double *g, int max_iter,
+            double *zout) {
+    Launch L = c->L();
+    const int n = c->N, n3 = 3 * c->N;
+    launch_tcg_init(L, n3, g, c->Dex, c->tp, c->tr, c->tz, c->td);
+    for (int i = 0; i < max_iter; ++i) {
+        launch_fill_zero(L, n3, c->tCd, &c->dsc->tcg_done);
+        launch_ebe_exani(L, c->E, c->td, c->tCd, nullptr, &c->dsc->tcg_done);
+        launch_hvec(L, n, m_ref, gF, c->tCd, c->td, c->tHd, &c->dsc->tcg_den, &c->dsc->tcg_done);
+        launch_tcg_update(L, n3, c->tp, c->tr, c->tz, c->td, c->tHd, c->Dex);
+        launch_tcg_dupdate(L, n3, c->td, c->tz);
+    }
+    launch_reduce(L, RED_NORMINF, n3, c->tp, nullptr, nullptr, nullptr, 0, 0, 0, &c->dsc->tcg_pmax);
+    launch_tcg_final(L, n3, c->tp, g, zout);
+    return 0;
+}
+
+int reduce(mm_ctx *c, RedOp op, int64_t n, const double *a, const double *b, const double *cc,
+           const double *d, double *out, int nout, double h0 = 0, double h1 = 0, double h2 = 0) {
+    Launch L = c->L();
+    launch_reduce(L, op, n, a, b, cc, d, h0, h1, h2, c->dsc->red);
+    int rc = sync_scalars(c);
+    if (rc) return rc;
+    for (int k = 0; k < nout; ++k) out[k] = c->hsc->red[k];
+    return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char *mm_last_error(void) { return g_err.c_str(); }
+
+int mm_ctx_create(const mm_mesh_desc *d, const mm_options *opts, mm_ctx **out) {
+    if (!d || !out) return fail(MM_E_ARG, "null argument");
+    *out = nullptr;
+    if (d->n_nodes <= 0 || d->n_elems <= 0 || !d->coords || !d->tets || !d->elem_mat ||
+        d->n_mats <= 0 || !d->mat_Js || !d->mat_A || !d->mat_K1 || !d->mat_u)
+        return fail(MM_E_ARG, "incomplete mesh description");
+    mm_options o{};
+    if (opts) o = *opts;
+    if (o.precision != MM_PREC_FP64 && o.precision != MM_PREC_MIXED)
+        return fail(MM_E_ARG, "unknown precision");
+    if (o.precision == MM_PREC_MIXED) return fail(MM_E_ARG, "mixed precision not available in this build");
+
+    HostOps H;
+    std::string err = assemble(*d, o.drop_tol, H, true);
+    if (!err.empty()) return fail(MM_E_MESH, err);
+
+    HIPC(hipSetDevice(o.device));
+    mm_ctx *c = new mm_ctx();
+    c->device = o.device;
+    c->precision = o.precision;
+    c->N = H.N;
+    c->NE = H.NE;
+    c->NEm = (int)H.mag_elems.size();
+    c->Nm = H.n_mag_nodes;
+    c->V = H.V;
+    if (o.pcg_batch > 0) c->pcg_batch = o.pcg_batch;
+    int rc = 0;
+    auto bail = [&](int code) {
+        delete c;
+        return code;
+    };
+    if (hipStreamCreateWithFlags(&c->s, hipStreamNonBlocking) != hipSuccess)
+        return bail(fail(MM_E_HIP, "hipStreamCreate failed"));
+
+    // operators
+    if ((rc = c->upload(&c->meas, H.meas))) return bail(rc);
+    if ((rc = c->upload(&c->Dex, H.Dex))) return bail(rc);
+    if ((rc = c->upload(&c->diagK, H.diagK))) return bail(rc);
+    {
+        std::vector<double> inv(H.N);
+        for (int i = 0; i < H.N; ++i) inv[i] = 1.0 / H.diagK[i];
+        if ((rc = c->upload(&c->invDK, inv))) return bail(rc);
+    }
+    {
+        std::vector<int> sp, scol;
+        std::vector<double> sval;
+        csr_to_sell(H.N, H.k_rowptr, H.k_col, H.k_val, sp, scol, sval);
+        int *d_sp, *d_col;
+        double *d_val;
+        if ((rc = c->upload(&d_sp, sp)) || (rc = c->upload(&d_col, scol)) || (rc = c->upload(&d_val, sval)))
+            return bail(rc);
+        c->K.n_rows = H.N;
+        c->K.n_slices = (int)sp.size() - 1;
+        c->K.slice_ptr = d_sp;
+        c->K.col = d_col;
+        c->K.val = d_val;
+        c->k_nnz = (int64_t)H.k_val.size();
+        c->k_entries = (int64_t)sval.size();
+        // algorithmic bytes of one SpMV: stored entries (val + col), row pointers,
+        // x read once, y written once
+        c->spmv_bytes = c->k_nnz * (8 + 4) + 4 * (int64_t)(H.N + 1) + 2 * 8 * (int64_t)H.N;
+    }
+    {
+        std::vector<int4> t4(c->NEm);
+        for (int q = 0; q < c->NEm; ++q)
+            t4[q] = make_int4(H.mag_tets[4 * q], H.mag_tets[4 * q + 1], H.mag_tets[4 * q + 2],
+                              H.mag_tets[4 * q + 3]);
+        int4 *dt;
+        double *dG, *dv;
+        int *dm;
+        if ((rc = c->upload(&dt, t4)) || (rc = c->upload(&dG, H.mag_G)) ||
+            (rc = c->upload(&dv, H.mag_vol)) || (rc = c->upload(&dm, H.mag_mat)))
+            return bail(rc);
+        c->E.n = c->NEm;
+        c->E.tets = dt;
+        c->E.G = dG;
+        c->E.vol = dv;
+        c->E.mat = dm;
+    }
+    {
+        const double mu0 = 4.0e-7 * M_PI, size = d->length_scale > 0 ? d->length_scale : 1e-9;
+        std::vector<MatCoef> mc(d->n_mats);
+        for (int m = 0; m < d->n_mats; ++m) {
+            const bool mag = d->mat_Js[m] > 0;
+            const double Aeff = mag ? mu0 * d->mat_A[m] / (size * size) : 0.0;
+            const double Keff = mag ? mu0 * d->mat_K1[m] : 0.0;
+            mc[m].cex = 2.0 * Aeff / H.V;
+            mc[m].cK = -2.0 * Keff / H.V;
+            mc[m].Js = mag ? d->mat_Js[m] : 0.0;
+            for (int a = 0; a < 3; ++a) mc[m].u[a] = mag ? d->mat_u[3 * m + a] : 0.0;
+        }
+        if ((rc = c->upload(&c->mats, mc))) return bail(rc);
+        c->n_mats = d->n_mats;
+    }
+    if ((rc = c->alloc(&c->dsc, 1))) return bail(rc);
+    HIPC(hipMemset(c->dsc, 0, sizeof(DevScalars)));
+    if (hipHostMalloc((void **)&c->hsc, sizeof(DevScalars), hipHostMallocDefault) != hipSuccess)
+        return bail(fail(MM_E_HIP, "hipHostMalloc failed"));
+    if ((rc = c->alloc(&c->red.partials, 4 * (size_t)kMaxPartials))) return bail(rc);
+    if ((rc = c->alloc(&c->red.counter, 1))) return bail(rc);
+    HIPC(hipMemset(c->red.counter, 0, sizeof(unsigned)));
+
+    const size_t n = H.N, n3 = 3 * n;
+    double **vN[] = {&c->b, &c->r, &c->z, &c->p, &c->q};
+    for (double **v : vN)
+        if ((rc = c->alloc(v, n))) return bail(rc);
+    double **v3[] = {&c->gC, &c->gd, &c->tp, &c->tr, &c->tz, &c->td, &c->tCd, &c->tHd,
+                     &c->X,  &c->X0, &c->X1, &c->G,  &c->G1, &c->Dd, &c->Z1, &c->gtmp};
+    for (double **v : v3)
+        if ((rc = c->alloc(v, n3))) return bail(rc);
+    HIPC(hipMemset(c->gd, 0, n3 * sizeof(double)));
+    HIPC(hipDeviceSynchronize());
+    *out = c;
+    return 0;
+}
+
+int mm_ctx_destroy(mm_ctx *c) {
+    if (!c) return fail(MM_E_ARG, "null context");
+    delete c;
+    return 0;
+}
+
+int mm_ctx_info(mm_ctx *c, mm_info *o) {
+    if (!c || !o) return fail(MM_E_ARG, "null argument");
+    o->n_nodes = c->N;
+    o->n_elems = c->NE;
+    o->n_mag_elems = c->NEm;
+    o->n_mag_nodes = c->Nm;
+    o->k_nnz = c->k_nnz;
+    o->k_sell_entries = c->k_entries;
+    o->volume = c->V;
+    o->structured = 0;
+    o->precision = c->precision;
+    o->device_bytes = (int64_t)c->bytes;
+    return 0;
+}
+
+int mm_get_meas_host(mm_ctx *c, double *meas_host) {
+    if (!c || !meas_host) return fail(MM_E_ARG, "null argument");
+    HIPC(hipMemcpy(meas_host, c->meas, c->N * sizeof(double), hipMemcpyDeviceToHost));
+    return 0;
+}
+
+int mm_get_diag_host(mm_ctx *c, double *dex_host, double *diagK_host) {
+    if (!c) return fail(MM_E_ARG, "null argument");
+    if (dex_host) HIPC(hipMemcpy(dex_host, c->Dex, 3 * c->N * sizeof(double), hipMemcpyDeviceToHost));
+    if (diagK_host) HIPC(hipMemcpy(diagK_host, c->diagK, c->N * sizeof(double), hipMemcpyDeviceToHost));
+    return 0;
+}
+
+int mm_external_eg(mm_ctx *c, const double *m, double hext, const double hdir[3], double *E_host,
+                   double *g) {
+    if (!c || !m || !hdir || !E_host) return fail(MM_E_ARG, "null argument");
+    Launch L = c->L();
+    launch_external(L, c->N, m, c->meas, -hext * hdir[0] / c->V, -hext * hdir[1] / c->V,
+                    -hext * hdir[2] / c->V, g, &c->dsc->energy);
+    int rc = sync_scalars(c);
+    if (rc) return rc;
+    *E_host = c->hsc->energy;
+    return 0;
+}
+
+int mm_exani_eg(mm_ctx *c, const double *m, double *E_host, double *g) {
+    if (!c || !m || !E_host || !g) return fail(MM_E_ARG, "null argument");
+    Launch L = c->L();
+    HIPC(hipMemsetAsync(g, 0, 3 * (size_t)c->N * sizeof(double), c->s));
+    launch_ebe_exani(L, c->E, m, g, nullptr, nullptr);
+    launch_reduce(L, RED_DOT, 3 * (int64_t)c->N, m, g, nullptr, nullptr, 0, 0, 0, c->dsc->red);
+    int rc = sync_scalars(c);
+    if (rc) return rc;
+    *E_host = 0.5 * c->hsc->red[0];
+    return 0;
+}
+
+int mm_hmag_eg(mm_ctx *c, const double *m, double *u, double tol, double *E_host, double *g,
+               int32_t *pcg_iters_host) {
+    if (!c || !m || !u || !E_host || !g) return fail(MM_E_ARG, "null argument");
+    Launch L = c->L();
+    HIPC(hipMemsetAsync(c->b, 0, (size_t)c->N * sizeof(double), c->s));
+    HIPC(hipMemsetAsync(c->gC, 0, 3 * (size_t)c->N * sizeof(double), c->s));
+    launch_ebe_exani(L, c->E, m, c->gC, c->b, nullptr);
+    int it = 0;
+    int rc = pcg(c, c->b, u, tol, 0.0, 10000, &it);
+    if (rc < 0) return rc;
+    HIPC(hipMemsetAsync(g, 0, 3 * (size_t)c->N * sizeof(double), c->s));
+    launch_ebe_grad(L, c->E, u, g, 1.0 / c->V);
+    launch_reduce(L, RED_DOT, 3 * (int64_t)c->N, m, g, nullptr, nullptr, 0, 0, 0, c->dsc->red);
+    int rc2 = sync_scalars(c);
+    if (rc2) return rc2;
+    *E_host = 0.5 * c->hsc->red[0];
+    if (pcg_iters_host) *pcg_iters_host = it;
+    return rc;
+}
+
+int mm_total_eg(mm_ctx *c, const double *m, double hext, const double hdir[3], double tol_u,
+                int32_t hmag_on, double *u, double *E_host, double *g_proj, double *g_full,
+                int32_t *pcg_iters_host) {
+    if (!c || !m || !hdir || !u || !E_host || !g_proj || !g_full) return fail(MM_E_ARG, "null argument");
+    int it = 0;
+    int rc = total_eg(c, m, hext, hdir, tol_u, hmag_on, u, E_host, g_proj, g_full, &it);
+    if (pcg_iters_host) *pcg_iters_host = it;
+    return rc;
+}
+
+int mm_pcg(mm_ctx *c, const double *b, double *x, double tol, double atol, int32_t maxiter,
+           int32_t *iters_host) {
+    if (!c || !b || !x) return fail(MM_E_ARG, "null argument");
+    if (maxiter < 0) return fail(MM_E_ARG, "maxiter < 0");
+    int it = 0;
+    int rc = pcg(c, b, x, tol, atol, maxiter, &it);
+    if (iters_host) *iters_host = it;
+    return rc;
+}
+
+int mm_precond(mm_ctx *c, const double *m_ref, const double *g_full, const double *g,
+               int32_t max_iter, double *z, int32_t *iters_host) {
+    if (!c || !m_ref || !g_full || !g || !z) return fail(MM_E_ARG, "null argument");
+    if (max_iter < 0) return fail(MM_E_ARG, "max_iter < 0");
+    int rc = precond(c, m_ref, g_full, g, max_iter, z);
+    if (rc) return rc;
+    rc = sync_scalars(c);
+    if (rc) return rc;
+    if (iters_host) *iters_host = max_iter > 0 ? c->hsc->tcg_iter : 0;
+    return 0;
+}
+
+int mm_update_m(mm_ctx *c, const double *x, double alpha, const double *d, double *out) {
+    if (!c || !x || !d || !out) return fail(MM_E_ARG, "null argument");
+    launch_update_m(c->L(), c->N, x, alpha, d, out);
+    HIPC(hipStreamSynchronize(c->s));
+    return 0;
+}
+
+int mm_dot(mm_ctx *c, const double *a, const double *b, int64_t n, double *out) {
+    if (!c || !a || !b || !out || n < 0) return fail(MM_E_ARG, "bad argument");
+    return reduce(c, RED_DOT, n, a, b, nullptr, nullptr, out, 1);
+}
+
+int mm_norm_inf(mm_ctx *c, const double *a, int64_t n, double *out) {
+    if (!c || !a || !out || n < 0) return fail(MM_E_ARG, "bad argument");
+    return reduce(c, RED_NORMINF, n, a, nullptr, nullptr, nullptr, out, 1);
+}
+
+int mm_compute_mh(mm_ctx *c, const double *m, const double hdir[3], double *mh) {
+    if (!c || !m || !hdir || !mh) return fail(MM_E_ARG, "null argument");
+    double s = 0;
+    int rc = reduce(c, RED_MH, c->N, m, c->meas, nullptr, nullptr, &s, 1, hdir[0], hdir[1], hdir[2]);
+    if (rc) return rc;
+    *mh = s / c->V;
+    return 0;
+}
+
+int mm_apply_C(mm_ctx *c, const double *v, double *y) {
+    if (!c || !v || !y) return fail(MM_E_ARG, "null argument");
+    HIPC(hipMemsetAsync(y, 0, 3 * (size_t)c->N * sizeof(double), c->s));
+    launch_ebe_exani(c->L(), c->E, v, y, nullptr, nullptr);
+    HIPC(hipStreamSynchronize(c->s));
+    return 0;
+}
+
+int mm_apply_K(mm_ctx *c, const double *x, double *y) {
+    if (!c || !x || !y) return fail(MM_E_ARG, "null argument");
+    launch_sell_spmv(c->L(), c->K, x, y, nullptr, nullptr);
+    HIPC(hipStreamSynchronize(c->s));
+    return 0;
+}
+
+int mm_divergence(mm_ctx *c, const double *m, double *b) {
+    if (!c || !m || !b) return fail(MM_E_ARG, "null argument");
+    HIPC(hipMemsetAsync(b, 0, (size_t)c->N * sizeof(double), c->s));
+    HIPC(hipMemsetAsync(c->gtmp, 0, 3 * (size_t)c->N * sizeof(double), c->s));
+    launch_ebe_exani(c->L(), c->E, m, c->gtmp, b, nullptr);
+    HIPC(hipStreamSynchronize(c->s));
+    return 0;
+}
+
+int mm_gradient(mm_ctx *c, const double *u, double *g) {
+    if (!c || !u || !g) return fail(MM_E_ARG, "null argument");
+    HIPC(hipMemsetAsync(g, 0, 3 * (size_t)c->N * sizeof(double), c->s));
+    launch_ebe_grad(c->L(), c->E, u, g, 1.0 / c->V);
+    HIPC(hipStreamSynchronize(c->s));
+    return 0;
+}
+
+int mm_minimize(mm_ctx *c, double *m_inout, double hext, const double hdir[3],
+                const mm_min_params *P, const double *m_ref, double *u, double *gF, double *E_host,
+                mm_stats *st) {
+    if (!c || !m_inout || !hdir || !P || !m_ref || !u || !gF || !E_host)
+        return fail(MM_E_ARG, "null argument");
+    const int n = c->N;
+    const int64_t n3 = 3 * (int64_t)n;
+    const size_t bytes3 = n3 * sizeof(double);
+    Launch L = c->L();
+    const double tol = P->tol_fun;
+    const int iter_max = P->iter_max > 0 ? P->iter_max : 100000;
+    mm_stats loc{};
+    int rc = 0;
+    double *X = c->X, *X0 = c->X0, *X1 = c->X1, *G = c->G, *G1 = c->G1, *D = c->Dd, *Z1 = c->Z1;
+    HIPC(hipMemcpyAsync(X, m_inout, bytes3, hipMemcpyDeviceToDevice, c->s));
+    HIPC(hipMemsetAsync(&c->dsc->red[15], 0, sizeof(double), c->s));
+
+    auto feval = [&](const double *x, double *E, double *g) -> int {
+        int it = 0;
+        int r = total_eg(c, x, hext, hdir, P->tol_u, P->hmag_on, u, E, g, gF, &it);
+        loc.fcalls++;
+        loc.pcg_iters += it;
+        loc.pcg_calls += P->hmag_on > 0;
+        loc.c_applies++;
+        return r < 0 ? r : 0;
+    };
+    auto minv = [&](const double *g, double *z) -> int {
+        loc.c_applies += P->precond_iter;
+        return precond(c, m_ref, gF, g, P->precond_iter, z);
+    };
+
+    const double FMAX = DBL_MAX;
+    double f0 = FMAX, f = 0.0;
+    if ((rc = feval(X, &f, G))) return rc;
+    if ((rc = minv(G, Z1))) return rc;
+    launch_axpby(L, n3, -1.0, Z1, 0.0, nullptr, D);  // d = -M_inv(g)
+    int64_t k = 1;
+    HIPC(hipMemcpyAsync(X0, X, bytes3, hipMemcpyDeviceToDevice, c->s));
+    const double sqrt_tol = std::sqrt(tol), cbrt_tol = std::cbrt(tol);
+    const double sqrt_eps = std::sqrt(DBL_EPSILON);
+    for (;;) {
+        double s3[3];
+        if ((rc = reduce(c, RED_STOP3, n3, X0, X, G, nullptr, s3, 3))) return rc;
+        const bool a = (f0 - f) > tol * (1 + std::fabs(f));
+        const bool b = (s3[0] > sqrt_tol * (1 + s3[1])) || k == 0;
+        const bool cc = s3[2] > cbrt_tol * (1 + std::fabs(f));
+        if (!((a || b || cc) && k < iter_max)) break;
+        // restart if d is not a sufficient descent direction (minimizers.py:171-173)
+        double dg3[3];
+        if ((rc = reduce(c, RED_DG3, n3, D, G, nullptr, nullptr, dg3, 3))) return rc;
+        if (dg3[0] > -0.001 * std::sqrt(dg3[1]) * std::sqrt(dg3[2]))
+            launch_axpby(L, n3, -1.0, G, 0.0, nullptr, D);
+        // ---- line search (minimizers.py:92-137)
+        double f1 = 0.0;
+        {
+            const double rho = 0.5, cA = 1e-4, delta = 0.1, eps = 1e-6;
+            const int max_ls = 20;
+            double h3[3];
+            if ((rc = reduce(c, RED_H3, n3, X, D, nullptr, nullptr, h3, 3))) return rc;
+            const double t1 = 2.0 * sqrt_eps * (1.0 + std::sqrt(h3[0])) / std::sqrt(h3[1]);
+            const double t2 = sqrt_eps / h3[2];
+            const double h = t1 < t2 ? t1 : t2;
+            launch_update_m(L, n, X, h, D, X1);
+            if ((rc = feval(X1, &f1, G1))) return rc;
+            double gd2[2];
+            if ((rc = reduce(c, RED_GD2, n3, G, G1, D, nullptr, gd2, 2))) return rc;
+            const double gd = gd2[0];
+            double alpha;
+            {  // alpha_init (minimizers.py:34-57), c = 0.1, o = 0.99
+                double term1 = -h * (gd / (gd2[1] - gd));
+                if (term1 < 0) term1 = 0.1 * std::fabs(term1);
+                double term2 = 2.0 * (f - f0) / gd;
+                if (term2 < 0) term2 = 1.0e30;
+                alpha = term1 < 0.99 * term2 ? term1 : term2;
+            }
+            launch_update_m(L, n, X, alpha, D, X1);
+            if ((rc = feval(X1, &f1, G1))) return rc;
+            double gd1;
+            if ((rc = reduce(c, RED_DOT, n3, G1, D, nullptr, nullptr, &gd1, 1))) return rc;
+            int i = 0;
+            for (;;) {
+                const double eps_k = eps * std::fabs(f);
+                const bool wolfe = ((2 * delta - 1) * gd >= gd1) && (f1 <= f + eps_k);
+                const bool done = wolfe || (f1 <= f + cA * alpha * gd) || (i > max_ls);
+                if (done) break;
+                alpha *= rho;
+                launch_update_m(L, n, X, alpha, D, X1);
+                if ((rc = feval(X1, &f1, G1))) return rc;
+                if ((rc = reduce(c, RED_DOT, n3, G1, D, nullptr, nullptr, &gd1, 1))) return rc;
+                ++i;
+                loc.backtracks++;
+            }
+        }
+        // ---- HS+ direction (minimizers.py:175-184)
+        if ((rc = minv(G1, Z1))) return rc;
+        double yzd[2];
+        if ((rc = reduce(c, RED_YZD, n3, G, G1, Z1, D, yzd, 2))) return rc;
+        const double beta = std::max(yzd[0] / yzd[1], 0.0);
+        launch_axpby(L, n3, -1.0, Z1, beta, D, D);
+        if ((rc = reduce(c, RED_DG3, n3, D, G, nullptr, nullptr, dg3, 3))) return rc;
+        if (dg3[0] > -0.001 * std::sqrt(dg3[1]) * std::sqrt(dg3[2]))
+            launch_axpby(L, n3, -1.0, Z1, 0.0, nullptr, D);
+        // shift state: x0 <- x, x <- x1, f0 <- f, f <- f1, g <- g1
+        std::swap(X0, X);
+        std::swap(X, X1);
+        std::swap(G, G1);
+        f0 = f;
+        f = f1;
+        ++k;
+    }
+    HIPC(hipMemcpyAsync(m_inout, X, bytes3, hipMemcpyDeviceToDevice, c->s));
+    if ((rc = sync_scalars(c))) return rc;
+    loc.ncg_iters = k;
+    loc.tcg_iters = (int64_t)c->hsc->red[15];
+    *E_host = f;
+    if (st) {
+        st->ncg_iters += loc.ncg_iters;
+        st->fcalls += loc.fcalls;
+        st->pcg_iters += loc.pcg_iters;
+        st->pcg_calls += loc.pcg_calls;
+        st->tcg_iters += loc.tcg_iters;
+        st->c_applies += loc.c_applies;
+        st->backtracks += loc.backtracks;
+    }
+    // keep the rotated buffers consistent for the next call
+    c->X = X;
+    c->X0 = X0;
+    c->X1 = X1;
+    c->G = G;
+    c->G1 = G1;
+    return 0;
+}
+
+int mm_timing_enable(mm_ctx *c, int32_t on) {
+    if (!c) return fail(MM_E_ARG, "null context");
+    c->timing = on != 0;
+    if (c->timing && c->evpool.empty()) {
+        c->evpool.resize(2 * 4096);
+        for (auto &e : c->evpool) HIPC(hipEventCreate(&e));
+    }
+    c->spmv_launches = 0;
+    c->spmv_ms = 0.0;
+    return 0;
+}
+
+int mm_timing_get(mm_ctx *c, mm_kernel_timing *o) {
+    if (!c || !o) return fail(MM_E_ARG, "null argument");
+    o->spmv_launches = c->spmv_launches;
+    o->spmv_ms = c->spmv_ms;
+    o->spmv_bytes = c->spmv_bytes;
+    return 0;
+}
+
+}  // extern "C"

@@ -1,0 +1,139 @@
+"""GPU (HIP, gfx950) path against the FP64 oracle and the reference fixtures, on the
+reference's test mesh (9,698 nodes, 53,275 Tet4, shell transform).
+
+Tolerances: operators to 1e-12 relative (FP64 with a different summation order);
+energies to 1e-10 relative; the full loop's M.h to 1e-6 (the .dat fixture) and the VTU
+``m`` under the reference's own np.allclose.
+"""
+import numpy as np
+import pytest
+
+from conftest import read_csv_values
+
+pytestmark = pytest.mark.gpu
+
+
+def rel(a, b):
+    a, b = np.asarray(a), np.asarray(b)
+    return float(np.max(np.abs(a - b)) / max(np.max(np.abs(b)), 1e-300))
+
+
+@pytest.fixture(scope="module")
+def gpu(cube_arrays, cube_krn):
+    from mammos_mumag_amd import Model
+    print("[gpu] building the test-mesh context", flush=True)
+    return Model(*cube_arrays, cube_krn)
+
+
+def test_setup_matches_oracle(gpu, oracle_cube):
+    assert gpu.N == oracle_cube.N
+    assert abs(gpu.V / oracle_cube.V - 1) < 1e-14
+    assert rel(gpu.meas, oracle_cube.meas) < 1e-14
+    assert gpu.bnd["airbox"] == oracle_cube.airbox
+    assert np.array_equal(gpu.bnd["dirichlet"], oracle_cube.dirichlet)
+    assert gpu.bnd["R"] == oracle_cube.R
+
+
+def test_operators(gpu, oracle_cube):
+    O = oracle_cube
+    rng = np.random.default_rng(3)
+    v = rng.normal(size=3 * O.N)
+    x = rng.normal(size=O.N)
+    y = gpu.apply_C(gpu.to_device(v)).cpu().numpy()
+    assert rel(y, O.C @ v) < 1e-12
+    y = gpu.apply_K(gpu.to_device(x)).cpu().numpy()
+    assert rel(y, O.K @ x) < 1e-12
+    b = gpu.divergence(gpu.to_device(v)).cpu().numpy()
+    assert rel(b, O.Dxyz[0] @ v[0::3] + O.Dxyz[1] @ v[1::3] + O.Dxyz[2] @ v[2::3]) < 1e-12
+    g = gpu.gradient(gpu.to_device(x)).cpu().numpy()
+    ref = np.stack([O.Gxyz[a] @ x for a in range(3)], axis=1).ravel()
+    assert rel(g, ref) < 1e-12
+
+
+def test_pcg_jax_semantics(gpu, oracle_cube):
+    from oracle.solvers import jax_cg
+    O = oracle_cube
+    m = O.uniform_m([0.0, 0.0, 1.0])
+    b = O.Dxyz[2] @ m[2::3]
+    x_ref, it_ref = jax_cg(O.K, O.diagK, b, np.zeros(O.N), tol=1e-10)
+    x = gpu.empty1()
+    it, rc = gpu.pcg(gpu.to_device(b), x, 1e-10)
+    assert rc == 0
+    assert abs(it - it_ref) <= 1
+    assert rel(x.cpu().numpy(), x_ref) < 1e-9
+    # maxiter cap returns the warning code and exactly maxiter iterations
+    x = gpu.empty1()
+    it, rc = gpu.pcg(gpu.to_device(b), x, 1e-14, maxiter=5)
+    assert it == 5 and rc == 1
+    # a converged start does nothing
+    x = gpu.to_device(x_ref)
+    it, rc = gpu.pcg(gpu.to_device(b), x, 1e-6)
+    assert it == 0 and np.array_equal(x.cpu().numpy(), x_ref)
+
+
+def test_energies_match_fixtures(gpu, cube_params, golden):
+    from oracle.io import mu0_output
+    mu0 = mu0_output()
+    p = cube_params
+    m = gpu.to_device(gpu.uniform_m([0.0, 0.0, 1.0]))
+    xm, _ = gpu.vortex_xm()
+    E, _ = gpu.exani_eg(gpu.to_device(xm))
+    assert abs(E / mu0 / read_csv_values(golden / "exani_cube_vortex.csv")[0] - 1) < 1e-10
+    E, _ = gpu.exani_eg(m)
+    assert abs(E / mu0 / read_csv_values(golden / "exani_cube_uniform.csv")[0] - 1) < 1e-10
+    E = gpu.external_eg(gpu.to_device(gpu.uniform_m(p["m"])), p["hstart"] - p["hstep"], p["h"])
+    assert abs(E / mu0 / read_csv_values(golden / "external_cube.csv")[0] - 1) < 1e-10
+    E, _, _ = gpu.hmag_eg(m, gpu.empty1(), 1e-12)
+    assert abs(E / mu0 / read_csv_values(golden / "hmag_cube.csv")[1] - 1) < 1e-10
+
+
+def test_total_eg_and_precond(gpu, oracle_cube, cube_params):
+    from oracle.energies import total_eg
+    from oracle.loop import Minimizer
+    O, p = oracle_cube, cube_params
+    rng = np.random.default_rng(4)
+    m = O.uniform_m([0.1, 0.2, 0.97]) + 0.05 * rng.normal(size=3 * O.N) * np.repeat(O.meas > 0, 3)
+    m = (m.reshape(-1, 3) / np.where(np.linalg.norm(m.reshape(-1, 3), axis=1) == 0, 1,
+                                      np.linalg.norm(m.reshape(-1, 3), axis=1))[:, None]).ravel()
+    E_ref, gp_ref, u_ref, gf_ref = total_eg(O, m, 1e-10, 0.7, p["h"], 1, np.zeros(O.N))
+    u = gpu.empty1()
+    E, gp, gf, it = gpu.total_eg(gpu.to_device(m), 0.7, p["h"], 1e-10, 1, u)
+    assert abs(E / E_ref - 1) < 1e-10
+    assert rel(gf.cpu().numpy(), gf_ref) < 1e-9
+    assert rel(gp.cpu().numpy(), gp_ref) < 1e-9
+    assert rel(u.cpu().numpy(), u_ref) < 1e-9
+    # M_inv on the same state (truncated CG with the initial-m operator)
+    mini = Minimizer(O, p, O.uniform_m(p["m"]))
+    mini.gF = gf_ref
+    z_ref = mini.M_inv(gp_ref)
+    z, it = gpu.M_inv(gpu.to_device(O.uniform_m(p["m"])), gpu.to_device(gf_ref), gpu.to_device(gp_ref),
+                      p["precond_iter"])
+    assert 1 <= it <= p["precond_iter"]
+    assert rel(z.cpu().numpy(), z_ref) < 1e-8
+    # retraction and M.h
+    d = rng.normal(size=3 * O.N)
+    from oracle.minimizers import update_m
+    out = gpu.update_m(gpu.to_device(m), 0.3, gpu.to_device(d)).cpu().numpy()
+    assert rel(out, update_m(m, 0.3, d)) < 1e-14
+    from oracle.loop import compute_mh
+    assert abs(gpu.compute_mh(gpu.to_device(m), p["h"]) / compute_mh(O, m, p["h"]) - 1) < 1e-13
+
+
+def test_loop_matches_reference_fixture(gpu, cube_params, golden):
+    """The reference's test_loop criterion on the GPU path: loop/cube.dat + cube_0001.vtu."""
+    from mammos_mumag_amd.loop import run_loop
+    from oracle.io import mu0_output
+    saved = {}
+    rec, stats, _ = run_loop(gpu, cube_params, gpu.uniform_m(cube_params["m"]),
+                             save_callback=lambda c, m, u: saved.setdefault(c, (m.cpu().numpy(), u.cpu().numpy())))
+    print("[gpu] loop stats", stats, flush=True)
+    ref = np.loadtxt(golden / "loop_cube.dat")
+    got = np.array([[r[0], r[1], r[2], r[3] / mu0_output()] for r in rec])
+    assert got.shape == ref.shape
+    assert np.array_equal(got[:, :2], ref[:, :2])
+    assert np.max(np.abs(got[:, 2] / ref[:, 2] - 1)) < 1e-10
+    assert np.max(np.abs(got[:, 3] / ref[:, 3] - 1)) < 1e-10
+    assert np.allclose(ref, got)
+    z = np.load(golden / "cube_0001_vtu.npz")
+    assert sorted(saved) == [1]
+    assert np.allclose(z["m"], saved[1][0].reshape(-1, 3))
