@@ -1,0 +1,216 @@
+"""Benchmark: hysteresis field steps per second on one MI355X.
+
+One step = one field value of the hysteresis sweep: the full nonlinear-CG
+minimisation (``minimize``, loop.py:58-65 of the reference) at that field, on the
+synthetic 64-grain Voronoi cube of 125^3 voxels (~2.0 M magnet nodes, 3.65 M nodes,
+21.5 M Kuhn tetrahedra) from +8 T in -0.05 T steps.  The first W field steps are
+warm-up; the next K are timed.  Inputs are resident in HBM before timing starts.
+
+Prints one JSON line.  The FP64 oracle is timed on the host cores as a reported
+baseline in a child process, concurrently with the GPU run.
+"""
+import argparse
+import json
+import os
+import pathlib
+import subprocess
+import sys
+import tempfile
+import time
+
+ROOT = pathlib.Path(__file__).resolve().parent
+for _p in (ROOT, ROOT / "mammos-mumag_amd"):
+    if str(_p) not in sys.path:
+        sys.path.insert(0, str(_p))
+
+METRIC = "hysteresis field steps/s @2M nodes: learned+FP32 vs plain FP64 on one MI355X"
+HBM_PEAK_GBS = 8000.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+# --------------------------------------------------------------------------- CPU leg
+def cpu_worker(cfg, out_path, seed):
+    """Unit costs of the FP64 oracle on this host, at the benchmark's own mesh."""
+    import numpy as np
+
+    from mammos_mumag_amd.generate import build_config, loop_params
+    from oracle.assemble import Model
+    from oracle.energies import total_eg
+    from oracle.loop import Minimizer
+    from oracle.solvers import jax_cg
+
+    t0 = time.perf_counter()
+    g = build_config(cfg, seed=seed)
+    M = Model(g.coords(), g.tets(), g.elem_tags(), g.krn())
+    t_setup = time.perf_counter() - t0
+    p = loop_params(cfg)
+    m = M.uniform_m(p["m"])
+    b = M.Dxyz[0] @ m[0::3] + M.Dxyz[1] @ m[1::3] + M.Dxyz[2] @ m[2::3]
+    n_pcg = 20
+    t0 = time.perf_counter()
+    u, it = jax_cg(M.K, M.diagK, b, np.zeros(M.N), tol=1e-30, maxiter=n_pcg)
+    t_pcg = (time.perf_counter() - t0) / it
+    # one evaluation without PCG iterations (tol 1: the start already satisfies it)
+    reps = 3
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        E, gp, _, gF = total_eg(M, m, 1.0, p["hstart"], p["h"], 1, u)
+    t_eval = (time.perf_counter() - t0) / reps
+    mini = Minimizer(M, p, m)
+    mini.gF = gF
+    t0 = time.perf_counter()
+    mini.M_inv(gp)
+    n_tcg = mini.stats["tcg_iters"]
+    t_tcg = (time.perf_counter() - t0) / max(n_tcg, 1)
+    sample_s = t_pcg * it + t_eval * reps + t_tcg * n_tcg
+    with open(out_path, "w") as f:
+        json.dump(dict(t_pcg_iter=t_pcg, t_eval=t_eval, t_tcg_iter=t_tcg, setup_s=t_setup,
+                       sample_s=sample_s, n_pcg=it, n_tcg=n_tcg, cores=1), f)
+
+
+def start_cpu_leg(cfg, seed):
+    fd, path = tempfile.mkstemp(suffix=".json", prefix="mumag_cpu_")
+    os.close(fd)
+    env = dict(os.environ, OMP_NUM_THREADS="1", OPENBLAS_NUM_THREADS="1", MKL_NUM_THREADS="1")
+    proc = subprocess.Popen([sys.executable, str(ROOT / "bench.py"), "--cpu-worker", path,
+                             "--config", cfg, "--seed", str(seed)], env=env)
+    return proc, path
+
+
+# --------------------------------------------------------------------------- GPU leg
+def gpu_run(args):
+    import numpy as np
+    import torch
+
+    from mammos_mumag_amd import Model, _lib
+    from mammos_mumag_amd.generate import build_config, loop_params
+
+    t0 = time.perf_counter()
+    g = build_config(args.config, seed=args.seed)
+    n_nodes, n_elems = len(g.xs) * len(g.ys) * len(g.zs), 6 * (len(g.xs) - 1) * (len(g.ys) - 1) * (len(g.zs) - 1)
+    log(f"[bench] mesh {args.config}: {n_nodes} nodes, {n_elems} tets ({time.perf_counter() - t0:.1f} s)")
+    t0 = time.perf_counter()
+    model = Model.from_structured(g, structured=not args.general, precision=args.precision,
+                                  inner_tol=args.inner_tol)
+    log(f"[bench] context: {model.info} ({time.perf_counter() - t0:.1f} s)")
+    p = loop_params(args.config)
+    m = model.to_device(model.uniform_m(p["m"]))
+    m_ref = m.clone()
+    u = model.empty1()
+    gF = model.empty3()
+    hext = p["hstart"]
+    st = _lib.Stats()
+    for w in range(args.warmup):
+        t1 = time.perf_counter()
+        E = model.minimize(m, hext, p["h"], p, m_ref, u, gF, st)
+        log(f"[bench] warmup step {w}: h={hext:.2f} T E={E:.6e} {time.perf_counter() - t1:.3f} s {st.as_dict()}")
+        hext += p["hstep"]
+    st = _lib.Stats()
+    model.timing(True)
+    torch.cuda.synchronize()
+    t_start = time.perf_counter()
+    mh = None
+    for k in range(args.steps):
+        E = model.minimize(m, hext, p["h"], p, m_ref, u, gF, st)
+        hext += p["hstep"]
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t_start
+    timing = model.timing_get()
+    model.timing(False)
+    mh = model.compute_mh(m, p["h"])
+    log(f"[bench] timed {args.steps} steps: {elapsed:.3f} s, last h={hext - p['hstep']:.2f} T, "
+        f"M.h={mh:.6f} T, E={E:.6e}, {st.as_dict()}, {timing}")
+    return dict(elapsed=elapsed, stats=st.as_dict(), timing=timing, info=model.info,
+                n_nodes=n_nodes, n_elems=n_elems, n_mag_nodes=int(np.sum(model.meas > 0)))
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--config", default="cfg2")
+    ap.add_argument("--precision", default="fp64", choices=["fp64", "mixed"])
+    ap.add_argument("--seed", type=int, default=1000)
+    ap.add_argument("--general", action="store_true", help="unstructured kernels (SELL K, element loops)")
+    ap.add_argument("--inner-tol", type=float, default=0.0)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-worker", default=None, help=argparse.SUPPRESS)
+    args = ap.parse_args()
+    if args.cpu_worker:
+        cpu_worker(args.config, args.cpu_worker, args.seed)
+        return 0
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only (this build has no multi-GPU path); got --gpus {args.gpus}",
+              file=sys.stderr)
+        return 2
+    cpu = None
+    if not args.no_cpu_baseline:
+        cpu = start_cpu_leg(args.config, args.seed)
+    res = gpu_run(args)
+    st = res["stats"]
+    steps = args.steps
+    value = steps / res["elapsed"]
+    tm = res["timing"]
+    roof = None
+    kernels = []
+    prec = "FP32 inner solve" if args.precision == "mixed" else "FP64"
+    kname = ("k_grid_poisson (7-point Poisson stencil)" if res["info"]["structured"]
+             else "k_sell_spmv (Poisson K, SELL-64)")
+    for key, name in (("spmv", f"{kname}, {prec}"), ("update", f"k_pcg_update (fused PCG update + 2 dots), {prec}")):
+        if tm[f"{key}_launches"] > 0:
+            avg_s = tm[f"{key}_ms"] / tm[f"{key}_launches"] / 1e3
+            kernels.append(dict(kernel=name, total_ms=tm[f"{key}_ms"], avg_s=avg_s,
+                                bytes=tm[f"{key}_bytes"], launches=tm[f"{key}_launches"]))
+    if kernels:
+        k = max(kernels, key=lambda d: d["total_ms"])  # the dominant kernel of the timed steps
+        achieved = k["bytes"] / k["avg_s"] / 1e9
+        roof = dict(bound="hbm", achieved=round(achieved, 1), peak=HBM_PEAK_GBS, unit="GB/s",
+                    frac=round(achieved / HBM_PEAK_GBS, 4), traffic=None, kernel=k["kernel"],
+                    bytes_per_launch=k["bytes"], avg_launch_us=round(k["avg_s"] * 1e6, 2),
+                    launches=k["launches"],
+                    others=[dict(kernel=o["kernel"], avg_launch_us=round(o["avg_s"] * 1e6, 2),
+                                 achieved_GBs=round(o["bytes"] / o["avg_s"] / 1e9, 1))
+                            for o in kernels if o is not k])
+    cpu_res = None
+    if cpu is not None:
+        proc, path = cpu
+        proc.wait()
+        try:
+            with open(path) as f:
+                c = json.load(f)
+            per_step = (st["pcg_iters"] * c["t_pcg_iter"] + st["fcalls"] * c["t_eval"]
+                        + st["tcg_iters"] * c["t_tcg_iter"]) / steps
+            cpu_res = dict(value=round(1.0 / per_step, 6), unit="field steps/s", cores=c["cores"],
+                           kind="port",
+                           sample=(f"FP64 numpy/scipy oracle on the same {args.config} mesh: unit costs "
+                                   f"timed over {c['n_pcg']} PCG iterations, 3 PCG-free evaluations and "
+                                   f"{c['n_tcg']} truncated-CG iterations ({c['sample_s']:.1f} s), "
+                                   f"scaled by this run's per-step counts; single thread"))
+        except Exception as e:  # the baseline is reported, not required
+            log(f"[bench] cpu baseline failed: {e}")
+        finally:
+            os.unlink(path)
+    out = {
+        "metric": METRIC, "value": round(value, 6), "unit": "field steps/s", "n_gpus": 1,
+        "steps": steps, "warmup": args.warmup, "ms_per_step": round(1e3 * res["elapsed"] / steps, 3),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64" if args.precision == "fp64" else "f32+f64",
+        "data": "synthetic",
+        "config": {"workload": f"{args.config}: 64-grain Voronoi cube 250 nm, 125^3 voxels x 6 Kuhn tets, "
+                               f"{res['n_mag_nodes']} magnet nodes / {res['n_nodes']} nodes / "
+                               f"{res['n_elems']} tets, field steps from +8 T by -0.05 T",
+                   "precision": args.precision, "form": "plain (no learned warm start)",
+                   "per_step": {k: round(v / steps, 2) for k, v in st.items()}},
+        "roofline": roof,
+        "cpu_baseline": cpu_res,
+    }
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

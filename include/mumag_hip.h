@@ -55,13 +55,20 @@ typedef struct {
     double shell_R;           /* inner radius of the shell transform (if shell != NULL) */
     double shell_Rinf;        /* outer radius */
     double length_scale;      /* metres per coordinate unit; the reference fixes 1e-9 */
-    /* Optional structured Kuhn-grid description (all zero = unstructured). When set, the
-     * mesh MUST be the one produced by the structured generator: nodes ordered
-     * x-fastest on an (nx+1)(ny+1)(nz+1) rectilinear grid, 6 tets per voxel. */
+    /* Optional structured Kuhn-grid description (grid_n all zero = unstructured).  When
+     * set, the mesh must be the one the structured generator writes: nodes numbered
+     * i + (nx+1) (j + (ny+1) k) at (grid_x[i], grid_y[j], grid_z[k]); elements voxel-major
+     * (voxel v = vx + nx (vy + ny vz)) with the 6 Kuhn tetrahedra of each voxel in the
+     * generator's order; the magnetic elements exactly the voxels of the cube
+     * [grid_mag0, grid_mag0 + grid_magn)^3, of uniform edge; an air box (Dirichlet on the
+     * six faces).  mm_ctx_create validates all of this and fails with MM_E_MESH if a
+     * structured context was requested for a mesh that does not match. */
     int32_t grid_n[3];        /* voxels per axis of the whole grid */
     const double *grid_x;     /* host f64[nx+1], f64[ny+1], f64[nz+1] node coordinates */
     const double *grid_y;
     const double *grid_z;
+    int32_t grid_mag0[3];     /* first voxel of the magnet block per axis */
+    int32_t grid_magn;        /* magnet voxels per axis */
 } mm_mesh_desc;
 
 typedef struct {
@@ -69,7 +76,9 @@ typedef struct {
     int32_t device;           /* HIP device ordinal */
     double drop_tol;          /* drop |K_ij| < drop_tol * max(|K_ii|,|K_jj|); 0 keeps all */
     int32_t pcg_batch;        /* PCG iterations launched between host checks (0 = default) */
-    int32_t use_structured;   /* 1: use the Kuhn-grid stencil kernels when grid_n is set */
+    int32_t use_structured;   /* 1: use the Kuhn-grid stencil kernels (needs grid_n) */
+    double inner_tol;         /* MM_PREC_MIXED: relative residual reduction of each FP32
+                                 inner solve (0 = default 1e-3) */
 } mm_options;
 
 typedef struct {
@@ -164,11 +173,16 @@ int mm_apply_K(mm_ctx *ctx, const double *x, double *y);
 int mm_divergence(mm_ctx *ctx, const double *m, double *b);
 int mm_gradient(mm_ctx *ctx, const double *u, double *g);
 
-/* Event-timed statistics of the dominant kernels since the last reset. */
+/* HIP-event timing of the PCG kernels since the last mm_timing_enable: the K apply
+ * (SELL SpMV or 7-point stencil, FP64 or the FP32 inner solve) and the fused vector
+ * update.  Only launches that did work (before convergence) are counted. */
 typedef struct {
-    int64_t spmv_launches;    /* K SpMV launches inside PCG */
-    double spmv_ms;           /* summed device time of those launches (HIP events) */
-    int64_t spmv_bytes;       /* algorithmic bytes of one K SpMV launch */
+    int64_t spmv_launches;
+    double spmv_ms;           /* summed device time of those launches */
+    int64_t spmv_bytes;       /* algorithmic bytes of one launch */
+    int64_t update_launches;
+    double update_ms;
+    int64_t update_bytes;
 } mm_kernel_timing;
 int mm_timing_enable(mm_ctx *ctx, int32_t on);
 int mm_timing_get(mm_ctx *ctx, mm_kernel_timing *out);

@@ -1,5 +1,6 @@
 // C ABI of libmumag_hip.so: context creation, the energy/gradient evaluation,
-// the Jacobi PCG, the truncated-CG preconditioner and the nonlinear-CG minimiser.
+// the Jacobi PCG (FP64, or FP32-inner / FP64-outer refinement), the truncated-CG
+// preconditioner and the nonlinear-CG minimiser.
 //
 // The minimiser is a host-side control loop over device kernels; each iteration
 // returns only the few scalars the reference's control flow branches on.
@@ -15,6 +16,7 @@
 #include "../../include/mumag_hip.h"
 #include "assemble.h"
 #include "common.h"
+#include "structured.h"
 
 using namespace mm;
 
@@ -34,20 +36,34 @@ int fail(int code, const std::string &msg) {
             return fail(MM_E_HIP, std::string(#x) + ": " + hipGetErrorString(e_));         \
     } while (0)
 
+struct KernelTimer {
+    std::vector<hipEvent_t> ev;  // start/stop pairs, one per launch of a solve
+    int64_t launches = 0;
+    double ms = 0.0;
+    int64_t bytes = 0;           // algorithmic bytes of one launch
+};
+
 struct mm_ctx {
     int device = 0;
     hipStream_t s = nullptr;
     int precision = MM_PREC_FP64;
-    int N = 0, NE = 0, NEm = 0, Nm = 0;
+    bool structured = false;
+    int64_t N = 0, NE = 0, NEm = 0, Nm = 0;
     double V = 0.0;
     int pcg_batch = 16;
+    double inner_tol = 1e-3;
     int64_t k_nnz = 0, k_entries = 0;
     size_t bytes = 0;
     std::vector<void *> allocs;
 
-    double *meas = nullptr, *Dex = nullptr, *invDK = nullptr, *diagK = nullptr;
+    double *meas = nullptr, *Dex = nullptr, *invDK = nullptr;
+    float *invDK32 = nullptr;
     SellDev<double> K;
+    SellDev<float> K32;
     ElemDev E;
+    GridDev G;
+    PoissonGrid<double> P;
+    PoissonGrid<float> P32;
     MatCoef *mats = nullptr;
     int n_mats = 0;
     DevScalars *dsc = nullptr;
@@ -56,18 +72,16 @@ struct mm_ctx {
 
     // workspace (N)
     double *b = nullptr, *r = nullptr, *z = nullptr, *p = nullptr, *q = nullptr;
+    float *rhs32 = nullptr, *r32 = nullptr, *z32 = nullptr, *p32 = nullptr, *q32 = nullptr, *dx32 = nullptr;
     // workspace (3N)
     double *gC = nullptr, *gd = nullptr;
     double *tp = nullptr, *tr = nullptr, *tz = nullptr, *td = nullptr, *tCd = nullptr, *tHd = nullptr;
-    double *X = nullptr, *X0 = nullptr, *X1 = nullptr, *G = nullptr, *G1 = nullptr, *Dd = nullptr,
+    double *X = nullptr, *X0 = nullptr, *X1 = nullptr, *Gp = nullptr, *G1 = nullptr, *Dd = nullptr,
            *Z1 = nullptr, *gtmp = nullptr;
 
-    // timing of the K SpMV inside PCG
+    // event timing of the PCG kernels (K apply and vector update)
     bool timing = false;
-    std::vector<hipEvent_t> evpool;
-    int64_t spmv_launches = 0;
-    double spmv_ms = 0.0;
-    int64_t spmv_bytes = 0;
+    KernelTimer t_spmv, t_upd;
 
     Launch L() const {
         Launch l;
@@ -75,7 +89,6 @@ struct mm_ctx {
         l.red = red;
         l.dsc = dsc;
         l.mats = mats;
-        l.n_mats = n_mats;
         return l;
     }
 
@@ -87,6 +100,8 @@ struct mm_ctx {
         allocs.push_back(p_);
         bytes += count * sizeof(T);
         *ptr = (T *)p_;
+        e = hipMemset(p_, 0, std::max<size_t>(count, 1) * sizeof(T));
+        if (e != hipSuccess) return fail(MM_E_HIP, std::string("hipMemset: ") + hipGetErrorString(e));
         return 0;
     }
     template <typename T>
@@ -100,7 +115,8 @@ struct mm_ctx {
         if (s) (void)hipStreamSynchronize(s);
         for (void *a : allocs) (void)hipFree(a);
         if (hsc) (void)hipHostFree(hsc);
-        for (hipEvent_t e : evpool) (void)hipEventDestroy(e);
+        for (hipEvent_t e : t_spmv.ev) (void)hipEventDestroy(e);
+        for (hipEvent_t e : t_upd.ev) (void)hipEventDestroy(e);
         if (s) (void)hipStreamDestroy(s);
     }
 };
@@ -113,63 +129,160 @@ int sync_scalars(mm_ctx *c) {
     return 0;
 }
 
-// jax.scipy.sparse.linalg.cg semantics (see kernels.hip); x is updated in place.
-int pcg(mm_ctx *c, const double *b, double *x, double tol, double atol, int maxiter, int *iters) {
+// ---------------------------------------------------------------- operators
+
+void op_K(mm_ctx *c, const double *x, double *y, double *dot, const int *skip) {
+    if (c->structured)
+        launch_grid_poisson(c->L(), c->P, x, y, dot, skip);
+    else
+        launch_sell_spmv(c->L(), c->K, x, y, dot, skip);
+}
+void op_K32(mm_ctx *c, const float *x, float *y, double *dot, const int *skip) {
+    if (c->structured)
+        launch_grid_poisson32(c->L(), c->P32, x, y, dot, skip);
+    else
+        launch_sell_spmv32(c->L(), c->K32, x, y, dot, skip);
+}
+// y = C v and optionally b = Dx vx + Dy vy + Dz vz.  Structured kernels overwrite the
+// magnet entries only (the others stay zero); the element kernels accumulate.
+void op_C(mm_ctx *c, const double *v, double *y, double *b, const int *skip) {
     Launch L = c->L();
-    const int n = c->N;
-    launch_sell_spmv(L, c->K, x, c->q, nullptr, nullptr);
-    launch_pcg_init(L, n, b, c->q, c->r, c->z, c->p, c->invDK, tol, atol, maxiter);
-    int done = 0;
+    if (c->structured) {
+        launch_grid_exani(L, c->G, c->precision == MM_PREC_MIXED, v, y, b, skip);
+    } else {
+        launch_fill_zero(L, 3 * c->N, y, skip);
+        if (b) launch_fill_zero(L, c->N, b, skip);
+        launch_ebe_exani(L, c->E, v, y, b, skip);
+    }
+}
+void op_grad(mm_ctx *c, const double *u, double *g) {
+    Launch L = c->L();
+    if (c->structured) {
+        launch_grid_grad(L, c->G, c->precision == MM_PREC_MIXED, u, g);
+    } else {
+        launch_fill_zero(L, 3 * c->N, g, nullptr);
+        launch_ebe_grad(L, c->E, u, g, 1.0 / c->V);
+    }
+}
+
+// event pair idx of a timer (ignored past the pool)
+void tick(mm_ctx *c, KernelTimer &t, int idx, bool start) {
+    if (!c->timing || idx >= (int)t.ev.size() / 2) return;
+    (void)hipEventRecord(t.ev[2 * idx + (start ? 0 : 1)], c->s);
+}
+// add the durations of pairs [from, from + count) (after the stream has synchronised)
+int collect(KernelTimer &t, int from, int count) {
+    const int pool = (int)t.ev.size() / 2;
+    for (int j = from; j < std::min(from + count, pool); ++j) {
+        float ms = 0.f;
+        HIPC(hipEventElapsedTime(&ms, t.ev[2 * j], t.ev[2 * j + 1]));
+        t.ms += ms;
+        t.launches++;
+    }
+    return 0;
+}
+
+// One PCG solve on the device: batches of B iterations are queued, the host checks
+// the done flag between batches; launches past convergence are no-ops.
+template <typename T, typename KOP, typename UPD, typename PUPD>
+int run_iterations(mm_ctx *c, KOP kop, UPD upd, PUPD pupd, int *iters) {
     const int B = c->pcg_batch;
-    int launched = 0;
-    const int pool = (int)c->evpool.size() / 2;
+    int launched = 0, done = 0;
     while (!done) {
-        for (int k = 0; k < B; ++k) {
-            const bool tm = c->timing && launched < pool;
-            if (tm) HIPC(hipEventRecord(c->evpool[2 * launched], c->s));
-            launch_sell_spmv(L, c->K, c->p, c->q, &c->dsc->pcg_pq, &c->dsc->pcg_done);
-            if (tm) HIPC(hipEventRecord(c->evpool[2 * launched + 1], c->s));
-            ++launched;
-            launch_pcg_update(L, n, x, c->r, c->z, c->p, c->q, c->invDK);
-            launch_pcg_pupdate(L, n, c->p, c->z);
+        for (int k = 0; k < B; ++k, ++launched) {
+            tick(c, c->t_spmv, launched, true);
+            kop();
+            tick(c, c->t_spmv, launched, false);
+            tick(c, c->t_upd, launched, true);
+            upd();
+            tick(c, c->t_upd, launched, false);
+            pupd();
         }
         int rc = sync_scalars(c);
         if (rc) return rc;
         done = c->hsc->pcg_done;
     }
+    const int used = c->hsc->pcg_iter;
     if (c->timing) {
-        // launches past convergence are no-ops: count the first pcg_iter only
-        const int used = std::min(c->hsc->pcg_iter, pool);
-        for (int j = 0; j < used; ++j) {
-            float ms = 0.f;
-            HIPC(hipEventElapsedTime(&ms, c->evpool[2 * j], c->evpool[2 * j + 1]));
-            c->spmv_ms += ms;
-            c->spmv_launches++;
-        }
+        int rc = collect(c->t_spmv, 0, used);
+        if (!rc) rc = collect(c->t_upd, 0, used);
+        if (rc) return rc;
     }
-    if (iters) *iters = c->hsc->pcg_iter;
-    if (c->hsc->pcg_iter >= maxiter && c->hsc->pcg_rr > c->hsc->pcg_bound) return MM_W_MAXITER;
+    *iters = used;
     return 0;
+}
+
+// jax.scipy.sparse.linalg.cg semantics, FP64; x is updated in place.
+int pcg64(mm_ctx *c, const double *b, double *x, double tol, double atol, int maxiter, int *iters) {
+    Launch L = c->L();
+    const int64_t n = c->N;
+    op_K(c, x, c->q, nullptr, nullptr);
+    launch_pcg_init(L, n, b, c->q, c->r, c->z, c->p, c->invDK, tol, atol, maxiter);
+    int it = 0;
+    int rc = run_iterations<double>(
+        c, [&] { op_K(c, c->p, c->q, &c->dsc->pcg_pq, &c->dsc->pcg_done); },
+        [&] { launch_pcg_update(L, n, x, c->r, c->z, c->p, c->q, c->invDK); },
+        [&] { launch_pcg_pupdate(L, n, c->p, c->z); }, &it);
+    if (rc) return rc;
+    if (iters) *iters = it;
+    if (it >= maxiter && c->hsc->pcg_rr > c->hsc->pcg_bound) return MM_W_MAXITER;
+    return 0;
+}
+
+// Mixed precision: FP64 outer residual r = b - K x (stop when r.r <= max(tol^2 b.b,
+// atol^2)), FP32 inner Jacobi-PCG for K dx = r from dx = 0 to a relative residual
+// inner_tol, x += dx.  The answer meets the FP64 tolerance on the true residual.
+int pcg_mixed(mm_ctx *c, const double *b, double *x, double tol, double atol, int maxiter, int *iters) {
+    Launch L = c->L();
+    const int64_t n = c->N;
+    op_K(c, x, c->q, nullptr, nullptr);
+    launch_residual(L, n, b, c->q, c->rhs32, tol, atol, true);
+    int rc = sync_scalars(c);
+    if (rc) return rc;
+    int inner_total = 0;
+    const int max_outer = 50;
+    int outer = 0;
+    while (!c->hsc->ir_done && outer < max_outer && inner_total < maxiter) {
+        launch_fill_zero32(L, n, c->dx32, nullptr);
+        launch_pcg32_init(L, n, c->rhs32, c->r32, c->z32, c->p32, c->invDK32, c->inner_tol,
+                          maxiter - inner_total);
+        int it = 0;
+        rc = run_iterations<float>(
+            c, [&] { op_K32(c, c->p32, c->q32, &c->dsc->pcg_pq, &c->dsc->pcg_done); },
+            [&] { launch_pcg32_update(L, n, c->dx32, c->r32, c->z32, c->p32, c->q32, c->invDK32); },
+            [&] { launch_pcg32_pupdate(L, n, c->p32, c->z32); }, &it);
+        if (rc) return rc;
+        inner_total += it;
+        launch_add_correction(L, n, x, c->dx32);
+        op_K(c, x, c->q, nullptr, nullptr);
+        launch_residual(L, n, b, c->q, c->rhs32, tol, atol, false);
+        if ((rc = sync_scalars(c))) return rc;
+        ++outer;
+    }
+    if (iters) *iters = inner_total;
+    if (!c->hsc->ir_done) return MM_W_MAXITER;
+    return 0;
+}
+
+int pcg(mm_ctx *c, const double *b, double *x, double tol, double atol, int maxiter, int *iters) {
+    if (c->precision == MM_PREC_MIXED) return pcg_mixed(c, b, x, tol, atol, maxiter, iters);
+    return pcg64(c, b, x, tol, atol, maxiter, iters);
 }
 
 // E, g_proj, g_full for magnetisation m (energies.py:67-99).
 int total_eg(mm_ctx *c, const double *m, double hext, const double *hdir, double tol_u,
              int hmag_on, double *u, double *E, double *gproj, double *gfull, int *pcg_it) {
     Launch L = c->L();
-    const int n = c->N;
-    HIPC(hipMemsetAsync(c->gC, 0, 3 * (size_t)n * sizeof(double), c->s));
-    if (hmag_on > 0) HIPC(hipMemsetAsync(c->b, 0, (size_t)n * sizeof(double), c->s));
-    launch_ebe_exani(L, c->E, m, c->gC, hmag_on > 0 ? c->b : nullptr, nullptr);
+    op_C(c, m, c->gC, hmag_on > 0 ? c->b : nullptr, nullptr);
     int it = 0, rc = 0;
     if (hmag_on > 0) {
         rc = pcg(c, c->b, u, tol_u, 0.0, 10000, &it);
         if (rc < 0) return rc;
-        HIPC(hipMemsetAsync(c->gd, 0, 3 * (size_t)n * sizeof(double), c->s));
-        launch_ebe_grad(L, c->E, u, c->gd, 1.0 / c->V);
+        op_grad(c, u, c->gd);
     }
     const double zc0 = -hext * hdir[0] / c->V, zc1 = -hext * hdir[1] / c->V,
                  zc2 = -hext * hdir[2] / c->V;
-    launch_finish_eg(L, n, m, c->meas, zc0, zc1, zc2, c->gC, c->gd, gfull, gproj, &c->dsc->energy,
+    launch_finish_eg(L, c->N, m, c->meas, zc0, zc1, zc2, c->gC, c->gd, gfull, gproj, &c->dsc->energy,
                      hmag_on > 0);
     int rc2 = sync_scalars(c);
     if (rc2) return rc2;
@@ -182,11 +295,10 @@ int total_eg(mm_ctx *c, const double *m, double hext, const double *hdir, double
 int precond(mm_ctx *c, const double *m_ref, const double *gF, const double *g, int max_iter,
             double *zout) {
     Launch L = c->L();
-    const int n = c->N, n3 = 3 * c->N;
+    const int64_t n = c->N, n3 = 3 * c->N;
     launch_tcg_init(L, n3, g, c->Dex, c->tp, c->tr, c->tz, c->td);
     for (int i = 0; i < max_iter; ++i) {
-        launch_fill_zero(L, n3, c->tCd, &c->dsc->tcg_done);
-        launch_ebe_exani(L, c->E, c->td, c->tCd, nullptr, &c->dsc->tcg_done);
+        op_C(c, c->td, c->tCd, nullptr, &c->dsc->tcg_done);
         launch_hvec(L, n, m_ref, gF, c->tCd, c->td, c->tHd, &c->dsc->tcg_den, &c->dsc->tcg_done);
         launch_tcg_update(L, n3, c->tp, c->tr, c->tz, c->td, c->tHd, c->Dex);
         launch_tcg_dupdate(L, n3, c->td, c->tz);
@@ -198,12 +310,32 @@ int precond(mm_ctx *c, const double *m_ref, const double *gF, const double *g, i
 
 int reduce(mm_ctx *c, RedOp op, int64_t n, const double *a, const double *b, const double *cc,
            const double *d, double *out, int nout, double h0 = 0, double h1 = 0, double h2 = 0) {
-    Launch L = c->L();
-    launch_reduce(L, op, n, a, b, cc, d, h0, h1, h2, c->dsc->red);
+    launch_reduce(c->L(), op, n, a, b, cc, d, h0, h1, h2, c->dsc->red);
     int rc = sync_scalars(c);
     if (rc) return rc;
     for (int k = 0; k < nout; ++k) out[k] = c->hsc->red[k];
     return 0;
+}
+
+std::vector<MatCoef> material_table(const mm_mesh_desc *d, double V) {
+    const double mu0 = 4.0e-7 * M_PI, size = d->length_scale > 0 ? d->length_scale : 1e-9;
+    std::vector<MatCoef> mc(d->n_mats + 1);
+    for (int m = 0; m < d->n_mats; ++m) {
+        const bool mag = d->mat_Js[m] > 0;
+        const double Aeff = mag ? mu0 * d->mat_A[m] / (size * size) : 0.0;
+        const double Keff = mag ? mu0 * d->mat_K1[m] : 0.0;
+        mc[m].cex = 2.0 * Aeff / V;
+        mc[m].cK = -2.0 * Keff / V;
+        mc[m].Js = mag ? d->mat_Js[m] : 0.0;
+        for (int a = 0; a < 3; ++a) mc[m].u[a] = mag ? d->mat_u[3 * m + a] : 0.0;
+    }
+    mc[d->n_mats] = MatCoef{0.0, 0.0, 0.0, {0.0, 0.0, 0.0}};  // voxels outside the magnet block
+    return mc;
+}
+
+template <typename T>
+std::vector<T> convert(const std::vector<double> &v) {
+    return std::vector<T>(v.begin(), v.end());
 }
 
 }  // namespace
@@ -222,22 +354,35 @@ int mm_ctx_create(const mm_mesh_desc *d, const mm_options *opts, mm_ctx **out) {
     if (opts) o = *opts;
     if (o.precision != MM_PREC_FP64 && o.precision != MM_PREC_MIXED)
         return fail(MM_E_ARG, "unknown precision");
-    if (o.precision == MM_PREC_MIXED) return fail(MM_E_ARG, "mixed precision not available in this build");
+    for (int64_t i = 0; i < 4 * (int64_t)d->n_elems; ++i)
+        if (d->tets[i] < 0 || d->tets[i] >= d->n_nodes) return fail(MM_E_MESH, "element node index out of range");
+    for (int64_t e = 0; e < d->n_elems; ++e)
+        if (d->elem_mat[e] < 0 || d->elem_mat[e] >= d->n_mats)
+            return fail(MM_E_MESH, "element material index out of range");
 
+    const bool structured = o.use_structured != 0;
     HostOps H;
-    std::string err = assemble(*d, o.drop_tol, H, true);
-    if (!err.empty()) return fail(MM_E_MESH, err);
+    StructHost S;
+    if (structured) {
+        std::string err = structured_setup(*d, S);
+        if (!err.empty()) return fail(MM_E_MESH, err);
+        if (!(S.V > 0.0)) return fail(MM_E_MESH, "no magnetic element (all Js <= 0)");
+    } else {
+        std::string err = assemble(*d, o.drop_tol, H, true);
+        if (!err.empty()) return fail(MM_E_MESH, err);
+    }
 
     HIPC(hipSetDevice(o.device));
     mm_ctx *c = new mm_ctx();
     c->device = o.device;
     c->precision = o.precision;
-    c->N = H.N;
-    c->NE = H.NE;
-    c->NEm = (int)H.mag_elems.size();
-    c->Nm = H.n_mag_nodes;
-    c->V = H.V;
+    c->structured = structured;
+    c->N = d->n_nodes;
+    c->NE = d->n_elems;
+    c->V = structured ? S.V : H.V;
+    c->Nm = structured ? S.n_mag_nodes : H.n_mag_nodes;
     if (o.pcg_batch > 0) c->pcg_batch = o.pcg_batch;
+    if (o.inner_tol > 0) c->inner_tol = o.inner_tol;
     int rc = 0;
     auto bail = [&](int code) {
         delete c;
@@ -246,83 +391,129 @@ int mm_ctx_create(const mm_mesh_desc *d, const mm_options *opts, mm_ctx **out) {
     if (hipStreamCreateWithFlags(&c->s, hipStreamNonBlocking) != hipSuccess)
         return bail(fail(MM_E_HIP, "hipStreamCreate failed"));
 
-    // operators
-    if ((rc = c->upload(&c->meas, H.meas))) return bail(rc);
-    if ((rc = c->upload(&c->Dex, H.Dex))) return bail(rc);
-    if ((rc = c->upload(&c->diagK, H.diagK))) return bail(rc);
+    const std::vector<double> &meas = structured ? S.meas : H.meas;
+    const std::vector<double> &Dex = structured ? S.Dex : H.Dex;
+    const std::vector<double> &diagK = structured ? S.diagK : H.diagK;
+    if ((rc = c->upload(&c->meas, meas))) return bail(rc);
+    if ((rc = c->upload(&c->Dex, Dex))) return bail(rc);
     {
-        std::vector<double> inv(H.N);
-        for (int i = 0; i < H.N; ++i) inv[i] = 1.0 / H.diagK[i];
+        std::vector<double> inv(c->N);
+        for (int64_t i = 0; i < c->N; ++i) inv[i] = 1.0 / diagK[i];
         if ((rc = c->upload(&c->invDK, inv))) return bail(rc);
+        if ((rc = c->upload(&c->invDK32, convert<float>(inv)))) return bail(rc);
     }
     {
+        std::vector<MatCoef> mc = material_table(d, c->V);
+        if ((rc = c->upload(&c->mats, mc))) return bail(rc);
+        c->n_mats = d->n_mats;
+    }
+    if (structured) {
+        // 7-point Poisson stencil: voxel sizes and 1 / (6 h) per axis, FP64 and FP32
+        auto up = [&](const std::vector<double> &h, const double **ph, const double **pinv,
+                      const float **ph32, const float **pinv32) -> int {
+            std::vector<double> inv(h.size());
+            for (size_t i = 0; i < h.size(); ++i) inv[i] = 1.0 / (6.0 * h[i]);
+            double *a, *b;
+            float *a32, *b32;
+            int r_;
+            if ((r_ = c->upload(&a, h)) || (r_ = c->upload(&b, inv)) || (r_ = c->upload(&a32, convert<float>(h))) ||
+                (r_ = c->upload(&b32, convert<float>(inv))))
+                return r_;
+            *ph = a;
+            *pinv = b;
+            *ph32 = a32;
+            *pinv32 = b32;
+            return 0;
+        };
+        c->P.nx = c->P32.nx = S.nx;
+        c->P.ny = c->P32.ny = S.ny;
+        c->P.nz = c->P32.nz = S.nz;
+        if ((rc = up(S.hx, &c->P.hx, &c->P.inv6hx, &c->P32.hx, &c->P32.inv6hx)) ||
+            (rc = up(S.hy, &c->P.hy, &c->P.inv6hy, &c->P32.hy, &c->P32.inv6hy)) ||
+            (rc = up(S.hz, &c->P.hz, &c->P.inv6hz, &c->P32.hz, &c->P32.inv6hz)))
+            return bail(rc);
+        uint16_t *vm;
+        if ((rc = c->upload(&vm, S.vmat))) return bail(rc);
+        c->G.nx = S.nx;
+        c->G.ny = S.ny;
+        c->G.m0x = S.m0x;
+        c->G.m0y = S.m0y;
+        c->G.m0z = S.m0z;
+        c->G.mn = S.mn;
+        c->G.n_mats = d->n_mats;
+        c->G.vmat = vm;
+        c->G.fS = S.h / 6.0;
+        c->G.fM = S.h * S.h * S.h / 120.0;
+        c->G.fB = S.h * S.h / 24.0;
+        c->G.fG = S.h * S.h / (24.0 * c->V);
+        c->NEm = 6 * (int64_t)S.mn * S.mn * S.mn;
+    } else {
         std::vector<int> sp, scol;
         std::vector<double> sval;
-        csr_to_sell(H.N, H.k_rowptr, H.k_col, H.k_val, sp, scol, sval);
+        csr_to_sell((int)c->N, H.k_rowptr, H.k_col, H.k_val, sp, scol, sval);
         int *d_sp, *d_col;
         double *d_val;
-        if ((rc = c->upload(&d_sp, sp)) || (rc = c->upload(&d_col, scol)) || (rc = c->upload(&d_val, sval)))
+        float *d_val32;
+        if ((rc = c->upload(&d_sp, sp)) || (rc = c->upload(&d_col, scol)) || (rc = c->upload(&d_val, sval)) ||
+            (rc = c->upload(&d_val32, convert<float>(sval))))
             return bail(rc);
-        c->K.n_rows = H.N;
-        c->K.n_slices = (int)sp.size() - 1;
-        c->K.slice_ptr = d_sp;
-        c->K.col = d_col;
+        c->K.n_rows = c->K32.n_rows = (int)c->N;
+        c->K.n_slices = c->K32.n_slices = (int)sp.size() - 1;
+        c->K.slice_ptr = c->K32.slice_ptr = d_sp;
+        c->K.col = c->K32.col = d_col;
         c->K.val = d_val;
+        c->K32.val = d_val32;
         c->k_nnz = (int64_t)H.k_val.size();
         c->k_entries = (int64_t)sval.size();
-        // algorithmic bytes of one SpMV: stored entries (val + col), row pointers,
-        // x read once, y written once
-        c->spmv_bytes = c->k_nnz * (8 + 4) + 4 * (int64_t)(H.N + 1) + 2 * 8 * (int64_t)H.N;
-    }
-    {
+        c->NEm = (int64_t)H.mag_elems.size();
         std::vector<int4> t4(c->NEm);
-        for (int q = 0; q < c->NEm; ++q)
-            t4[q] = make_int4(H.mag_tets[4 * q], H.mag_tets[4 * q + 1], H.mag_tets[4 * q + 2],
-                              H.mag_tets[4 * q + 3]);
+        for (int64_t q = 0; q < c->NEm; ++q)
+            t4[q] = make_int4(H.mag_tets[4 * q], H.mag_tets[4 * q + 1], H.mag_tets[4 * q + 2], H.mag_tets[4 * q + 3]);
         int4 *dt;
         double *dG, *dv;
         int *dm;
-        if ((rc = c->upload(&dt, t4)) || (rc = c->upload(&dG, H.mag_G)) ||
-            (rc = c->upload(&dv, H.mag_vol)) || (rc = c->upload(&dm, H.mag_mat)))
+        if ((rc = c->upload(&dt, t4)) || (rc = c->upload(&dG, H.mag_G)) || (rc = c->upload(&dv, H.mag_vol)) ||
+            (rc = c->upload(&dm, H.mag_mat)))
             return bail(rc);
-        c->E.n = c->NEm;
+        c->E.n = (int)c->NEm;
         c->E.tets = dt;
         c->E.G = dG;
         c->E.vol = dv;
         c->E.mat = dm;
     }
-    {
-        const double mu0 = 4.0e-7 * M_PI, size = d->length_scale > 0 ? d->length_scale : 1e-9;
-        std::vector<MatCoef> mc(d->n_mats);
-        for (int m = 0; m < d->n_mats; ++m) {
-            const bool mag = d->mat_Js[m] > 0;
-            const double Aeff = mag ? mu0 * d->mat_A[m] / (size * size) : 0.0;
-            const double Keff = mag ? mu0 * d->mat_K1[m] : 0.0;
-            mc[m].cex = 2.0 * Aeff / H.V;
-            mc[m].cK = -2.0 * Keff / H.V;
-            mc[m].Js = mag ? d->mat_Js[m] : 0.0;
-            for (int a = 0; a < 3; ++a) mc[m].u[a] = mag ? d->mat_u[3 * m + a] : 0.0;
-        }
-        if ((rc = c->upload(&c->mats, mc))) return bail(rc);
-        c->n_mats = d->n_mats;
-    }
+    // algorithmic bytes of one K apply in the PCG loop: stored entries (value + column),
+    // row pointers, x read once, y written once; the stencil stores no matrix.
+    const int s_k = c->precision == MM_PREC_MIXED ? 4 : 8;
+    c->t_spmv.bytes = structured ? 2 * (int64_t)s_k * c->N
+                                 : c->k_nnz * (s_k + 4) + 4 * (c->N + 1) + 2 * (int64_t)s_k * c->N;
+    // PCG update: reads x, r, p, q, invD; writes x, r, z
+    c->t_upd.bytes = 8 * (int64_t)s_k * c->N;
+
     if ((rc = c->alloc(&c->dsc, 1))) return bail(rc);
-    HIPC(hipMemset(c->dsc, 0, sizeof(DevScalars)));
+    {
+        DevScalars init{};
+        init.pcg_done = 1;
+        init.tcg_done = 1;
+        init.ir_done = 1;
+        HIPC(hipMemcpy(c->dsc, &init, sizeof(DevScalars), hipMemcpyHostToDevice));
+    }
     if (hipHostMalloc((void **)&c->hsc, sizeof(DevScalars), hipHostMallocDefault) != hipSuccess)
         return bail(fail(MM_E_HIP, "hipHostMalloc failed"));
     if ((rc = c->alloc(&c->red.partials, 4 * (size_t)kMaxPartials))) return bail(rc);
-    if ((rc = c->alloc(&c->red.counter, 1))) return bail(rc);
-    HIPC(hipMemset(c->red.counter, 0, sizeof(unsigned)));
 
-    const size_t n = H.N, n3 = 3 * n;
+    const size_t n = c->N, n3 = 3 * n;
     double **vN[] = {&c->b, &c->r, &c->z, &c->p, &c->q};
     for (double **v : vN)
         if ((rc = c->alloc(v, n))) return bail(rc);
+    if (c->precision == MM_PREC_MIXED) {
+        float **vF[] = {&c->rhs32, &c->r32, &c->z32, &c->p32, &c->q32, &c->dx32};
+        for (float **v : vF)
+            if ((rc = c->alloc(v, n))) return bail(rc);
+    }
     double **v3[] = {&c->gC, &c->gd, &c->tp, &c->tr, &c->tz, &c->td, &c->tCd, &c->tHd,
-                     &c->X,  &c->X0, &c->X1, &c->G,  &c->G1, &c->Dd, &c->Z1, &c->gtmp};
+                     &c->X,  &c->X0, &c->X1, &c->Gp, &c->G1, &c->Dd, &c->Z1, &c->gtmp};
     for (double **v : v3)
         if ((rc = c->alloc(v, n3))) return bail(rc);
-    HIPC(hipMemset(c->gd, 0, n3 * sizeof(double)));
     HIPC(hipDeviceSynchronize());
     *out = c;
     return 0;
@@ -343,7 +534,7 @@ int mm_ctx_info(mm_ctx *c, mm_info *o) {
     o->k_nnz = c->k_nnz;
     o->k_sell_entries = c->k_entries;
     o->volume = c->V;
-    o->structured = 0;
+    o->structured = c->structured ? 1 : 0;
     o->precision = c->precision;
     o->device_bytes = (int64_t)c->bytes;
     return 0;
@@ -358,15 +549,18 @@ int mm_get_meas_host(mm_ctx *c, double *meas_host) {
 int mm_get_diag_host(mm_ctx *c, double *dex_host, double *diagK_host) {
     if (!c) return fail(MM_E_ARG, "null argument");
     if (dex_host) HIPC(hipMemcpy(dex_host, c->Dex, 3 * c->N * sizeof(double), hipMemcpyDeviceToHost));
-    if (diagK_host) HIPC(hipMemcpy(diagK_host, c->diagK, c->N * sizeof(double), hipMemcpyDeviceToHost));
+    if (diagK_host) {
+        std::vector<double> inv(c->N);
+        HIPC(hipMemcpy(inv.data(), c->invDK, c->N * sizeof(double), hipMemcpyDeviceToHost));
+        for (int64_t i = 0; i < c->N; ++i) diagK_host[i] = 1.0 / inv[i];
+    }
     return 0;
 }
 
 int mm_external_eg(mm_ctx *c, const double *m, double hext, const double hdir[3], double *E_host,
                    double *g) {
     if (!c || !m || !hdir || !E_host) return fail(MM_E_ARG, "null argument");
-    Launch L = c->L();
-    launch_external(L, c->N, m, c->meas, -hext * hdir[0] / c->V, -hext * hdir[1] / c->V,
+    launch_external(c->L(), c->N, m, c->meas, -hext * hdir[0] / c->V, -hext * hdir[1] / c->V,
                     -hext * hdir[2] / c->V, g, &c->dsc->energy);
     int rc = sync_scalars(c);
     if (rc) return rc;
@@ -376,10 +570,9 @@ int mm_external_eg(mm_ctx *c, const double *m, double hext, const double hdir[3]
 
 int mm_exani_eg(mm_ctx *c, const double *m, double *E_host, double *g) {
     if (!c || !m || !E_host || !g) return fail(MM_E_ARG, "null argument");
-    Launch L = c->L();
     HIPC(hipMemsetAsync(g, 0, 3 * (size_t)c->N * sizeof(double), c->s));
-    launch_ebe_exani(L, c->E, m, g, nullptr, nullptr);
-    launch_reduce(L, RED_DOT, 3 * (int64_t)c->N, m, g, nullptr, nullptr, 0, 0, 0, c->dsc->red);
+    op_C(c, m, g, nullptr, nullptr);
+    launch_reduce(c->L(), RED_DOT, 3 * c->N, m, g, nullptr, nullptr, 0, 0, 0, c->dsc->red);
     int rc = sync_scalars(c);
     if (rc) return rc;
     *E_host = 0.5 * c->hsc->red[0];
@@ -389,16 +582,13 @@ int mm_exani_eg(mm_ctx *c, const double *m, double *E_host, double *g) {
 int mm_hmag_eg(mm_ctx *c, const double *m, double *u, double tol, double *E_host, double *g,
                int32_t *pcg_iters_host) {
     if (!c || !m || !u || !E_host || !g) return fail(MM_E_ARG, "null argument");
-    Launch L = c->L();
-    HIPC(hipMemsetAsync(c->b, 0, (size_t)c->N * sizeof(double), c->s));
-    HIPC(hipMemsetAsync(c->gC, 0, 3 * (size_t)c->N * sizeof(double), c->s));
-    launch_ebe_exani(L, c->E, m, c->gC, c->b, nullptr);
+    op_C(c, m, c->gC, c->b, nullptr);
     int it = 0;
     int rc = pcg(c, c->b, u, tol, 0.0, 10000, &it);
     if (rc < 0) return rc;
     HIPC(hipMemsetAsync(g, 0, 3 * (size_t)c->N * sizeof(double), c->s));
-    launch_ebe_grad(L, c->E, u, g, 1.0 / c->V);
-    launch_reduce(L, RED_DOT, 3 * (int64_t)c->N, m, g, nullptr, nullptr, 0, 0, 0, c->dsc->red);
+    op_grad(c, u, g);
+    launch_reduce(c->L(), RED_DOT, 3 * c->N, m, g, nullptr, nullptr, 0, 0, 0, c->dsc->red);
     int rc2 = sync_scalars(c);
     if (rc2) return rc2;
     *E_host = 0.5 * c->hsc->red[0];
@@ -467,14 +657,14 @@ int mm_compute_mh(mm_ctx *c, const double *m, const double hdir[3], double *mh) 
 int mm_apply_C(mm_ctx *c, const double *v, double *y) {
     if (!c || !v || !y) return fail(MM_E_ARG, "null argument");
     HIPC(hipMemsetAsync(y, 0, 3 * (size_t)c->N * sizeof(double), c->s));
-    launch_ebe_exani(c->L(), c->E, v, y, nullptr, nullptr);
+    op_C(c, v, y, nullptr, nullptr);
     HIPC(hipStreamSynchronize(c->s));
     return 0;
 }
 
 int mm_apply_K(mm_ctx *c, const double *x, double *y) {
     if (!c || !x || !y) return fail(MM_E_ARG, "null argument");
-    launch_sell_spmv(c->L(), c->K, x, y, nullptr, nullptr);
+    op_K(c, x, y, nullptr, nullptr);
     HIPC(hipStreamSynchronize(c->s));
     return 0;
 }
@@ -482,8 +672,7 @@ int mm_apply_K(mm_ctx *c, const double *x, double *y) {
 int mm_divergence(mm_ctx *c, const double *m, double *b) {
     if (!c || !m || !b) return fail(MM_E_ARG, "null argument");
     HIPC(hipMemsetAsync(b, 0, (size_t)c->N * sizeof(double), c->s));
-    HIPC(hipMemsetAsync(c->gtmp, 0, 3 * (size_t)c->N * sizeof(double), c->s));
-    launch_ebe_exani(c->L(), c->E, m, c->gtmp, b, nullptr);
+    op_C(c, m, c->gtmp, b, nullptr);
     HIPC(hipStreamSynchronize(c->s));
     return 0;
 }
@@ -491,7 +680,7 @@ int mm_divergence(mm_ctx *c, const double *m, double *b) {
 int mm_gradient(mm_ctx *c, const double *u, double *g) {
     if (!c || !u || !g) return fail(MM_E_ARG, "null argument");
     HIPC(hipMemsetAsync(g, 0, 3 * (size_t)c->N * sizeof(double), c->s));
-    launch_ebe_grad(c->L(), c->E, u, g, 1.0 / c->V);
+    op_grad(c, u, g);
     HIPC(hipStreamSynchronize(c->s));
     return 0;
 }
@@ -501,17 +690,18 @@ int mm_minimize(mm_ctx *c, double *m_inout, double hext, const double hdir[3],
                 mm_stats *st) {
     if (!c || !m_inout || !hdir || !P || !m_ref || !u || !gF || !E_host)
         return fail(MM_E_ARG, "null argument");
-    const int n = c->N;
-    const int64_t n3 = 3 * (int64_t)n;
+    const int64_t n = c->N;
+    const int64_t n3 = 3 * n;
     const size_t bytes3 = n3 * sizeof(double);
     Launch L = c->L();
     const double tol = P->tol_fun;
     const int iter_max = P->iter_max > 0 ? P->iter_max : 100000;
     mm_stats loc{};
     int rc = 0;
-    double *X = c->X, *X0 = c->X0, *X1 = c->X1, *G = c->G, *G1 = c->G1, *D = c->Dd, *Z1 = c->Z1;
+    double *X = c->X, *X0 = c->X0, *X1 = c->X1, *G = c->Gp, *G1 = c->G1, *D = c->Dd, *Z1 = c->Z1;
     HIPC(hipMemcpyAsync(X, m_inout, bytes3, hipMemcpyDeviceToDevice, c->s));
-    HIPC(hipMemsetAsync(&c->dsc->red[15], 0, sizeof(double), c->s));
+    if ((rc = sync_scalars(c))) return rc;
+    const long long tcg0 = c->hsc->tcg_total;
 
     auto feval = [&](const double *x, double *E, double *g) -> int {
         int it = 0;
@@ -546,8 +736,7 @@ int mm_minimize(mm_ctx *c, double *m_inout, double hext, const double hdir[3],
         // restart if d is not a sufficient descent direction (minimizers.py:171-173)
         double dg3[3];
         if ((rc = reduce(c, RED_DG3, n3, D, G, nullptr, nullptr, dg3, 3))) return rc;
-        if (dg3[0] > -0.001 * std::sqrt(dg3[1]) * std::sqrt(dg3[2]))
-            launch_axpby(L, n3, -1.0, G, 0.0, nullptr, D);
+        if (dg3[0] > -0.001 * std::sqrt(dg3[1]) * std::sqrt(dg3[2])) launch_axpby(L, n3, -1.0, G, 0.0, nullptr, D);
         // ---- line search (minimizers.py:92-137)
         double f1 = 0.0;
         {
@@ -596,8 +785,7 @@ int mm_minimize(mm_ctx *c, double *m_inout, double hext, const double hdir[3],
         const double beta = std::max(yzd[0] / yzd[1], 0.0);
         launch_axpby(L, n3, -1.0, Z1, beta, D, D);
         if ((rc = reduce(c, RED_DG3, n3, D, G, nullptr, nullptr, dg3, 3))) return rc;
-        if (dg3[0] > -0.001 * std::sqrt(dg3[1]) * std::sqrt(dg3[2]))
-            launch_axpby(L, n3, -1.0, Z1, 0.0, nullptr, D);
+        if (dg3[0] > -0.001 * std::sqrt(dg3[1]) * std::sqrt(dg3[2])) launch_axpby(L, n3, -1.0, Z1, 0.0, nullptr, D);
         // shift state: x0 <- x, x <- x1, f0 <- f, f <- f1, g <- g1
         std::swap(X0, X);
         std::swap(X, X1);
@@ -609,7 +797,7 @@ int mm_minimize(mm_ctx *c, double *m_inout, double hext, const double hdir[3],
     HIPC(hipMemcpyAsync(m_inout, X, bytes3, hipMemcpyDeviceToDevice, c->s));
     if ((rc = sync_scalars(c))) return rc;
     loc.ncg_iters = k;
-    loc.tcg_iters = (int64_t)c->hsc->red[15];
+    loc.tcg_iters = c->hsc->tcg_total - tcg0;
     *E_host = f;
     if (st) {
         st->ncg_iters += loc.ncg_iters;
@@ -620,11 +808,11 @@ int mm_minimize(mm_ctx *c, double *m_inout, double hext, const double hdir[3],
         st->c_applies += loc.c_applies;
         st->backtracks += loc.backtracks;
     }
-    // keep the rotated buffers consistent for the next call
+    // keep the rotated buffers for the next call
     c->X = X;
     c->X0 = X0;
     c->X1 = X1;
-    c->G = G;
+    c->Gp = G;
     c->G1 = G1;
     return 0;
 }
@@ -632,20 +820,25 @@ int mm_minimize(mm_ctx *c, double *m_inout, double hext, const double hdir[3],
 int mm_timing_enable(mm_ctx *c, int32_t on) {
     if (!c) return fail(MM_E_ARG, "null context");
     c->timing = on != 0;
-    if (c->timing && c->evpool.empty()) {
-        c->evpool.resize(2 * 4096);
-        for (auto &e : c->evpool) HIPC(hipEventCreate(&e));
+    for (KernelTimer *t : {&c->t_spmv, &c->t_upd}) {
+        if (c->timing && t->ev.empty()) {
+            t->ev.resize(2 * 10240);
+            for (auto &e : t->ev) HIPC(hipEventCreate(&e));
+        }
+        t->launches = 0;
+        t->ms = 0.0;
     }
-    c->spmv_launches = 0;
-    c->spmv_ms = 0.0;
     return 0;
 }
 
 int mm_timing_get(mm_ctx *c, mm_kernel_timing *o) {
     if (!c || !o) return fail(MM_E_ARG, "null argument");
-    o->spmv_launches = c->spmv_launches;
-    o->spmv_ms = c->spmv_ms;
-    o->spmv_bytes = c->spmv_bytes;
+    o->spmv_launches = c->t_spmv.launches;
+    o->spmv_ms = c->t_spmv.ms;
+    o->spmv_bytes = c->t_spmv.bytes;
+    o->update_launches = c->t_upd.launches;
+    o->update_ms = c->t_upd.ms;
+    o->update_bytes = c->t_upd.bytes;
     return 0;
 }
 

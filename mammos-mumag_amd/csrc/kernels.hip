@@ -1,16 +1,17 @@
 // HIP kernels for gfx950 (CDNA4): the per-field-step hot path of mammos-mumag.
 //
-// Streaming kernels use 256-thread blocks (4 wavefronts of 64).  Every scalar that
-// an iteration needs (alpha, beta, convergence flags) is produced on the device by
-// the "last block" of the preceding reduction, so solver iterations are launched
-// back to back without a host round trip.  Reductions are deterministic: each block
-// writes one partial per output, and the last block to finish sums the partials in
-// a fixed order.
+// Streaming kernels use 256-thread blocks (4 wavefronts of 64) with grid-stride loops
+// over at most kMaxPartials blocks.  Reductions are deterministic and free of
+// contended atomics: every block writes one partial per output slot, and a one-block
+// finisher kernel (k_fin) sums the partials in a fixed order and derives the scalars
+// an iteration needs (alpha, beta, convergence flags) on the device, so solver
+// iterations are queued back to back without a host round trip.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
 
 #include "common.h"
+#include "kuhn_tables.h"
 
 namespace mm {
 
@@ -27,17 +28,11 @@ __device__ __forceinline__ double wave_max(double v) {
     return v;
 }
 
-__device__ __forceinline__ double load_agent(const double *p) {
-    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-
-// Reduce NV values per thread over the grid.  Bit i of maxmask selects max instead
-// of sum for slot i.  The last block calls fin(vals) on thread 0 with the totals.
-// Must be reached by every thread of every block (blockDim.x == kBlock).
-template <int NV, typename Fin>
-__device__ void grid_reduce(double (&v)[NV], unsigned maxmask, const Reducer &R, Fin fin) {
+// Block-level reduction of NV values; thread 0 writes partials[i * kMaxPartials + block].
+// Bit i of maxmask selects max (identity 0, used on absolute values) instead of sum.
+template <int NV>
+__device__ __forceinline__ void block_partials(double (&v)[NV], unsigned maxmask, double *partials) {
     __shared__ double sh[NV][kBlock / kWave];
-    __shared__ int is_last;
     const int lane = threadIdx.x & (kWave - 1), w = threadIdx.x / kWave;
 #pragma unroll
     for (int i = 0; i < NV; ++i) v[i] = ((maxmask >> i) & 1u) ? wave_max(v[i]) : wave_sum(v[i]);
@@ -50,38 +45,11 @@ __device__ void grid_reduce(double (&v)[NV], unsigned maxmask, const Reducer &R,
 #pragma unroll
         for (int i = 0; i < NV; ++i) {
             double a = sh[i][0];
+#pragma unroll
             for (int k = 1; k < kBlock / kWave; ++k)
                 a = ((maxmask >> i) & 1u) ? fmax(a, sh[i][k]) : a + sh[i][k];
-            R.partials[i * kMaxPartials + blockIdx.x] = a;
+            partials[i * kMaxPartials + blockIdx.x] = a;
         }
-        __threadfence();
-        unsigned prev = atomicAdd(R.counter, 1u);
-        is_last = (prev == gridDim.x - 1);
-    }
-    __syncthreads();
-    if (!is_last) return;
-    __threadfence();
-    double tot[NV];
-#pragma unroll
-    for (int i = 0; i < NV; ++i) {
-        const bool mx = (maxmask >> i) & 1u;
-        double a = mx ? 0.0 : 0.0;
-        for (int j = threadIdx.x; j < (int)gridDim.x; j += kBlock) {
-            double p = load_agent(&R.partials[i * kMaxPartials + j]);
-            a = mx ? fmax(a, p) : a + p;
-        }
-        a = mx ? wave_max(a) : wave_sum(a);
-        __syncthreads();
-        if (lane == 0) sh[i][w] = a;
-        __syncthreads();
-        a = sh[i][0];
-        for (int k = 1; k < kBlock / kWave; ++k) a = mx ? fmax(a, sh[i][k]) : a + sh[i][k];
-        tot[i] = a;
-    }
-    if (threadIdx.x == 0) {
-        fin(tot);
-        *R.counter = 0u;
-        __threadfence();
     }
 }
 
@@ -92,12 +60,105 @@ static inline int grid_for(int64_t n, int per_thread = 1) {
     return (int)b;
 }
 
+// One block: totals of the partials, then the scalar logic selected by op.
+__global__ __launch_bounds__(kBlock) void k_fin(const double *__restrict__ partials, FinArgs a,
+                                                DevScalars *dsc) {
+    if (a.skip && *a.skip) return;
+    __shared__ double sh[kBlock / kWave];
+    const int lane = threadIdx.x & (kWave - 1), w = threadIdx.x / kWave;
+    double tot[4] = {0.0, 0.0, 0.0, 0.0};
+    for (int i = 0; i < a.nv; ++i) {
+        const bool mx = (a.maxmask >> i) & 1u;
+        double s = 0.0;
+        for (int j = threadIdx.x; j < a.nparts; j += kBlock) {
+            const double p = partials[i * kMaxPartials + j];
+            s = mx ? fmax(s, p) : s + p;
+        }
+        s = mx ? wave_max(s) : wave_sum(s);
+        if (lane == 0) sh[w] = s;
+        __syncthreads();
+        double t = sh[0];
+        for (int k = 1; k < kBlock / kWave; ++k) t = mx ? fmax(t, sh[k]) : t + sh[k];
+        tot[i] = t;
+        __syncthreads();
+    }
+    if (threadIdx.x != 0) return;
+    switch (a.op) {
+        case FIN_STORE:
+            for (int i = 0; i < a.nv; ++i) a.out[i] = tot[i];
+            break;
+        case FIN_PCG_INIT: {  // tot = (r.z, r.r, b.b)
+            dsc->pcg_gamma = tot[0];
+            dsc->pcg_rr = tot[1];
+            dsc->pcg_bs = tot[2];
+            const double bound = fmax(a.tol * a.tol * tot[2], a.atol * a.atol);
+            dsc->pcg_bound = bound;
+            dsc->pcg_iter = 0;
+            dsc->pcg_maxiter = a.maxiter;
+            dsc->pcg_done = !((tot[1] > bound) && (0 < a.maxiter));
+            break;
+        }
+        case FIN_PCG_UPDATE: {  // tot = (r.z, r.r)
+            dsc->pcg_beta = tot[0] / dsc->pcg_gamma;
+            dsc->pcg_gamma = tot[0];
+            dsc->pcg_rr = tot[1];
+            const int it = dsc->pcg_iter + 1;
+            dsc->pcg_iter = it;
+            dsc->pcg_total += 1;
+            dsc->pcg_done = !((tot[1] > dsc->pcg_bound) && (it < dsc->pcg_maxiter));
+            break;
+        }
+        case FIN_TCG_INIT: {  // tot = (g.g, r.z)
+            const double nrm = sqrt(tot[0]);
+            dsc->tcg_eps = fmin(0.5, sqrt(nrm) * nrm);
+            dsc->tcg_rz = tot[1];
+            dsc->tcg_done = 0;
+            dsc->tcg_neg = 0;
+            dsc->tcg_iter = 0;
+            break;
+        }
+        case FIN_TCG_UPDATE: {  // tot = (r'.r', r'.z')
+            const bool neg = dsc->tcg_den <= 0.0;
+            const bool conv = sqrt(tot[0]) <= dsc->tcg_eps;
+            const double rz = dsc->tcg_rz;
+            dsc->tcg_beta = (!neg && !conv) ? tot[1] / (rz + 1e-16) : 0.0;
+            if (!neg) dsc->tcg_rz = tot[1];
+            dsc->tcg_neg = neg;
+            dsc->tcg_done = neg || conv;
+            dsc->tcg_iter += 1;
+            dsc->tcg_total += 1;
+            break;
+        }
+        case FIN_IR_RESIDUAL: {  // outer residual of the refinement PCG: tot = (r.r, b.b)
+            dsc->ir_rr = tot[0];
+            if (a.maxiter >= 0) dsc->ir_bound = fmax(a.tol * a.tol * tot[1], a.atol * a.atol);
+            dsc->ir_done = tot[0] <= dsc->ir_bound;
+            dsc->ir_outer += 1;
+            break;
+        }
+        case FIN_PCG32_INIT: {  // inner FP32 PCG from x = 0: tot = (r.z, r.r, r.r)
+            dsc->pcg_gamma = tot[0];
+            dsc->pcg_rr = tot[1];
+            dsc->pcg_bs = tot[1];
+            dsc->pcg_bound = a.tol * a.tol * tot[1];
+            dsc->pcg_iter = 0;
+            dsc->pcg_maxiter = a.maxiter;
+            dsc->pcg_done = (dsc->ir_done != 0) || !((tot[1] > dsc->pcg_bound) && (0 < a.maxiter));
+            break;
+        }
+    }
+}
+
+void launch_fin(const Launch &L, const FinArgs &a) {
+    hipLaunchKernelGGL(k_fin, dim3(1), dim3(kBlock), 0, L.s, L.red.partials, a, L.dsc);
+}
+
 // ------------------------------------------------------------------ SELL SpMV
 
-template <typename T, typename TX>
-__global__ __launch_bounds__(kBlock) void k_sell_spmv(SellDev<T> A, const TX *__restrict__ x,
-                                                      TX *__restrict__ y, double *dot_out,
-                                                      const int *skip, Reducer R) {
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_sell_spmv(SellDev<T> A, const T *__restrict__ x,
+                                                      T *__restrict__ y, int want_dot,
+                                                      const int *skip, double *partials) {
     if (skip && *skip) return;
     const int lane = threadIdx.x & (kWave - 1);
     double dot = 0.0;
@@ -106,130 +167,289 @@ __global__ __launch_bounds__(kBlock) void k_sell_spmv(SellDev<T> A, const TX *__
         const int beg = A.slice_ptr[slice], end = A.slice_ptr[slice + 1];
         const int row = slice * kWave + lane;
         T acc = 0;
-        for (int k = beg + lane; k < end; k += kWave) acc += A.val[k] * (T)x[A.col[k]];
+        for (int k = beg + lane; k < end; k += kWave) acc += A.val[k] * x[A.col[k]];
         if (row < A.n_rows) {
-            y[row] = (TX)acc;
+            y[row] = acc;
             dot += (double)acc * (double)x[row];
         }
     }
-    if (dot_out) {
+    if (want_dot) {
         double v[1] = {dot};
-        grid_reduce<1>(v, 0u, R, [=](double *t) { *dot_out = t[0]; });
+        block_partials<1>(v, 0u, partials);
     }
+}
+
+template <typename T>
+static int sell_blocks(const SellDev<T> &A) {
+    return std::max(1, std::min((A.n_slices + kBlock / kWave - 1) / (kBlock / kWave), kMaxPartials));
 }
 
 void launch_sell_spmv(const Launch &L, const SellDev<double> &A, const double *x, double *y,
                       double *dot_out, const int *skip_flag) {
-    int nb = std::min((A.n_slices + kBlock / kWave - 1) / (kBlock / kWave), kMaxPartials);
-    if (nb < 1) nb = 1;
-    hipLaunchKernelGGL((k_sell_spmv<double, double>), dim3(nb), dim3(kBlock), 0, L.s, A, x, y,
-                       dot_out, skip_flag, L.red);
+    const int nb = sell_blocks(A);
+    hipLaunchKernelGGL(k_sell_spmv<double>, dim3(nb), dim3(kBlock), 0, L.s, A, x, y, dot_out != nullptr,
+                       skip_flag, L.red.partials);
+    if (dot_out) launch_fin(L, FinArgs::store(dot_out, 1, 0u, nb, skip_flag));
 }
 void launch_sell_spmv32(const Launch &L, const SellDev<float> &A, const float *x, float *y,
                         double *dot_out, const int *skip_flag) {
-    int nb = std::min((A.n_slices + kBlock / kWave - 1) / (kBlock / kWave), kMaxPartials);
-    if (nb < 1) nb = 1;
-    hipLaunchKernelGGL((k_sell_spmv<float, float>), dim3(nb), dim3(kBlock), 0, L.s, A, x, y,
-                       dot_out, skip_flag, L.red);
+    const int nb = sell_blocks(A);
+    hipLaunchKernelGGL(k_sell_spmv<float>, dim3(nb), dim3(kBlock), 0, L.s, A, x, y, dot_out != nullptr,
+                       skip_flag, L.red.partials);
+    if (dot_out) launch_fin(L, FinArgs::store(dot_out, 1, 0u, nb, skip_flag));
 }
 
-// ------------------------------------------------------------------ PCG, FP64
+// ------------------------------------------------------------------ 7-point Poisson stencil
+// On the Kuhn-split rectilinear grid every tetrahedron couples its nodes only along
+// axis-aligned edges (the barycentric gradients of non-consecutive path vertices are
+// orthogonal).  The edge (P, P + e_d) lies in 4 voxels; in each it belongs to 2 Kuhn
+// tetrahedra if its two transverse local coordinates are equal, else to 1, and each
+// contributes -vol_t / h_d^2 = -(h_d h_d1 h_d2 / 6) / h_d^2.  The diagonal is minus the
+// sum of the six couplings; Dirichlet rows are identity and couplings to Dirichlet
+// nodes are dropped (escript nullifyRowsAndCols semantics).
+
+template <typename T>
+__device__ __forceinline__ T edge_coef(const T *h1, const T *h2, int a, int b, int n1, int n2, T inv6hd) {
+    // transverse voxels a-1 (local coordinate 1) and a (local 0); same for b
+    T s = 0;
+    const bool a0 = a - 1 >= 0, a1 = a < n1, b0 = b - 1 >= 0, b1 = b < n2;
+    if (a0 && b0) s += (T)2 * h1[a - 1] * h2[b - 1];
+    if (a0 && b1) s += h1[a - 1] * h2[b];
+    if (a1 && b0) s += h1[a] * h2[b - 1];
+    if (a1 && b1) s += (T)2 * h1[a] * h2[b];
+    return -s * inv6hd;
+}
+
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_grid_poisson(PoissonGrid<T> P, const T *__restrict__ x,
+                                                         T *__restrict__ y, int want_dot,
+                                                         const int *skip, double *partials) {
+    if (skip && *skip) return;
+    const int nx = P.nx, ny = P.ny, nz = P.nz;
+    const int64_t n = (int64_t)nx * ny * nz;
+    double dot = 0.0;
+    for (int64_t idx = blockIdx.x * (int64_t)kBlock + threadIdx.x; idx < n;
+         idx += (int64_t)gridDim.x * kBlock) {
+        const int i = (int)(idx % nx);
+        const int j = (int)((idx / nx) % ny);
+        const int k = (int)(idx / ((int64_t)nx * ny));
+        T out;
+        const T xi = x[idx];
+        if (i == 0 || j == 0 || k == 0 || i == nx - 1 || j == ny - 1 || k == nz - 1) {
+            out = xi;  // Dirichlet row
+        } else {
+            // voxel sizes hx[0..nx-2] etc.
+            const T cxm = edge_coef(P.hy, P.hz, j, k, ny - 1, nz - 1, P.inv6hx[i - 1]);
+            const T cxp = edge_coef(P.hy, P.hz, j, k, ny - 1, nz - 1, P.inv6hx[i]);
+            const T cym = edge_coef(P.hz, P.hx, k, i, nz - 1, nx - 1, P.inv6hy[j - 1]);
+            const T cyp = edge_coef(P.hz, P.hx, k, i, nz - 1, nx - 1, P.inv6hy[j]);
+            const T czm = edge_coef(P.hx, P.hy, i, j, nx - 1, ny - 1, P.inv6hz[k - 1]);
+            const T czp = edge_coef(P.hx, P.hy, i, j, nx - 1, ny - 1, P.inv6hz[k]);
+            const T diag = -(cxm + cxp + cym + cyp + czm + czp);
+            const int64_t sy = nx, sz = (int64_t)nx * ny;
+            T acc = diag * xi;
+            // couplings to Dirichlet neighbours are dropped
+            if (i - 1 > 0) acc += cxm * x[idx - 1];
+            if (i + 1 < nx - 1) acc += cxp * x[idx + 1];
+            if (j - 1 > 0) acc += cym * x[idx - sy];
+            if (j + 1 < ny - 1) acc += cyp * x[idx + sy];
+            if (k - 1 > 0) acc += czm * x[idx - sz];
+            if (k + 1 < nz - 1) acc += czp * x[idx + sz];
+            out = acc;
+        }
+        y[idx] = out;
+        dot += (double)out * (double)xi;
+    }
+    if (want_dot) {
+        double v[1] = {dot};
+        block_partials<1>(v, 0u, partials);
+    }
+}
+
+template <typename T>
+static int poisson_blocks(const PoissonGrid<T> &P) {
+    return grid_for((int64_t)P.nx * P.ny * P.nz, 2);
+}
+
+void launch_grid_poisson(const Launch &L, const PoissonGrid<double> &P, const double *x, double *y,
+                         double *dot_out, const int *skip_flag) {
+    const int nb = poisson_blocks(P);
+    hipLaunchKernelGGL(k_grid_poisson<double>, dim3(nb), dim3(kBlock), 0, L.s, P, x, y,
+                       dot_out != nullptr, skip_flag, L.red.partials);
+    if (dot_out) launch_fin(L, FinArgs::store(dot_out, 1, 0u, nb, skip_flag));
+}
+void launch_grid_poisson32(const Launch &L, const PoissonGrid<float> &P, const float *x, float *y,
+                           double *dot_out, const int *skip_flag) {
+    const int nb = poisson_blocks(P);
+    hipLaunchKernelGGL(k_grid_poisson<float>, dim3(nb), dim3(kBlock), 0, L.s, P, x, y,
+                       dot_out != nullptr, skip_flag, L.red.partials);
+    if (dot_out) launch_fin(L, FinArgs::store(dot_out, 1, 0u, nb, skip_flag));
+}
+
+// ------------------------------------------------------------------ PCG
 // jax.scipy.sparse.linalg.cg with M(x) = x * (1/D):
 //   r = b - A x0; z = M r; p = z; gamma = r.z; bound = max(tol^2 b.b, atol^2)
 //   while r.r > bound and k < maxiter: q = A p; alpha = gamma/(p.q); x += alpha p;
 //       r -= alpha q; z = M r; gamma' = r.z; p = z + (gamma'/gamma) p; k += 1
+// T = double (FP64 path) or float (inner solve of the mixed path; FP64 dots).
 
-__global__ __launch_bounds__(kBlock) void k_pcg_init(int n, const double *__restrict__ b,
-                                                     const double *__restrict__ Ax0,
-                                                     double *__restrict__ r, double *__restrict__ z,
-                                                     double *__restrict__ p,
-                                                     const double *__restrict__ invD, double tol,
-                                                     double atol, int maxiter, DevScalars *dsc,
-                                                     Reducer R) {
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_pcg_init(int64_t n, const T *__restrict__ b,
+                                                     const T *__restrict__ Ax0, T *__restrict__ r,
+                                                     T *__restrict__ z, T *__restrict__ p,
+                                                     const T *__restrict__ invD, double *partials) {
     double s[3] = {0.0, 0.0, 0.0};
-    for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) {
-        const double bi = b[i];
-        const double ri = bi - Ax0[i];
-        const double zi = ri * invD[i];
+    for (int64_t i = blockIdx.x * (int64_t)kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock) {
+        const T bi = b[i];
+        const T ri = Ax0 ? bi - Ax0[i] : bi;
+        const T zi = ri * invD[i];
         r[i] = ri;
         z[i] = zi;
         p[i] = zi;
-        s[0] += ri * zi;
-        s[1] += ri * ri;
-        s[2] += bi * bi;
+        s[0] += (double)ri * (double)zi;
+        s[1] += (double)ri * (double)ri;
+        s[2] += (double)bi * (double)bi;
     }
-    grid_reduce<3>(s, 0u, R, [=](double *t) {
-        dsc->pcg_gamma = t[0];
-        dsc->pcg_rr = t[1];
-        dsc->pcg_bs = t[2];
-        const double bound = fmax(tol * tol * t[2], atol * atol);
-        dsc->pcg_bound = bound;
-        dsc->pcg_iter = 0;
-        dsc->pcg_maxiter = maxiter;
-        dsc->pcg_done = !((t[1] > bound) && (0 < maxiter));
-    });
+    block_partials<3>(s, 0u, partials);
 }
 
-__global__ __launch_bounds__(kBlock) void k_pcg_update(int n, double *__restrict__ x,
-                                                       double *__restrict__ r,
-                                                       double *__restrict__ z,
-                                                       const double *__restrict__ p,
-                                                       const double *__restrict__ q,
-                                                       const double *__restrict__ invD,
-                                                       DevScalars *dsc, Reducer R) {
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_pcg_update(int64_t n, T *__restrict__ x, T *__restrict__ r,
+                                                       T *__restrict__ z, const T *__restrict__ p,
+                                                       const T *__restrict__ q,
+                                                       const T *__restrict__ invD, const DevScalars *dsc,
+                                                       double *partials) {
     if (dsc->pcg_done) return;
-    const double alpha = dsc->pcg_gamma / dsc->pcg_pq;
+    const T alpha = (T)(dsc->pcg_gamma / dsc->pcg_pq);
     double s[2] = {0.0, 0.0};
-    for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) {
-        const double pi = p[i];
-        x[i] = x[i] + alpha * pi;
-        const double ri = r[i] - alpha * q[i];
-        const double zi = ri * invD[i];
+    for (int64_t i = blockIdx.x * (int64_t)kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock) {
+        x[i] = x[i] + alpha * p[i];
+        const T ri = r[i] - alpha * q[i];
+        const T zi = ri * invD[i];
         r[i] = ri;
         z[i] = zi;
-        s[0] += ri * zi;
-        s[1] += ri * ri;
+        s[0] += (double)ri * (double)zi;
+        s[1] += (double)ri * (double)ri;
     }
-    grid_reduce<2>(s, 0u, R, [=](double *t) {
-        dsc->pcg_beta = t[0] / dsc->pcg_gamma;
-        dsc->pcg_gamma = t[0];
-        dsc->pcg_rr = t[1];
-        const int it = dsc->pcg_iter + 1;
-        dsc->pcg_iter = it;
-        dsc->pcg_done = !((t[1] > dsc->pcg_bound) && (it < dsc->pcg_maxiter));
-    });
+    block_partials<2>(s, 0u, partials);
 }
 
-__global__ __launch_bounds__(kBlock) void k_pcg_pupdate(int n, double *__restrict__ p,
-                                                        const double *__restrict__ z,
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_pcg_pupdate(int64_t n, T *__restrict__ p, const T *__restrict__ z,
                                                         const DevScalars *dsc) {
     if (dsc->pcg_done) return;
-    const double beta = dsc->pcg_beta;
-    for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock)
+    const T beta = (T)dsc->pcg_beta;
+    for (int64_t i = blockIdx.x * (int64_t)kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock)
         p[i] = z[i] + beta * p[i];
 }
 
-void launch_pcg_init(const Launch &L, int n, const double *b, const double *Ax0, double *r,
-                     double *z, double *p, const double *invD, double tol, double atol,
-                     int maxiter) {
-    hipLaunchKernelGGL(k_pcg_init, dim3(grid_for(n, 4)), dim3(kBlock), 0, L.s, n, b, Ax0, r, z, p,
-                       invD, tol, atol, maxiter, L.dsc, L.red);
-}
-void launch_pcg_update(const Launch &L, int n, double *x, double *r, double *z, const double *p,
-                       const double *q, const double *invD) {
-    hipLaunchKernelGGL(k_pcg_update, dim3(grid_for(n, 4)), dim3(kBlock), 0, L.s, n, x, r, z, p, q,
-                       invD, L.dsc, L.red);
-}
-void launch_pcg_pupdate(const Launch &L, int n, double *p, const double *z) {
-    hipLaunchKernelGGL(k_pcg_pupdate, dim3(grid_for(n, 4)), dim3(kBlock), 0, L.s, n, p, z, L.dsc);
+template <typename T>
+static void pcg_init_t(const Launch &L, int64_t n, const T *b, const T *Ax0, T *r, T *z, T *p,
+                       const T *invD, const FinArgs &fa) {
+    const int nb = grid_for(n, 4);
+    hipLaunchKernelGGL(k_pcg_init<T>, dim3(nb), dim3(kBlock), 0, L.s, n, b, Ax0, r, z, p, invD,
+                       L.red.partials);
+    FinArgs a = fa;
+    a.nparts = nb;
+    launch_fin(L, a);
 }
 
-// ------------------------------------------------------------------ element kernels
+void launch_pcg_init(const Launch &L, int64_t n, const double *b, const double *Ax0, double *r,
+                     double *z, double *p, const double *invD, double tol, double atol, int maxiter) {
+    FinArgs a{};
+    a.op = FIN_PCG_INIT;
+    a.nv = 3;
+    a.tol = tol;
+    a.atol = atol;
+    a.maxiter = maxiter;
+    pcg_init_t<double>(L, n, b, Ax0, r, z, p, invD, a);
+}
+void launch_pcg32_init(const Launch &L, int64_t n, const float *b, float *r, float *z, float *p,
+                       const float *invD, double tol, int maxiter) {
+    FinArgs a{};
+    a.op = FIN_PCG32_INIT;
+    a.nv = 3;
+    a.tol = tol;
+    a.maxiter = maxiter;
+    pcg_init_t<float>(L, n, b, nullptr, r, z, p, invD, a);
+}
+
+template <typename T>
+static void pcg_update_t(const Launch &L, int64_t n, T *x, T *r, T *z, const T *p, const T *q,
+                         const T *invD) {
+    const int nb = grid_for(n, 4);
+    hipLaunchKernelGGL(k_pcg_update<T>, dim3(nb), dim3(kBlock), 0, L.s, n, x, r, z, p, q, invD, L.dsc,
+                       L.red.partials);
+    FinArgs a{};
+    a.op = FIN_PCG_UPDATE;
+    a.nv = 2;
+    a.nparts = nb;
+    a.skip = &L.dsc->pcg_done;
+    launch_fin(L, a);
+}
+void launch_pcg_update(const Launch &L, int64_t n, double *x, double *r, double *z, const double *p,
+                       const double *q, const double *invD) {
+    pcg_update_t<double>(L, n, x, r, z, p, q, invD);
+}
+void launch_pcg32_update(const Launch &L, int64_t n, float *x, float *r, float *z, const float *p,
+                         const float *q, const float *invD) {
+    pcg_update_t<float>(L, n, x, r, z, p, q, invD);
+}
+void launch_pcg_pupdate(const Launch &L, int64_t n, double *p, const double *z) {
+    hipLaunchKernelGGL(k_pcg_pupdate<double>, dim3(grid_for(n, 4)), dim3(kBlock), 0, L.s, n, p, z, L.dsc);
+}
+void launch_pcg32_pupdate(const Launch &L, int64_t n, float *p, const float *z) {
+    hipLaunchKernelGGL(k_pcg_pupdate<float>, dim3(grid_for(n, 4)), dim3(kBlock), 0, L.s, n, p, z, L.dsc);
+}
+
+// Outer residual of the refinement PCG (FP64): r32 = (float)(b - Ax); partials (r.r, b.b).
+__global__ __launch_bounds__(kBlock) void k_residual(int64_t n, const double *__restrict__ b,
+                                                     const double *__restrict__ Ax,
+                                                     float *__restrict__ r32, double *partials) {
+    double s[2] = {0.0, 0.0};
+    for (int64_t i = blockIdx.x * (int64_t)kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock) {
+        const double bi = b[i];
+        const double ri = bi - Ax[i];
+        r32[i] = (float)ri;
+        s[0] += ri * ri;
+        s[1] += bi * bi;
+    }
+    block_partials<2>(s, 0u, partials);
+}
+void launch_residual(const Launch &L, int64_t n, const double *b, const double *Ax, float *r32,
+                     double tol, double atol, bool first) {
+    const int nb = grid_for(n, 4);
+    hipLaunchKernelGGL(k_residual, dim3(nb), dim3(kBlock), 0, L.s, n, b, Ax, r32, L.red.partials);
+    FinArgs a{};
+    a.op = FIN_IR_RESIDUAL;
+    a.nv = 2;
+    a.nparts = nb;
+    a.tol = tol;
+    a.atol = atol;
+    a.maxiter = first ? 0 : -1;
+    a.skip = first ? nullptr : &L.dsc->ir_done;
+    launch_fin(L, a);
+}
+
+// x += (double) dx; skipped once the outer residual has converged.
+__global__ __launch_bounds__(kBlock) void k_add_correction(int64_t n, double *__restrict__ x,
+                                                           const float *__restrict__ dx,
+                                                           const DevScalars *dsc) {
+    if (dsc->ir_done) return;
+    for (int64_t i = blockIdx.x * (int64_t)kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock)
+        x[i] += (double)dx[i];
+}
+void launch_add_correction(const Launch &L, int64_t n, double *x, const float *dx) {
+    hipLaunchKernelGGL(k_add_correction, dim3(grid_for(n, 4)), dim3(kBlock), 0, L.s, n, x, dx, L.dsc);
+}
+
+// ------------------------------------------------------------------ element kernels (general meshes)
 // Element e with nodes t0..t3, barycentric gradients G_k, volume vol, material m:
 //   exchange:    y_k,a += (2A/V) vol  G_k . grad(v_a)
 //   anisotropy:  y_k   += (-2K/V) (vol/20) u (u.v_k + sum_l u.v_l)
 //   divergence:  b_k   += Js vol G_k . mean(v)
 //   gradient:    g_k   += Js vol/(4V) grad(u)        (same for the 4 nodes)
+// Accumulated with FP64 atomics into zeroed outputs.
 
 struct ElemLocal {
     double G[4][3];
@@ -240,7 +460,7 @@ __device__ __forceinline__ void load_G(const ElemDev &E, int e, ElemLocal &L) {
 #pragma unroll
     for (int k = 1; k < 4; ++k)
 #pragma unroll
-        for (int a = 0; a < 3; ++a) L.G[k][a] = E.G[(3 * (k - 1) + a) * n + e];
+        for (int a = 0; a < 3; ++a) L.G[k][a] = E.G[(3 * (k - 1) + a) * (int64_t)n + e];
 #pragma unroll
     for (int a = 0; a < 3; ++a) L.G[0][a] = -(L.G[1][a] + L.G[2][a] + L.G[3][a]);
 }
@@ -333,6 +553,164 @@ void launch_ebe_grad(const Launch &L, const ElemDev &E, const double *u, double 
                        L.mats, u, g, invV);
 }
 
+// ------------------------------------------------------------------ structured magnet kernels
+// Magnet block of mn^3 voxels (edge h) inside the rectilinear grid; magnet node
+// (i,j,k), 0 <= i,j,k <= mn, is grid node (m0x+i, m0y+j, m0z+k).  Node-centric: each
+// thread sums the contributions of the (up to) 8 voxels around its node with the exact
+// Kuhn voxel tables (kuhn_tables.h); voxels outside the block use the all-zero
+// material n_mats, so neighbour loads are clamped instead of branched.  No atomics.
+
+__device__ __forceinline__ int64_t gnode(const GridDev &G, int i, int j, int k) {
+    return (int64_t)(G.m0x + i) + (int64_t)G.nx * ((G.m0y + j) + (int64_t)G.ny * (G.m0z + k));
+}
+
+// y = C v at magnet nodes; with DIV also b = Dx vx + Dy vy + Dz vz at magnet nodes.
+template <typename TC, bool DIV>
+__global__ __launch_bounds__(kBlock) void k_grid_exani(GridDev G, const MatCoef *__restrict__ mats,
+                                                       const double *__restrict__ v,
+                                                       double *__restrict__ y, double *__restrict__ b,
+                                                       const int *skip) {
+    if (skip && *skip) return;
+    const int n1 = G.mn + 1;
+    const int64_t nmag = (int64_t)n1 * n1 * n1;
+    const int64_t t = blockIdx.x * (int64_t)kBlock + threadIdx.x;
+    if (t >= nmag) return;
+    const int i = (int)(t % n1), j = (int)((t / n1) % n1), k = (int)(t / ((int64_t)n1 * n1));
+    TC nv[27][3];
+#pragma unroll
+    for (int dz = -1; dz <= 1; ++dz)
+#pragma unroll
+        for (int dy = -1; dy <= 1; ++dy)
+#pragma unroll
+            for (int dx = -1; dx <= 1; ++dx) {
+                const int ii = min(max(i + dx, 0), G.mn), jj = min(max(j + dy, 0), G.mn),
+                          kk = min(max(k + dz, 0), G.mn);
+                const int64_t g = gnode(G, ii, jj, kk);
+                const int s = (dx + 1) + 3 * (dy + 1) + 9 * (dz + 1);
+#pragma unroll
+                for (int a = 0; a < 3; ++a) nv[s][a] = (TC)v[3 * g + a];
+            }
+    TC ex[3] = {0, 0, 0}, an[3] = {0, 0, 0}, bd = 0;
+#pragma unroll
+    for (int w = 0; w < 8; ++w) {
+        const int vx = w & 1, vy = (w >> 1) & 1, vz = (w >> 2) & 1;  // voxel (i-1+vx, j-1+vy, k-1+vz)
+        const int xi = i - 1 + vx, yj = j - 1 + vy, zk = k - 1 + vz;
+        const bool inside = xi >= 0 && yj >= 0 && zk >= 0 && xi < G.mn && yj < G.mn && zk < G.mn;
+        const int mat = inside ? (int)G.vmat[xi + (int64_t)G.mn * (yj + (int64_t)G.mn * zk)] : G.n_mats;
+        const MatCoef mc = mats[mat];
+        const int p = (1 - vx) + 2 * (1 - vy) + 4 * (1 - vz);  // corner of the node in this voxel
+        TC sS[3] = {0, 0, 0}, sM = 0, sB = 0;
+        const TC u0 = (TC)mc.u[0], u1 = (TC)mc.u[1], u2 = (TC)mc.u[2];
+#pragma unroll
+        for (int c = 0; c < 8; ++c) {
+            const int cx = c & 1, cy = (c >> 1) & 1, cz = (c >> 2) & 1;
+            const int s = (cx - (1 - vx) + 1) + 3 * (cy - (1 - vy) + 1) + 9 * (cz - (1 - vz) + 1);
+            const int KS = kKS[p][c], KM = kKM[p][c];
+            if (KS != 0) {
+#pragma unroll
+                for (int a = 0; a < 3; ++a) sS[a] += (TC)KS * nv[s][a];
+            }
+            if (KM != 0) sM += (TC)KM * (u0 * nv[s][0] + u1 * nv[s][1] + u2 * nv[s][2]);
+            if (DIV) {
+                const int B0 = kKB[p][c][0], B1 = kKB[p][c][1], B2 = kKB[p][c][2];
+                if (B0 != 0) sB += (TC)B0 * nv[s][0];
+                if (B1 != 0) sB += (TC)B1 * nv[s][1];
+                if (B2 != 0) sB += (TC)B2 * nv[s][2];
+            }
+        }
+        const TC cex = (TC)mc.cex, cK = (TC)mc.cK;
+#pragma unroll
+        for (int a = 0; a < 3; ++a) ex[a] += cex * sS[a];
+        an[0] += cK * sM * u0;
+        an[1] += cK * sM * u1;
+        an[2] += cK * sM * u2;
+        if (DIV) bd += (TC)mc.Js * sB;
+    }
+    const int64_t g = gnode(G, i, j, k);
+    const TC fS = (TC)G.fS, fM = (TC)G.fM;
+#pragma unroll
+    for (int a = 0; a < 3; ++a) y[3 * g + a] = (double)(fS * ex[a] + fM * an[a]);
+    if (DIV) b[g] = (double)((TC)G.fB * bd);
+}
+
+// g = interleave(Gx u, Gy u, Gz u) at magnet nodes.
+template <typename TC>
+__global__ __launch_bounds__(kBlock) void k_grid_grad(GridDev G, const MatCoef *__restrict__ mats,
+                                                      const double *__restrict__ u,
+                                                      double *__restrict__ g) {
+    const int n1 = G.mn + 1;
+    const int64_t nmag = (int64_t)n1 * n1 * n1;
+    const int64_t t = blockIdx.x * (int64_t)kBlock + threadIdx.x;
+    if (t >= nmag) return;
+    const int i = (int)(t % n1), j = (int)((t / n1) % n1), k = (int)(t / ((int64_t)n1 * n1));
+    TC nu[27];
+#pragma unroll
+    for (int dz = -1; dz <= 1; ++dz)
+#pragma unroll
+        for (int dy = -1; dy <= 1; ++dy)
+#pragma unroll
+            for (int dx = -1; dx <= 1; ++dx) {
+                const int ii = min(max(i + dx, 0), G.mn), jj = min(max(j + dy, 0), G.mn),
+                          kk = min(max(k + dz, 0), G.mn);
+                nu[(dx + 1) + 3 * (dy + 1) + 9 * (dz + 1)] = (TC)u[gnode(G, ii, jj, kk)];
+            }
+    TC acc[3] = {0, 0, 0};
+#pragma unroll
+    for (int w = 0; w < 8; ++w) {
+        const int vx = w & 1, vy = (w >> 1) & 1, vz = (w >> 2) & 1;
+        const int xi = i - 1 + vx, yj = j - 1 + vy, zk = k - 1 + vz;
+        const bool inside = xi >= 0 && yj >= 0 && zk >= 0 && xi < G.mn && yj < G.mn && zk < G.mn;
+        const int mat = inside ? (int)G.vmat[xi + (int64_t)G.mn * (yj + (int64_t)G.mn * zk)] : G.n_mats;
+        const TC js = (TC)mats[mat].Js;
+        const int p = (1 - vx) + 2 * (1 - vy) + 4 * (1 - vz);
+        TC s[3] = {0, 0, 0};
+#pragma unroll
+        for (int c = 0; c < 8; ++c) {
+            const int cx = c & 1, cy = (c >> 1) & 1, cz = (c >> 2) & 1;
+            const int sidx = (cx - (1 - vx) + 1) + 3 * (cy - (1 - vy) + 1) + 9 * (cz - (1 - vz) + 1);
+#pragma unroll
+            for (int a = 0; a < 3; ++a) {
+                const int B = kKB[c][p][a];
+                if (B != 0) s[a] += (TC)B * nu[sidx];
+            }
+        }
+#pragma unroll
+        for (int a = 0; a < 3; ++a) acc[a] += js * s[a];
+    }
+    const int64_t gn = gnode(G, i, j, k);
+    const TC f = (TC)G.fG;
+#pragma unroll
+    for (int a = 0; a < 3; ++a) g[3 * gn + a] = (double)(f * acc[a]);
+}
+
+static inline int grid_mag_blocks(const GridDev &G) {
+    const int64_t n1 = G.mn + 1;
+    return (int)((n1 * n1 * n1 + kBlock - 1) / kBlock);
+}
+
+void launch_grid_exani(const Launch &L, const GridDev &G, bool fp32, const double *v, double *y,
+                       double *b, const int *skip_flag) {
+    const dim3 g(grid_mag_blocks(G)), blk(kBlock);
+    if (fp32) {
+        if (b)
+            hipLaunchKernelGGL((k_grid_exani<float, true>), g, blk, 0, L.s, G, L.mats, v, y, b, skip_flag);
+        else
+            hipLaunchKernelGGL((k_grid_exani<float, false>), g, blk, 0, L.s, G, L.mats, v, y, b, skip_flag);
+    } else {
+        if (b)
+            hipLaunchKernelGGL((k_grid_exani<double, true>), g, blk, 0, L.s, G, L.mats, v, y, b, skip_flag);
+        else
+            hipLaunchKernelGGL((k_grid_exani<double, false>), g, blk, 0, L.s, G, L.mats, v, y, b, skip_flag);
+    }
+}
+void launch_grid_grad(const Launch &L, const GridDev &G, bool fp32, const double *u, double *g) {
+    const dim3 gr(grid_mag_blocks(G)), blk(kBlock);
+    if (fp32)
+        hipLaunchKernelGGL(k_grid_grad<float>, gr, blk, 0, L.s, G, L.mats, u, g);
+    else
+        hipLaunchKernelGGL(k_grid_grad<double>, gr, blk, 0, L.s, G, L.mats, u, g);
+}
+
 // ------------------------------------------------------------------ node kernels
 
 __device__ __forceinline__ void cross3(const double *a, const double *b, double *c) {
@@ -342,140 +720,122 @@ __device__ __forceinline__ void cross3(const double *a, const double *b, double 
 }
 
 // g = gz + gC (+ gd); E = m.gz + m.gC/2 (+ m.gd/2); g_proj = m x (g x m)
-__global__ __launch_bounds__(kBlock) void k_finish_eg(int n, const double *__restrict__ m,
+__global__ __launch_bounds__(kBlock) void k_finish_eg(int64_t n, const double *__restrict__ m,
                                                       const double *__restrict__ meas, double zc0,
                                                       double zc1, double zc2,
                                                       const double *__restrict__ gC,
                                                       const double *__restrict__ gd,
                                                       double *__restrict__ g_full,
-                                                      double *__restrict__ g_proj, double *E_out,
-                                                      int with_hmag, Reducer R) {
+                                                      double *__restrict__ g_proj, int with_hmag,
+                                                      double *partials) {
     double e = 0.0;
     const double zc[3] = {zc0, zc1, zc2};
-    for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) {
+    for (int64_t i = blockIdx.x * (int64_t)kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock) {
         const double mi = meas[i];
         double mm[3], g[3];
 #pragma unroll
         for (int a = 0; a < 3; ++a) {
-            mm[a] = m[3 * (int64_t)i + a];
+            mm[a] = m[3 * i + a];
             const double gz = zc[a] * mi;
-            const double gc = gC[3 * (int64_t)i + a];
-            const double gdm = with_hmag ? gd[3 * (int64_t)i + a] : 0.0;
+            const double gc = gC[3 * i + a];
+            const double gdm = with_hmag ? gd[3 * i + a] : 0.0;
             g[a] = gz + gc + gdm;
             e += mm[a] * gz + 0.5 * mm[a] * gc + 0.5 * mm[a] * gdm;
-            g_full[3 * (int64_t)i + a] = g[a];
+            g_full[3 * i + a] = g[a];
         }
         double c[3], pr[3];
         cross3(g, mm, c);
         cross3(mm, c, pr);
 #pragma unroll
-        for (int a = 0; a < 3; ++a) g_proj[3 * (int64_t)i + a] = pr[a];
+        for (int a = 0; a < 3; ++a) g_proj[3 * i + a] = pr[a];
     }
     double v[1] = {e};
-    grid_reduce<1>(v, 0u, R, [=](double *t) { *E_out = t[0]; });
+    block_partials<1>(v, 0u, partials);
 }
 
-void launch_finish_eg(const Launch &L, int n, const double *m, const double *meas, double zc0,
+void launch_finish_eg(const Launch &L, int64_t n, const double *m, const double *meas, double zc0,
                       double zc1, double zc2, const double *gC, const double *gd, double *g_full,
                       double *g_proj, double *E_out, int with_hmag) {
-    hipLaunchKernelGGL(k_finish_eg, dim3(grid_for(n, 2)), dim3(kBlock), 0, L.s, n, m, meas, zc0,
-                       zc1, zc2, gC, gd, g_full, g_proj, E_out, with_hmag, L.red);
+    const int nb = grid_for(n, 2);
+    hipLaunchKernelGGL(k_finish_eg, dim3(nb), dim3(kBlock), 0, L.s, n, m, meas, zc0, zc1, zc2, gC, gd,
+                       g_full, g_proj, with_hmag, L.red.partials);
+    launch_fin(L, FinArgs::store(E_out, 1, 0u, nb, nullptr));
 }
 
-__global__ __launch_bounds__(kBlock) void k_external(int n, const double *__restrict__ m,
+__global__ __launch_bounds__(kBlock) void k_external(int64_t n, const double *__restrict__ m,
                                                      const double *__restrict__ meas, double zc0,
                                                      double zc1, double zc2,
-                                                     double *__restrict__ g, double *E_out,
-                                                     Reducer R) {
+                                                     double *__restrict__ g, double *partials) {
     double e = 0.0;
     const double zc[3] = {zc0, zc1, zc2};
-    for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) {
+    for (int64_t i = blockIdx.x * (int64_t)kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock) {
         const double mi = meas[i];
 #pragma unroll
         for (int a = 0; a < 3; ++a) {
             const double gz = zc[a] * mi;
-            if (g) g[3 * (int64_t)i + a] = gz;
-            e += m[3 * (int64_t)i + a] * gz;
+            if (g) g[3 * i + a] = gz;
+            e += m[3 * i + a] * gz;
         }
     }
     double v[1] = {e};
-    grid_reduce<1>(v, 0u, R, [=](double *t) { *E_out = t[0]; });
+    block_partials<1>(v, 0u, partials);
 }
-void launch_external(const Launch &L, int n, const double *m, const double *meas, double zc0,
+void launch_external(const Launch &L, int64_t n, const double *m, const double *meas, double zc0,
                      double zc1, double zc2, double *g, double *E_out) {
-    hipLaunchKernelGGL(k_external, dim3(grid_for(n, 2)), dim3(kBlock), 0, L.s, n, m, meas, zc0,
-                       zc1, zc2, g, E_out, L.red);
+    const int nb = grid_for(n, 2);
+    hipLaunchKernelGGL(k_external, dim3(nb), dim3(kBlock), 0, L.s, n, m, meas, zc0, zc1, zc2, g,
+                       L.red.partials);
+    launch_fin(L, FinArgs::store(E_out, 1, 0u, nb, nullptr));
 }
 
-__global__ __launch_bounds__(kBlock) void k_project(int n, const double *__restrict__ m,
-                                                    const double *__restrict__ g,
-                                                    double *__restrict__ out) {
-    const int i = blockIdx.x * kBlock + threadIdx.x;
-    if (i >= n) return;
-    double mm[3], gg[3], c[3], pr[3];
-#pragma unroll
-    for (int a = 0; a < 3; ++a) {
-        mm[a] = m[3 * (int64_t)i + a];
-        gg[a] = g[3 * (int64_t)i + a];
-    }
-    cross3(gg, mm, c);
-    cross3(mm, c, pr);
-#pragma unroll
-    for (int a = 0; a < 3; ++a) out[3 * (int64_t)i + a] = pr[a];
-}
-void launch_project(const Launch &L, int n, const double *m, const double *g, double *out) {
-    hipLaunchKernelGGL(k_project, dim3((n + kBlock - 1) / kBlock), dim3(kBlock), 0, L.s, n, m, g,
-                       out);
-}
-
-// ------------------------------------------------------------------ truncated CG (M_inv)
 // H v = m0 x ((C v) x m0) - (m0 . gF)_i v_i ;  den = d . H d
-
-__global__ __launch_bounds__(kBlock) void k_hvec(int n, const double *__restrict__ m0,
+__global__ __launch_bounds__(kBlock) void k_hvec(int64_t n, const double *__restrict__ m0,
                                                  const double *__restrict__ gF,
                                                  const double *__restrict__ Cd,
                                                  const double *__restrict__ d,
-                                                 double *__restrict__ Hd, double *den_out,
-                                                 const int *skip, Reducer R) {
+                                                 double *__restrict__ Hd, const int *skip,
+                                                 double *partials) {
     if (skip && *skip) return;
     double den = 0.0;
-    for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) {
+    for (int64_t i = blockIdx.x * (int64_t)kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock) {
         double mm[3], cd[3], dd[3], c[3], pr[3];
         double mg = 0.0;
 #pragma unroll
         for (int a = 0; a < 3; ++a) {
-            mm[a] = m0[3 * (int64_t)i + a];
-            cd[a] = Cd[3 * (int64_t)i + a];
-            dd[a] = d[3 * (int64_t)i + a];
+            mm[a] = m0[3 * i + a];
+            cd[a] = Cd[3 * i + a];
+            dd[a] = d[3 * i + a];
         }
 #pragma unroll
-        for (int a = 0; a < 3; ++a) mg += mm[a] * gF[3 * (int64_t)i + a];
+        for (int a = 0; a < 3; ++a) mg += mm[a] * gF[3 * i + a];
         cross3(cd, mm, c);
         cross3(mm, c, pr);
 #pragma unroll
         for (int a = 0; a < 3; ++a) {
             const double h = pr[a] - mg * dd[a];
-            Hd[3 * (int64_t)i + a] = h;
+            Hd[3 * i + a] = h;
             den += dd[a] * h;
         }
     }
     double v[1] = {den};
-    grid_reduce<1>(v, 0u, R, [=](double *t) { *den_out = t[0]; });
+    block_partials<1>(v, 0u, partials);
 }
-void launch_hvec(const Launch &L, int n, const double *m0, const double *gF, const double *Cd,
+void launch_hvec(const Launch &L, int64_t n, const double *m0, const double *gF, const double *Cd,
                  const double *d, double *Hd, double *den_out, const int *skip_flag) {
-    hipLaunchKernelGGL(k_hvec, dim3(grid_for(n, 2)), dim3(kBlock), 0, L.s, n, m0, gF, Cd, d, Hd,
-                       den_out, skip_flag, L.red);
+    const int nb = grid_for(n, 2);
+    hipLaunchKernelGGL(k_hvec, dim3(nb), dim3(kBlock), 0, L.s, n, m0, gF, Cd, d, Hd, skip_flag,
+                       L.red.partials);
+    launch_fin(L, FinArgs::store(den_out, 1, 0u, nb, skip_flag));
 }
 
-// grad_fk = -g: p = 0, r = grad_fk, z = r / D, d = -z; eps = min(0.5, sqrt(|g|)|g|)
-__global__ __launch_bounds__(kBlock) void k_tcg_init(int n3, const double *__restrict__ g,
+// grad_fk = -g: p = 0, r = grad_fk, z = r / D, d = -z
+__global__ __launch_bounds__(kBlock) void k_tcg_init(int64_t n3, const double *__restrict__ g,
                                                      const double *__restrict__ D,
                                                      double *__restrict__ p, double *__restrict__ r,
                                                      double *__restrict__ z, double *__restrict__ d,
-                                                     DevScalars *dsc, Reducer R) {
+                                                     double *partials) {
     double s[2] = {0.0, 0.0};
-    for (int i = blockIdx.x * kBlock + threadIdx.x; i < n3; i += gridDim.x * kBlock) {
+    for (int64_t i = blockIdx.x * (int64_t)kBlock + threadIdx.x; i < n3; i += (int64_t)gridDim.x * kBlock) {
         const double ri = -g[i];
         const double zi = ri / D[i];
         p[i] = 0.0;
@@ -485,29 +845,22 @@ __global__ __launch_bounds__(kBlock) void k_tcg_init(int n3, const double *__res
         s[0] += ri * ri;
         s[1] += ri * zi;
     }
-    grid_reduce<2>(s, 0u, R, [=](double *t) {
-        const double nrm = sqrt(t[0]);
-        dsc->tcg_eps = fmin(0.5, sqrt(nrm) * nrm);
-        dsc->tcg_rz = t[1];
-        dsc->tcg_done = 0;
-        dsc->tcg_neg = 0;
-        dsc->tcg_iter = 0;
-    });
+    block_partials<2>(s, 0u, partials);
 }
 
-__global__ __launch_bounds__(kBlock) void k_tcg_update(int n3, double *__restrict__ p,
+__global__ __launch_bounds__(kBlock) void k_tcg_update(int64_t n3, double *__restrict__ p,
                                                        double *__restrict__ r,
                                                        double *__restrict__ z,
                                                        const double *__restrict__ d,
                                                        const double *__restrict__ Hd,
                                                        const double *__restrict__ D,
-                                                       DevScalars *dsc, Reducer R) {
+                                                       const DevScalars *dsc, double *partials) {
     if (dsc->tcg_done) return;
     const double den = dsc->tcg_den;
     const bool neg = den <= 0.0;
     const double alpha = neg ? 0.0 : dsc->tcg_rz / den;
     double s[2] = {0.0, 0.0};
-    for (int i = blockIdx.x * kBlock + threadIdx.x; i < n3; i += gridDim.x * kBlock) {
+    for (int64_t i = blockIdx.x * (int64_t)kBlock + threadIdx.x; i < n3; i += (int64_t)gridDim.x * kBlock) {
         const double rn = r[i] + alpha * Hd[i];
         const double zn = rn / D[i];
         if (!neg) {
@@ -518,86 +871,86 @@ __global__ __launch_bounds__(kBlock) void k_tcg_update(int n3, double *__restric
         s[0] += rn * rn;
         s[1] += rn * zn;
     }
-    grid_reduce<2>(s, 0u, R, [=](double *t) {
-        const bool conv = sqrt(t[0]) <= dsc->tcg_eps;
-        const double rz = dsc->tcg_rz;
-        dsc->tcg_beta = (!neg && !conv) ? t[1] / (rz + 1e-16) : 0.0;
-        if (!neg) dsc->tcg_rz = t[1];
-        dsc->tcg_neg = neg;
-        dsc->tcg_done = neg || conv;
-        dsc->tcg_iter += 1;
-        dsc->red[15] += 1.0;  // cumulative truncated-CG iterations
-    });
+    block_partials<2>(s, 0u, partials);
 }
 
-__global__ __launch_bounds__(kBlock) void k_tcg_dupdate(int n3, double *__restrict__ d,
+__global__ __launch_bounds__(kBlock) void k_tcg_dupdate(int64_t n3, double *__restrict__ d,
                                                         const double *__restrict__ z,
                                                         const DevScalars *dsc) {
     if (dsc->tcg_done) return;
     const double beta = dsc->tcg_beta;
-    for (int i = blockIdx.x * kBlock + threadIdx.x; i < n3; i += gridDim.x * kBlock)
+    for (int64_t i = blockIdx.x * (int64_t)kBlock + threadIdx.x; i < n3; i += (int64_t)gridDim.x * kBlock)
         d[i] = -z[i] + beta * d[i];
 }
 
 // z_out = (all p == 0) ? -grad_fk (= g) : p
-__global__ __launch_bounds__(kBlock) void k_tcg_final(int n3, const double *__restrict__ p,
+__global__ __launch_bounds__(kBlock) void k_tcg_final(int64_t n3, const double *__restrict__ p,
                                                       const double *__restrict__ g,
                                                       double *__restrict__ zout,
                                                       const DevScalars *dsc) {
     const bool zero = dsc->tcg_pmax == 0.0;
-    for (int i = blockIdx.x * kBlock + threadIdx.x; i < n3; i += gridDim.x * kBlock)
+    for (int64_t i = blockIdx.x * (int64_t)kBlock + threadIdx.x; i < n3; i += (int64_t)gridDim.x * kBlock)
         zout[i] = zero ? g[i] : p[i];
 }
 
-void launch_tcg_init(const Launch &L, int n3, const double *g, const double *D, double *p,
+void launch_tcg_init(const Launch &L, int64_t n3, const double *g, const double *D, double *p,
                      double *r, double *z, double *d) {
-    hipLaunchKernelGGL(k_tcg_init, dim3(grid_for(n3, 4)), dim3(kBlock), 0, L.s, n3, g, D, p, r, z,
-                       d, L.dsc, L.red);
+    const int nb = grid_for(n3, 4);
+    hipLaunchKernelGGL(k_tcg_init, dim3(nb), dim3(kBlock), 0, L.s, n3, g, D, p, r, z, d, L.red.partials);
+    FinArgs a{};
+    a.op = FIN_TCG_INIT;
+    a.nv = 2;
+    a.nparts = nb;
+    launch_fin(L, a);
 }
-void launch_tcg_update(const Launch &L, int n3, double *p, double *r, double *z, double *d,
+void launch_tcg_update(const Launch &L, int64_t n3, double *p, double *r, double *z, double *d,
                        const double *Hd, const double *D) {
-    hipLaunchKernelGGL(k_tcg_update, dim3(grid_for(n3, 4)), dim3(kBlock), 0, L.s, n3, p, r, z, d,
-                       Hd, D, L.dsc, L.red);
+    const int nb = grid_for(n3, 4);
+    hipLaunchKernelGGL(k_tcg_update, dim3(nb), dim3(kBlock), 0, L.s, n3, p, r, z, d, Hd, D, L.dsc,
+                       L.red.partials);
+    FinArgs a{};
+    a.op = FIN_TCG_UPDATE;
+    a.nv = 2;
+    a.nparts = nb;
+    a.skip = &L.dsc->tcg_done;
+    launch_fin(L, a);
 }
-void launch_tcg_dupdate(const Launch &L, int n3, double *d, const double *z) {
+void launch_tcg_dupdate(const Launch &L, int64_t n3, double *d, const double *z) {
     hipLaunchKernelGGL(k_tcg_dupdate, dim3(grid_for(n3, 4)), dim3(kBlock), 0, L.s, n3, d, z, L.dsc);
 }
-void launch_tcg_final(const Launch &L, int n3, const double *p, const double *g, double *zout) {
-    hipLaunchKernelGGL(k_tcg_final, dim3(grid_for(n3, 4)), dim3(kBlock), 0, L.s, n3, p, g, zout,
-                       L.dsc);
+void launch_tcg_final(const Launch &L, int64_t n3, const double *p, const double *g, double *zout) {
+    hipLaunchKernelGGL(k_tcg_final, dim3(grid_for(n3, 4)), dim3(kBlock), 0, L.s, n3, p, g, zout, L.dsc);
 }
 
 // ------------------------------------------------------------------ vector kernels
 
 // out_i = normalize(x_i + alpha d_i), zero-length vectors divided by 1 (jax_tools.py:4-18)
-__global__ __launch_bounds__(kBlock) void k_update_m(int n, const double *__restrict__ x,
+__global__ __launch_bounds__(kBlock) void k_update_m(int64_t n, const double *__restrict__ x,
                                                      double alpha, const double *__restrict__ d,
                                                      double *__restrict__ out) {
-    const int i = blockIdx.x * kBlock + threadIdx.x;
-    if (i >= n) return;
-    double v[3];
+    for (int64_t i = blockIdx.x * (int64_t)kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock) {
+        double v[3];
 #pragma unroll
-    for (int a = 0; a < 3; ++a) v[a] = x[3 * (int64_t)i + a] + alpha * d[3 * (int64_t)i + a];
-    double nrm = sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
-    if (nrm == 0.0) nrm = 1.0;
+        for (int a = 0; a < 3; ++a) v[a] = x[3 * i + a] + alpha * d[3 * i + a];
+        double nrm = sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
+        if (nrm == 0.0) nrm = 1.0;
 #pragma unroll
-    for (int a = 0; a < 3; ++a) out[3 * (int64_t)i + a] = v[a] / nrm;
+        for (int a = 0; a < 3; ++a) out[3 * i + a] = v[a] / nrm;
+    }
 }
-void launch_update_m(const Launch &L, int n, const double *x, double alpha, const double *d,
+void launch_update_m(const Launch &L, int64_t n, const double *x, double alpha, const double *d,
                      double *out) {
-    hipLaunchKernelGGL(k_update_m, dim3((n + kBlock - 1) / kBlock), dim3(kBlock), 0, L.s, n, x,
-                       alpha, d, out);
+    hipLaunchKernelGGL(k_update_m, dim3(grid_for(n, 2)), dim3(kBlock), 0, L.s, n, x, alpha, d, out);
 }
 
 // out may alias y (d = -z + beta d): each element is read before it is written.
 __global__ __launch_bounds__(kBlock) void k_axpby(int64_t n, double a, const double *__restrict__ x,
                                                   double b, const double *y, double *out) {
-    for (int64_t i = blockIdx.x * (int64_t)kBlock + threadIdx.x; i < n;
-         i += (int64_t)gridDim.x * kBlock)
+    for (int64_t i = blockIdx.x * (int64_t)kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock)
         out[i] = (y ? a * x[i] + b * y[i] : a * x[i]);
 }
-void launch_axpby(const Launch &L, int64_t n, double a, const double *x, double b,
-                  const double *y, double *out) {
+void launch_axpby(const Launch &L, int64_t n, double a, const double *x, double b, const double *y,
+                  double *out) {
     hipLaunchKernelGGL(k_axpby, dim3(grid_for(n, 4)), dim3(kBlock), 0, L.s, n, a, x, b, y, out);
 }
 
@@ -607,8 +960,8 @@ template <int OP>
 __global__ __launch_bounds__(kBlock) void k_reduce(int64_t n, const double *__restrict__ a,
                                                    const double *__restrict__ b,
                                                    const double *__restrict__ c,
-                                                   const double *__restrict__ d, double h0,
-                                                   double h1, double h2, double *out, Reducer R) {
+                                                   const double *__restrict__ d, double h0, double h1,
+                                                   double h2, double *partials) {
     constexpr int NV = (OP == RED_STOP3 || OP == RED_DG3 || OP == RED_H3) ? 3
                        : (OP == RED_GD2 || OP == RED_YZD)                 ? 2
                                                                           : 1;
@@ -616,8 +969,7 @@ __global__ __launch_bounds__(kBlock) void k_reduce(int64_t n, const double *__re
     double s[NV];
 #pragma unroll
     for (int k = 0; k < NV; ++k) s[k] = 0.0;
-    for (int64_t i = blockIdx.x * (int64_t)kBlock + threadIdx.x; i < n;
-         i += (int64_t)gridDim.x * kBlock) {
+    for (int64_t i = blockIdx.x * (int64_t)kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock) {
         if constexpr (OP == RED_DOT) {
             s[0] += a[i] * b[i];
         } else if constexpr (OP == RED_NORMINF) {
@@ -647,42 +999,52 @@ __global__ __launch_bounds__(kBlock) void k_reduce(int64_t n, const double *__re
             s[0] += a[i] * a[i];
         }
     }
-    grid_reduce<NV>(s, MASK, R, [=](double *t) {
-#pragma unroll
-        for (int k = 0; k < NV; ++k) out[k] = t[k];
-    });
+    block_partials<NV>(s, MASK, partials);
 }
 
 void launch_reduce(const Launch &L, RedOp op, int64_t n, const double *a, const double *b,
-                   const double *c, const double *d, double h0, double h1, double h2,
-                   double *out) {
-    const dim3 g(grid_for(n, 8)), blk(kBlock);
+                   const double *c, const double *d, double h0, double h1, double h2, double *out) {
+    const int nb = grid_for(n, 8);
+    const dim3 g(nb), blk(kBlock);
+    int nv = 1;
+    unsigned mask = 0;
     switch (op) {
-#define MM_RED_CASE(OPC)                                                                        \
-    case OPC:                                                                                   \
-        hipLaunchKernelGGL(k_reduce<OPC>, g, blk, 0, L.s, n, a, b, c, d, h0, h1, h2, out, L.red); \
+#define MM_RED_CASE(OPC, NV, MASK)                                                                     \
+    case OPC:                                                                                          \
+        hipLaunchKernelGGL(k_reduce<OPC>, g, blk, 0, L.s, n, a, b, c, d, h0, h1, h2, L.red.partials); \
+        nv = NV;                                                                                       \
+        mask = MASK;                                                                                   \
         break;
-        MM_RED_CASE(RED_DOT)
-        MM_RED_CASE(RED_NORMINF)
-        MM_RED_CASE(RED_STOP3)
-        MM_RED_CASE(RED_DG3)
-        MM_RED_CASE(RED_H3)
-        MM_RED_CASE(RED_GD2)
-        MM_RED_CASE(RED_YZD)
-        MM_RED_CASE(RED_MH)
-        MM_RED_CASE(RED_SS)
+        MM_RED_CASE(RED_DOT, 1, 0u)
+        MM_RED_CASE(RED_NORMINF, 1, 1u)
+        MM_RED_CASE(RED_STOP3, 3, 7u)
+        MM_RED_CASE(RED_DG3, 3, 0u)
+        MM_RED_CASE(RED_H3, 3, 4u)
+        MM_RED_CASE(RED_GD2, 2, 0u)
+        MM_RED_CASE(RED_YZD, 2, 0u)
+        MM_RED_CASE(RED_MH, 1, 0u)
+        MM_RED_CASE(RED_SS, 1, 0u)
 #undef MM_RED_CASE
     }
+    launch_fin(L, FinArgs::store(out, nv, mask, nb, nullptr));
 }
 
 __global__ void k_fill_zero(int64_t n, double *__restrict__ x, const int *skip) {
     if (skip && *skip) return;
-    for (int64_t i = blockIdx.x * (int64_t)kBlock + threadIdx.x; i < n;
-         i += (int64_t)gridDim.x * kBlock)
+    for (int64_t i = blockIdx.x * (int64_t)kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock)
         x[i] = 0.0;
 }
 void launch_fill_zero(const Launch &L, int64_t n, double *x, const int *skip_flag) {
     hipLaunchKernelGGL(k_fill_zero, dim3(grid_for(n, 8)), dim3(kBlock), 0, L.s, n, x, skip_flag);
+}
+
+__global__ void k_fill_zero32(int64_t n, float *__restrict__ x, const int *skip) {
+    if (skip && *skip) return;
+    for (int64_t i = blockIdx.x * (int64_t)kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock)
+        x[i] = 0.0f;
+}
+void launch_fill_zero32(const Launch &L, int64_t n, float *x, const int *skip_flag) {
+    hipLaunchKernelGGL(k_fill_zero32, dim3(grid_for(n, 8)), dim3(kBlock), 0, L.s, n, x, skip_flag);
 }
 
 }  // namespace mm

@@ -32,13 +32,14 @@ class MeshDesc(ctypes.Structure):
         ("length_scale", ctypes.c_double),
         ("grid_n", ctypes.c_int32 * 3),
         ("grid_x", c_double_p), ("grid_y", c_double_p), ("grid_z", c_double_p),
+        ("grid_mag0", ctypes.c_int32 * 3), ("grid_magn", ctypes.c_int32),
     ]
 
 
 class Options(ctypes.Structure):
     _fields_ = [("precision", ctypes.c_int32), ("device", ctypes.c_int32),
                 ("drop_tol", ctypes.c_double), ("pcg_batch", ctypes.c_int32),
-                ("use_structured", ctypes.c_int32)]
+                ("use_structured", ctypes.c_int32), ("inner_tol", ctypes.c_double)]
 
 
 class Info(ctypes.Structure):
@@ -67,7 +68,8 @@ class Stats(ctypes.Structure):
 
 class KernelTiming(ctypes.Structure):
     _fields_ = [("spmv_launches", ctypes.c_int64), ("spmv_ms", ctypes.c_double),
-                ("spmv_bytes", ctypes.c_int64)]
+                ("spmv_bytes", ctypes.c_int64), ("update_launches", ctypes.c_int64),
+                ("update_ms", ctypes.c_double), ("update_bytes", ctypes.c_int64)]
 
 
 VP = ctypes.c_void_p

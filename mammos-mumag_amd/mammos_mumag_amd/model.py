@@ -71,7 +71,9 @@ class Model:
     """Operators of one mesh + material set on the GPU."""
 
     def __init__(self, coords, tets, elem_tags, krn, precision="fp64", device=0, drop_tol=1e-12,
-                 pcg_batch=0, length_scale=1.0e-9):
+                 pcg_batch=0, length_scale=1.0e-9, grid=None, inner_tol=0.0):
+        """``grid`` (optional): dict(xs, ys, zs, mag0, magn) of a structured Kuhn-grid mesh
+        written by ``generate``; the library then validates it and uses its stencil kernels."""
         self.coords = np.ascontiguousarray(coords, dtype=np.float64)
         self.tets = np.ascontiguousarray(tets, dtype=np.int32)
         self.elem_tags = np.ascontiguousarray(elem_tags, dtype=np.int32)
@@ -113,11 +115,24 @@ class Model:
         desc.shell = shell.ctypes.data_as(_lib.c_uint8_p) if shell is not None else None
         desc.shell_R, desc.shell_Rinf = self.bnd["R"], self.bnd["Rinf"]
         desc.length_scale = length_scale
+        if grid is not None:
+            gx, gy, gz = (np.ascontiguousarray(grid[k], dtype=np.float64) for k in ("xs", "ys", "zs"))
+            keep += [gx, gy, gz]
+            desc.grid_n[0], desc.grid_n[1], desc.grid_n[2] = len(gx) - 1, len(gy) - 1, len(gz) - 1
+            desc.grid_x = gx.ctypes.data_as(_lib.c_double_p)
+            desc.grid_y = gy.ctypes.data_as(_lib.c_double_p)
+            desc.grid_z = gz.ctypes.data_as(_lib.c_double_p)
+            for a in range(3):
+                desc.grid_mag0[a] = int(grid["mag0"][a])
+            desc.grid_magn = int(grid["magn"])
+        self.grid = grid
         opts = _lib.Options()
         opts.precision = {"fp64": _lib.MM_PREC_FP64, "mixed": _lib.MM_PREC_MIXED}[precision]
         opts.device = device
         opts.drop_tol = drop_tol
         opts.pcg_batch = pcg_batch
+        opts.use_structured = 1 if grid is not None else 0
+        opts.inner_tol = inner_tol
         self._ctx = ctypes.c_void_p()
         _lib.check(lib.mm_ctx_create(ctypes.byref(desc), ctypes.byref(opts), ctypes.byref(self._ctx)),
                    "mm_ctx_create")
@@ -132,6 +147,15 @@ class Model:
         self.device = device
 
     # -------------------------------------------------------------- construction
+    @classmethod
+    def from_structured(cls, mesh, structured=True, **kw):
+        """From a ``generate.StructuredMesh``; ``structured=False`` forces the general path."""
+        grid = None
+        if structured:
+            grid = dict(xs=mesh.xs, ys=mesh.ys, zs=mesh.zs, mag0=[b[0] for b in mesh.magnet_box],
+                        magn=mesh.magnet_box[0][1])
+        return cls(mesh.coords(), mesh.tets(), mesh.elem_tags(), mesh.krn(), grid=grid, **kw)
+
     @classmethod
     def from_files(cls, base, **kw):
         coords, tets, etags = read_fly(base + ".fly")
@@ -340,4 +364,4 @@ class Model:
     def timing_get(self):
         t = _lib.KernelTiming()
         _lib.check(self._lib.mm_timing_get(self._ctx, ctypes.byref(t)), "mm_timing_get")
-        return dict(spmv_launches=t.spmv_launches, spmv_ms=t.spmv_ms, spmv_bytes=t.spmv_bytes)
+        return {k: getattr(t, k) for k, _ in _lib.KernelTiming._fields_}

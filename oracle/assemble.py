@@ -51,10 +51,10 @@ def element_geometry(coords, tets):
 
 
 def _coo_scalar(tets, local, n):
-    """Assemble per-element 4x4 blocks into an n x n CSR matrix."""
+    """Assemble per-element 4x4 blocks into an n x n CSR matrix (duplicates summed)."""
     rows = np.repeat(tets, 4, axis=1).ravel()
     cols = np.tile(tets, (1, 4)).ravel()
-    return sp.coo_matrix((local.reshape(-1), (rows, cols)), shape=(n, n)).tocsr()
+    return sp.csr_matrix((local.reshape(-1), (rows, cols)), shape=(n, n))
 
 
 def t3d_tensor(xq, R, Rinf):
@@ -114,20 +114,22 @@ class Model:
         V = self.V
         # meas (escript_tools.py:13-17)
         self.meas = np.bincount(tets.ravel(), weights=np.repeat(self.Js * vol / 4.0, 4), minlength=N)
-        # exchange (scalar) and the 6 anisotropy blocks
-        GG = np.einsum("eka,ela->ekl", G, G)
-        Sloc = (2.0 * self.A / V * vol)[:, None, None] * GG
-        S = _coo_scalar(tets, Sloc, N)
+        # only magnetic elements carry exchange, anisotropy, Dx and Gx coefficients
+        tm, Gm, vm = tets[mag], G[mag], vol[mag]
+        GG = np.einsum("eka,ela->ekl", Gm, Gm)
+        S = _coo_scalar(tm, (2.0 * self.A[mag] / V * vm)[:, None, None] * GG, N)
+        del GG
         self.S = S
         mass = (np.ones((4, 4)) + np.eye(4)) / 20.0
-        cK = -2.0 * self.K / V * vol
+        cK = -2.0 * self.K[mag] / V * vm
+        um = self.u[mag]
         C = sp.kron(S, sp.identity(3), format="csr")
         for a in range(3):
             for b in range(3):
-                w = cK * self.u[:, a] * self.u[:, b]
+                w = cK * um[:, a] * um[:, b]
                 if not np.any(w):
                     continue
-                Mab = _coo_scalar(tets, w[:, None, None] * mass[None], N)
+                Mab = _coo_scalar(tm, w[:, None, None] * mass[None], N)
                 E = sp.coo_matrix(([1.0], ([a], [b])), shape=(3, 3))
                 C = C + sp.kron(Mab, E, format="csr")
         self.C = C.tocsr()
@@ -137,10 +139,11 @@ class Model:
         self.Dex = D
         # divergence / gradient operators (matrix.py:9-35)
         self.Dxyz = []
+        jv = self.Js[mag] * vm / 4.0
         for a in range(3):
-            loc = np.repeat((self.Js * vol / 4.0)[:, None] * G[:, :, a], 4, axis=1).reshape(-1, 4, 4)
+            loc = np.repeat(jv[:, None] * Gm[:, :, a], 4, axis=1).reshape(-1, 4, 4)
             # loc[e, k, l] = Js vol/4 d_a phi_k   (row k test, column l trial)
-            self.Dxyz.append(_coo_scalar(tets, loc, N))
+            self.Dxyz.append(_coo_scalar(tm, loc, N))
         self.Gxyz = [(Dm.T / V).tocsr() for Dm in self.Dxyz]
         self._poisson(vol, G)
 

@@ -1,0 +1,133 @@
+"""GPU structured (Kuhn-grid stencil) path and mixed precision against the FP64 oracle,
+on a small generated 4-grain Voronoi cube (24 nm, 8^3 magnet voxels, graded air box).
+
+FP64: operators to 1e-12 relative, PCG with jax CG semantics, one NCG field step and a
+short loop to 1e-9.  Mixed precision (FP32 operators and inner PCG, FP64 dots, energies
+and outer residual): energies to 1e-6 relative, M.h to 1e-6 absolute.
+"""
+import numpy as np
+import pytest
+
+from mammos_mumag_amd import generate
+
+pytestmark = pytest.mark.gpu
+
+
+def rel(a, b):
+    a, b = np.asarray(a), np.asarray(b)
+    return float(np.max(np.abs(a - b)) / max(np.max(np.abs(b)), 1e-300))
+
+
+@pytest.fixture(scope="module")
+def mesh():
+    return generate.build(edge=24.0, n=8, layers=3, grains=4, cone=15.0, seed=7)
+
+
+@pytest.fixture(scope="module")
+def oracle(mesh):
+    from oracle.assemble import Model
+    return Model(mesh.coords(), mesh.tets(), mesh.elem_tags(), mesh.krn())
+
+
+@pytest.fixture(scope="module")
+def gpu_s(mesh):
+    from mammos_mumag_amd import Model
+    print("[gpu] structured context", flush=True)
+    return Model.from_structured(mesh)
+
+
+@pytest.fixture(scope="module")
+def gpu_g(mesh):
+    from mammos_mumag_amd import Model
+    print("[gpu] general context on the generated mesh", flush=True)
+    return Model.from_structured(mesh, structured=False)
+
+
+@pytest.fixture(scope="module")
+def gpu_mixed(mesh):
+    from mammos_mumag_amd import Model
+    print("[gpu] mixed-precision structured context", flush=True)
+    return Model.from_structured(mesh, precision="mixed")
+
+
+def _diag(model):
+    import ctypes
+    from mammos_mumag_amd import _lib
+    dex = np.empty(3 * model.N)
+    dk = np.empty(model.N)
+    _lib.check(model._lib.mm_get_diag_host(model._ctx, dex.ctypes.data_as(_lib.c_double_p),
+                                           dk.ctypes.data_as(_lib.c_double_p)), "diag")
+    return dex, dk
+
+
+@pytest.mark.parametrize("which", ["gpu_s", "gpu_g"])
+def test_operators(which, request, oracle):
+    model = request.getfixturevalue(which)
+    O = oracle
+    assert model.info["structured"] == (1 if which == "gpu_s" else 0)
+    assert abs(model.V / O.V - 1) < 1e-14
+    assert rel(model.meas, O.meas) < 1e-13
+    dex, dk = _diag(model)
+    assert rel(dex, O.Dex) < 1e-13
+    assert rel(dk, O.diagK) < 1e-12
+    rng = np.random.default_rng(11)
+    v = rng.normal(size=3 * O.N)
+    x = rng.normal(size=O.N)
+    assert rel(model.apply_C(model.to_device(v)).cpu().numpy(), O.C @ v) < 1e-12
+    assert rel(model.apply_K(model.to_device(x)).cpu().numpy(), O.K @ x) < 1e-12
+    b = model.divergence(model.to_device(v)).cpu().numpy()
+    assert rel(b, O.Dxyz[0] @ v[0::3] + O.Dxyz[1] @ v[1::3] + O.Dxyz[2] @ v[2::3]) < 1e-12
+    g = model.gradient(model.to_device(x)).cpu().numpy()
+    assert rel(g, np.stack([O.Gxyz[a] @ x for a in range(3)], 1).ravel()) < 1e-12
+
+
+def test_pcg_structured(gpu_s, oracle):
+    from oracle.solvers import jax_cg
+    O = oracle
+    m = O.uniform_m([0.0, 0.0, 1.0])
+    b = O.Dxyz[2] @ m[2::3]
+    x_ref, it_ref = jax_cg(O.K, O.diagK, b, np.zeros(O.N), tol=1e-10)
+    x = gpu_s.empty1()
+    it, rc = gpu_s.pcg(gpu_s.to_device(b), x, 1e-10)
+    assert rc == 0 and abs(it - it_ref) <= 1
+    assert rel(x.cpu().numpy(), x_ref) < 1e-9
+
+
+def test_loop_steps_structured(gpu_s, oracle):
+    from mammos_mumag_amd.loop import run_loop
+    from oracle.loop import run_loop as oracle_loop
+    p = generate.loop_params("cfg0")
+    rec_o, mini = oracle_loop(oracle, p, oracle.uniform_m(p["m"]), max_steps=4)
+    rec_g, stats, _ = run_loop(gpu_s, p, gpu_s.uniform_m(p["m"]), max_steps=4, save=False)
+    print("[gpu] oracle", mini.stats, "gpu", stats, flush=True)
+    for ro, rg in zip(rec_o, rec_g):
+        assert ro[0] == rg[0] and ro[1] == rg[1]
+        assert abs(rg[2] - ro[2]) < 1e-9
+        assert abs(rg[3] / ro[3] - 1) < 1e-9
+
+
+def test_mixed_precision_errors(gpu_mixed, oracle):
+    """Mixed precision keeps the FP64 answer to FP32 operator accuracy; the PCG meets the
+    FP64 tolerance on the true residual."""
+    from mammos_mumag_amd.loop import run_loop
+    from oracle.energies import total_eg
+    O = oracle
+    p = generate.loop_params("cfg0")
+    rng = np.random.default_rng(5)
+    m = O.uniform_m([0.1, 0.2, 0.97]) + 0.05 * rng.normal(size=3 * O.N) * np.repeat(O.meas > 0, 3)
+    n = np.linalg.norm(m.reshape(-1, 3), axis=1)
+    m = (m.reshape(-1, 3) / np.where(n == 0, 1, n)[:, None]).ravel()
+    E_ref, gp_ref, u_ref, _ = total_eg(O, m, 1e-10, 1.5, p["h"], 1, np.zeros(O.N))
+    u = gpu_mixed.empty1()
+    E, gp, gf, it = gpu_mixed.total_eg(gpu_mixed.to_device(m), 1.5, p["h"], 1e-10, 1, u)
+    print(f"[gpu] mixed: dE/E={abs(E / E_ref - 1):.2e} du={rel(u.cpu().numpy(), u_ref):.2e} "
+          f"dg={rel(gp.cpu().numpy(), gp_ref):.2e} inner iters={it}", flush=True)
+    assert abs(E / E_ref - 1) < 1e-6
+    assert rel(u.cpu().numpy(), u_ref) < 1e-5
+    assert rel(gp.cpu().numpy(), gp_ref) < 1e-5
+    rec, stats, _ = run_loop(gpu_mixed, p, gpu_mixed.uniform_m(p["m"]), max_steps=3, save=False)
+    from oracle.loop import run_loop as oracle_loop
+    rec_o, _ = oracle_loop(O, p, O.uniform_m(p["m"]), max_steps=3)
+    for ro, rg in zip(rec_o, rec):
+        assert abs(rg[2] - ro[2]) < 1e-6
+        assert abs(rg[3] / ro[3] - 1) < 1e-6
