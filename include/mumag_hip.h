@@ -77,8 +77,8 @@ typedef struct {
     double drop_tol;          /* drop |K_ij| < drop_tol * max(|K_ii|,|K_jj|); 0 keeps all */
     int32_t pcg_batch;        /* PCG iterations launched between host checks (0 = default) */
     int32_t use_structured;   /* 1: use the Kuhn-grid stencil kernels (needs grid_n) */
-    double inner_tol;         /* MM_PREC_MIXED: relative residual reduction of each FP32
-                                 inner solve (0 = default 1e-3) */
+    double inner_tol;         /* MM_PREC_MIXED: FP32 residual reduction between two FP64
+                                 reliable updates (0 = default 0.1) */
 } mm_options;
 
 typedef struct {

@@ -1,5 +1,5 @@
 // C ABI of libmumag_hip.so: context creation, the energy/gradient evaluation,
-// the Jacobi PCG (FP64, or FP32-inner / FP64-outer refinement), the truncated-CG
+// the Jacobi PCG (FP64, or FP32 with FP64 reliable updates), the truncated-CG
 // preconditioner and the nonlinear-CG minimiser.
 //
 // The minimiser is a host-side control loop over device kernels; each iteration
@@ -51,7 +51,7 @@ struct mm_ctx {
     int64_t N = 0, NE = 0, NEm = 0, Nm = 0;
     double V = 0.0;
     int pcg_batch = 16;
-    double inner_tol = 1e-3;
+    double ru_delta = 0.1;   // mixed precision: residual reduction between reliable updates
     int64_t k_nnz = 0, k_entries = 0;
     size_t bytes = 0;
     std::vector<void *> allocs;
@@ -72,7 +72,7 @@ struct mm_ctx {
 
     // workspace (N)
     double *b = nullptr, *r = nullptr, *z = nullptr, *p = nullptr, *q = nullptr;
-    float *rhs32 = nullptr, *r32 = nullptr, *z32 = nullptr, *p32 = nullptr, *q32 = nullptr, *dx32 = nullptr;
+    float *r32 = nullptr, *z32 = nullptr, *p32 = nullptr, *q32 = nullptr, *dx32 = nullptr;
     // workspace (3N)
     double *gC = nullptr, *gd = nullptr;
     double *tp = nullptr, *tr = nullptr, *tz = nullptr, *td = nullptr, *tCd = nullptr, *tHd = nullptr;
@@ -229,37 +229,55 @@ int pcg64(mm_ctx *c, const double *b, double *x, double tol, double atol, int ma
     return 0;
 }
 
-// Mixed precision: FP64 outer residual r = b - K x (stop when r.r <= max(tol^2 b.b,
-// atol^2)), FP32 inner Jacobi-PCG for K dx = r from dx = 0 to a relative residual
-// inner_tol, x += dx.  The answer meets the FP64 tolerance on the true residual.
+// Mixed precision with reliable updates: the Jacobi-PCG recursion (alpha, beta, r, z, p
+// and the correction dx) runs in FP32 with FP64 dot products; whenever the FP32
+// recursive residual has dropped by ru_delta since the last replacement, or claims
+// convergence, x += dx in FP64 and the residual is replaced by the true FP64 residual
+// b - K x, keeping the search direction p.  Stops when the true residual satisfies
+// r.r <= max(tol^2 b.b, atol^2), the FP64 path's bound.
 int pcg_mixed(mm_ctx *c, const double *b, double *x, double tol, double atol, int maxiter, int *iters) {
     Launch L = c->L();
     const int64_t n = c->N;
+    const double delta2 = c->ru_delta * c->ru_delta;
     op_K(c, x, c->q, nullptr, nullptr);
-    launch_residual(L, n, b, c->q, c->rhs32, tol, atol, true);
+    launch_ru_residual(L, n, b, c->q, c->r32, c->z32, c->p32, c->invDK32, tol, atol, maxiter, true);
+    launch_fill_zero32(L, n, c->dx32, nullptr);
     int rc = sync_scalars(c);
     if (rc) return rc;
-    int inner_total = 0;
-    const int max_outer = 50;
-    int outer = 0;
-    while (!c->hsc->ir_done && outer < max_outer && inner_total < maxiter) {
-        launch_fill_zero32(L, n, c->dx32, nullptr);
-        launch_pcg32_init(L, n, c->rhs32, c->r32, c->z32, c->p32, c->invDK32, c->inner_tol,
-                          maxiter - inner_total);
+    int total = 0;
+    while (!c->hsc->ir_done) {
         int it = 0;
-        rc = run_iterations<float>(
-            c, [&] { op_K32(c, c->p32, c->q32, &c->dsc->pcg_pq, &c->dsc->pcg_done); },
-            [&] { launch_pcg32_update(L, n, c->dx32, c->r32, c->z32, c->p32, c->q32, c->invDK32); },
-            [&] { launch_pcg32_pupdate(L, n, c->p32, c->z32); }, &it);
-        if (rc) return rc;
-        inner_total += it;
+        // FP32 iterations until the recursive residual has dropped by ru_delta (checked
+        // between batches) or claims convergence
+        const int B = c->pcg_batch;
+        int launched = 0;
+        const double rmax2 = c->hsc->ir_rmax2;
+        for (;;) {
+            for (int k = 0; k < B; ++k, ++launched) {
+                tick(c, c->t_spmv, launched, true);
+                op_K32(c, c->p32, c->q32, &c->dsc->pcg_pq, &c->dsc->pcg_done);
+                tick(c, c->t_spmv, launched, false);
+                tick(c, c->t_upd, launched, true);
+                launch_pcg32_update(L, n, c->dx32, c->r32, c->z32, c->p32, c->q32, c->invDK32);
+                tick(c, c->t_upd, launched, false);
+                launch_pcg32_pupdate(L, n, c->p32, c->z32);
+            }
+            if ((rc = sync_scalars(c))) return rc;
+            if (c->hsc->pcg_done || c->hsc->pcg_rr < delta2 * rmax2) break;
+        }
+        it = c->hsc->pcg_iter - total;
+        if (c->timing) {
+            if ((rc = collect(c->t_spmv, 0, it)) || (rc = collect(c->t_upd, 0, it))) return rc;
+        }
+        total = c->hsc->pcg_iter;
+        // reliable update: fold dx into x, replace r by the true residual
         launch_add_correction(L, n, x, c->dx32);
         op_K(c, x, c->q, nullptr, nullptr);
-        launch_residual(L, n, b, c->q, c->rhs32, tol, atol, false);
+        launch_ru_residual(L, n, b, c->q, c->r32, c->z32, c->p32, c->invDK32, tol, atol, maxiter, false);
         if ((rc = sync_scalars(c))) return rc;
-        ++outer;
+        if (!c->hsc->ir_done && c->hsc->pcg_iter >= maxiter) break;
     }
-    if (iters) *iters = inner_total;
+    if (iters) *iters = c->hsc->pcg_iter;
     if (!c->hsc->ir_done) return MM_W_MAXITER;
     return 0;
 }
@@ -382,7 +400,7 @@ int mm_ctx_create(const mm_mesh_desc *d, const mm_options *opts, mm_ctx **out) {
     c->V = structured ? S.V : H.V;
     c->Nm = structured ? S.n_mag_nodes : H.n_mag_nodes;
     if (o.pcg_batch > 0) c->pcg_batch = o.pcg_batch;
-    if (o.inner_tol > 0) c->inner_tol = o.inner_tol;
+    if (o.inner_tol > 0) c->ru_delta = o.inner_tol;
     int rc = 0;
     auto bail = [&](int code) {
         delete c;
@@ -408,15 +426,18 @@ int mm_ctx_create(const mm_mesh_desc *d, const mm_options *opts, mm_ctx **out) {
         c->n_mats = d->n_mats;
     }
     if (structured) {
-        // 7-point Poisson stencil: voxel sizes and 1 / (6 h) per axis, FP64 and FP32
+        // 7-point Poisson stencil: voxel sizes padded with zeros and 1 / (6 h), FP64 and FP32
         auto up = [&](const std::vector<double> &h, const double **ph, const double **pinv,
                       const float **ph32, const float **pinv32) -> int {
-            std::vector<double> inv(h.size());
-            for (size_t i = 0; i < h.size(); ++i) inv[i] = 1.0 / (6.0 * h[i]);
+            std::vector<double> hp(h.size() + 2, 0.0), inv(h.size() + 2, 0.0);
+            for (size_t i = 0; i < h.size(); ++i) {
+                hp[i + 1] = h[i];
+                inv[i + 1] = 1.0 / (6.0 * h[i]);
+            }
             double *a, *b;
             float *a32, *b32;
             int r_;
-            if ((r_ = c->upload(&a, h)) || (r_ = c->upload(&b, inv)) || (r_ = c->upload(&a32, convert<float>(h))) ||
+            if ((r_ = c->upload(&a, hp)) || (r_ = c->upload(&b, inv)) || (r_ = c->upload(&a32, convert<float>(hp))) ||
                 (r_ = c->upload(&b32, convert<float>(inv))))
                 return r_;
             *ph = a;
@@ -428,9 +449,9 @@ int mm_ctx_create(const mm_mesh_desc *d, const mm_options *opts, mm_ctx **out) {
         c->P.nx = c->P32.nx = S.nx;
         c->P.ny = c->P32.ny = S.ny;
         c->P.nz = c->P32.nz = S.nz;
-        if ((rc = up(S.hx, &c->P.hx, &c->P.inv6hx, &c->P32.hx, &c->P32.inv6hx)) ||
-            (rc = up(S.hy, &c->P.hy, &c->P.inv6hy, &c->P32.hy, &c->P32.inv6hy)) ||
-            (rc = up(S.hz, &c->P.hz, &c->P.inv6hz, &c->P32.hz, &c->P32.inv6hz)))
+        if ((rc = up(S.hx, &c->P.hxp, &c->P.ixp, &c->P32.hxp, &c->P32.ixp)) ||
+            (rc = up(S.hy, &c->P.hyp, &c->P.iyp, &c->P32.hyp, &c->P32.iyp)) ||
+            (rc = up(S.hz, &c->P.hzp, &c->P.izp, &c->P32.hzp, &c->P32.izp)))
             return bail(rc);
         uint16_t *vm;
         if ((rc = c->upload(&vm, S.vmat))) return bail(rc);
@@ -506,7 +527,7 @@ int mm_ctx_create(const mm_mesh_desc *d, const mm_options *opts, mm_ctx **out) {
     for (double **v : vN)
         if ((rc = c->alloc(v, n))) return bail(rc);
     if (c->precision == MM_PREC_MIXED) {
-        float **vF[] = {&c->rhs32, &c->r32, &c->z32, &c->p32, &c->q32, &c->dx32};
+        float **vF[] = {&c->r32, &c->z32, &c->p32, &c->q32, &c->dx32};
         for (float **v : vF)
             if ((rc = c->alloc(v, n))) return bail(rc);
     }

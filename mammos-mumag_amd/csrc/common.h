@@ -22,8 +22,8 @@ struct DevScalars {
     double tcg_eps, tcg_rz, tcg_den, tcg_beta, tcg_pmax;
     int tcg_done, tcg_neg, tcg_iter, tcg_pad;
     long long tcg_total;
-    // outer residual of the mixed-precision refinement
-    double ir_rr, ir_bound;
+    // true (FP64) residual of the mixed-precision reliable-update PCG
+    double ir_rr, ir_bound, ir_rmax2;
     int ir_done, ir_outer;
     // generic reduction slots and the energy of the last evaluation
     double red[8];
@@ -70,11 +70,13 @@ struct GridDev {
 };
 
 // 7-point Poisson stencil on the rectilinear grid (voxel sizes per axis).
+// hxp[i] = size of voxel i-1 along x for i in [1, nx-1], 0 at i = 0 and i = nx;
+// ixp[i] = 1 / (6 hxp[i]) (0 for the pads).  Same for y and z.
 template <typename T>
 struct PoissonGrid {
     int nx = 0, ny = 0, nz = 0;        // nodes per axis
-    const T *hx = nullptr, *hy = nullptr, *hz = nullptr;              // voxel sizes
-    const T *inv6hx = nullptr, *inv6hy = nullptr, *inv6hz = nullptr;  // 1 / (6 h)
+    const T *hxp = nullptr, *hyp = nullptr, *hzp = nullptr;
+    const T *ixp = nullptr, *iyp = nullptr, *izp = nullptr;
 };
 
 struct Reducer {
@@ -87,8 +89,8 @@ enum FinOp {
     FIN_PCG_UPDATE = 2,
     FIN_TCG_INIT = 3,
     FIN_TCG_UPDATE = 4,
-    FIN_IR_RESIDUAL = 5,
-    FIN_PCG32_INIT = 6,
+    FIN_RU_INIT = 5,
+    FIN_RU_REPLACE = 6,
 };
 
 struct FinArgs {
@@ -137,14 +139,13 @@ void launch_pcg_init(const Launch &L, int64_t n, const double *b, const double *
 void launch_pcg_update(const Launch &L, int64_t n, double *x, double *r, double *z, const double *p,
                        const double *q, const double *invD);
 void launch_pcg_pupdate(const Launch &L, int64_t n, double *p, const double *z);
-void launch_pcg32_init(const Launch &L, int64_t n, const float *b, float *r, float *z, float *p,
-                       const float *invD, double tol, int maxiter);
 void launch_pcg32_update(const Launch &L, int64_t n, float *x, float *r, float *z, const float *p,
                          const float *q, const float *invD);
 void launch_pcg32_pupdate(const Launch &L, int64_t n, float *p, const float *z);
-void launch_residual(const Launch &L, int64_t n, const double *b, const double *Ax, float *r32,
-                     double tol, double atol, bool first);
-void launch_add_correction(const Launch &L, int64_t n, double *x, const float *dx);
+void launch_ru_residual(const Launch &L, int64_t n, const double *b, const double *Ax, float *r32,
+                        float *z32, float *p32, const float *invD32, double tol, double atol,
+                        int maxiter, bool init);
+void launch_add_correction(const Launch &L, int64_t n, double *x, float *dx);
 
 // Element-by-element operators (general meshes), accumulated into zeroed outputs.
 void launch_ebe_exani(const Launch &L, const ElemDev &E, const double *v, double *y, double *b,

@@ -28,11 +28,13 @@ __device__ __forceinline__ double wave_max(double v) {
     return v;
 }
 
-// Block-level reduction of NV values; thread 0 writes partials[i * kMaxPartials + block].
-// Bit i of maxmask selects max (identity 0, used on absolute values) instead of sum.
+// Block-level reduction of NV values; thread 0 writes partials[i * kMaxPartials + block]
+// with the linear block index of a 1-, 2- or 3-D grid.  Bit i of maxmask selects max
+// (identity 0, used on absolute values) instead of sum.
 template <int NV>
 __device__ __forceinline__ void block_partials(double (&v)[NV], unsigned maxmask, double *partials) {
     __shared__ double sh[NV][kBlock / kWave];
+    const int bid = blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z);
     const int lane = threadIdx.x & (kWave - 1), w = threadIdx.x / kWave;
 #pragma unroll
     for (int i = 0; i < NV; ++i) v[i] = ((maxmask >> i) & 1u) ? wave_max(v[i]) : wave_sum(v[i]);
@@ -48,7 +50,7 @@ __device__ __forceinline__ void block_partials(double (&v)[NV], unsigned maxmask
 #pragma unroll
             for (int k = 1; k < kBlock / kWave; ++k)
                 a = ((maxmask >> i) & 1u) ? fmax(a, sh[i][k]) : a + sh[i][k];
-            partials[i * kMaxPartials + blockIdx.x] = a;
+            partials[i * kMaxPartials + bid] = a;
         }
     }
 }
@@ -129,21 +131,23 @@ __global__ __launch_bounds__(kBlock) void k_fin(const double *__restrict__ parti
             dsc->tcg_total += 1;
             break;
         }
-        case FIN_IR_RESIDUAL: {  // outer residual of the refinement PCG: tot = (r.r, b.b)
+        case FIN_RU_INIT:  // tot = (r.r FP64, r32.z32, b.b): fixes the bound
+        case FIN_RU_REPLACE: {
+            if (a.op == FIN_RU_INIT) {
+                const double bound = fmax(a.tol * a.tol * tot[2], a.atol * a.atol);
+                dsc->ir_bound = bound;
+                dsc->pcg_bound = bound;
+                dsc->pcg_iter = 0;
+                dsc->pcg_maxiter = a.maxiter;
+                dsc->ir_outer = 0;
+            }
             dsc->ir_rr = tot[0];
-            if (a.maxiter >= 0) dsc->ir_bound = fmax(a.tol * a.tol * tot[1], a.atol * a.atol);
+            dsc->ir_rmax2 = tot[0];
             dsc->ir_done = tot[0] <= dsc->ir_bound;
             dsc->ir_outer += 1;
-            break;
-        }
-        case FIN_PCG32_INIT: {  // inner FP32 PCG from x = 0: tot = (r.z, r.r, r.r)
-            dsc->pcg_gamma = tot[0];
-            dsc->pcg_rr = tot[1];
-            dsc->pcg_bs = tot[1];
-            dsc->pcg_bound = a.tol * a.tol * tot[1];
-            dsc->pcg_iter = 0;
-            dsc->pcg_maxiter = a.maxiter;
-            dsc->pcg_done = (dsc->ir_done != 0) || !((tot[1] > dsc->pcg_bound) && (0 < a.maxiter));
+            dsc->pcg_gamma = tot[1];
+            dsc->pcg_rr = tot[0];
+            dsc->pcg_done = dsc->ir_done || dsc->pcg_iter >= dsc->pcg_maxiter;
             break;
         }
     }
@@ -205,19 +209,19 @@ void launch_sell_spmv32(const Launch &L, const SellDev<float> &A, const float *x
 // orthogonal).  The edge (P, P + e_d) lies in 4 voxels; in each it belongs to 2 Kuhn
 // tetrahedra if its two transverse local coordinates are equal, else to 1, and each
 // contributes -vol_t / h_d^2 = -(h_d h_d1 h_d2 / 6) / h_d^2.  The diagonal is minus the
-// sum of the six couplings; Dirichlet rows are identity and couplings to Dirichlet
-// nodes are dropped (escript nullifyRowsAndCols semantics).
+// sum of the six couplings; Dirichlet rows (the box faces) are identity and couplings
+// to Dirichlet nodes are dropped (escript nullifyRowsAndCols semantics).
+//
+// Voxel sizes are stored padded with a zero at both ends (hp[i] = size of voxel i-1),
+// and ip[i] = 1 / (6 hp[i]) (0 for the pads), so no branch is needed for missing
+// voxels.  Blocks are 64 x 4 tiles of an (x, y) plane that march through a contiguous
+// range of z planes, so the z neighbours come from cache.
 
 template <typename T>
-__device__ __forceinline__ T edge_coef(const T *h1, const T *h2, int a, int b, int n1, int n2, T inv6hd) {
-    // transverse voxels a-1 (local coordinate 1) and a (local 0); same for b
-    T s = 0;
-    const bool a0 = a - 1 >= 0, a1 = a < n1, b0 = b - 1 >= 0, b1 = b < n2;
-    if (a0 && b0) s += (T)2 * h1[a - 1] * h2[b - 1];
-    if (a0 && b1) s += h1[a - 1] * h2[b];
-    if (a1 && b0) s += h1[a] * h2[b - 1];
-    if (a1 && b1) s += (T)2 * h1[a] * h2[b];
-    return -s * inv6hd;
+__device__ __forceinline__ T edge_sum(const T *__restrict__ h1p, const T *__restrict__ h2p, int a, int b) {
+    // transverse voxels a-1 / a (local coordinate 1 / 0) and b-1 / b; 2 tets when equal
+    const T l1 = h1p[a], u1 = h1p[a + 1], l2 = h2p[b], u2 = h2p[b + 1];
+    return (T)2 * l1 * l2 + l1 * u2 + u1 * l2 + (T)2 * u1 * u2;
 }
 
 template <typename T>
@@ -226,39 +230,40 @@ __global__ __launch_bounds__(kBlock) void k_grid_poisson(PoissonGrid<T> P, const
                                                          const int *skip, double *partials) {
     if (skip && *skip) return;
     const int nx = P.nx, ny = P.ny, nz = P.nz;
-    const int64_t n = (int64_t)nx * ny * nz;
+    const int i = blockIdx.x * 64 + (threadIdx.x & 63);
+    const int j = blockIdx.y * 4 + (threadIdx.x >> 6);
+    const int chunk = (nz + gridDim.z - 1) / gridDim.z;
+    const int k0 = blockIdx.z * chunk, k1 = min(nz, k0 + chunk);
+    const int64_t sy = nx, sz = (int64_t)nx * ny;
     double dot = 0.0;
-    for (int64_t idx = blockIdx.x * (int64_t)kBlock + threadIdx.x; idx < n;
-         idx += (int64_t)gridDim.x * kBlock) {
-        const int i = (int)(idx % nx);
-        const int j = (int)((idx / nx) % ny);
-        const int k = (int)(idx / ((int64_t)nx * ny));
-        T out;
-        const T xi = x[idx];
-        if (i == 0 || j == 0 || k == 0 || i == nx - 1 || j == ny - 1 || k == nz - 1) {
-            out = xi;  // Dirichlet row
-        } else {
-            // voxel sizes hx[0..nx-2] etc.
-            const T cxm = edge_coef(P.hy, P.hz, j, k, ny - 1, nz - 1, P.inv6hx[i - 1]);
-            const T cxp = edge_coef(P.hy, P.hz, j, k, ny - 1, nz - 1, P.inv6hx[i]);
-            const T cym = edge_coef(P.hz, P.hx, k, i, nz - 1, nx - 1, P.inv6hy[j - 1]);
-            const T cyp = edge_coef(P.hz, P.hx, k, i, nz - 1, nx - 1, P.inv6hy[j]);
-            const T czm = edge_coef(P.hx, P.hy, i, j, nx - 1, ny - 1, P.inv6hz[k - 1]);
-            const T czp = edge_coef(P.hx, P.hy, i, j, nx - 1, ny - 1, P.inv6hz[k]);
-            const T diag = -(cxm + cxp + cym + cyp + czm + czp);
-            const int64_t sy = nx, sz = (int64_t)nx * ny;
-            T acc = diag * xi;
-            // couplings to Dirichlet neighbours are dropped
-            if (i - 1 > 0) acc += cxm * x[idx - 1];
-            if (i + 1 < nx - 1) acc += cxp * x[idx + 1];
-            if (j - 1 > 0) acc += cym * x[idx - sy];
-            if (j + 1 < ny - 1) acc += cyp * x[idx + sy];
-            if (k - 1 > 0) acc += czm * x[idx - sz];
-            if (k + 1 < nz - 1) acc += czp * x[idx + sz];
-            out = acc;
+    if (i < nx && j < ny) {
+        const bool edge_ij = i == 0 || j == 0 || i == nx - 1 || j == ny - 1;
+        // x-edge sums depend on (j, k), y-edge sums on (k, i), z-edge sums on (i, j)
+        const T sz_ij = edge_sum(P.hxp, P.hyp, i, j);
+        const T mxm = (i - 1 > 0) ? (T)1 : (T)0, mxp = (i + 1 < nx - 1) ? (T)1 : (T)0;
+        const T mym = (j - 1 > 0) ? (T)1 : (T)0, myp = (j + 1 < ny - 1) ? (T)1 : (T)0;
+        const T ixm = P.ixp[i], ixp = P.ixp[i + 1], iym = P.iyp[j], iyp = P.iyp[j + 1];
+        for (int k = k0; k < k1; ++k) {
+            const int64_t idx = i + sy * j + sz * k;
+            const T xi = x[idx];
+            T out;
+            if (edge_ij || k == 0 || k == nz - 1) {
+                out = xi;  // Dirichlet row
+            } else {
+                const T sx = edge_sum(P.hyp, P.hzp, j, k);
+                const T syy = edge_sum(P.hzp, P.hxp, k, i);
+                const T cxm = -sx * ixm, cxp = -sx * ixp;
+                const T cym = -syy * iym, cyp = -syy * iyp;
+                const T czm = -sz_ij * P.izp[k], czp = -sz_ij * P.izp[k + 1];
+                const T diag = -(cxm + cxp + cym + cyp + czm + czp);
+                const T mzm = (k - 1 > 0) ? (T)1 : (T)0, mzp = (k + 1 < nz - 1) ? (T)1 : (T)0;
+                out = diag * xi + mxm * cxm * x[idx - 1] + mxp * cxp * x[idx + 1] +
+                      mym * cym * x[idx - sy] + myp * cyp * x[idx + sy] + mzm * czm * x[idx - sz] +
+                      mzp * czp * x[idx + sz];
+            }
+            y[idx] = out;
+            dot += (double)out * (double)xi;
         }
-        y[idx] = out;
-        dot += (double)out * (double)xi;
     }
     if (want_dot) {
         double v[1] = {dot};
@@ -267,23 +272,30 @@ __global__ __launch_bounds__(kBlock) void k_grid_poisson(PoissonGrid<T> P, const
 }
 
 template <typename T>
-static int poisson_blocks(const PoissonGrid<T> &P) {
-    return grid_for((int64_t)P.nx * P.ny * P.nz, 2);
+static dim3 poisson_grid(const PoissonGrid<T> &P) {
+    const int gx = (P.nx + 63) / 64, gy = (P.ny + 3) / 4;
+    int gz = std::max(1, std::min(P.nz, 1024 / std::max(1, gx * gy)));
+    while (gx * gy * gz > kMaxPartials && gz > 1) --gz;
+    return dim3(gx, gy, gz);
+}
+template <typename T>
+static int nblocks(const dim3 &g) {
+    return (int)(g.x * g.y * g.z);
 }
 
 void launch_grid_poisson(const Launch &L, const PoissonGrid<double> &P, const double *x, double *y,
                          double *dot_out, const int *skip_flag) {
-    const int nb = poisson_blocks(P);
-    hipLaunchKernelGGL(k_grid_poisson<double>, dim3(nb), dim3(kBlock), 0, L.s, P, x, y,
-                       dot_out != nullptr, skip_flag, L.red.partials);
-    if (dot_out) launch_fin(L, FinArgs::store(dot_out, 1, 0u, nb, skip_flag));
+    const dim3 g = poisson_grid(P);
+    hipLaunchKernelGGL(k_grid_poisson<double>, g, dim3(kBlock), 0, L.s, P, x, y, dot_out != nullptr,
+                       skip_flag, L.red.partials);
+    if (dot_out) launch_fin(L, FinArgs::store(dot_out, 1, 0u, nblocks<double>(g), skip_flag));
 }
 void launch_grid_poisson32(const Launch &L, const PoissonGrid<float> &P, const float *x, float *y,
                            double *dot_out, const int *skip_flag) {
-    const int nb = poisson_blocks(P);
-    hipLaunchKernelGGL(k_grid_poisson<float>, dim3(nb), dim3(kBlock), 0, L.s, P, x, y,
-                       dot_out != nullptr, skip_flag, L.red.partials);
-    if (dot_out) launch_fin(L, FinArgs::store(dot_out, 1, 0u, nb, skip_flag));
+    const dim3 g = poisson_grid(P);
+    hipLaunchKernelGGL(k_grid_poisson<float>, g, dim3(kBlock), 0, L.s, P, x, y, dot_out != nullptr,
+                       skip_flag, L.red.partials);
+    if (dot_out) launch_fin(L, FinArgs::store(dot_out, 1, 0u, nblocks<float>(g), skip_flag));
 }
 
 // ------------------------------------------------------------------ PCG
@@ -364,16 +376,6 @@ void launch_pcg_init(const Launch &L, int64_t n, const double *b, const double *
     a.maxiter = maxiter;
     pcg_init_t<double>(L, n, b, Ax0, r, z, p, invD, a);
 }
-void launch_pcg32_init(const Launch &L, int64_t n, const float *b, float *r, float *z, float *p,
-                       const float *invD, double tol, int maxiter) {
-    FinArgs a{};
-    a.op = FIN_PCG32_INIT;
-    a.nv = 3;
-    a.tol = tol;
-    a.maxiter = maxiter;
-    pcg_init_t<float>(L, n, b, nullptr, r, z, p, invD, a);
-}
-
 template <typename T>
 static void pcg_update_t(const Launch &L, int64_t n, T *x, T *r, T *z, const T *p, const T *q,
                          const T *invD) {
@@ -402,44 +404,67 @@ void launch_pcg32_pupdate(const Launch &L, int64_t n, float *p, const float *z) 
     hipLaunchKernelGGL(k_pcg_pupdate<float>, dim3(grid_for(n, 4)), dim3(kBlock), 0, L.s, n, p, z, L.dsc);
 }
 
-// Outer residual of the refinement PCG (FP64): r32 = (float)(b - Ax); partials (r.r, b.b).
-__global__ __launch_bounds__(kBlock) void k_residual(int64_t n, const double *__restrict__ b,
-                                                     const double *__restrict__ Ax,
-                                                     float *__restrict__ r32, double *partials) {
-    double s[2] = {0.0, 0.0};
+// Mixed precision, reliable updates: the FP32 iteration runs on the correction dx;
+// the true residual is recomputed in FP64 from x whenever the FP32 recursive residual
+// has dropped by a factor, keeping the search direction p.
+//   r = b - Ax (FP64), r32 = (float) r, z32 = r32 / D (as r32 * invD32); with INIT also
+//   p32 = z32.  Partials: (r.r in FP64, r32.z32, b.b).
+template <bool INIT>
+__global__ __launch_bounds__(kBlock) void k_ru_residual(int64_t n, const double *__restrict__ b,
+                                                        const double *__restrict__ Ax,
+                                                        float *__restrict__ r32, float *__restrict__ z32,
+                                                        float *__restrict__ p32,
+                                                        const float *__restrict__ invD32,
+                                                        const int *skip, double *partials) {
+    if (skip && *skip) return;
+    double s[3] = {0.0, 0.0, 0.0};
     for (int64_t i = blockIdx.x * (int64_t)kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock) {
         const double bi = b[i];
         const double ri = bi - Ax[i];
-        r32[i] = (float)ri;
+        const float rf = (float)ri;
+        const float zf = rf * invD32[i];
+        r32[i] = rf;
+        z32[i] = zf;
+        if (INIT) p32[i] = zf;
         s[0] += ri * ri;
-        s[1] += bi * bi;
+        s[1] += (double)rf * (double)zf;
+        s[2] += bi * bi;
     }
-    block_partials<2>(s, 0u, partials);
+    block_partials<3>(s, 0u, partials);
 }
-void launch_residual(const Launch &L, int64_t n, const double *b, const double *Ax, float *r32,
-                     double tol, double atol, bool first) {
+void launch_ru_residual(const Launch &L, int64_t n, const double *b, const double *Ax, float *r32,
+                        float *z32, float *p32, const float *invD32, double tol, double atol,
+                        int maxiter, bool init) {
     const int nb = grid_for(n, 4);
-    hipLaunchKernelGGL(k_residual, dim3(nb), dim3(kBlock), 0, L.s, n, b, Ax, r32, L.red.partials);
+    const int *skip = init ? nullptr : &L.dsc->ir_done;
+    if (init)
+        hipLaunchKernelGGL(k_ru_residual<true>, dim3(nb), dim3(kBlock), 0, L.s, n, b, Ax, r32, z32, p32,
+                           invD32, skip, L.red.partials);
+    else
+        hipLaunchKernelGGL(k_ru_residual<false>, dim3(nb), dim3(kBlock), 0, L.s, n, b, Ax, r32, z32, p32,
+                           invD32, skip, L.red.partials);
     FinArgs a{};
-    a.op = FIN_IR_RESIDUAL;
-    a.nv = 2;
+    a.op = init ? FIN_RU_INIT : FIN_RU_REPLACE;
+    a.nv = 3;
     a.nparts = nb;
     a.tol = tol;
     a.atol = atol;
-    a.maxiter = first ? 0 : -1;
-    a.skip = first ? nullptr : &L.dsc->ir_done;
+    a.maxiter = maxiter;
+    a.skip = skip;
     launch_fin(L, a);
 }
 
-// x += (double) dx; skipped once the outer residual has converged.
+// x += (double) dx; dx = 0.  Skipped once the true residual has converged.
 __global__ __launch_bounds__(kBlock) void k_add_correction(int64_t n, double *__restrict__ x,
-                                                           const float *__restrict__ dx,
+                                                           float *__restrict__ dx,
                                                            const DevScalars *dsc) {
     if (dsc->ir_done) return;
-    for (int64_t i = blockIdx.x * (int64_t)kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock)
+    for (int64_t i = blockIdx.x * (int64_t)kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock) {
         x[i] += (double)dx[i];
+        dx[i] = 0.0f;
+    }
 }
-void launch_add_correction(const Launch &L, int64_t n, double *x, const float *dx) {
+void launch_add_correction(const Launch &L, int64_t n, double *x, float *dx) {
     hipLaunchKernelGGL(k_add_correction, dim3(grid_for(n, 4)), dim3(kBlock), 0, L.s, n, x, dx, L.dsc);
 }
 
