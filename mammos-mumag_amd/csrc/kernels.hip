@@ -236,20 +236,26 @@ __global__ __launch_bounds__(kBlock) void k_grid_poisson(PoissonGrid<T> P, const
     const int k0 = blockIdx.z * chunk, k1 = min(nz, k0 + chunk);
     const int64_t sy = nx, sz = (int64_t)nx * ny;
     double dot = 0.0;
-    if (i < nx && j < ny) {
+    // the last chunks can be empty (k0 >= nz) when gridDim.z * chunk > nz
+    if (i < nx && j < ny && k0 < k1) {
         const bool edge_ij = i == 0 || j == 0 || i == nx - 1 || j == ny - 1;
         // x-edge sums depend on (j, k), y-edge sums on (k, i), z-edge sums on (i, j)
         const T sz_ij = edge_sum(P.hxp, P.hyp, i, j);
         const T mxm = (i - 1 > 0) ? (T)1 : (T)0, mxp = (i + 1 < nx - 1) ? (T)1 : (T)0;
         const T mym = (j - 1 > 0) ? (T)1 : (T)0, myp = (j + 1 < ny - 1) ? (T)1 : (T)0;
         const T ixm = P.ixp[i], ixp = P.ixp[i + 1], iym = P.iyp[j], iyp = P.iyp[j + 1];
+        const int64_t base = i + sy * j;
+        // sliding window along z: (k-1, k, k+1) values of this column stay in registers
+        T xm = k0 > 0 ? x[base + sz * (k0 - 1)] : (T)0;
+        T xc = x[base + sz * k0];
         for (int k = k0; k < k1; ++k) {
-            const int64_t idx = i + sy * j + sz * k;
-            const T xi = x[idx];
+            const int64_t idx = base + sz * k;
+            const T xp = k + 1 < nz ? x[idx + sz] : (T)0;
             T out;
             if (edge_ij || k == 0 || k == nz - 1) {
-                out = xi;  // Dirichlet row
+                out = xc;  // Dirichlet row
             } else {
+                const T xw = x[idx - 1], xe = x[idx + 1], xs = x[idx - sy], xn = x[idx + sy];
                 const T sx = edge_sum(P.hyp, P.hzp, j, k);
                 const T syy = edge_sum(P.hzp, P.hxp, k, i);
                 const T cxm = -sx * ixm, cxp = -sx * ixp;
@@ -257,12 +263,13 @@ __global__ __launch_bounds__(kBlock) void k_grid_poisson(PoissonGrid<T> P, const
                 const T czm = -sz_ij * P.izp[k], czp = -sz_ij * P.izp[k + 1];
                 const T diag = -(cxm + cxp + cym + cyp + czm + czp);
                 const T mzm = (k - 1 > 0) ? (T)1 : (T)0, mzp = (k + 1 < nz - 1) ? (T)1 : (T)0;
-                out = diag * xi + mxm * cxm * x[idx - 1] + mxp * cxp * x[idx + 1] +
-                      mym * cym * x[idx - sy] + myp * cyp * x[idx + sy] + mzm * czm * x[idx - sz] +
-                      mzp * czp * x[idx + sz];
+                out = diag * xc + mxm * cxm * xw + mxp * cxp * xe + mym * cym * xs + myp * cyp * xn +
+                      mzm * czm * xm + mzp * czp * xp;
             }
             y[idx] = out;
-            dot += (double)out * (double)xi;
+            dot += (double)out * (double)xc;
+            xm = xc;
+            xc = xp;
         }
     }
     if (want_dot) {
@@ -274,7 +281,7 @@ __global__ __launch_bounds__(kBlock) void k_grid_poisson(PoissonGrid<T> P, const
 template <typename T>
 static dim3 poisson_grid(const PoissonGrid<T> &P) {
     const int gx = (P.nx + 63) / 64, gy = (P.ny + 3) / 4;
-    int gz = std::max(1, std::min(P.nz, 1024 / std::max(1, gx * gy)));
+    int gz = std::max(1, std::min(P.nz, kMaxPartials / std::max(1, gx * gy)));
     while (gx * gy * gz > kMaxPartials && gz > 1) --gz;
     return dim3(gx, gy, gz);
 }

@@ -131,3 +131,16 @@ def test_mixed_precision_errors(gpu_mixed, oracle):
     for ro, rg in zip(rec_o, rec):
         assert abs(rg[2] - ro[2]) < 1e-6
         assert abs(rg[3] / ro[3] - 1) < 1e-6
+
+
+def test_poisson_stencil_chunked_grid():
+    """A 66^3-node grid, where the z-march leaves the last block chunks empty: the stencil
+    must touch nothing outside the grid and cover every row once (regression)."""
+    from mammos_mumag_amd import Model
+    from oracle.assemble import Model as OracleModel
+    g = generate.build(edge=39.0, n=39, layers=13, grains=2, cone=15.0, seed=3)
+    assert len(g.xs) == 66
+    model = Model.from_structured(g)
+    O = OracleModel(g.coords(), g.tets(), g.elem_tags(), g.krn())
+    x = np.random.default_rng(2).normal(size=O.N)
+    assert rel(model.apply_K(model.to_device(x)).cpu().numpy(), O.K @ x) < 1e-12
