@@ -71,8 +71,9 @@ struct mm_ctx {
     Reducer red;
 
     // workspace (N)
-    double *b = nullptr, *r = nullptr, *z = nullptr, *p = nullptr, *q = nullptr;
-    float *r32 = nullptr, *z32 = nullptr, *p32 = nullptr, *q32 = nullptr, *dx32 = nullptr;
+    double *b = nullptr, *r = nullptr, *z = nullptr, *p = nullptr, *p2 = nullptr, *q = nullptr;
+    float *r32 = nullptr, *z32 = nullptr, *p32 = nullptr, *p32b = nullptr, *q32 = nullptr, *dx32 = nullptr;
+    double *partsA = nullptr, *partsB = nullptr;  // partials of the fused PCG kernels
     // workspace (3N)
     double *gC = nullptr, *gd = nullptr;
     double *tp = nullptr, *tr = nullptr, *tz = nullptr, *td = nullptr, *tCd = nullptr, *tHd = nullptr;
@@ -137,12 +138,6 @@ void op_K(mm_ctx *c, const double *x, double *y, double *dot, const int *skip) {
     else
         launch_sell_spmv(c->L(), c->K, x, y, dot, skip);
 }
-void op_K32(mm_ctx *c, const float *x, float *y, double *dot, const int *skip) {
-    if (c->structured)
-        launch_grid_poisson32(c->L(), c->P32, x, y, dot, skip);
-    else
-        launch_sell_spmv32(c->L(), c->K32, x, y, dot, skip);
-}
 // y = C v and optionally b = Dx vx + Dy vy + Dz vz.  Structured kernels overwrite the
 // magnet entries only (the others stay zero); the element kernels accumulate.
 void op_C(mm_ctx *c, const double *v, double *y, double *b, const int *skip) {
@@ -182,48 +177,56 @@ int collect(KernelTimer &t, int from, int count) {
     return 0;
 }
 
-// One PCG solve on the device: batches of B iterations are queued, the host checks
-// the done flag between batches; launches past convergence are no-ops.
-template <typename T, typename KOP, typename UPD, typename PUPD>
-int run_iterations(mm_ctx *c, KOP kop, UPD upd, PUPD pupd, int *iters) {
-    const int B = c->pcg_batch;
-    int launched = 0, done = 0;
-    while (!done) {
-        for (int k = 0; k < B; ++k, ++launched) {
-            tick(c, c->t_spmv, launched, true);
-            kop();
-            tick(c, c->t_spmv, launched, false);
-            tick(c, c->t_upd, launched, true);
-            upd();
-            tick(c, c->t_upd, launched, false);
-            pupd();
-        }
-        int rc = sync_scalars(c);
-        if (rc) return rc;
-        done = c->hsc->pcg_done;
-    }
-    const int used = c->hsc->pcg_iter;
-    if (c->timing) {
-        int rc = collect(c->t_spmv, 0, used);
-        if (!rc) rc = collect(c->t_upd, 0, used);
-        if (rc) return rc;
-    }
-    *iters = used;
-    return 0;
+// One fused PCG iteration (2 kernels).  p alternates between two buffers.
+template <typename T>
+void pcg_iteration(mm_ctx *c, int i, int mode, double tol, double atol, int maxiter, T *z, T *const pb[2],
+                   T *q, T *x, T *r, const T *invD, const SellDev<T> &A, const PoissonGrid<T> &P, int *nB) {
+    Launch L = c->L();
+    PcgIterArgs a;
+    a.parts_in = c->partsB;
+    a.nparts_in = *nB;
+    a.parts_out = c->partsA;
+    a.iter = i;
+    a.mode = mode;
+    a.tol = tol;
+    a.atol = atol;
+    a.maxiter = maxiter;
+    const T *pold = pb[i & 1];
+    T *pnew = pb[(i + 1) & 1];
+    tick(c, c->t_spmv, i, true);
+    const int nA = c->structured ? launch_ka_grid<T>(L, P, a, z, pold, pnew, q)
+                                 : launch_ka_sell<T>(L, A, a, z, pold, pnew, q);
+    tick(c, c->t_spmv, i, false);
+    PcgIterArgs bb = a;
+    bb.parts_in = c->partsA;
+    bb.nparts_in = nA;
+    bb.parts_out = c->partsB;
+    tick(c, c->t_upd, i, true);
+    *nB = launch_kb<T>(L, c->N, bb, x, r, z, pnew, q, invD);
+    tick(c, c->t_upd, i, false);
 }
 
-// jax.scipy.sparse.linalg.cg semantics, FP64; x is updated in place.
+// jax.scipy.sparse.linalg.cg semantics, FP64; x is updated in place.  Batches of
+// pcg_batch iterations are queued; the host reads the stop flag between batches and
+// launches past convergence are no-ops.
 int pcg64(mm_ctx *c, const double *b, double *x, double tol, double atol, int maxiter, int *iters) {
     Launch L = c->L();
     const int64_t n = c->N;
     op_K(c, x, c->q, nullptr, nullptr);
-    launch_pcg_init(L, n, b, c->q, c->r, c->z, c->p, c->invDK, tol, atol, maxiter);
-    int it = 0;
-    int rc = run_iterations<double>(
-        c, [&] { op_K(c, c->p, c->q, &c->dsc->pcg_pq, &c->dsc->pcg_done); },
-        [&] { launch_pcg_update(L, n, x, c->r, c->z, c->p, c->q, c->invDK); },
-        [&] { launch_pcg_pupdate(L, n, c->p, c->z); }, &it);
-    if (rc) return rc;
+    int nB = launch_pcg0<double>(L, n, b, c->q, c->r, c->z, c->invDK, c->partsB);
+    double *const pb[2] = {c->p, c->p2};
+    int i = 0, rc = 0;
+    for (;;) {
+        for (int k = 0; k < c->pcg_batch; ++k, ++i)
+            pcg_iteration<double>(c, i, i == 0 ? PCG_FIRST : PCG_NORMAL, tol, atol, maxiter, c->z, pb, c->q, x,
+                                  c->r, c->invDK, c->K, c->P, &nB);
+        if ((rc = sync_scalars(c))) return rc;
+        if (c->hsc->pcg_done) break;
+    }
+    const int it = c->hsc->pcg_iter;
+    if (c->timing) {
+        if ((rc = collect(c->t_spmv, 0, it)) || (rc = collect(c->t_upd, 0, it))) return rc;
+    }
     if (iters) *iters = it;
     if (it >= maxiter && c->hsc->pcg_rr > c->hsc->pcg_bound) return MM_W_MAXITER;
     return 0;
@@ -233,7 +236,8 @@ int pcg64(mm_ctx *c, const double *b, double *x, double tol, double atol, int ma
 // and the correction dx) runs in FP32 with FP64 dot products; whenever the FP32
 // recursive residual has dropped by ru_delta since the last replacement, or claims
 // convergence, x += dx in FP64 and the residual is replaced by the true FP64 residual
-// b - K x, keeping the search direction p.  Stops when the true residual satisfies
+// b - K x; the search direction continues as p = z + beta p with gamma from the true
+// residual.  Stops when the true residual satisfies
 // r.r <= max(tol^2 b.b, atol^2), the FP64 path's bound.
 int pcg_mixed(mm_ctx *c, const double *b, double *x, double tol, double atol, int maxiter, int *iters) {
     Launch L = c->L();
@@ -244,40 +248,38 @@ int pcg_mixed(mm_ctx *c, const double *b, double *x, double tol, double atol, in
     launch_fill_zero32(L, n, c->dx32, nullptr);
     int rc = sync_scalars(c);
     if (rc) return rc;
-    int total = 0;
+    float *const pb[2] = {c->p32, c->p32b};
+    int i = 0, nB = 0, mode = PCG_RESTART;
     while (!c->hsc->ir_done) {
-        int it = 0;
-        // FP32 iterations until the recursive residual has dropped by ru_delta (checked
-        // between batches) or claims convergence
-        const int B = c->pcg_batch;
-        int launched = 0;
+        // FP32 iterations until the recursive residual has dropped by ru_delta since the
+        // last true residual (checked between batches) or claims convergence
         const double rmax2 = c->hsc->ir_rmax2;
+        const int seg0 = i;
         for (;;) {
-            for (int k = 0; k < B; ++k, ++launched) {
-                tick(c, c->t_spmv, launched, true);
-                op_K32(c, c->p32, c->q32, &c->dsc->pcg_pq, &c->dsc->pcg_done);
-                tick(c, c->t_spmv, launched, false);
-                tick(c, c->t_upd, launched, true);
-                launch_pcg32_update(L, n, c->dx32, c->r32, c->z32, c->p32, c->q32, c->invDK32);
-                tick(c, c->t_upd, launched, false);
-                launch_pcg32_pupdate(L, n, c->p32, c->z32);
+            for (int k = 0; k < c->pcg_batch; ++k, ++i) {
+                pcg_iteration<float>(c, i, mode, tol, atol, maxiter, c->z32, pb, c->q32, c->dx32, c->r32,
+                                     c->invDK32, c->K32, c->P32, &nB);
+                mode = PCG_NORMAL;
             }
             if ((rc = sync_scalars(c))) return rc;
             if (c->hsc->pcg_done || c->hsc->pcg_rr < delta2 * rmax2) break;
         }
-        it = c->hsc->pcg_iter - total;
+        // the stopped iteration index; the next segment continues from it with p kept
+        // (the kernels queued after it in the batch did nothing)
+        if (c->hsc->pcg_done) i = c->hsc->pcg_iter;
         if (c->timing) {
-            if ((rc = collect(c->t_spmv, 0, it)) || (rc = collect(c->t_upd, 0, it))) return rc;
+            const int used = i - seg0;  // event pairs are indexed by the global iteration number
+            if ((rc = collect(c->t_spmv, seg0, used)) || (rc = collect(c->t_upd, seg0, used))) return rc;
         }
-        total = c->hsc->pcg_iter;
         // reliable update: fold dx into x, replace r by the true residual
         launch_add_correction(L, n, x, c->dx32);
         op_K(c, x, c->q, nullptr, nullptr);
         launch_ru_residual(L, n, b, c->q, c->r32, c->z32, c->p32, c->invDK32, tol, atol, maxiter, false);
         if ((rc = sync_scalars(c))) return rc;
-        if (!c->hsc->ir_done && c->hsc->pcg_iter >= maxiter) break;
+        if (!c->hsc->ir_done && i >= maxiter) break;
+        mode = PCG_KEEP;
     }
-    if (iters) *iters = c->hsc->pcg_iter;
+    if (iters) *iters = i;
     if (!c->hsc->ir_done) return MM_W_MAXITER;
     return 0;
 }
@@ -523,11 +525,13 @@ int mm_ctx_create(const mm_mesh_desc *d, const mm_options *opts, mm_ctx **out) {
     if ((rc = c->alloc(&c->red.partials, 4 * (size_t)kMaxPartials))) return bail(rc);
 
     const size_t n = c->N, n3 = 3 * n;
-    double **vN[] = {&c->b, &c->r, &c->z, &c->p, &c->q};
+    if ((rc = c->alloc(&c->partsA, 3 * (size_t)kMaxPartials)) || (rc = c->alloc(&c->partsB, 3 * (size_t)kMaxPartials)))
+        return bail(rc);
+    double **vN[] = {&c->b, &c->r, &c->z, &c->p, &c->p2, &c->q};
     for (double **v : vN)
         if ((rc = c->alloc(v, n))) return bail(rc);
     if (c->precision == MM_PREC_MIXED) {
-        float **vF[] = {&c->r32, &c->z32, &c->p32, &c->q32, &c->dx32};
+        float **vF[] = {&c->r32, &c->z32, &c->p32, &c->p32b, &c->q32, &c->dx32};
         for (float **v : vF)
             if ((rc = c->alloc(v, n))) return bail(rc);
     }

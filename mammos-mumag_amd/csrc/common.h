@@ -10,21 +10,22 @@ namespace mm {
 constexpr int kWave = 64;          // CDNA wavefront width; SELL slice height
 constexpr int kBlock = 256;        // threads per block of the streaming kernels
 constexpr int kMaxPartials = 2048; // blocks of a reducing kernel (one partial each)
+constexpr int kFusedBlocks = 256;  // blocks (of 1024 threads) of the fused PCG kernels
 
 // Scalars that live on the device so that iterative solvers can run many
 // iterations without a host round trip.  One instance per context.
 struct DevScalars {
-    // Jacobi PCG (jax CG semantics); the inner FP32 solve of the mixed path reuses it
-    double pcg_gamma, pcg_pq, pcg_rr, pcg_bound, pcg_bs, pcg_beta;
-    int pcg_iter, pcg_maxiter, pcg_done, pcg_pad;
-    long long pcg_total;
+    // Jacobi PCG (jax CG semantics); the FP32 iteration of the mixed path reuses it.
+    // gamma = r.z is double-buffered by iteration parity.
+    double pcg_gam[2], pcg_rr, pcg_bound, pcg_bs, pcg_beta;
+    int pcg_iter, pcg_done;
     // truncated CG (M_inv)
     double tcg_eps, tcg_rz, tcg_den, tcg_beta, tcg_pmax;
     int tcg_done, tcg_neg, tcg_iter, tcg_pad;
     long long tcg_total;
     // true (FP64) residual of the mixed-precision reliable-update PCG
-    double ir_rr, ir_bound, ir_rmax2;
-    int ir_done, ir_outer;
+    double ir_rr, ir_bound, ir_rmax2, ru_gamma;
+    int ir_done, ir_outer, ru_maxiter, ru_pad;
     // generic reduction slots and the energy of the last evaluation
     double red[8];
     double energy;
@@ -85,8 +86,6 @@ struct Reducer {
 
 enum FinOp {
     FIN_STORE = 0,
-    FIN_PCG_INIT = 1,
-    FIN_PCG_UPDATE = 2,
     FIN_TCG_INIT = 3,
     FIN_TCG_UPDATE = 4,
     FIN_RU_INIT = 5,
@@ -113,6 +112,18 @@ struct FinArgs {
     }
 };
 
+// Arguments of one fused PCG iteration (pcg_fused.hip).
+enum PcgMode { PCG_FIRST = 0, PCG_NORMAL = 1, PCG_RESTART = 2, PCG_KEEP = 3 };
+struct PcgIterArgs {
+    const double *parts_in = nullptr;  // partials of the previous kernel (slots of kMaxPartials)
+    int nparts_in = 0;
+    double *parts_out = nullptr;
+    int iter = 0;                      // iteration index i (host-known)
+    int mode = PCG_NORMAL;
+    double tol = 0, atol = 0;          // FIRST: bound = max(tol^2 b.b, atol^2)
+    int maxiter = 0;
+};
+
 struct Launch {
     hipStream_t s;
     Reducer red;
@@ -133,15 +144,19 @@ void launch_grid_poisson(const Launch &L, const PoissonGrid<double> &P, const do
 void launch_grid_poisson32(const Launch &L, const PoissonGrid<float> &P, const float *x, float *y,
                            double *dot_out, const int *skip_flag);
 
-// PCG building blocks (jax CG semantics), FP64 and the FP32 inner solve.
-void launch_pcg_init(const Launch &L, int64_t n, const double *b, const double *Ax0, double *r,
-                     double *z, double *p, const double *invD, double tol, double atol, int maxiter);
-void launch_pcg_update(const Launch &L, int64_t n, double *x, double *r, double *z, const double *p,
-                       const double *q, const double *invD);
-void launch_pcg_pupdate(const Launch &L, int64_t n, double *p, const double *z);
-void launch_pcg32_update(const Launch &L, int64_t n, float *x, float *r, float *z, const float *p,
-                         const float *q, const float *invD);
-void launch_pcg32_pupdate(const Launch &L, int64_t n, float *p, const float *z);
+// Fused PCG iteration (pcg_fused.hip); each returns the number of blocks (= partials).
+template <typename T>
+int launch_pcg0(const Launch &L, int64_t n, const T *b, const T *Ax0, T *r, T *z, const T *invD,
+                double *parts_out);
+template <typename T>
+int launch_ka_grid(const Launch &L, const PoissonGrid<T> &P, const PcgIterArgs &a, const T *z,
+                   const T *pold, T *pnew, T *q);
+template <typename T>
+int launch_ka_sell(const Launch &L, const SellDev<T> &A, const PcgIterArgs &a, const T *z,
+                   const T *pold, T *pnew, T *q);
+template <typename T>
+int launch_kb(const Launch &L, int64_t n, const PcgIterArgs &a, T *x, T *r, T *z, const T *p,
+              const T *q, const T *invD);
 void launch_ru_residual(const Launch &L, int64_t n, const double *b, const double *Ax, float *r32,
                         float *z32, float *p32, const float *invD32, double tol, double atol,
                         int maxiter, bool init);

@@ -89,27 +89,6 @@ __global__ __launch_bounds__(kBlock) void k_fin(const double *__restrict__ parti
         case FIN_STORE:
             for (int i = 0; i < a.nv; ++i) a.out[i] = tot[i];
             break;
-        case FIN_PCG_INIT: {  // tot = (r.z, r.r, b.b)
-            dsc->pcg_gamma = tot[0];
-            dsc->pcg_rr = tot[1];
-            dsc->pcg_bs = tot[2];
-            const double bound = fmax(a.tol * a.tol * tot[2], a.atol * a.atol);
-            dsc->pcg_bound = bound;
-            dsc->pcg_iter = 0;
-            dsc->pcg_maxiter = a.maxiter;
-            dsc->pcg_done = !((tot[1] > bound) && (0 < a.maxiter));
-            break;
-        }
-        case FIN_PCG_UPDATE: {  // tot = (r.z, r.r)
-            dsc->pcg_beta = tot[0] / dsc->pcg_gamma;
-            dsc->pcg_gamma = tot[0];
-            dsc->pcg_rr = tot[1];
-            const int it = dsc->pcg_iter + 1;
-            dsc->pcg_iter = it;
-            dsc->pcg_total += 1;
-            dsc->pcg_done = !((tot[1] > dsc->pcg_bound) && (it < dsc->pcg_maxiter));
-            break;
-        }
         case FIN_TCG_INIT: {  // tot = (g.g, r.z)
             const double nrm = sqrt(tot[0]);
             dsc->tcg_eps = fmin(0.5, sqrt(nrm) * nrm);
@@ -138,16 +117,16 @@ __global__ __launch_bounds__(kBlock) void k_fin(const double *__restrict__ parti
                 dsc->ir_bound = bound;
                 dsc->pcg_bound = bound;
                 dsc->pcg_iter = 0;
-                dsc->pcg_maxiter = a.maxiter;
+                dsc->ru_maxiter = a.maxiter;
                 dsc->ir_outer = 0;
             }
             dsc->ir_rr = tot[0];
             dsc->ir_rmax2 = tot[0];
             dsc->ir_done = tot[0] <= dsc->ir_bound;
             dsc->ir_outer += 1;
-            dsc->pcg_gamma = tot[1];
+            dsc->ru_gamma = tot[1];
             dsc->pcg_rr = tot[0];
-            dsc->pcg_done = dsc->ir_done || dsc->pcg_iter >= dsc->pcg_maxiter;
+            dsc->pcg_done = dsc->ir_done || dsc->pcg_iter >= dsc->ru_maxiter;
             break;
         }
     }
@@ -305,112 +284,7 @@ void launch_grid_poisson32(const Launch &L, const PoissonGrid<float> &P, const f
     if (dot_out) launch_fin(L, FinArgs::store(dot_out, 1, 0u, nblocks<float>(g), skip_flag));
 }
 
-// ------------------------------------------------------------------ PCG
-// jax.scipy.sparse.linalg.cg with M(x) = x * (1/D):
-//   r = b - A x0; z = M r; p = z; gamma = r.z; bound = max(tol^2 b.b, atol^2)
-//   while r.r > bound and k < maxiter: q = A p; alpha = gamma/(p.q); x += alpha p;
-//       r -= alpha q; z = M r; gamma' = r.z; p = z + (gamma'/gamma) p; k += 1
-// T = double (FP64 path) or float (inner solve of the mixed path; FP64 dots).
-
-template <typename T>
-__global__ __launch_bounds__(kBlock) void k_pcg_init(int64_t n, const T *__restrict__ b,
-                                                     const T *__restrict__ Ax0, T *__restrict__ r,
-                                                     T *__restrict__ z, T *__restrict__ p,
-                                                     const T *__restrict__ invD, double *partials) {
-    double s[3] = {0.0, 0.0, 0.0};
-    for (int64_t i = blockIdx.x * (int64_t)kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock) {
-        const T bi = b[i];
-        const T ri = Ax0 ? bi - Ax0[i] : bi;
-        const T zi = ri * invD[i];
-        r[i] = ri;
-        z[i] = zi;
-        p[i] = zi;
-        s[0] += (double)ri * (double)zi;
-        s[1] += (double)ri * (double)ri;
-        s[2] += (double)bi * (double)bi;
-    }
-    block_partials<3>(s, 0u, partials);
-}
-
-template <typename T>
-__global__ __launch_bounds__(kBlock) void k_pcg_update(int64_t n, T *__restrict__ x, T *__restrict__ r,
-                                                       T *__restrict__ z, const T *__restrict__ p,
-                                                       const T *__restrict__ q,
-                                                       const T *__restrict__ invD, const DevScalars *dsc,
-                                                       double *partials) {
-    if (dsc->pcg_done) return;
-    const T alpha = (T)(dsc->pcg_gamma / dsc->pcg_pq);
-    double s[2] = {0.0, 0.0};
-    for (int64_t i = blockIdx.x * (int64_t)kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock) {
-        x[i] = x[i] + alpha * p[i];
-        const T ri = r[i] - alpha * q[i];
-        const T zi = ri * invD[i];
-        r[i] = ri;
-        z[i] = zi;
-        s[0] += (double)ri * (double)zi;
-        s[1] += (double)ri * (double)ri;
-    }
-    block_partials<2>(s, 0u, partials);
-}
-
-template <typename T>
-__global__ __launch_bounds__(kBlock) void k_pcg_pupdate(int64_t n, T *__restrict__ p, const T *__restrict__ z,
-                                                        const DevScalars *dsc) {
-    if (dsc->pcg_done) return;
-    const T beta = (T)dsc->pcg_beta;
-    for (int64_t i = blockIdx.x * (int64_t)kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock)
-        p[i] = z[i] + beta * p[i];
-}
-
-template <typename T>
-static void pcg_init_t(const Launch &L, int64_t n, const T *b, const T *Ax0, T *r, T *z, T *p,
-                       const T *invD, const FinArgs &fa) {
-    const int nb = grid_for(n, 4);
-    hipLaunchKernelGGL(k_pcg_init<T>, dim3(nb), dim3(kBlock), 0, L.s, n, b, Ax0, r, z, p, invD,
-                       L.red.partials);
-    FinArgs a = fa;
-    a.nparts = nb;
-    launch_fin(L, a);
-}
-
-void launch_pcg_init(const Launch &L, int64_t n, const double *b, const double *Ax0, double *r,
-                     double *z, double *p, const double *invD, double tol, double atol, int maxiter) {
-    FinArgs a{};
-    a.op = FIN_PCG_INIT;
-    a.nv = 3;
-    a.tol = tol;
-    a.atol = atol;
-    a.maxiter = maxiter;
-    pcg_init_t<double>(L, n, b, Ax0, r, z, p, invD, a);
-}
-template <typename T>
-static void pcg_update_t(const Launch &L, int64_t n, T *x, T *r, T *z, const T *p, const T *q,
-                         const T *invD) {
-    const int nb = grid_for(n, 4);
-    hipLaunchKernelGGL(k_pcg_update<T>, dim3(nb), dim3(kBlock), 0, L.s, n, x, r, z, p, q, invD, L.dsc,
-                       L.red.partials);
-    FinArgs a{};
-    a.op = FIN_PCG_UPDATE;
-    a.nv = 2;
-    a.nparts = nb;
-    a.skip = &L.dsc->pcg_done;
-    launch_fin(L, a);
-}
-void launch_pcg_update(const Launch &L, int64_t n, double *x, double *r, double *z, const double *p,
-                       const double *q, const double *invD) {
-    pcg_update_t<double>(L, n, x, r, z, p, q, invD);
-}
-void launch_pcg32_update(const Launch &L, int64_t n, float *x, float *r, float *z, const float *p,
-                         const float *q, const float *invD) {
-    pcg_update_t<float>(L, n, x, r, z, p, q, invD);
-}
-void launch_pcg_pupdate(const Launch &L, int64_t n, double *p, const double *z) {
-    hipLaunchKernelGGL(k_pcg_pupdate<double>, dim3(grid_for(n, 4)), dim3(kBlock), 0, L.s, n, p, z, L.dsc);
-}
-void launch_pcg32_pupdate(const Launch &L, int64_t n, float *p, const float *z) {
-    hipLaunchKernelGGL(k_pcg_pupdate<float>, dim3(grid_for(n, 4)), dim3(kBlock), 0, L.s, n, p, z, L.dsc);
-}
-
+// ------------------------------------------------------------------ reliable updates (mixed precision)
 // Mixed precision, reliable updates: the FP32 iteration runs on the correction dx;
 // the true residual is recomputed in FP64 from x whenever the FP32 recursive residual
 // has dropped by a factor, keeping the search direction p.
