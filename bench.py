@@ -25,6 +25,14 @@ for _p in (ROOT, ROOT / "mammos-mumag_amd"):
 
 METRIC = "hysteresis field steps/s @2M nodes: learned+FP32 vs plain FP64 on one MI355X"
 HBM_PEAK_GBS = 8000.0
+# HBM bytes per launch of the PCG kernels on the benchmark grid, from rocprofv3 --pmc
+# FETCH_SIZE and WRITE_SIZE (KiB per dispatch) scaled by the same counters on a 512 MiB
+# device copy (bench.py --traffic-probe); filled in from the committed profile summaries.
+TRAFFIC = {}
+
+
+def key_of(k):
+    return k["key"]
 
 
 def log(*a):
@@ -123,6 +131,15 @@ def gpu_run(args):
     mh = model.compute_mh(m, p["h"])
     log(f"[bench] timed {args.steps} steps: {elapsed:.3f} s, last h={hext - p['hstep']:.2f} T, "
         f"M.h={mh:.6f} T, E={E:.6e}, {st.as_dict()}, {timing}")
+    if args.traffic_probe:
+        # reference dispatch for the HBM byte counters: a device copy of 512 MiB
+        # (reads and writes 524288 KiB)
+        src = torch.ones(64 << 20, dtype=torch.float64, device="cuda")
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        log("[bench] traffic probe: copied 512 MiB")
     return dict(elapsed=elapsed, stats=st.as_dict(), timing=timing, info=model.info,
                 n_nodes=n_nodes, n_elems=n_elems, n_mag_nodes=int(np.sum(model.meas > 0)))
 
@@ -138,6 +155,8 @@ def main():
     ap.add_argument("--general", action="store_true", help="unstructured kernels (SELL K, element loops)")
     ap.add_argument("--inner-tol", type=float, default=0.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--traffic-probe", action="store_true",
+                    help="after the run, copy 512 MiB on the device (calibrates the FETCH_SIZE / WRITE_SIZE counters)")
     ap.add_argument("--cpu-worker", default=None, help=argparse.SUPPRESS)
     args = ap.parse_args()
     if args.cpu_worker:
@@ -158,20 +177,21 @@ def main():
     roof = None
     kernels = []
     prec = "FP32 inner solve" if args.precision == "mixed" else "FP64"
-    kname = ("k_grid_poisson (7-point Poisson stencil)" if res["info"]["structured"]
-             else "k_sell_spmv (Poisson K, SELL-64)")
-    for key, name in (("spmv", f"{kname}, {prec}"), ("update", f"k_pcg_update (fused PCG update + 2 dots), {prec}")):
+    kname = ("k_ka_grid (7-point Poisson stencil fused with p = z + beta p, p.q partials)"
+             if res["info"]["structured"] else "k_ka_sell (SELL-64 K apply fused with p = z + beta p, p.q partials)")
+    for key, name in (("spmv", f"{kname}, {prec}"),
+                      ("update", f"k_kb (x, r, z update, r.z and r.r partials), {prec}")):
         if tm[f"{key}_launches"] > 0:
             avg_s = tm[f"{key}_ms"] / tm[f"{key}_launches"] / 1e3
-            kernels.append(dict(kernel=name, total_ms=tm[f"{key}_ms"], avg_s=avg_s,
+            kernels.append(dict(key=key, kernel=name, total_ms=tm[f"{key}_ms"], avg_s=avg_s,
                                 bytes=tm[f"{key}_bytes"], launches=tm[f"{key}_launches"]))
     if kernels:
         k = max(kernels, key=lambda d: d["total_ms"])  # the dominant kernel of the timed steps
         achieved = k["bytes"] / k["avg_s"] / 1e9
         roof = dict(bound="hbm", achieved=round(achieved, 1), peak=HBM_PEAK_GBS, unit="GB/s",
-                    frac=round(achieved / HBM_PEAK_GBS, 4), traffic=None, kernel=k["kernel"],
-                    bytes_per_launch=k["bytes"], avg_launch_us=round(k["avg_s"] * 1e6, 2),
-                    launches=k["launches"],
+                    frac=round(achieved / HBM_PEAK_GBS, 4), traffic=TRAFFIC.get((args.precision, key_of(k))),
+                    kernel=k["kernel"], bytes_per_launch=k["bytes"], avg_launch_us=round(k["avg_s"] * 1e6, 2),
+                    sampled_launches=k["launches"],
                     others=[dict(kernel=o["kernel"], avg_launch_us=round(o["avg_s"] * 1e6, 2),
                                  achieved_GBs=round(o["bytes"] / o["avg_s"] / 1e9, 1))
                             for o in kernels if o is not k])

@@ -173,9 +173,11 @@ int mm_apply_K(mm_ctx *ctx, const double *x, double *y);
 int mm_divergence(mm_ctx *ctx, const double *m, double *b);
 int mm_gradient(mm_ctx *ctx, const double *u, double *g);
 
-/* HIP-event timing of the PCG kernels since the last mm_timing_enable: the K apply
- * (SELL SpMV or 7-point stencil, FP64 or the FP32 inner solve) and the fused vector
- * update.  Only launches that did work (before convergence) are counted. */
+/* HIP-event timing of the two fused PCG kernels since the last mm_timing_enable: the
+ * K apply with the search-direction update (SELL SpMV or 7-point stencil, FP64 or the
+ * FP32 inner solve) and the vector update.  One iteration in 16 is timed (the counts
+ * are of the sampled launches); only launches that did work (before convergence) are
+ * counted. */
 typedef struct {
     int64_t spmv_launches;
     double spmv_ms;           /* summed device time of those launches */

@@ -160,15 +160,28 @@ void op_grad(mm_ctx *c, const double *u, double *g) {
     }
 }
 
-// event pair idx of a timer (ignored past the pool)
-void tick(mm_ctx *c, KernelTimer &t, int idx, bool start) {
-    if (!c->timing || idx >= (int)t.ev.size() / 2) return;
-    (void)hipEventRecord(t.ev[2 * idx + (start ? 0 : 1)], c->s);
+// The PCG kernels are timed on a sample: iteration i is bracketed by an event pair when
+// i % kTimerStride == kTimerPhase (mid-batch, away from the host synchronisation), so
+// that the events barely perturb the timed solve.  Pair index i / kTimerStride; pairs
+// past the pool are not recorded.
+constexpr int kTimerStride = 16, kTimerPhase = 7;
+// the launch of iteration i, with the timer's event pair attached when i is sampled
+Launch timed(mm_ctx *c, KernelTimer &t, int i) {
+    Launch l = c->L();
+    if (!c->timing || i % kTimerStride != kTimerPhase) return l;
+    const int idx = i / kTimerStride;
+    if (idx >= (int)t.ev.size() / 2) return l;
+    l.t0 = t.ev[2 * idx];
+    l.t1 = t.ev[2 * idx + 1];
+    return l;
 }
-// add the durations of pairs [from, from + count) (after the stream has synchronised)
+// add the durations of the sampled iterations in [from, from + count) (after the stream
+// has synchronised)
 int collect(KernelTimer &t, int from, int count) {
     const int pool = (int)t.ev.size() / 2;
-    for (int j = from; j < std::min(from + count, pool); ++j) {
+    for (int i = from; i < from + count; ++i) {
+        if (i % kTimerStride != kTimerPhase || i / kTimerStride >= pool) continue;
+        const int j = i / kTimerStride;
         float ms = 0.f;
         HIPC(hipEventElapsedTime(&ms, t.ev[2 * j], t.ev[2 * j + 1]));
         t.ms += ms;
@@ -181,7 +194,6 @@ int collect(KernelTimer &t, int from, int count) {
 template <typename T>
 void pcg_iteration(mm_ctx *c, int i, int mode, double tol, double atol, int maxiter, T *z, T *const pb[2],
                    T *q, T *x, T *r, const T *invD, const SellDev<T> &A, const PoissonGrid<T> &P, int *nB) {
-    Launch L = c->L();
     PcgIterArgs a;
     a.parts_in = c->partsB;
     a.nparts_in = *nB;
@@ -193,17 +205,14 @@ void pcg_iteration(mm_ctx *c, int i, int mode, double tol, double atol, int maxi
     a.maxiter = maxiter;
     const T *pold = pb[i & 1];
     T *pnew = pb[(i + 1) & 1];
-    tick(c, c->t_spmv, i, true);
-    const int nA = c->structured ? launch_ka_grid<T>(L, P, a, z, pold, pnew, q)
-                                 : launch_ka_sell<T>(L, A, a, z, pold, pnew, q);
-    tick(c, c->t_spmv, i, false);
+    const Launch LA = timed(c, c->t_spmv, i);
+    const int nA = c->structured ? launch_ka_grid<T>(LA, P, a, z, pold, pnew, q)
+                                 : launch_ka_sell<T>(LA, A, a, z, pold, pnew, q);
     PcgIterArgs bb = a;
     bb.parts_in = c->partsA;
     bb.nparts_in = nA;
     bb.parts_out = c->partsB;
-    tick(c, c->t_upd, i, true);
-    *nB = launch_kb<T>(L, c->N, bb, x, r, z, pnew, q, invD);
-    tick(c, c->t_upd, i, false);
+    *nB = launch_kb<T>(timed(c, c->t_upd, i), c->N, bb, x, r, z, pnew, q, invD);
 }
 
 // jax.scipy.sparse.linalg.cg semantics, FP64; x is updated in place.  Batches of
@@ -387,6 +396,8 @@ int mm_ctx_create(const mm_mesh_desc *d, const mm_options *opts, mm_ctx **out) {
         std::string err = structured_setup(*d, S);
         if (!err.empty()) return fail(MM_E_MESH, err);
         if (!(S.V > 0.0)) return fail(MM_E_MESH, "no magnetic element (all Js <= 0)");
+        if (!ka_grid_fits(d->grid_n[0], d->grid_n[1], d->grid_n[2]))
+            return fail(MM_E_MESH, "structured: grid too large for the PCG tiles; use use_structured = 0");
     } else {
         std::string err = assemble(*d, o.drop_tol, H, true);
         if (!err.empty()) return fail(MM_E_MESH, err);
@@ -504,11 +515,12 @@ int mm_ctx_create(const mm_mesh_desc *d, const mm_options *opts, mm_ctx **out) {
         c->E.vol = dv;
         c->E.mat = dm;
     }
-    // algorithmic bytes of one K apply in the PCG loop: stored entries (value + column),
-    // row pointers, x read once, y written once; the stencil stores no matrix.
+    // algorithmic bytes of one KA launch of the PCG loop: z and p_old read once, p and
+    // q = K p written once, plus the stored matrix for SELL (entries: value + column,
+    // slice pointers); the stencil stores no matrix.
     const int s_k = c->precision == MM_PREC_MIXED ? 4 : 8;
-    c->t_spmv.bytes = structured ? 2 * (int64_t)s_k * c->N
-                                 : c->k_nnz * (s_k + 4) + 4 * (c->N + 1) + 2 * (int64_t)s_k * c->N;
+    c->t_spmv.bytes = structured ? 4 * (int64_t)s_k * c->N
+                                 : c->k_nnz * (s_k + 4) + 4 * (c->N + 1) + 4 * (int64_t)s_k * c->N;
     // PCG update: reads x, r, p, q, invD; writes x, r, z
     c->t_upd.bytes = 8 * (int64_t)s_k * c->N;
 
@@ -847,7 +859,7 @@ int mm_timing_enable(mm_ctx *c, int32_t on) {
     c->timing = on != 0;
     for (KernelTimer *t : {&c->t_spmv, &c->t_upd}) {
         if (c->timing && t->ev.empty()) {
-            t->ev.resize(2 * 10240);
+            t->ev.resize(2 * 1024);   // sampled pairs: iterations up to 16 x 1024 per solve
             for (auto &e : t->ev) HIPC(hipEventCreate(&e));
         }
         t->launches = 0;

@@ -129,6 +129,9 @@ struct Launch {
     Reducer red;
     DevScalars *dsc;
     const MatCoef *mats;
+    // when set, the fused PCG launchers stamp this event pair from the dispatch itself
+    // (hipExtLaunchKernelGGL), timing the kernel alone
+    hipEvent_t t0 = nullptr, t1 = nullptr;
 };
 
 // ---------------- kernel launchers (kernels.hip) ----------------
@@ -148,6 +151,8 @@ void launch_grid_poisson32(const Launch &L, const PoissonGrid<float> &P, const f
 template <typename T>
 int launch_pcg0(const Launch &L, int64_t n, const T *b, const T *Ax0, T *r, T *z, const T *invD,
                 double *parts_out);
+// The structured PCG tiles fit: blocks within the partial slots, byte offsets in 32 bits.
+bool ka_grid_fits(int nx, int ny, int nz);
 template <typename T>
 int launch_ka_grid(const Launch &L, const PoissonGrid<T> &P, const PcgIterArgs &a, const T *z,
                    const T *pold, T *pnew, T *q);

@@ -15,6 +15,7 @@
 // (after a reliable update of the mixed path: the search direction is kept, i.e.
 // p = z + beta p with gamma, r.r and z from the replaced residual).
 // T = double (FP64 path) or float (mixed path: FP32 vectors, FP64 sums).
+#include <hip/hip_ext.h>
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -25,7 +26,22 @@ namespace mm {
 
 namespace {
 
-constexpr int kFB = 1024;  // threads per block of the fused kernels (16 wavefronts)
+constexpr int kFB = 1024;  // threads per block of the streaming fused kernels (16 wavefronts)
+// Tile rows, target block count and waves per SIMD of the structured KA (compile-time
+// tunables; the defaults are the measured best on MI355X).
+#ifndef MM_KA_TY
+#define MM_KA_TY 8
+#endif
+#ifndef MM_KA_BLOCKS
+#define MM_KA_BLOCKS 512
+#endif
+#ifndef MM_KA_WAVES
+#define MM_KA_WAVES 0
+#endif
+#ifndef MM_KB_BLOCKS
+#define MM_KB_BLOCKS 256
+#endif
+constexpr int kKaTY = MM_KA_TY;  // rows of a KA tile on the structured grid (64 x kKaTY nodes)
 
 __device__ __forceinline__ double fwave_sum(double v) {
 #pragma unroll
@@ -34,9 +50,10 @@ __device__ __forceinline__ double fwave_sum(double v) {
 }
 
 // Sum of n partials (fixed order: per-thread strided sums, then a fixed tree).
+template <int NT>
 __device__ double block_sum_parts(const double *__restrict__ parts, int n, double *sh) {
     double s = 0.0;
-    for (int j = threadIdx.x; j < n; j += kFB) s += parts[j];
+    for (int j = threadIdx.x; j < n; j += NT) s += parts[j];
     s = fwave_sum(s);
     const int lane = threadIdx.x & (kWave - 1), w = threadIdx.x / kWave;
     __syncthreads();
@@ -44,15 +61,14 @@ __device__ double block_sum_parts(const double *__restrict__ parts, int n, doubl
     __syncthreads();
     double t = 0.0;
 #pragma unroll
-    for (int k = 0; k < kFB / kWave; ++k) t += sh[k];
+    for (int k = 0; k < NT / kWave; ++k) t += sh[k];
     return t;
 }
 
-// Block reduction of NV values into out[i * kMaxPartials + block].
-template <int NV>
+// Block reduction of NV values into out[i * kMaxPartials + blockIdx.x].
+template <int NV, int NT>
 __device__ void fblock_partials(double (&v)[NV], double *sh, double *out) {
     const int lane = threadIdx.x & (kWave - 1), w = threadIdx.x / kWave;
-    const int bid = blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z);
 #pragma unroll
     for (int i = 0; i < NV; ++i) {
         double s = fwave_sum(v[i]);
@@ -61,45 +77,70 @@ __device__ void fblock_partials(double (&v)[NV], double *sh, double *out) {
         __syncthreads();
         if (threadIdx.x == 0) {
             double t = 0.0;
-            for (int k = 0; k < kFB / kWave; ++k) t += sh[k];
-            out[i * kMaxPartials + bid] = t;
+            for (int k = 0; k < NT / kWave; ++k) t += sh[k];
+            out[i * kMaxPartials + blockIdx.x] = t;
         }
     }
 }
 
-// Block-uniform read of the stop flag: block 0 of the same KA kernel may write it while
-// other blocks start, so one thread reads it and broadcasts it through LDS.
-__device__ bool block_done(const DevScalars *dsc, int *flag) {
-    if (threadIdx.x == 0) *flag = dsc->pcg_done;
-    __syncthreads();
-    return *flag != 0;
-}
-
 // Scalars of iteration i from the partials of the previous KB (or the init / replaced
-// residual).  Returns beta; sets *stop.  Block 0 thread 0 publishes.
-__device__ double ka_prologue(const PcgIterArgs &a, DevScalars *dsc, double *sh, bool *stop) {
-    double beta = 0.0, gamma, rr;
-    if (a.mode == PCG_KEEP || a.mode == PCG_RESTART) {
-        gamma = dsc->ru_gamma;
-        rr = dsc->ir_rr;
-        if (a.mode == PCG_KEEP) beta = gamma / dsc->pcg_gam[(a.iter + 1) & 1];
-    } else {
-        gamma = block_sum_parts(a.parts_in, a.nparts_in, sh);
-        rr = block_sum_parts(a.parts_in + kMaxPartials, a.nparts_in, sh);
-        if (a.mode == PCG_NORMAL) beta = gamma / dsc->pcg_gam[(a.iter + 1) & 1];
+// residual).  Returns beta; sets *stop (block-uniform).  Every load is issued before the
+// one barrier.  The stop flag may be written by block 0 of this same kernel while other
+// blocks start (to the value they would compute themselves), so thread 0 reads it and
+// broadcasts it through LDS.  Block 0 thread 0 publishes the scalars.  sh holds
+// 3 * NT / kWave doubles.
+template <int NT>
+__device__ double ka_prologue(const PcgIterArgs &a, DevScalars *dsc, double *sh, int *flag, bool *stop) {
+    constexpr int NW = NT / kWave;
+    if (threadIdx.x == 0) *flag = dsc->pcg_done;
+    const bool from_parts = !(a.mode == PCG_KEEP || a.mode == PCG_RESTART);
+    const bool first = a.mode == PCG_FIRST;
+    const double gprev = dsc->pcg_gam[(a.iter + 1) & 1];
+    const double bound_prev = dsc->pcg_bound;
+    const double ru_gamma = dsc->ru_gamma, ir_rr = dsc->ir_rr;
+    double s0 = 0.0, s1 = 0.0, s2 = 0.0;
+    if (from_parts) {
+        for (int jj = threadIdx.x; jj < a.nparts_in; jj += NT) {
+            s0 += a.parts_in[jj];
+            s1 += a.parts_in[kMaxPartials + jj];
+            if (first) s2 += a.parts_in[2 * kMaxPartials + jj];
+        }
     }
-    double bound = dsc->pcg_bound;
-    if (a.mode == PCG_FIRST) {
-        const double bb = block_sum_parts(a.parts_in + 2 * kMaxPartials, a.nparts_in, sh);
-        bound = fmax(a.tol * a.tol * bb, a.atol * a.atol);
+    s0 = fwave_sum(s0);
+    s1 = fwave_sum(s1);
+    s2 = fwave_sum(s2);
+    const int lane = threadIdx.x & (kWave - 1), w = threadIdx.x / kWave;
+    if (lane == 0) {
+        sh[w] = s0;
+        sh[NW + w] = s1;
+        sh[2 * NW + w] = s2;
     }
+    __syncthreads();
+    if (*flag != 0) {
+        *stop = true;
+        return 0.0;
+    }
+    double gamma = 0.0, rr = 0.0, bb = 0.0;
+#pragma unroll
+    for (int k = 0; k < NW; ++k) {
+        gamma += sh[k];
+        rr += sh[NW + k];
+        bb += sh[2 * NW + k];
+    }
+    double beta = 0.0;
+    if (!from_parts) {
+        gamma = ru_gamma;
+        rr = ir_rr;
+    }
+    if (a.mode == PCG_NORMAL || a.mode == PCG_KEEP) beta = gamma / gprev;
+    const double bound = first ? fmax(a.tol * a.tol * bb, a.atol * a.atol) : bound_prev;
     const bool done = !((rr > bound) && (a.iter < a.maxiter));
-    if (threadIdx.x == 0 && blockIdx.x == 0 && blockIdx.y == 0 && blockIdx.z == 0) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) {
         dsc->pcg_gam[a.iter & 1] = gamma;
         dsc->pcg_beta = beta;
         dsc->pcg_rr = rr;
         dsc->pcg_iter = a.iter;
-        if (a.mode == PCG_FIRST) {
+        if (first) {
             dsc->pcg_bound = bound;
             dsc->pcg_bs = fmax(rr, 0.0);
         }
@@ -109,73 +150,188 @@ __device__ double ka_prologue(const PcgIterArgs &a, DevScalars *dsc, double *sh,
     return beta;
 }
 
-template <typename T>
-__device__ __forceinline__ T edge_sum_f(const T *__restrict__ h1p, const T *__restrict__ h2p, int a, int b) {
-    const T l1 = h1p[a], u1 = h1p[a + 1], l2 = h2p[b], u2 = h2p[b + 1];
-    return (T)2 * l1 * l2 + l1 * u2 + u1 * l2 + (T)2 * u1 * u2;
+// Logical tile of block b of nb: blocks are dispatched round-robin over the 8 XCDs
+// (block b runs on XCD b % 8), so consecutive logical tiles are given to one XCD and
+// the z-neighbouring chunks of one (x, y) column share that XCD's L2.  A bijection on
+// [0, nb).
+__device__ __forceinline__ int xcd_tile(int b, int nb) {
+    const int q = nb / 8, r = nb % 8, x = b % 8, l = b / 8;
+    return x < r ? x * (q + 1) + l : r * (q + 1) + (x - r) * q + l;
 }
 
-// KA on the structured grid: 64 x 16 tiles of an (x, y) plane marching through a range
-// of z planes (see k_grid_poisson in kernels.hip for the stencil coefficients).
+// Uniform load through the scalar path (constant address space: the spacing arrays are
+// read-only for the whole run).
 template <typename T>
-__global__ __launch_bounds__(kFB) void k_ka_grid(PoissonGrid<T> P, PcgIterArgs a,
-                                                 const T *__restrict__ z, const T *__restrict__ pold,
-                                                 T *__restrict__ pnew, T *__restrict__ q,
-                                                 DevScalars *dsc) {
-    __shared__ double sh[kFB / kWave];
+__device__ __forceinline__ T uload(const T *p, int k) {
+    return ((const __attribute__((address_space(4))) T *)p)[k];
+}
+
+// Element access at a 32-bit byte offset (vectors of the structured path hold fewer
+// than 2^29 entries, checked at set-up), so that addresses are a scalar base plus one
+// 32-bit register.
+template <typename T>
+__device__ __forceinline__ T ld(const T *base, uint32_t off) {
+    return *(const T *)((const char *)base + off);
+}
+template <typename T>
+__device__ __forceinline__ void st(T *base, uint32_t off, T v) {
+    *(T *)((char *)base + off) = v;
+}
+
+// Raw z / p_old pair of one node and plane.
+template <typename T>
+struct Raw {
+    T z, p;
+};
+
+// KA on the structured grid: a block owns a 64 x TY tile of (x, y) and a chunk of at
+// most kKaChunk z planes, marching upwards.  Per plane, p = z + beta p_old is formed
+// once per node of the tile and of its one-node halo ring (south / north rows, west /
+// east columns) into a double-buffered LDS plane; the in-plane neighbours come from
+// LDS, the z neighbours from registers.  Raw z / p_old loads run two planes ahead of
+// their use (own node: plane k + 3 while plane k is computed; halo: plane k + 2), in
+// two register slots that alternate between even and odd steps; the chunk's z
+// spacings are staged in LDS.
+//
+// Coefficients (as k_grid_poisson in kernels.hip): with A = 2 hz_l + hz_u and
+// B = hz_l + 2 hz_u of plane k, the x-edge factor is sx = hy_l A + hy_u B, the y-edge
+// factor syy = hx_l A + hx_u B, the z-edge factor sz = edge sum of (hx, hy); the
+// off-diagonal entries are -sx / (6 h), ..., the diagonal their negated sum, and the
+// entries towards Dirichlet nodes (first / last index on an axis) are dropped.
+// Grid: 1D, gx * gy * gz blocks, logical tile = (z chunk fastest, then x tile, then y).
+constexpr int kKaChunk = 256;
+template <typename T, int TY>
+__global__ __launch_bounds__(64 * TY)
+#if MM_KA_WAVES > 0
+__attribute__((amdgpu_waves_per_eu(MM_KA_WAVES)))
+#endif
+void k_ka_grid(PoissonGrid<T> P, PcgIterArgs a, int gx, int gz, const T *__restrict__ z,
+               const T *__restrict__ pold, T *__restrict__ pnew, T *__restrict__ q, DevScalars *dsc) {
+    static_assert(TY >= 3 && TY <= 16, "halo lanes need 3 <= TY; blocks of at most 1024 threads");
+    constexpr int NT = 64 * TY;
+    __shared__ double sh[3 * NT / kWave];
+    __shared__ T sp[2][TY + 2][66];
+    __shared__ T szc[2][kKaChunk + 1];   // hz, 1 / (6 hz) of the chunk's planes k0 .. k1
     __shared__ int flag;
-    if (block_done(dsc, &flag)) return;
+    const int nx = P.nx, ny = P.ny, nz = P.nz;
+    const int t = xcd_tile(blockIdx.x, gridDim.x);
+    const int cz = t % gz, bx = (t / gz) % gx, by = t / (gz * gx);
+    const int tx = threadIdx.x & 63;
+    const int ty = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);   // one row per wave
+    const int i = bx * 64 + tx, j = by * TY + ty;
+    const int chunk = (nz + gz - 1) / gz;
+    const int k0 = cz * chunk, k1 = min(nz, k0 + chunk);   // block-uniform; empty when k0 >= nz
+    const uint32_t es = sizeof(T);
+    const uint32_t sz = (uint32_t)nx * (uint32_t)ny * es;  // plane stride in bytes
+    const bool in = i < nx && j < ny;
+    const bool run = k0 < k1;
+    const bool pz = a.mode == PCG_FIRST || a.mode == PCG_RESTART;  // p = z, else z + beta p_old
+    // halo node of this thread (at most one): south row (ty == 0), north row
+    // (ty == TY - 1), west / east columns (ty == 1, lanes [0, TY) and [32, 32 + TY))
+    int hi = 0, hj = 0, hly = -1, hlx = 0;
+    if (ty == 0) {
+        hi = i; hj = by * TY - 1; hly = 0; hlx = tx + 1;
+    } else if (ty == TY - 1) {
+        hi = i; hj = by * TY + TY; hly = TY + 1; hlx = tx + 1;
+    } else if (ty == 1 && tx < TY) {
+        hi = bx * 64 - 1; hj = by * TY + tx; hly = tx + 1; hlx = 0;
+    } else if (ty == 1 && tx >= 32 && tx < 32 + TY) {
+        hi = bx * 64 + 64; hj = by * TY + tx - 32; hly = tx - 31; hlx = 65;
+    }
+    const bool hin = hly >= 0 && hi >= 0 && hi < nx && hj >= 0 && hj < ny;
+    const uint32_t ob = in ? (uint32_t)(i + nx * j) * es : 0u;
+    const uint32_t hb = hin ? (uint32_t)(hi + nx * hj) * es : 0u;
+
+    auto raw = [&](uint32_t off) -> Raw<T> {
+        Raw<T> r;
+        r.z = ld(z, off);
+        r.p = pz ? (T)0 : ld(pold, off);
+        return r;
+    };
+    // Own plane m is used (as the upper neighbour of plane m - 1) iff m <= k1 and m < nz;
+    // halo plane m iff m < k1.  Loads issued before the scalars are known.
+    Raw<T> om1{0, 0}, o0{0, 0}, oA{0, 0}, oB{0, 0}, hA{0, 0}, hB{0, 0};
+    if (run && in) {
+        if (k0 > 0) om1 = raw(ob + (uint32_t)(k0 - 1) * sz);
+        o0 = raw(ob + (uint32_t)k0 * sz);
+        if (k0 + 1 < nz) oA = raw(ob + (uint32_t)(k0 + 1) * sz);
+        if (k0 + 2 <= k1 && k0 + 2 < nz) oB = raw(ob + (uint32_t)(k0 + 2) * sz);
+    }
+    if (run && hin) {
+        hA = raw(hb + (uint32_t)k0 * sz);
+        if (k0 + 1 < k1) hB = raw(hb + (uint32_t)(k0 + 1) * sz);
+    }
+    if (run) {
+        for (int e = threadIdx.x; e <= k1 - k0; e += NT) {   // k1 - k0 <= kKaChunk
+            szc[0][e] = P.hzp[k0 + e];
+            szc[1][e] = P.izp[k0 + e];
+        }
+    }
+
     bool stop;
-    const double bd = ka_prologue(a, dsc, sh, &stop);
+    const double bd = ka_prologue<NT>(a, dsc, sh, &flag, &stop);   // (its barrier publishes szc)
     if (stop) return;
     const T beta = (T)bd;
-    // p = z (FIRST, RESTART), z + beta p (NORMAL, KEEP)
-    const int pm = (a.mode == PCG_FIRST || a.mode == PCG_RESTART) ? 0 : 2;
-    const int nx = P.nx, ny = P.ny, nz = P.nz;
-    const int i = blockIdx.x * 64 + (threadIdx.x & 63);
-    const int j = blockIdx.y * 16 + (threadIdx.x >> 6);
-    const int chunk = (nz + gridDim.z - 1) / gridDim.z;
-    const int k0 = blockIdx.z * chunk, k1 = min(nz, k0 + chunk);
-    const int64_t sy = nx, sz = (int64_t)nx * ny;
-    auto pv = [&](int64_t idx) -> T { return pm == 0 ? z[idx] : pm == 1 ? pold[idx] : z[idx] + beta * pold[idx]; };
+    auto comb = [&](const Raw<T> &r) -> T { return pz ? r.z : r.z + beta * r.p; };
     double dot = 0.0;
-    // empty trailing chunks (k0 >= nz) load nothing
-    if (i < nx && j < ny && k0 < k1) {
-        const bool edge_ij = i == 0 || j == 0 || i == nx - 1 || j == ny - 1;
-        const T sz_ij = edge_sum_f(P.hxp, P.hyp, i, j);
-        const T mxm = (i - 1 > 0) ? (T)1 : (T)0, mxp = (i + 1 < nx - 1) ? (T)1 : (T)0;
-        const T mym = (j - 1 > 0) ? (T)1 : (T)0, myp = (j + 1 < ny - 1) ? (T)1 : (T)0;
-        const T ixm = P.ixp[i], ixp = P.ixp[i + 1], iym = P.iyp[j], iyp = P.iyp[j + 1];
-        const int64_t base = i + sy * j;
-        T xm = k0 > 0 ? pv(base + sz * (k0 - 1)) : (T)0;
-        T xc = pv(base + sz * k0);
-        for (int k = k0; k < k1; ++k) {
-            const int64_t idx = base + sz * k;
-            const T xp = k + 1 < nz ? pv(idx + sz) : (T)0;
-            T out;
-            if (edge_ij || k == 0 || k == nz - 1) {
-                out = xc;
-            } else {
-                const T xw = pv(idx - 1), xe = pv(idx + 1), xs = pv(idx - sy), xn = pv(idx + sy);
-                const T sx = edge_sum_f(P.hyp, P.hzp, j, k);
-                const T syy = edge_sum_f(P.hzp, P.hxp, k, i);
-                const T cxm = -sx * ixm, cxp = -sx * ixp;
-                const T cym = -syy * iym, cyp = -syy * iyp;
-                const T czm = -sz_ij * P.izp[k], czp = -sz_ij * P.izp[k + 1];
-                const T diag = -(cxm + cxp + cym + cyp + czm + czp);
-                const T mzm = (k - 1 > 0) ? (T)1 : (T)0, mzp = (k + 1 < nz - 1) ? (T)1 : (T)0;
-                out = diag * xc + mxm * cxm * xw + mxp * cxp * xe + mym * cym * xs + myp * cyp * xn +
-                      mzm * czm * xm + mzp * czp * xp;
+    if (run) {
+        const bool edge_ij = !in || i == 0 || j == 0 || i == nx - 1 || j == ny - 1;
+        // y factors are wave-uniform (scalar path), x factors per lane
+        const bool jin = j > 0 && j < ny - 1;
+        const T hyl = jin ? uload(P.hyp, j) : (T)0, hyu = jin ? uload(P.hyp, j + 1) : (T)0;
+        const T b0 = jin ? uload(P.iyp, j) : (T)0, b1 = jin ? uload(P.iyp, j + 1) : (T)0;
+        const T iys = b0 + b1, iym = j - 1 > 0 ? b0 : (T)0, iyp = j + 1 < ny - 1 ? b1 : (T)0;
+        T hxl = 0, hxu = 0, ixs = 0, ixm = 0, ixp = 0;
+        if (!edge_ij) {
+            hxl = P.hxp[i]; hxu = P.hxp[i + 1];
+            const T a0 = P.ixp[i], a1 = P.ixp[i + 1];
+            ixs = a0 + a1;
+            ixm = i - 1 > 0 ? a0 : (T)0; ixp = i + 1 < nx - 1 ? a1 : (T)0;
+        }
+        const T szij = (T)2 * hxl * hyl + hxl * hyu + hxu * hyl + (T)2 * hxu * hyu;
+        T pm = comb(om1);   // plane k - 1 (0 below the grid or outside)
+        T pc = comb(o0);    // plane k
+
+        // One plane: oN holds plane k + 1 (refilled with plane k + 3), hN the halo of
+        // plane k (refilled with plane k + 2).
+        auto step = [&](int k, Raw<T> &oN, Raw<T> &hN) {
+            const T pp = comb(oN);
+            const T hv = comb(hN);
+            if (in) oN = (k + 3 <= k1 && k + 3 < nz) ? raw(ob + (uint32_t)(k + 3) * sz) : Raw<T>{0, 0};
+            if (hin && k + 2 < k1) hN = raw(hb + (uint32_t)(k + 2) * sz);
+            T(*buf)[66] = sp[k & 1];
+            buf[ty + 1][tx + 1] = pc;
+            if (hly >= 0) buf[hly][hlx] = hin ? hv : (T)0;
+            __syncthreads();
+            if (in) {
+                T out = pc;
+                if (!(edge_ij || k == 0 || k == nz - 1)) {
+                    const T xw = buf[ty + 1][tx], xe = buf[ty + 1][tx + 2];
+                    const T xs = buf[ty][tx + 1], xn = buf[ty + 2][tx + 1];
+                    const T hzl = szc[0][k - k0], hzu = szc[0][k - k0 + 1];
+                    const T izl = szc[1][k - k0], izu = szc[1][k - k0 + 1];
+                    const T A = (T)2 * hzl + hzu, B = hzl + (T)2 * hzu;
+                    const T izm = k - 1 > 0 ? izl : (T)0, izq = k + 1 < nz - 1 ? izu : (T)0;
+                    const T sx = hyl * A + hyu * B, syy = hxl * A + hxu * B;
+                    const T diag = sx * ixs + syy * iys + szij * (izl + izu);
+                    out = diag * pc - sx * (ixm * xw + ixp * xe) - syy * (iym * xs + iyp * xn) -
+                          szij * (izm * pm + izq * pp);
+                }
+                const uint32_t off = ob + (uint32_t)k * sz;
+                st(pnew, off, pc);
+                st(q, off, out);
+                dot += (double)out * (double)pc;
             }
-            pnew[idx] = xc;
-            q[idx] = out;
-            dot += (double)out * (double)xc;
-            xm = xc;
-            xc = xp;
+            pm = pc;
+            pc = pp;
+        };
+        for (int k = k0; k < k1; k += 2) {
+            step(k, oA, hA);
+            if (k + 1 < k1) step(k + 1, oB, hB);
         }
     }
     double v[1] = {dot};
-    fblock_partials<1>(v, sh, a.parts_out);
+    fblock_partials<1, NT>(v, sh, a.parts_out);
 }
 
 // KA on a SELL-64 matrix (general meshes): 16 slices per block.
@@ -183,15 +339,14 @@ template <typename T>
 __global__ __launch_bounds__(kFB) void k_ka_sell(SellDev<T> A, PcgIterArgs a, const T *__restrict__ z,
                                                  const T *__restrict__ pold, T *__restrict__ pnew,
                                                  T *__restrict__ q, DevScalars *dsc) {
-    __shared__ double sh[kFB / kWave];
+    __shared__ double sh[3 * kFB / kWave];
     __shared__ int flag;
-    if (block_done(dsc, &flag)) return;
     bool stop;
-    const double bd = ka_prologue(a, dsc, sh, &stop);
+    const double bd = ka_prologue<kFB>(a, dsc, sh, &flag, &stop);
     if (stop) return;
     const T beta = (T)bd;
-    const int pm = (a.mode == PCG_FIRST || a.mode == PCG_RESTART) ? 0 : 2;
-    auto pv = [&](int c) -> T { return pm == 0 ? z[c] : pm == 1 ? pold[c] : z[c] + beta * pold[c]; };
+    const bool pz = a.mode == PCG_FIRST || a.mode == PCG_RESTART;
+    auto pv = [&](int c) -> T { return pz ? z[c] : z[c] + beta * pold[c]; };
     const int lane = threadIdx.x & (kWave - 1);
     double dot = 0.0;
     for (int slice = blockIdx.x * (kFB / kWave) + threadIdx.x / kWave; slice < A.n_slices;
@@ -211,7 +366,7 @@ __global__ __launch_bounds__(kFB) void k_ka_sell(SellDev<T> A, PcgIterArgs a, co
         }
     }
     double v[1] = {dot};
-    fblock_partials<1>(v, sh, a.parts_out);
+    fblock_partials<1, kFB>(v, sh, a.parts_out);
 }
 
 // KB: alpha = gamma_i / (p.q); x += alpha p; r -= alpha q; z = r * invD; (r.z, r.r).
@@ -222,7 +377,7 @@ __global__ __launch_bounds__(kFB) void k_kb(int64_t n, PcgIterArgs a, T *__restr
                                             const T *__restrict__ invD, DevScalars *dsc) {
     __shared__ double sh[kFB / kWave];
     if (dsc->pcg_done) return;
-    const double pq = block_sum_parts(a.parts_in, a.nparts_in, sh);
+    const double pq = block_sum_parts<kFB>(a.parts_in, a.nparts_in, sh);
     const T alpha = (T)(dsc->pcg_gam[a.iter & 1] / pq);
     double s[2] = {0.0, 0.0};
     for (int64_t i = blockIdx.x * (int64_t)kFB + threadIdx.x; i < n; i += (int64_t)gridDim.x * kFB) {
@@ -234,7 +389,7 @@ __global__ __launch_bounds__(kFB) void k_kb(int64_t n, PcgIterArgs a, T *__restr
         s[0] += (double)ri * (double)zi;
         s[1] += (double)ri * (double)ri;
     }
-    fblock_partials<2>(s, sh, a.parts_out);
+    fblock_partials<2, kFB>(s, sh, a.parts_out);
 }
 
 // Init: r = b - A x0, z = r * invD; partials (r.z, r.r, b.b) for KA(0) in FIRST mode.
@@ -256,23 +411,48 @@ __global__ __launch_bounds__(kFB) void k_pcg0(int64_t n, const T *__restrict__ b
         s[1] += (double)ri * (double)ri;
         s[2] += (double)bi * (double)bi;
     }
-    fblock_partials<3>(s, sh, parts_out);
+    fblock_partials<3, kFB>(s, sh, parts_out);
 }
 
+// Blocks of the structured KA: 64 x kKaTY tiles times gz z-chunks, gz chosen so that
+// the launch has at most kKaBlocks blocks, at least one chunk per tile column and at
+// most kMaxPartials blocks.
+constexpr int kKaBlocks = MM_KA_BLOCKS;
+struct KaDims {
+    int gx, gy, gz;
+};
 template <typename T>
-dim3 ka_grid_dims(const PoissonGrid<T> &P) {
-    const int gx = (P.nx + 63) / 64, gy = (P.ny + 15) / 16;
-    const int per_plane = gx * gy;
-    int gz = std::max(1, std::min(P.nz, (kFusedBlocks + per_plane - 1) / per_plane));
-    while (per_plane * gz > kMaxPartials && gz > 1) --gz;
-    return dim3(gx, gy, gz);
+KaDims ka_grid_dims(const PoissonGrid<T> &P) {
+    KaDims d;
+    d.gx = (P.nx + 63) / 64;
+    d.gy = (P.ny + kKaTY - 1) / kKaTY;
+    const int per_plane = d.gx * d.gy;
+    d.gz = std::max(1, std::min(P.nz, kKaBlocks / per_plane));
+    d.gz = std::max(d.gz, (P.nz + kKaChunk - 1) / kKaChunk);   // chunks of at most kKaChunk planes
+    return d;
 }
 
 int streaming_blocks(int64_t n) {
-    return (int)std::max<int64_t>(1, std::min<int64_t>(kFusedBlocks, (n + kFB - 1) / kFB));
+    return (int)std::max<int64_t>(1, std::min<int64_t>(MM_KB_BLOCKS, (n + kFB - 1) / kFB));
+}
+
+// Launch, timed by the dispatch's own event pair when the caller asks for it.
+template <typename K, typename... Args>
+void launch_timed(const Launch &L, K kernel, dim3 grid, dim3 block, Args... args) {
+    if (L.t0)
+        hipExtLaunchKernelGGL(kernel, grid, block, 0, L.s, L.t0, L.t1, 0, args...);
+    else
+        hipLaunchKernelGGL(kernel, grid, block, 0, L.s, args...);
 }
 
 }  // namespace
+
+bool ka_grid_fits(int nx, int ny, int nz) {
+    const int64_t per_plane = (int64_t)((nx + 63) / 64) * ((ny + kKaTY - 1) / kKaTY);
+    const int64_t gz = std::max<int64_t>(1, (nz + kKaChunk - 1) / kKaChunk);
+    const int64_t n = (int64_t)nx * ny * nz;
+    return per_plane * gz <= kMaxPartials && n * (int64_t)sizeof(double) < ((int64_t)1 << 32);
+}
 
 template <typename T>
 int launch_pcg0(const Launch &L, int64_t n, const T *b, const T *Ax0, T *r, T *z, const T *invD,
@@ -285,16 +465,18 @@ int launch_pcg0(const Launch &L, int64_t n, const T *b, const T *Ax0, T *r, T *z
 template <typename T>
 int launch_ka_grid(const Launch &L, const PoissonGrid<T> &P, const PcgIterArgs &a, const T *z,
                    const T *pold, T *pnew, T *q) {
-    const dim3 g = ka_grid_dims(P);
-    hipLaunchKernelGGL(k_ka_grid<T>, g, dim3(kFB), 0, L.s, P, a, z, pold, pnew, q, L.dsc);
-    return (int)(g.x * g.y * g.z);
+    const KaDims d = ka_grid_dims(P);
+    const int nb = d.gx * d.gy * d.gz;
+    launch_timed(L, k_ka_grid<T, kKaTY>, dim3(nb), dim3(64 * kKaTY), P, a, d.gx, d.gz, z, pold, pnew, q,
+                 L.dsc);
+    return nb;
 }
 
 template <typename T>
 int launch_ka_sell(const Launch &L, const SellDev<T> &A, const PcgIterArgs &a, const T *z,
                    const T *pold, T *pnew, T *q) {
     const int nb = std::max(1, std::min((A.n_slices + kFB / kWave - 1) / (kFB / kWave), kFusedBlocks));
-    hipLaunchKernelGGL(k_ka_sell<T>, dim3(nb), dim3(kFB), 0, L.s, A, a, z, pold, pnew, q, L.dsc);
+    launch_timed(L, k_ka_sell<T>, dim3(nb), dim3(kFB), A, a, z, pold, pnew, q, L.dsc);
     return nb;
 }
 
@@ -302,7 +484,7 @@ template <typename T>
 int launch_kb(const Launch &L, int64_t n, const PcgIterArgs &a, T *x, T *r, T *z, const T *p,
               const T *q, const T *invD) {
     const int nb = streaming_blocks(n);
-    hipLaunchKernelGGL(k_kb<T>, dim3(nb), dim3(kFB), 0, L.s, n, a, x, r, z, p, q, invD, L.dsc);
+    launch_timed(L, k_kb<T>, dim3(nb), dim3(kFB), n, a, x, r, z, p, q, invD, L.dsc);
     return nb;
 }
 
