@@ -25,10 +25,14 @@ for _p in (ROOT, ROOT / "mammos-mumag_amd"):
 
 METRIC = "hysteresis field steps/s @2M nodes: learned+FP32 vs plain FP64 on one MI355X"
 HBM_PEAK_GBS = 8000.0
-# HBM bytes per launch of the PCG kernels on the benchmark grid, from rocprofv3 --pmc
-# FETCH_SIZE and WRITE_SIZE (KiB per dispatch) scaled by the same counters on a 512 MiB
-# device copy (bench.py --traffic-probe); filled in from the committed profile summaries.
-TRAFFIC = {}
+# HBM bytes per launch of the two PCG kernels on the voronoi64_250nm grid (structured
+# path), from rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE (median KiB per dispatch) scaled
+# by the same counters on the 512 MiB device copy of --traffic-probe (factors 0.5000 and
+# 1.0000); see profiles/pcg_kernels_{fp64,mixed}_voronoi125.md.
+TRAFFIC = {
+    ("fp64", "spmv"): 134047365, ("fp64", "update"): 233836081,
+    ("mixed", "spmv"): 67436398, ("mixed", "update"): 116955348,
+}
 
 
 def key_of(k):
@@ -149,7 +153,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--config", default="cfg2")
+    ap.add_argument("--config", default="voronoi64_250nm")
     ap.add_argument("--precision", default="fp64", choices=["fp64", "mixed"])
     ap.add_argument("--seed", type=int, default=1000)
     ap.add_argument("--general", action="store_true", help="unstructured kernels (SELL K, element loops)")
@@ -189,7 +193,9 @@ def main():
         k = max(kernels, key=lambda d: d["total_ms"])  # the dominant kernel of the timed steps
         achieved = k["bytes"] / k["avg_s"] / 1e9
         roof = dict(bound="hbm", achieved=round(achieved, 1), peak=HBM_PEAK_GBS, unit="GB/s",
-                    frac=round(achieved / HBM_PEAK_GBS, 4), traffic=TRAFFIC.get((args.precision, key_of(k))),
+                    frac=round(achieved / HBM_PEAK_GBS, 4),
+                    traffic=(TRAFFIC.get((args.precision, key_of(k)))
+                             if args.config == "voronoi64_250nm" and res["info"]["structured"] else None),
                     kernel=k["kernel"], bytes_per_launch=k["bytes"], avg_launch_us=round(k["avg_s"] * 1e6, 2),
                     sampled_launches=k["launches"],
                     others=[dict(kernel=o["kernel"], avg_launch_us=round(o["avg_s"] * 1e6, 2),
@@ -214,15 +220,18 @@ def main():
             log(f"[bench] cpu baseline failed: {e}")
         finally:
             os.unlink(path)
+    from mammos_mumag_amd.generate import CONFIGS
+    cfgd = CONFIGS[args.config]
     out = {
         "metric": METRIC, "value": round(value, 6), "unit": "field steps/s", "n_gpus": 1,
         "steps": steps, "warmup": args.warmup, "ms_per_step": round(1e3 * res["elapsed"] / steps, 3),
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "f64" if args.precision == "fp64" else "f32+f64",
         "data": "synthetic",
-        "config": {"workload": f"{args.config}: 64-grain Voronoi cube 250 nm, 125^3 voxels x 6 Kuhn tets, "
+        "config": {"workload": f"{args.config}: {cfgd['grains']}-grain Voronoi cube {cfgd['edge']:g} nm, "
+                               f"{cfgd['n']}^3 voxels x 6 Kuhn tets in a graded air box, "
                                f"{res['n_mag_nodes']} magnet nodes / {res['n_nodes']} nodes / "
-                               f"{res['n_elems']} tets, field steps from +8 T by -0.05 T",
+                               f"{res['n_elems']} tets, field steps from {cfgd['hstart']:+g} T by {cfgd['hstep']:g} T",
                    "precision": args.precision, "form": "plain (no learned warm start)",
                    "per_step": {k: round(v / steps, 2) for k, v in st.items()}},
         "roofline": roof,

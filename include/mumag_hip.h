@@ -5,7 +5,7 @@
  * Conventions
  *  - Every entry point returns int: 0 = ok, < 0 = error (see MM_E_*), > 0 = warning
  *    (MM_W_MAXITER: an iteration cap was reached; the result is still returned, as the
- *    reference does, since it discards jax CG's info: scripts/solvers.py:326-336).
+ *    reference does, since it discards jax CG's info: scripts/solvers.py:222-231).
  *  - Vectors are caller-owned DEVICE pointers (e.g. torch.Tensor.data_ptr() on cuda:0)
  *    unless the name ends in _host.  Nodal 3-vectors are interleaved f64[3N]
  *    ([3i+a] = component a of node i), scalars f64[N], in mesh node order.
@@ -90,7 +90,7 @@ typedef struct {
     int64_t device_bytes;            /* device memory held by the context */
 } mm_info;
 
-/* Minimiser parameters: scripts/tools.py:136-192 (read_params) and loop.py:58-65. */
+/* Minimiser parameters: scripts/tools.py:64-120 (read_params) and loop.py:58-65. */
 typedef struct {
     double tol_fun;           /* NCG tolerance (minimizers.py:164-166) */
     double tol_u;             /* PCG tolerance of the potential solve */
@@ -100,7 +100,7 @@ typedef struct {
 } mm_min_params;
 
 typedef struct {
-    int64_t ncg_iters;        /* NCG iterations (the reference's cg_iter, loop.py:394) */
+    int64_t ncg_iters;        /* NCG iterations (the reference's cg_iter, loop.py:64, 119) */
     int64_t fcalls;           /* total_eg evaluations (the reference's stats) */
     int64_t pcg_iters;        /* Poisson PCG iterations */
     int64_t pcg_calls;
@@ -134,13 +134,13 @@ int mm_total_eg(mm_ctx *ctx, const double *m, double hext, const double hdir[3],
                 double tol_u, int32_t hmag_on, double *u_inout, double *E_host,
                 double *g_proj, double *g_full, int32_t *pcg_iters_host);
 
-/* Jacobi PCG on K with jax.scipy.sparse.linalg.cg semantics (solvers.py:326-336):
+/* Jacobi PCG on K with jax.scipy.sparse.linalg.cg semantics (solvers.py:222-231):
  * stop when r.r <= max(tol^2 b.b, atol^2) or after maxiter iterations. */
 int mm_pcg(mm_ctx *ctx, const double *b, double *x_inout, double tol, double atol,
            int32_t maxiter, int32_t *iters_host);
 
-/* M_inv: truncated diagonally preconditioned CG (solvers.py:112-236) on
- * H v = m_ref x ((C v) x m_ref) - (m_ref . g_full)_i v_i (loop.py:315-330), called with
+/* M_inv: truncated diagonally preconditioned CG (solvers.py:7-131) on
+ * H v = m_ref x ((C v) x m_ref) - (m_ref . g_full)_i v_i (loop.py:40-55), called with
  * -g.  Writes z (the NCG then uses d = -z). */
 int mm_precond(mm_ctx *ctx, const double *m_ref, const double *g_full, const double *g,
                int32_t max_iter, double *z, int32_t *iters_host);
@@ -153,14 +153,15 @@ int mm_update_m(mm_ctx *ctx, const double *x, double alpha, const double *d, dou
 int mm_dot(mm_ctx *ctx, const double *a, const double *b, int64_t n, double *out_host);
 int mm_norm_inf(mm_ctx *ctx, const double *a, int64_t n, double *out_host);
 
-/* M.h = sum_i (h . m_i) meas_i / V in Tesla (loop.py:300-313 compute_mh). */
+/* M.h = sum_i (h . m_i) meas_i / V in Tesla (loop.py:25-38 compute_mh). */
 int mm_compute_mh(mm_ctx *ctx, const double *m, const double hdir[3], double *mh_host);
 
 /* One field step: hestenes_stiefel_ncg (minimizers.py:139-190) with line_search
  * (92-137) on total_eg and M_inv, i.e. loop.py:58-65 minimize.  m_inout is the
  * iterate; u_inout and gF_inout are the reference's alt_args (potential and full
  * gradient) carried across calls; m_ref is the preconditioner's fixed magnetisation
- * (the initial state, loop.py:325-328).  Writes the final energy. */
+ * (the module-level initial state that M_inv reads, loop.py:40-55 with 153-158).
+ * Writes the final energy. */
 int mm_minimize(mm_ctx *ctx, double *m_inout, double hext, const double hdir[3],
                 const mm_min_params *params, const double *m_ref, double *u_inout,
                 double *gF_inout, double *E_host, mm_stats *stats_inout);

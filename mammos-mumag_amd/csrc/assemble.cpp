@@ -115,7 +115,7 @@ std::string assemble(const mm_mesh_desc &d, double drop_tol, HostOps &o, bool wa
     for (int e = 0; e < NE; ++e)
         if (d.elem_mat[e] < 0 || d.elem_mat[e] >= d.n_mats) return "element material index out of range";
 
-    // Volume V (materials.py:36-37) in element order.
+    // Volume V (scripts/materials.py:36-37) in element order.
     double V = 0.0;
     std::vector<uint8_t> magnetic(NE);
     for (int e = 0; e < NE; ++e) {

@@ -320,7 +320,7 @@ int total_eg(mm_ctx *c, const double *m, double hext, const double *hdir, double
     return rc;
 }
 
-// M_inv (loop.py:315-330): truncated CG on H with grad_fk = -g.  No host sync.
+// M_inv (loop.py:40-55): truncated CG on H with grad_fk = -g.  No host sync.
 int precond(mm_ctx *c, const double *m_ref, const double *gF, const double *g, int max_iter,
             double *zout) {
     Launch L = c->L();

@@ -6,10 +6,10 @@
   ``Tri3_Contact``, ``Point1`` sections and ``Tags``.
 * ``.krn``: line i = i-th sorted element tag, ``theta phi K1 K2 Js A ...``
   (``scripts/materials.py:22-34``).
-* ``.p2``: INI read with the defaults of ``scripts/tools.py:136-192``.
+* ``.p2``: INI read with the defaults of ``scripts/tools.py:64-120``.
 * ``.dat``: ``int(counter) hext mh E/mu0`` (``scripts/tools.py:92-96``).
 * VTU: ASCII UnstructuredGrid as written by weipa's saveVTK
-  (``scripts/escript_tools.py:440-445``): points in node order, cells sorted
+  (``scripts/escript_tools.py:26-31``): points in node order, cells sorted
   stably by their smallest node index, ``%e`` values, ``tags`` cell data.
 * CSV: ``title`` line then ``name,value,explanation`` rows (``scripts/hmag.py:141-152``).
 """
@@ -100,7 +100,7 @@ def _normalize(v):
 
 
 def read_p2(path):
-    """``scripts/tools.py:136-192`` read_params, returned as a dict."""
+    """``scripts/tools.py:64-120`` read_params, returned as a dict."""
     config = configparser.ConfigParser({
         "mstep": 1.0, "state": "mxyz", "mfinal": -0.8, "hmag_on": 1, "tol_fun": 1e-10,
         "tol_hmag_factor": 1.0, "precond_iter": 10, "iter_max": 1000, "verbose": 1,

@@ -28,10 +28,10 @@ FIELD_DIR = (0.01745, 0.0, 0.99984)
 
 # workloads: magnet edge (nm), voxels per edge, air layers per side, grains, axis cone (deg)
 CONFIGS = {
-    "cfg0": dict(edge=40.0, n=16, layers=6, grains=1, cone=0.0, hstart=8.0, hfinal=-8.0, hstep=-0.1),
-    "cfg1": dict(edge=100.0, n=60, layers=11, grains=8, cone=15.0, hstart=8.0, hfinal=-8.0, hstep=-0.05),
-    "cfg2": dict(edge=250.0, n=125, layers=14, grains=64, cone=15.0, hstart=8.0, hfinal=-8.0, hstep=-0.05),
-    "cfg4": dict(edge=500.0, n=250, layers=16, grains=512, cone=90.0, hstart=6.0, hfinal=-6.0, hstep=-0.02),
+    "cube40_single": dict(edge=40.0, n=16, layers=6, grains=1, cone=0.0, hstart=8.0, hfinal=-8.0, hstep=-0.1),
+    "voronoi8_100nm": dict(edge=100.0, n=60, layers=11, grains=8, cone=15.0, hstart=8.0, hfinal=-8.0, hstep=-0.05),
+    "voronoi64_250nm": dict(edge=250.0, n=125, layers=14, grains=64, cone=15.0, hstart=8.0, hfinal=-8.0, hstep=-0.05),
+    "voronoi512_500nm": dict(edge=500.0, n=250, layers=16, grains=512, cone=90.0, hstart=6.0, hfinal=-6.0, hstep=-0.02),
 }
 
 

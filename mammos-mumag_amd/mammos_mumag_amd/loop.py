@@ -5,7 +5,7 @@ Per field value: one ``Model.minimize`` (the NCG of minimizers.py:139-190 with
 dropped by more than ``mstep`` since the last one (the first step always saves), and
 the record ``[counter, hext, M.h, E]`` is kept.  The potential ``u`` and the full
 gradient (the reference's ``alt_args``) carry over between field steps; the
-preconditioner uses the initial magnetisation throughout (loop.py:325-328).
+preconditioner uses the initial magnetisation throughout (loop.py:40-55 with 153-158).
 """
 import time
 

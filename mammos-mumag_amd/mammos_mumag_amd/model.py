@@ -4,8 +4,8 @@
 and materials, applies escript's boundary rules (scripts/matrix.py:49-70,
 scripts/trafo.py:4-16) and builds the device operators through the C ABI.  Its
 methods are the reference's hot-path functions on device vectors (torch tensors on
-``cuda``): ``total_eg`` (energies.py:67-99), ``M_inv`` (loop.py:315-330), ``update_m``
-(jax_tools.py:15-18), ``minimize`` (loop.py:333-340), ``compute_mh`` (loop.py:300-313).
+``cuda``): ``total_eg`` (energies.py:67-99), ``M_inv`` (loop.py:40-55), ``update_m``
+(jax_tools.py:15-18), ``minimize`` (loop.py:58-65), ``compute_mh`` (loop.py:25-38).
 """
 import ctypes
 import math
@@ -178,7 +178,7 @@ class Model:
         return self.meas > 0
 
     def uniform_m(self, v):
-        """getM state 0 (scripts/magnetization.py:246-262): mask * v, escript normalize."""
+        """getM state 0 (scripts/magnetization.py:54-71): mask * v, escript normalize."""
         m = np.zeros((self.N, 3))
         m[self.magnet_mask()] = v
         ln = np.linalg.norm(m, axis=1)
@@ -186,7 +186,7 @@ class Model:
         return (m * ((1.0 - z) / (ln + z))[:, None]).ravel()
 
     def state_m(self, v, state_id):
-        """Initial states of scripts/magnetization.py:205-262 (flower, vortex, twisted, uniform)."""
+        """Initial states of scripts/magnetization.py:9-71 (flower, vortex, twisted, uniform)."""
         x = self.coords
         if state_id == 4:
             raise ValueError("state 'random' uses escript's RandomData generator, which has no "
@@ -222,7 +222,7 @@ class Model:
         return (m * ((1.0 - z) / (ln + z))[:, None]).ravel()
 
     def vortex_xm(self):
-        """xM of scripts/magnetization.py:265-274 with scale = V**(1/3)."""
+        """xM of scripts/magnetization.py:73-82 with scale = V**(1/3)."""
         scale = self.V ** 0.3333333333333333
         k = 0.5 * math.pi / scale
         kr = k * (self.coords[:, 0] + self.coords[:, 1])
@@ -320,7 +320,7 @@ class Model:
         return out.value
 
     def minimize(self, m, hext, hdir, params, m_ref, u, gF, stats=None):
-        """One field step (loop.py:333-340); m, u, gF updated in place; returns E."""
+        """One field step (loop.py:58-65); m, u, gF updated in place; returns E."""
         P = _lib.MinParams()
         P.tol_fun, P.tol_u = params["tol_fun"], params["tol_u"]
         P.precond_iter, P.hmag_on = int(params["precond_iter"]), int(params["hmag_on"])

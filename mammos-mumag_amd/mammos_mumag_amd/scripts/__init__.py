@@ -38,19 +38,21 @@ def _name(argv, script):
 
 
 def _model(name):
+    """The device context of ``<name>``; ``MAMMOS_MUMAG_PRECISION`` = ``fp64`` (default) or
+    ``mixed`` selects the arithmetic."""
     from ..model import Model
-    return Model.from_files(name)
+    return Model.from_files(name, precision=os.environ.get("MAMMOS_MUMAG_PRECISION", "fp64"))
 
 
 def _krn_first(name):
-    """First .krn line (scripts/tools.py:111-134 read_Js / read_A / readAnisotropyEnergy)."""
+    """First .krn line (scripts/tools.py:39-62 read_Js / read_A / readAnisotropyEnergy)."""
     with open(name + ".krn") as f:
         ll = f.readline().split()
     return dict(theta=float(ll[0]), phi=float(ll[1]), K1=float(ll[2]), Js=float(ll[4]), A=float(ll[5]))
 
 
 def _aniso_energy(name, m):
-    """scripts/tools.py:123-134 (uses the script mu0 = 4 pi 1e-7)."""
+    """scripts/tools.py:51-62 (uses the script mu0 = 4 pi 1e-7)."""
     n = math.sqrt(sum(v * v for v in m))
     m = [v / n for v in m] if n else m
     k = _krn_first(name)

@@ -3,7 +3,7 @@
 ``run_loop / run_hmag / run_exani / run_external / run_mapping (outdir, name)`` copy
 ``<name>.fly``, ``<name>.krn`` (and ``<name>.p2``) into ``outdir`` and run the
 matching entry point in a child process with ``cwd=outdir``
-(simulation.py:87-113, 410-431).  The launcher is this Python interpreter running
+(simulation.py:87-113, 408-431).  The launcher is this Python interpreter running
 ``mammos_mumag_amd.scripts.<script>`` instead of ``run-escript``.  A non-zero exit
 raises ``RuntimeError`` with the child's stderr; ``info.json`` records the run.
 """

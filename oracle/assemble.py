@@ -103,7 +103,7 @@ class Model:
                 self.u[sel] = [math.sin(th) * math.cos(ph), math.sin(th) * math.sin(ph), math.cos(th)]
                 self.K[sel] = MU0_SCRIPT * mat["K1"]
                 self.Js[sel] = mat["Js"]
-                self.A[sel] = MU0_SCRIPT * mat["A"] / (1.0e-9 * 1.0e-9)  # size = 1e-9 (materials.py:13)
+                self.A[sel] = MU0_SCRIPT * mat["A"] / (1.0e-9 * 1.0e-9)  # size = 1e-9 (scripts/materials.py:13)
 
     def _assemble(self):
         N, tets = self.N, self.tets

@@ -9,9 +9,9 @@ Reference behaviour restated here:
   ``id tag n1 n2 n3 n4`` (1-based node ids).  Nodes are kept in file order.
 * ``.krn``: line i belongs to the i-th sorted element tag
   (``scripts/materials.py:22-34``); columns ``theta phi K1 K2 Js A``.
-* ``.p2``: ``scripts/tools.py:136-192`` read_params, same defaults.
+* ``.p2``: ``scripts/tools.py:64-120`` read_params, same defaults.
 * ``.dat``: ``scripts/tools.py:92-96`` write_mh.
-* VTU: ``scripts/escript_tools.py:440-445`` (weipa saveVTK): points in
+* VTU: ``scripts/escript_tools.py:26-31`` (weipa saveVTK): points in
   node order, cells sorted stably by their smallest node index, ASCII ``%e``.
 """
 import configparser
@@ -76,7 +76,7 @@ def _normalize(v):
 
 
 def read_p2(path):
-    """scripts/tools.py:136-192 read_params -> dict with the same tuple meaning."""
+    """scripts/tools.py:64-120 read_params -> dict with the same tuple meaning."""
     config = configparser.ConfigParser({
         "mstep": 1.0, "state": "mxyz", "mfinal": -0.8, "hmag_on": 1, "tol_fun": 1e-10,
         "tol_hmag_factor": 1.0, "precond_iter": 10, "iter_max": 1000, "verbose": 1,
@@ -92,7 +92,7 @@ def read_p2(path):
         m=m, mstep=float(field["mstep"]), mfinal=float(field["mfinal"]), state_id=state_id,
         h=h, hstart=float(field["hstart"]), hfinal=float(field["hfinal"]),
         hstep=float(field["hstep"]),
-        hmag_on=int(mini["hmag_on"]),  # read from [minimizer] only (tools.py:178)
+        hmag_on=int(mini["hmag_on"]),  # read from [minimizer] only (scripts/tools.py:106)
         tol_u=tol_fun * float(mini["tol_hmag_factor"]), tol_fun=tol_fun,
         precond_iter=int(mini["precond_iter"]), iter_max=int(mini["iter_max"]),
     )
@@ -114,7 +114,7 @@ def _fmt_rows(a):
 
 
 def write_vtu(path, coords, tets, elem_tags, point_data):
-    """weipa-style ASCII VTU (scripts/escript_tools.py:440-445)."""
+    """weipa-style ASCII VTU (scripts/escript_tools.py:26-31)."""
     order = np.argsort(tets.min(axis=1), kind="stable")
     t = tets[order]
     parts = ['<?xml version="1.0"?>',

@@ -1,6 +1,6 @@
 """Oracle linear solvers (FP64).  Test infrastructure only.
 
-* ``jax_cg`` restates ``scripts/solvers.py:326-336`` (jax_scipy_cg), i.e. JAX's
+* ``jax_cg`` restates ``scripts/solvers.py:222-231`` (jax_scipy_cg), i.e. JAX's
   ``jax.scipy.sparse.linalg.cg`` (third-party, JAX's public ``_cg_solve``) with
   the Jacobi preconditioner ``M(x) = x * (1 / D)``:
 
@@ -11,7 +11,7 @@
           z = M r; gamma' = r.z; p = z + (gamma'/gamma) p; gamma = gamma'; k += 1
 
   It returns the iterate; the reference discards convergence info.
-* ``truncated_cg`` restates ``scripts/solvers.py:112-236``
+* ``truncated_cg`` restates ``scripts/solvers.py:7-131``
   (truncated_cg_diag_precond_jit) including the ``p == 0 -> -grad_fk`` exit.
 """
 import numpy as np
@@ -42,7 +42,7 @@ def jax_cg(A, D, b, x0, tol=1e-5, atol=0.0, maxiter=10000):
 
 
 def truncated_cg(Bk, grad_fk, Dk, max_iter=100):
-    """Truncated PCG of Nocedal-Wright alg. 7.1 (solvers.py:112-236).
+    """Truncated PCG of Nocedal-Wright alg. 7.1 (solvers.py:7-131).
 
     ``Bk`` is a callable. Returns ``(p, iterations)``.
     """

@@ -96,7 +96,7 @@ def test_pcg_structured(gpu_s, oracle):
 def test_loop_steps_structured(gpu_s, oracle):
     from mammos_mumag_amd.loop import run_loop
     from oracle.loop import run_loop as oracle_loop
-    p = generate.loop_params("cfg0")
+    p = generate.loop_params("cube40_single")
     rec_o, mini = oracle_loop(oracle, p, oracle.uniform_m(p["m"]), max_steps=4)
     rec_g, stats, _ = run_loop(gpu_s, p, gpu_s.uniform_m(p["m"]), max_steps=4, save=False)
     print("[gpu] oracle", mini.stats, "gpu", stats, flush=True)
@@ -112,7 +112,7 @@ def test_mixed_precision_errors(gpu_mixed, oracle):
     from mammos_mumag_amd.loop import run_loop
     from oracle.energies import total_eg
     O = oracle
-    p = generate.loop_params("cfg0")
+    p = generate.loop_params("cube40_single")
     rng = np.random.default_rng(5)
     m = O.uniform_m([0.1, 0.2, 0.97]) + 0.05 * rng.normal(size=3 * O.N) * np.repeat(O.meas > 0, 3)
     n = np.linalg.norm(m.reshape(-1, 3), axis=1)
