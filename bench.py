@@ -6,8 +6,13 @@ synthetic 64-grain Voronoi cube of 125^3 voxels (~2.0 M magnet nodes, 3.65 M nod
 21.5 M Kuhn tetrahedra) from +8 T in -0.05 T steps.  The first W field steps are
 warm-up; the next K are timed.  Inputs are resident in HBM before timing starts.
 
+The value is the throughput of the form chosen by --precision (default: FP32 inner
+solve with FP64 outer refinement, FP64 dots / energies / line search); the plain FP64
+path then runs the same steps and the line carries its throughput, the speedup and the
+form's errors against it (M.h, energies, potential u, demag field G u).
+
 Prints one JSON line.  The FP64 oracle is timed on the host cores as a reported
-baseline in a child process, concurrently with the GPU run.
+baseline in a child process, concurrently with the GPU runs.
 """
 import argparse
 import json
@@ -61,7 +66,7 @@ def cpu_worker(cfg, out_path, seed):
     p = loop_params(cfg)
     m = M.uniform_m(p["m"])
     b = M.Dxyz[0] @ m[0::3] + M.Dxyz[1] @ m[1::3] + M.Dxyz[2] @ m[2::3]
-    n_pcg = 20
+    n_pcg = 100
     t0 = time.perf_counter()
     u, it = jax_cg(M.K, M.diagK, b, np.zeros(M.N), tol=1e-30, maxiter=n_pcg)
     t_pcg = (time.perf_counter() - t0) / it
@@ -93,7 +98,9 @@ def start_cpu_leg(cfg, seed):
 
 
 # --------------------------------------------------------------------------- GPU leg
-def gpu_run(args):
+def gpu_run(args, precision, traffic_probe=False):
+    """Warm-up and timed field steps of one form; returns throughput, counters, kernel
+    timing and the per-step records (M.h, E) plus the final potential and demag field."""
     import numpy as np
     import torch
 
@@ -103,11 +110,11 @@ def gpu_run(args):
     t0 = time.perf_counter()
     g = build_config(args.config, seed=args.seed)
     n_nodes, n_elems = len(g.xs) * len(g.ys) * len(g.zs), 6 * (len(g.xs) - 1) * (len(g.ys) - 1) * (len(g.zs) - 1)
-    log(f"[bench] mesh {args.config}: {n_nodes} nodes, {n_elems} tets ({time.perf_counter() - t0:.1f} s)")
+    log(f"[bench] {precision}: mesh {args.config}: {n_nodes} nodes, {n_elems} tets ({time.perf_counter() - t0:.1f} s)")
     t0 = time.perf_counter()
-    model = Model.from_structured(g, structured=not args.general, precision=args.precision,
+    model = Model.from_structured(g, structured=not args.general, precision=precision,
                                   inner_tol=args.inner_tol)
-    log(f"[bench] context: {model.info} ({time.perf_counter() - t0:.1f} s)")
+    log(f"[bench] {precision}: context {model.info} ({time.perf_counter() - t0:.1f} s)")
     p = loop_params(args.config)
     m = model.to_device(model.uniform_m(p["m"]))
     m_ref = m.clone()
@@ -118,24 +125,25 @@ def gpu_run(args):
     for w in range(args.warmup):
         t1 = time.perf_counter()
         E = model.minimize(m, hext, p["h"], p, m_ref, u, gF, st)
-        log(f"[bench] warmup step {w}: h={hext:.2f} T E={E:.6e} {time.perf_counter() - t1:.3f} s {st.as_dict()}")
+        log(f"[bench] {precision}: warmup step {w}: h={hext:.2f} T E={E:.6e} {time.perf_counter() - t1:.3f} s "
+            f"{st.as_dict()}")
         hext += p["hstep"]
     st = _lib.Stats()
     model.timing(True)
     torch.cuda.synchronize()
     t_start = time.perf_counter()
-    mh = None
+    energies = []
     for k in range(args.steps):
-        E = model.minimize(m, hext, p["h"], p, m_ref, u, gF, st)
+        energies.append(model.minimize(m, hext, p["h"], p, m_ref, u, gF, st))
         hext += p["hstep"]
     torch.cuda.synchronize()
     elapsed = time.perf_counter() - t_start
     timing = model.timing_get()
     model.timing(False)
     mh = model.compute_mh(m, p["h"])
-    log(f"[bench] timed {args.steps} steps: {elapsed:.3f} s, last h={hext - p['hstep']:.2f} T, "
-        f"M.h={mh:.6f} T, E={E:.6e}, {st.as_dict()}, {timing}")
-    if args.traffic_probe:
+    log(f"[bench] {precision}: timed {args.steps} steps: {elapsed:.3f} s, last h={hext - p['hstep']:.2f} T, "
+        f"M.h={mh:.6f} T, E={energies[-1]:.6e}, {st.as_dict()}, {timing}")
+    if traffic_probe:
         # reference dispatch for the HBM byte counters: a device copy of 512 MiB
         # (reads and writes 524288 KiB)
         src = torch.ones(64 << 20, dtype=torch.float64, device="cuda")
@@ -144,8 +152,53 @@ def gpu_run(args):
         dst.copy_(src)
         torch.cuda.synchronize()
         log("[bench] traffic probe: copied 512 MiB")
-    return dict(elapsed=elapsed, stats=st.as_dict(), timing=timing, info=model.info,
-                n_nodes=n_nodes, n_elems=n_elems, n_mag_nodes=int(np.sum(model.meas > 0)))
+    res = dict(elapsed=elapsed, stats=st.as_dict(), timing=timing, info=model.info,
+               n_nodes=n_nodes, n_elems=n_elems, n_mag_nodes=int(np.sum(model.meas > 0)),
+               mh=mh, energies=energies, u=u.cpu().numpy(), hd=model.gradient(u).cpu().numpy())
+    model.close()
+    del m, m_ref, u, gF, model
+    torch.cuda.empty_cache()
+    return res
+
+
+def roofline(res, precision, args):
+    """Dominant PCG kernel of the timed steps: algorithmic bytes / measured launch time."""
+    tm = res["timing"]
+    prec = "FP32 inner solve" if precision == "mixed" else "FP64"
+    kname = ("k_ka_grid (7-point Poisson stencil fused with p = z + beta p, p.q partials)"
+             if res["info"]["structured"] else "k_ka_sell (SELL-64 K apply fused with p = z + beta p, p.q partials)")
+    kernels = []
+    for key, name in (("spmv", f"{kname}, {prec}"),
+                      ("update", f"k_kb (x, r, z update, r.z and r.r partials), {prec}")):
+        if tm[f"{key}_launches"] > 0:
+            avg_s = tm[f"{key}_ms"] / tm[f"{key}_launches"] / 1e3
+            kernels.append(dict(key=key, kernel=name, total_ms=tm[f"{key}_ms"], avg_s=avg_s,
+                                bytes=tm[f"{key}_bytes"], launches=tm[f"{key}_launches"]))
+    if not kernels:
+        return None
+    # the dominant kernel: largest share of the step (average time x launches per step)
+    k = max(kernels, key=lambda d: d["avg_s"])
+    achieved = k["bytes"] / k["avg_s"] / 1e9
+    traffic = (TRAFFIC.get((precision, key_of(k)))
+               if args.config == "voronoi64_250nm" and res["info"]["structured"] else None)
+    return dict(bound="hbm", achieved=round(achieved, 1), peak=HBM_PEAK_GBS, unit="GB/s",
+                frac=round(achieved / HBM_PEAK_GBS, 4), traffic=traffic,
+                kernel=k["kernel"], bytes_per_launch=k["bytes"], avg_launch_us=round(k["avg_s"] * 1e6, 2),
+                sampled_launches=k["launches"],
+                others=[dict(kernel=o["kernel"], avg_launch_us=round(o["avg_s"] * 1e6, 2),
+                             achieved_GBs=round(o["bytes"] / o["avg_s"] / 1e9, 1))
+                        for o in kernels if o is not k])
+
+
+def form_errors(r, ref):
+    """Errors of a run against the FP64 run over the same field steps."""
+    import numpy as np
+    e = np.asarray(r["energies"])
+    e0 = np.asarray(ref["energies"])
+    return dict(mh_abs=float(abs(r["mh"] - ref["mh"])),
+                E_rel_max=float(np.max(np.abs(e / e0 - 1))),
+                u_rel=float(np.max(np.abs(r["u"] - ref["u"])) / np.max(np.abs(ref["u"]))),
+                hdemag_rel=float(np.max(np.abs(r["hd"] - ref["hd"])) / np.max(np.abs(ref["hd"]))))
 
 
 def main():
@@ -154,7 +207,10 @@ def main():
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--config", default="voronoi64_250nm")
-    ap.add_argument("--precision", default="fp64", choices=["fp64", "mixed"])
+    ap.add_argument("--precision", default="mixed", choices=["fp64", "mixed"],
+                    help="form whose throughput is the value (mixed: FP32 inner / FP64 outer)")
+    ap.add_argument("--no-compare", action="store_true",
+                    help="skip the plain FP64 run (speedup and errors of the form against it)")
     ap.add_argument("--seed", type=int, default=1000)
     ap.add_argument("--general", action="store_true", help="unstructured kernels (SELL K, element loops)")
     ap.add_argument("--inner-tol", type=float, default=0.0)
@@ -173,34 +229,15 @@ def main():
     cpu = None
     if not args.no_cpu_baseline:
         cpu = start_cpu_leg(args.config, args.seed)
-    res = gpu_run(args)
-    st = res["stats"]
+    res = gpu_run(args, args.precision, traffic_probe=args.traffic_probe)
+    ref = None
+    if args.precision != "fp64" and not args.no_compare:
+        ref = gpu_run(args, "fp64")
     steps = args.steps
     value = steps / res["elapsed"]
-    tm = res["timing"]
-    roof = None
-    kernels = []
-    prec = "FP32 inner solve" if args.precision == "mixed" else "FP64"
-    kname = ("k_ka_grid (7-point Poisson stencil fused with p = z + beta p, p.q partials)"
-             if res["info"]["structured"] else "k_ka_sell (SELL-64 K apply fused with p = z + beta p, p.q partials)")
-    for key, name in (("spmv", f"{kname}, {prec}"),
-                      ("update", f"k_kb (x, r, z update, r.z and r.r partials), {prec}")):
-        if tm[f"{key}_launches"] > 0:
-            avg_s = tm[f"{key}_ms"] / tm[f"{key}_launches"] / 1e3
-            kernels.append(dict(key=key, kernel=name, total_ms=tm[f"{key}_ms"], avg_s=avg_s,
-                                bytes=tm[f"{key}_bytes"], launches=tm[f"{key}_launches"]))
-    if kernels:
-        k = max(kernels, key=lambda d: d["total_ms"])  # the dominant kernel of the timed steps
-        achieved = k["bytes"] / k["avg_s"] / 1e9
-        roof = dict(bound="hbm", achieved=round(achieved, 1), peak=HBM_PEAK_GBS, unit="GB/s",
-                    frac=round(achieved / HBM_PEAK_GBS, 4),
-                    traffic=(TRAFFIC.get((args.precision, key_of(k)))
-                             if args.config == "voronoi64_250nm" and res["info"]["structured"] else None),
-                    kernel=k["kernel"], bytes_per_launch=k["bytes"], avg_launch_us=round(k["avg_s"] * 1e6, 2),
-                    sampled_launches=k["launches"],
-                    others=[dict(kernel=o["kernel"], avg_launch_us=round(o["avg_s"] * 1e6, 2),
-                                 achieved_GBs=round(o["bytes"] / o["avg_s"] / 1e9, 1))
-                            for o in kernels if o is not k])
+    roof = roofline(res, args.precision, args)
+    base = ref if ref is not None else res
+    st = base["stats"]   # the CPU baseline runs the plain FP64 method: scale by its counts
     cpu_res = None
     if cpu is not None:
         proc, path = cpu
@@ -215,7 +252,7 @@ def main():
                            sample=(f"FP64 numpy/scipy oracle on the same {args.config} mesh: unit costs "
                                    f"timed over {c['n_pcg']} PCG iterations, 3 PCG-free evaluations and "
                                    f"{c['n_tcg']} truncated-CG iterations ({c['sample_s']:.1f} s), "
-                                   f"scaled by this run's per-step counts; single thread"))
+                                   f"scaled by the plain FP64 run's per-step counts; single thread"))
         except Exception as e:  # the baseline is reported, not required
             log(f"[bench] cpu baseline failed: {e}")
         finally:
@@ -232,8 +269,14 @@ def main():
                                f"{cfgd['n']}^3 voxels x 6 Kuhn tets in a graded air box, "
                                f"{res['n_mag_nodes']} magnet nodes / {res['n_nodes']} nodes / "
                                f"{res['n_elems']} tets, field steps from {cfgd['hstart']:+g} T by {cfgd['hstep']:g} T",
-                   "precision": args.precision, "form": "plain (no learned warm start)",
-                   "per_step": {k: round(v / steps, 2) for k, v in st.items()}},
+                   "precision": args.precision,
+                   "form": ("FP32 inner solve + FP64 outer (no learned part)" if args.precision == "mixed"
+                            else "plain FP64"),
+                   "per_step": {k: round(v / steps, 2) for k, v in res["stats"].items()}},
+        "vs_fp64": (None if ref is None else dict(
+            fp64_value=round(steps / ref["elapsed"], 6), speedup=round(ref["elapsed"] / res["elapsed"], 4),
+            fp64_per_step={k: round(v / steps, 2) for k, v in ref["stats"].items()},
+            errors=form_errors(res, ref))),
         "roofline": roof,
         "cpu_baseline": cpu_res,
     }
