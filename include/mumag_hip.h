@@ -10,7 +10,8 @@
  *    unless the name ends in _host.  Nodal 3-vectors are interleaved f64[3N]
  *    ([3i+a] = component a of node i), scalars f64[N], in mesh node order.
  *  - The context owns the operators and all workspace; no entry point allocates after
- *    mm_ctx_create.  One HIP stream per context; calls return after the stream is
+ *    mm_ctx_create, except the set-up calls of the learned warm start (mm_mlp_create,
+ *    and mm_warm_start the first time it enables a network on a context).  One HIP stream per context; calls return after the stream is
  *    synchronised whenever a scalar is returned to the host.  A context is not
  *    thread-safe.
  *  - mm_last_error() returns a message for the last failure on the calling thread.
@@ -173,6 +174,47 @@ int mm_apply_C(mm_ctx *ctx, const double *v, double *y);
 int mm_apply_K(mm_ctx *ctx, const double *x, double *y);
 int mm_divergence(mm_ctx *ctx, const double *m, double *b);
 int mm_gradient(mm_ctx *ctx, const double *u, double *g);
+
+/* ---- Learned warm start (no counterpart in the reference: the learned part of the
+ * chosen form, for callers that run many potential solves on one structured grid).
+ *
+ * A per-node MLP (MM_MLP_IN features of a node and its 1-ring -> 128 -> 128 -> 128 -> 1,
+ * ReLU, bf16 MFMA with FP32 accumulation, FP32 master weights, Adam) predicts the
+ * scalar potential u from m.  With mm_warm_start enabled, every potential solve of
+ * mm_total_eg / mm_hmag_eg first moves u by the Galerkin step along the change of the
+ * prediction (the first step of a flexible PCG whose first preconditioner application
+ * is the network); the Jacobi PCG then ends at the same tolerance, so only the
+ * iteration count changes.  Structured (Kuhn grid) contexts only.
+ *
+ * Features (bf16, node-major [N][MM_MLP_IN]): m of the node and its 6 axis
+ * neighbours, coordinates / half box edge, mean magnetisation, magnetic fraction and
+ * mean Js of the 8 adjacent voxels.  Parameters: flat FP32 [MM_MLP_PARAMS], W_l
+ * row-major [in][out] then b_l, layers 1..4. */
+#define MM_MLP_IN 32
+#define MM_MLP_PARAMS 37377
+typedef struct mm_mlp mm_mlp;
+/* A network on ctx's device and stream, He-uniform initialised from seed; training
+ * batches of up to max_batch nodes.  Destroy it before its context. */
+int mm_mlp_create(mm_ctx *ctx, int64_t max_batch, uint64_t seed, mm_mlp **out);
+int mm_mlp_destroy(mm_mlp *net);
+int mm_mlp_set_params(mm_mlp *net, const float *params_host);   /* also resets Adam */
+int mm_mlp_get_params(mm_mlp *net, float *params_host_out);
+/* Features of every node for magnetisation m (device f64[3N]) into X (device u16[N*32]). */
+int mm_mlp_features(mm_ctx *ctx, const double *m, uint16_t *X);
+/* y[i] = network(X[i]) for n nodes (device pointers). */
+int mm_mlp_forward(mm_mlp *net, int64_t n, const uint16_t *X, float *y);
+/* Gradients of the loss mean((y - target)^2) over a batch of n <= max_batch nodes
+ * (flat, host) and the loss; no update. */
+int mm_mlp_grads(mm_mlp *net, int64_t n, const uint16_t *X, const float *target, float *grad_host,
+                 double *loss_host);
+/* One Adam step (lr, beta = 0.9 / 0.999) on a batch; returns when the step is done;
+ * loss_host (the batch loss before the step) may be NULL. */
+int mm_mlp_train_step(mm_mlp *net, int64_t n, const uint16_t *X, const float *target, float lr,
+                      double *loss_host);
+/* Enable (net != NULL) or disable the learned warm start; u = scale * network output. */
+int mm_warm_start(mm_ctx *ctx, mm_mlp *net, double scale);
+/* Warm-started solves since mm_warm_start and the last Galerkin coefficient. */
+int mm_warm_start_stats(mm_ctx *ctx, int64_t *calls_host, double *last_gamma_host);
 
 /* HIP-event timing of the two fused PCG kernels since the last mm_timing_enable: the
  * K apply with the search-direction update (SELL SpMV or 7-point stencil, FP64 or the

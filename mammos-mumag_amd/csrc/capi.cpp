@@ -16,6 +16,7 @@
 #include "../../include/mumag_hip.h"
 #include "assemble.h"
 #include "common.h"
+#include "mlp.h"
 #include "structured.h"
 
 using namespace mm;
@@ -84,6 +85,16 @@ struct mm_ctx {
     bool timing = false;
     KernelTimer t_spmv, t_upd;
 
+    // learned warm start (structured grids): network, features, predictions
+    FeatGrid F;
+    mm_mlp *ws_net = nullptr;
+    double ws_scale = 0.0;
+    uint16_t *ws_X = nullptr;        // [N][kMlpIn] features
+    float *ws_pred = nullptr, *ws_prev = nullptr;
+    bool ws_have_prev = false;
+    double *ws_red = nullptr;        // [0..2] mbar sums, [4] num, [5] unused, [6] den, [7] gamma
+    int64_t ws_calls = 0;
+
     Launch L() const {
         Launch l;
         l.s = s;
@@ -119,6 +130,49 @@ struct mm_ctx {
         for (hipEvent_t e : t_spmv.ev) (void)hipEventDestroy(e);
         for (hipEvent_t e : t_upd.ev) (void)hipEventDestroy(e);
         if (s) (void)hipStreamDestroy(s);
+    }
+};
+
+// The per-node MLP: FP32 master parameters with Adam moments, the bf16 copies the
+// kernels read, and the training workspace for batches of up to max_batch nodes.
+struct mm_mlp {
+    mm_ctx *ctx = nullptr;
+    float *p = nullptr, *g = nullptr, *m1 = nullptr, *m2 = nullptr;
+    uint16_t *w1t = nullptr, *w2t = nullptr, *w3t = nullptr, *w2n = nullptr, *w3n = nullptr;
+    int64_t max_batch = 0, ld_max = 0;
+    uint16_t *h[3] = {nullptr, nullptr, nullptr}, *dz[3] = {nullptr, nullptr, nullptr};
+    float *y = nullptr, *dy = nullptr;
+    double *loss = nullptr;
+    int step = 0;
+    std::vector<void *> allocs;
+    MlpDev dev() const {
+        MlpDev w;
+        w.w1t = w1t;
+        w.w2t = w2t;
+        w.w3t = w3t;
+        w.w2n = w2n;
+        w.w3n = w3n;
+        w.b1 = p + kMlpOffB1;
+        w.b2 = p + kMlpOffB2;
+        w.b3 = p + kMlpOffB3;
+        w.w4 = p + kMlpOffW4;
+        w.b4 = p + kMlpOffB4;
+        return w;
+    }
+    template <typename T>
+    int alloc(T **ptr, size_t count) {
+        void *q = nullptr;
+        hipError_t e = hipMalloc(&q, std::max<size_t>(count, 1) * sizeof(T));
+        if (e != hipSuccess) return fail(MM_E_HIP, std::string("hipMalloc: ") + hipGetErrorString(e));
+        allocs.push_back(q);
+        *ptr = (T *)q;
+        e = hipMemset(q, 0, std::max<size_t>(count, 1) * sizeof(T));
+        if (e != hipSuccess) return fail(MM_E_HIP, std::string("hipMemset: ") + hipGetErrorString(e));
+        return 0;
+    }
+    ~mm_mlp() {
+        if (ctx && ctx->s) (void)hipStreamSynchronize(ctx->s);
+        for (void *a : allocs) (void)hipFree(a);
     }
 };
 
@@ -298,6 +352,32 @@ int pcg(mm_ctx *c, const double *b, double *x, double tol, double atol, int maxi
     return pcg64(c, b, x, tol, atol, maxiter, iters);
 }
 
+// Learned warm start of the potential solve for b = D m: the network's prediction for m
+// gives a direction d (the change of the prediction since the previous solve, or on the
+// first solve the prediction minus u); u moves by the Galerkin step gamma d that
+// minimises the K-energy norm of the error along d, gamma = d.(b - K u) / d.K d.  This is
+// the first step of a flexible PCG whose first preconditioner application is the
+// network; the Jacobi PCG that follows ends at the same tolerance, so the answer is the
+// plain method's.  No host synchronisation.
+void learned_warm_start(mm_ctx *c, const double *m, const double *b, double *u) {
+    Launch L = c->L();
+    const int64_t n = c->N;
+    for (int a = 0; a < 3; ++a)
+        launch_reduce(L, RED_MH, n, m, c->meas, nullptr, nullptr, a == 0, a == 1, a == 2, c->ws_red + a);
+    launch_mlp_features(L, c->F, m, c->ws_red, c->ws_X);
+    launch_mlp_forward(L, c->ws_net->dev(), n, c->ws_X, c->ws_pred);
+    double *d = c->z, *Ku = c->q, *Kd = c->r;   // PCG workspace, free before the solve
+    launch_mlp_direction(L, c->F, c->ws_pred, c->ws_have_prev ? c->ws_prev : nullptr, u, c->ws_scale, d);
+    op_K(c, u, Ku, nullptr, nullptr);
+    op_K(c, d, Kd, nullptr, nullptr);
+    launch_reduce(L, RED_YZD, n, Ku, b, d, d, 0, 0, 0, c->ws_red + 4);
+    launch_reduce(L, RED_DOT, n, d, Kd, nullptr, nullptr, 0, 0, 0, c->ws_red + 6);
+    launch_mlp_galerkin(L, n, c->ws_red + 4, c->ws_red + 6, d, u, c->ws_red + 7);
+    std::swap(c->ws_pred, c->ws_prev);
+    c->ws_have_prev = true;
+    c->ws_calls++;
+}
+
 // E, g_proj, g_full for magnetisation m (energies.py:67-99).
 int total_eg(mm_ctx *c, const double *m, double hext, const double *hdir, double tol_u,
              int hmag_on, double *u, double *E, double *gproj, double *gfull, int *pcg_it) {
@@ -305,6 +385,7 @@ int total_eg(mm_ctx *c, const double *m, double hext, const double *hdir, double
     op_C(c, m, c->gC, hmag_on > 0 ? c->b : nullptr, nullptr);
     int it = 0, rc = 0;
     if (hmag_on > 0) {
+        if (c->ws_net) learned_warm_start(c, m, c->b, u);
         rc = pcg(c, c->b, u, tol_u, 0.0, 10000, &it);
         if (rc < 0) return rc;
         op_grad(c, u, c->gd);
@@ -481,6 +562,34 @@ int mm_ctx_create(const mm_mesh_desc *d, const mm_options *opts, mm_ctx **out) {
         c->G.fB = S.h * S.h / 24.0;
         c->G.fG = S.h * S.h / (24.0 * c->V);
         c->NEm = 6 * (int64_t)S.mn * S.mn * S.mn;
+        // the learned warm start's view of the grid
+        {
+            std::vector<double> ax(d->grid_x, d->grid_x + S.nx), ay(d->grid_y, d->grid_y + S.ny),
+                az(d->grid_z, d->grid_z + S.nz);
+            double *dx, *dy, *dz;
+            if ((rc = c->upload(&dx, ax)) || (rc = c->upload(&dy, ay)) || (rc = c->upload(&dz, az))) return bail(rc);
+            double half = 0.0, jsmax = 0.0;
+            for (double v : ax) half = std::max(half, std::fabs(v));
+            for (double v : ay) half = std::max(half, std::fabs(v));
+            for (double v : az) half = std::max(half, std::fabs(v));
+            for (int q = 0; q < d->n_mats; ++q) jsmax = std::max(jsmax, d->mat_Js[q]);
+            FeatGrid &F = c->F;
+            F.nx = S.nx;
+            F.ny = S.ny;
+            F.nz = S.nz;
+            F.x = dx;
+            F.y = dy;
+            F.z = dz;
+            F.inv_half = half > 0 ? 1.0 / half : 0.0;
+            F.inv_V = 1.0 / c->V;
+            F.inv_js = jsmax > 0 ? 1.0 / jsmax : 0.0;
+            F.m0x = S.m0x;
+            F.m0y = S.m0y;
+            F.m0z = S.m0z;
+            F.mn = S.mn;
+            F.vmat = vm;
+            if ((rc = c->alloc(&c->ws_red, 8))) return bail(rc);
+        }
     } else {
         std::vector<int> sp, scol;
         std::vector<double> sval;
@@ -620,6 +729,7 @@ int mm_hmag_eg(mm_ctx *c, const double *m, double *u, double tol, double *E_host
                int32_t *pcg_iters_host) {
     if (!c || !m || !u || !E_host || !g) return fail(MM_E_ARG, "null argument");
     op_C(c, m, c->gC, c->b, nullptr);
+    if (c->ws_net) learned_warm_start(c, m, c->b, u);
     int it = 0;
     int rc = pcg(c, c->b, u, tol, 0.0, 10000, &it);
     if (rc < 0) return rc;
@@ -851,6 +961,177 @@ int mm_minimize(mm_ctx *c, double *m_inout, double hext, const double hdir[3],
     c->X1 = X1;
     c->Gp = G;
     c->G1 = G1;
+    return 0;
+}
+
+// ---------------------------------------------------------------- learned warm start
+
+int mm_mlp_create(mm_ctx *c, int64_t max_batch, uint64_t seed, mm_mlp **out) {
+    if (!c || !out || max_batch <= 0) return fail(MM_E_ARG, "null argument or max_batch <= 0");
+    *out = nullptr;
+    HIPC(hipSetDevice(c->device));
+    mm_mlp *w = new mm_mlp();
+    w->ctx = c;
+    w->max_batch = max_batch;
+    w->ld_max = (max_batch + 31) / 32 * 32;
+    int rc = 0;
+    auto bail = [&](int code) {
+        delete w;
+        return code;
+    };
+    if ((rc = w->alloc(&w->p, kMlpParams)) || (rc = w->alloc(&w->g, kMlpParams)) ||
+        (rc = w->alloc(&w->m1, kMlpParams)) || (rc = w->alloc(&w->m2, kMlpParams)) ||
+        (rc = w->alloc(&w->w1t, (size_t)kMlpW * kMlpP1)) || (rc = w->alloc(&w->w2t, (size_t)kMlpW * kMlpPW)) ||
+        (rc = w->alloc(&w->w3t, (size_t)kMlpW * kMlpPW)) || (rc = w->alloc(&w->w2n, (size_t)kMlpW * kMlpPW)) ||
+        (rc = w->alloc(&w->w3n, (size_t)kMlpW * kMlpPW)) || (rc = w->alloc(&w->y, (size_t)w->ld_max)) ||
+        (rc = w->alloc(&w->dy, (size_t)w->ld_max)) || (rc = w->alloc(&w->loss, 1)))
+        return bail(rc);
+    for (int l = 0; l < 3; ++l)
+        if ((rc = w->alloc(&w->h[l], (size_t)kMlpW * w->ld_max)) || (rc = w->alloc(&w->dz[l], (size_t)kMlpW * w->ld_max)))
+            return bail(rc);
+    // He-uniform initialisation (splitmix64 stream), zero biases
+    std::vector<float> p(kMlpParams, 0.f);
+    uint64_t st = seed;
+    auto uni = [&st]() {
+        uint64_t z = (st += 0x9e3779b97f4a7c15ull);
+        z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ull;
+        z = (z ^ (z >> 27)) * 0x94d049bb133111ebull;
+        z ^= z >> 31;
+        return (double)(z >> 11) * (1.0 / 9007199254740992.0);
+    };
+    const int offs[4] = {kMlpOffW1, kMlpOffW2, kMlpOffW3, kMlpOffW4};
+    const int fan_in[4] = {kMlpIn, kMlpW, kMlpW, kMlpW}, count[4] = {kMlpIn * kMlpW, kMlpW * kMlpW, kMlpW * kMlpW, kMlpW};
+    for (int l = 0; l < 4; ++l) {
+        const double lim = std::sqrt((l < 3 ? 6.0 : 3.0) / fan_in[l]);
+        for (int i = 0; i < count[l]; ++i) p[offs[l] + i] = (float)((2.0 * uni() - 1.0) * lim);
+    }
+    if (hipMemcpy(w->p, p.data(), p.size() * sizeof(float), hipMemcpyHostToDevice) != hipSuccess)
+        return bail(fail(MM_E_HIP, "hipMemcpy of the initial parameters failed"));
+    launch_mlp_publish(c->L(), w->p, w->w1t, w->w2t, w->w3t, w->w2n, w->w3n);
+    if (hipStreamSynchronize(c->s) != hipSuccess) return bail(fail(MM_E_HIP, "stream synchronise failed"));
+    *out = w;
+    return 0;
+}
+
+int mm_mlp_destroy(mm_mlp *w) {
+    if (!w) return 0;
+    if (w->ctx && w->ctx->ws_net == w) w->ctx->ws_net = nullptr;
+    delete w;
+    return 0;
+}
+
+int mm_mlp_set_params(mm_mlp *w, const float *params_host) {
+    if (!w || !params_host) return fail(MM_E_ARG, "null argument");
+    mm_ctx *c = w->ctx;
+    HIPC(hipMemcpyAsync(w->p, params_host, kMlpParams * sizeof(float), hipMemcpyHostToDevice, c->s));
+    HIPC(hipMemsetAsync(w->m1, 0, kMlpParams * sizeof(float), c->s));
+    HIPC(hipMemsetAsync(w->m2, 0, kMlpParams * sizeof(float), c->s));
+    w->step = 0;
+    launch_mlp_publish(c->L(), w->p, w->w1t, w->w2t, w->w3t, w->w2n, w->w3n);
+    HIPC(hipStreamSynchronize(c->s));
+    c->ws_have_prev = false;
+    return 0;
+}
+
+int mm_mlp_get_params(mm_mlp *w, float *params_host) {
+    if (!w || !params_host) return fail(MM_E_ARG, "null argument");
+    HIPC(hipMemcpyAsync(params_host, w->p, kMlpParams * sizeof(float), hipMemcpyDeviceToHost, w->ctx->s));
+    HIPC(hipStreamSynchronize(w->ctx->s));
+    return 0;
+}
+
+int mm_mlp_features(mm_ctx *c, const double *m, uint16_t *X) {
+    if (!c || !m || !X) return fail(MM_E_ARG, "null argument");
+    if (!c->structured) return fail(MM_E_STATE, "the learned warm start needs a structured (Kuhn grid) context");
+    Launch L = c->L();
+    for (int a = 0; a < 3; ++a)
+        launch_reduce(L, RED_MH, c->N, m, c->meas, nullptr, nullptr, a == 0, a == 1, a == 2, c->ws_red + a);
+    launch_mlp_features(L, c->F, m, c->ws_red, X);
+    HIPC(hipStreamSynchronize(c->s));
+    return 0;
+}
+
+int mm_mlp_forward(mm_mlp *w, int64_t n, const uint16_t *X, float *y) {
+    if (!w || !X || !y || n < 0) return fail(MM_E_ARG, "null argument or n < 0");
+    launch_mlp_forward(w->ctx->L(), w->dev(), n, X, y);
+    HIPC(hipStreamSynchronize(w->ctx->s));
+    return 0;
+}
+
+namespace {
+// forward with stored activations, loss, backward, all parameter gradients into w->g
+int mlp_grads(mm_mlp *w, int64_t n, const uint16_t *X, const float *t) {
+    mm_ctx *c = w->ctx;
+    Launch L = c->L();
+    const int64_t ld = (n + 31) / 32 * 32;
+    HIPC(hipMemsetAsync(w->loss, 0, sizeof(double), c->s));
+    HIPC(hipMemsetAsync(w->g, 0, kMlpParams * sizeof(float), c->s));
+    const MlpDev d = w->dev();
+    launch_mlp_forward_train(L, d, n, X, w->y, w->h[0], w->h[1], w->h[2], ld);
+    launch_mlp_loss(L, n, w->y, t, w->dy, w->loss);
+    launch_mlp_backward(L, d, n, w->dy, w->h[0], w->h[1], w->h[2], w->dz[0], w->dz[1], w->dz[2], ld);
+    launch_mlp_param_grads(L, n, ld, X, w->dy, w->h[0], w->h[1], w->h[2], w->dz[0], w->dz[1], w->dz[2], w->g);
+    return 0;
+}
+}  // namespace
+
+int mm_mlp_grads(mm_mlp *w, int64_t n, const uint16_t *X, const float *target, float *grad_host,
+                 double *loss_host) {
+    if (!w || !X || !target || !grad_host) return fail(MM_E_ARG, "null argument");
+    if (n <= 0 || n > w->max_batch) return fail(MM_E_ARG, "batch size outside (0, max_batch]");
+    int rc = mlp_grads(w, n, X, target);
+    if (rc) return rc;
+    HIPC(hipMemcpyAsync(grad_host, w->g, kMlpParams * sizeof(float), hipMemcpyDeviceToHost, w->ctx->s));
+    double l = 0;
+    HIPC(hipMemcpyAsync(&l, w->loss, sizeof(double), hipMemcpyDeviceToHost, w->ctx->s));
+    HIPC(hipStreamSynchronize(w->ctx->s));
+    if (loss_host) *loss_host = l;
+    return 0;
+}
+
+int mm_mlp_train_step(mm_mlp *w, int64_t n, const uint16_t *X, const float *target, float lr, double *loss_host) {
+    if (!w || !X || !target) return fail(MM_E_ARG, "null argument");
+    if (n <= 0 || n > w->max_batch) return fail(MM_E_ARG, "batch size outside (0, max_batch]");
+    int rc = mlp_grads(w, n, X, target);
+    if (rc) return rc;
+    mm_ctx *c = w->ctx;
+    launch_mlp_adam(c->L(), w->p, w->g, w->m1, w->m2, lr, ++w->step);
+    launch_mlp_publish(c->L(), w->p, w->w1t, w->w2t, w->w3t, w->w2n, w->w3n);
+    c->ws_have_prev = false;
+    double l = 0.0;
+    HIPC(hipMemcpyAsync(&l, w->loss, sizeof(double), hipMemcpyDeviceToHost, c->s));
+    // the batch buffers are the caller's: return only when the step has read them
+    HIPC(hipStreamSynchronize(c->s));
+    if (loss_host) *loss_host = l;
+    return 0;
+}
+
+int mm_warm_start(mm_ctx *c, mm_mlp *w, double scale) {
+    if (!c) return fail(MM_E_ARG, "null context");
+    if (w && !c->structured) return fail(MM_E_STATE, "the learned warm start needs a structured (Kuhn grid) context");
+    if (w && w->ctx != c) return fail(MM_E_ARG, "the network belongs to another context");
+    if (w && !c->ws_X) {
+        if (c->alloc(&c->ws_X, (size_t)c->N * kMlpIn) || c->alloc(&c->ws_pred, (size_t)c->N) ||
+            c->alloc(&c->ws_prev, (size_t)c->N))
+            return MM_E_HIP;
+    }
+    c->ws_net = w;
+    c->ws_scale = scale;
+    c->ws_have_prev = false;
+    c->ws_calls = 0;
+    return 0;
+}
+
+int mm_warm_start_stats(mm_ctx *c, int64_t *calls, double *last_gamma) {
+    if (!c) return fail(MM_E_ARG, "null context");
+    if (calls) *calls = c->ws_calls;
+    if (last_gamma) {
+        *last_gamma = 0.0;
+        if (c->ws_red) {
+            HIPC(hipMemcpyAsync(last_gamma, c->ws_red + 7, sizeof(double), hipMemcpyDeviceToHost, c->s));
+            HIPC(hipStreamSynchronize(c->s));
+        }
+    }
     return 0;
 }
 

@@ -17,6 +17,7 @@ MM_PREC_FP64 = 0
 MM_PREC_MIXED = 1
 
 c_double_p = ctypes.POINTER(ctypes.c_double)
+c_float_p = ctypes.POINTER(ctypes.c_float)
 c_int32_p = ctypes.POINTER(ctypes.c_int32)
 c_uint8_p = ctypes.POINTER(ctypes.c_uint8)
 
@@ -101,7 +102,20 @@ SIGNATURES = {
     "mm_gradient": [VP, VP, VP],
     "mm_timing_enable": [VP, ctypes.c_int32],
     "mm_timing_get": [VP, ctypes.POINTER(KernelTiming)],
+    "mm_mlp_create": [VP, ctypes.c_int64, ctypes.c_uint64, ctypes.POINTER(VP)],
+    "mm_mlp_destroy": [VP],
+    "mm_mlp_set_params": [VP, c_float_p],
+    "mm_mlp_get_params": [VP, c_float_p],
+    "mm_mlp_features": [VP, VP, VP],
+    "mm_mlp_forward": [VP, ctypes.c_int64, VP, VP],
+    "mm_mlp_grads": [VP, ctypes.c_int64, VP, VP, c_float_p, c_double_p],
+    "mm_mlp_train_step": [VP, ctypes.c_int64, VP, VP, ctypes.c_float, c_double_p],
+    "mm_warm_start": [VP, VP, ctypes.c_double],
+    "mm_warm_start_stats": [VP, ctypes.POINTER(ctypes.c_int64), c_double_p],
 }
+
+MLP_IN = 32
+MLP_PARAMS = 37377
 
 _lib = None
 

@@ -162,7 +162,16 @@ class Model:
         tags = sorted(int(t) for t in np.unique(etags))
         return cls(coords, tets, etags, read_krn(base + ".krn", tags), **kw)
 
+    def _attach(self, net):
+        """Networks created on this context are destroyed before it."""
+        if not hasattr(self, "_nets"):
+            self._nets = []
+        self._nets.append(net)
+
     def close(self):
+        for net in getattr(self, "_nets", []):
+            net.close()
+        self._nets = []
         if getattr(self, "_ctx", None) and self._ctx.value:
             self._lib.mm_ctx_destroy(self._ctx)
             self._ctx = ctypes.c_void_p()
@@ -357,6 +366,19 @@ class Model:
         self._sync()
         _lib.check(self._lib.mm_gradient(self._ctx, _ptr(u, self.N), _ptr(g, 3 * self.N)), "mm_gradient")
         return g
+
+    def warm_start(self, net=None):
+        """Enable the learned warm start of every potential solve with ``net``
+        (``learned.Network``), or disable it (None)."""
+        _lib.check(self._lib.mm_warm_start(self._ctx, None if net is None else net._net,
+                                           1.0 if net is None else float(net.scale)), "mm_warm_start")
+
+    def warm_start_stats(self):
+        calls = ctypes.c_int64()
+        gamma = ctypes.c_double()
+        _lib.check(self._lib.mm_warm_start_stats(self._ctx, ctypes.byref(calls), ctypes.byref(gamma)),
+                   "mm_warm_start_stats")
+        return dict(calls=calls.value, last_gamma=gamma.value)
 
     def timing(self, on=True):
         _lib.check(self._lib.mm_timing_enable(self._ctx, int(on)), "mm_timing_enable")
