@@ -98,7 +98,7 @@ def start_cpu_leg(cfg, seed):
 
 
 # --------------------------------------------------------------------------- GPU leg
-def gpu_run(args, precision, traffic_probe=False):
+def gpu_run(args, precision, traffic_probe=False, learned=None):
     """Warm-up and timed field steps of one form; returns throughput, counters, kernel
     timing and the per-step records (M.h, E) plus the final potential and demag field."""
     import numpy as np
@@ -114,7 +114,13 @@ def gpu_run(args, precision, traffic_probe=False):
     t0 = time.perf_counter()
     model = Model.from_structured(g, structured=not args.general, precision=precision,
                                   inner_tol=args.inner_tol)
-    log(f"[bench] {precision}: context {model.info} ({time.perf_counter() - t0:.1f} s)")
+    tag = precision + (" + learned" if learned else "")
+    if learned:
+        from mammos_mumag_amd.learned import Network
+        net = Network(model, max_batch=32, seed=0)
+        net.load(learned)
+        model.warm_start(net)
+    log(f"[bench] {tag}: context {model.info} ({time.perf_counter() - t0:.1f} s)")
     p = loop_params(args.config)
     m = model.to_device(model.uniform_m(p["m"]))
     m_ref = m.clone()
@@ -125,7 +131,7 @@ def gpu_run(args, precision, traffic_probe=False):
     for w in range(args.warmup):
         t1 = time.perf_counter()
         E = model.minimize(m, hext, p["h"], p, m_ref, u, gF, st)
-        log(f"[bench] {precision}: warmup step {w}: h={hext:.2f} T E={E:.6e} {time.perf_counter() - t1:.3f} s "
+        log(f"[bench] {tag}: warmup step {w}: h={hext:.2f} T E={E:.6e} {time.perf_counter() - t1:.3f} s "
             f"{st.as_dict()}")
         hext += p["hstep"]
     st = _lib.Stats()
@@ -141,7 +147,7 @@ def gpu_run(args, precision, traffic_probe=False):
     timing = model.timing_get()
     model.timing(False)
     mh = model.compute_mh(m, p["h"])
-    log(f"[bench] {precision}: timed {args.steps} steps: {elapsed:.3f} s, last h={hext - p['hstep']:.2f} T, "
+    log(f"[bench] {tag}: timed {args.steps} steps: {elapsed:.3f} s, last h={hext - p['hstep']:.2f} T, "
         f"M.h={mh:.6f} T, E={energies[-1]:.6e}, {st.as_dict()}, {timing}")
     if traffic_probe:
         # reference dispatch for the HBM byte counters: a device copy of 512 MiB
@@ -152,7 +158,8 @@ def gpu_run(args, precision, traffic_probe=False):
         dst.copy_(src)
         torch.cuda.synchronize()
         log("[bench] traffic probe: copied 512 MiB")
-    res = dict(elapsed=elapsed, stats=st.as_dict(), timing=timing, info=model.info,
+    ws = model.warm_start_stats() if learned else None
+    res = dict(elapsed=elapsed, stats=st.as_dict(), timing=timing, info=model.info, warm_start=ws,
                n_nodes=n_nodes, n_elems=n_elems, n_mag_nodes=int(np.sum(model.meas > 0)),
                mh=mh, energies=energies, u=u.cpu().numpy(), hd=model.gradient(u).cpu().numpy())
     model.close()
@@ -209,6 +216,9 @@ def main():
     ap.add_argument("--config", default="voronoi64_250nm")
     ap.add_argument("--precision", default="mixed", choices=["fp64", "mixed"],
                     help="form whose throughput is the value (mixed: FP32 inner / FP64 outer)")
+    ap.add_argument("--learned", default="",
+                    help="trained warm-start network (.npz); default: the package's network for --config "
+                         "when present; 'none' disables")
     ap.add_argument("--no-compare", action="store_true",
                     help="skip the plain FP64 run (speedup and errors of the form against it)")
     ap.add_argument("--seed", type=int, default=1000)
@@ -229,9 +239,20 @@ def main():
     cpu = None
     if not args.no_cpu_baseline:
         cpu = start_cpu_leg(args.config, args.seed)
-    res = gpu_run(args, args.precision, traffic_probe=args.traffic_probe)
+    learned = None
+    if args.learned != "none":
+        cand = pathlib.Path(args.learned) if args.learned else (
+            ROOT / "mammos-mumag_amd" / "mammos_mumag_amd" / "data" / f"warmstart_{args.config}.npz")
+        if cand.exists() and not args.general:
+            learned = str(cand)
+        elif args.learned:
+            raise SystemExit(f"bench.py: trained warm-start network {cand} not found")
+    res = gpu_run(args, args.precision, traffic_probe=args.traffic_probe, learned=learned)
+    ablate = None
+    if learned and not args.no_compare:
+        ablate = gpu_run(args, args.precision)   # the same form without the network
     ref = None
-    if args.precision != "fp64" and not args.no_compare:
+    if (args.precision != "fp64" or learned) and not args.no_compare:
         ref = gpu_run(args, "fp64")
     steps = args.steps
     value = steps / res["elapsed"]
@@ -270,8 +291,14 @@ def main():
                                f"{res['n_mag_nodes']} magnet nodes / {res['n_nodes']} nodes / "
                                f"{res['n_elems']} tets, field steps from {cfgd['hstart']:+g} T by {cfgd['hstep']:g} T",
                    "precision": args.precision,
-                   "form": ("FP32 inner solve + FP64 outer (no learned part)" if args.precision == "mixed"
-                            else "plain FP64"),
+                   "form": (("learned warm start + " if learned else "")
+                            + ("FP32 inner solve + FP64 outer" if args.precision == "mixed" else "FP64")),
+                   "learned": (None if not learned else dict(
+                       network=os.path.relpath(learned, ROOT), warm_started_solves=res["warm_start"]["calls"],
+                       without_network=None if ablate is None else dict(
+                           value=round(steps / ablate["elapsed"], 6),
+                           pcg_iters_per_step=round(ablate["stats"]["pcg_iters"] / steps, 2),
+                           gain=round(ablate["elapsed"] / res["elapsed"], 4)))),
                    "per_step": {k: round(v / steps, 2) for k, v in res["stats"].items()}},
         "vs_fp64": (None if ref is None else dict(
             fp64_value=round(steps / ref["elapsed"], 6), speedup=round(ref["elapsed"] / res["elapsed"], 4),

@@ -161,3 +161,54 @@ def train(net, X, u, steps=2000, batch=16384, lr=1e-3, lr_min=1e-5, seed=0, log=
             if log:
                 log(f"[train] step {s}: loss {loss:.4e} (lr {lr_s:.2e}, {time.perf_counter() - t0:.1f} s)")
     return hist
+
+
+def train_for_config(config, seeds, steps, hstep, nodes_per_step, train_steps, batch, lr, out, log=print):
+    """Train the warm-start network for a generated workload: (m, u) pairs from
+    plain-FP64 loops of the microstructures ``seeds`` (none of them the evaluation
+    seed), then Adam on the GPU; saves parameters, output scale and provenance."""
+    torch = _torch()
+    from .generate import build_config, loop_params
+    from .model import Model
+    p = loop_params(config)
+    Xs, us = [], []
+    t0 = time.perf_counter()
+    for sd in seeds:
+        g = build_config(config, seed=sd)
+        model = Model.from_structured(g)
+        X, u = collect(model, p, steps, nodes_per_step, seed=sd, hstep=hstep)
+        Xs.append(X)
+        us.append(u)
+        model.close()
+        log(f"[learned] seed {sd}: {X.shape[0]} samples ({time.perf_counter() - t0:.1f} s)")
+    X = torch.cat(Xs)
+    u = torch.cat(us)
+    g = build_config(config, seed=seeds[0])
+    model = Model.from_structured(g)
+    net = Network(model, max_batch=batch, seed=0)
+    hist = train(net, X, u, steps=train_steps, batch=batch, lr=lr, log=log, log_every=max(train_steps // 10, 1))
+    net.save(out, config=config, seeds=np.asarray(seeds), steps=steps, hstep=hstep,
+             nodes_per_step=nodes_per_step, train_steps=train_steps, batch=batch, lr=lr,
+             loss_first=hist[0][1], loss_last=hist[-1][1])
+    log(f"[learned] saved {out}: loss {hist[0][1]:.3e} -> {hist[-1][1]:.3e} ({time.perf_counter() - t0:.1f} s)")
+    model.close()
+    return hist
+
+
+if __name__ == "__main__":
+    import argparse
+    import pathlib
+    ap = argparse.ArgumentParser(description="train the learned warm start for a generated workload")
+    ap.add_argument("--config", default="voronoi64_250nm")
+    ap.add_argument("--seeds", type=int, nargs="+", default=[2001, 2002, 2003, 2004])
+    ap.add_argument("--steps", type=int, default=40, help="field steps per loop")
+    ap.add_argument("--hstep", type=float, default=-0.4, help="field step (T) of the training loops")
+    ap.add_argument("--nodes", type=int, default=131072, help="sampled nodes per field step")
+    ap.add_argument("--train-steps", type=int, default=3000)
+    ap.add_argument("--batch", type=int, default=32768)
+    ap.add_argument("--lr", type=float, default=1e-3)
+    ap.add_argument("--out", default=None)
+    a = ap.parse_args()
+    out = a.out or str(pathlib.Path(__file__).resolve().parent / "data" / f"warmstart_{a.config}.npz")
+    pathlib.Path(out).parent.mkdir(exist_ok=True)
+    train_for_config(a.config, a.seeds, a.steps, a.hstep, a.nodes, a.train_steps, a.batch, a.lr, out)
