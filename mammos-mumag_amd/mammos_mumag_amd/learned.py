@@ -163,7 +163,8 @@ def train(net, X, u, steps=2000, batch=16384, lr=1e-3, lr_min=1e-5, seed=0, log=
     return hist
 
 
-def train_for_config(config, seeds, steps, hstep, nodes_per_step, train_steps, batch, lr, out, log=print):
+def train_for_config(config, seeds, steps, hstep, nodes_per_step, train_steps, batch, lr, out,
+                     log=lambda msg: print(msg, flush=True)):
     """Train the warm-start network for a generated workload: (m, u) pairs from
     plain-FP64 loops of the microstructures ``seeds`` (none of them the evaluation
     seed), then Adam on the GPU; saves parameters, output scale and provenance."""
@@ -176,7 +177,8 @@ def train_for_config(config, seeds, steps, hstep, nodes_per_step, train_steps, b
     for sd in seeds:
         g = build_config(config, seed=sd)
         model = Model.from_structured(g)
-        X, u = collect(model, p, steps, nodes_per_step, seed=sd, hstep=hstep)
+        X, u = collect(model, p, steps, nodes_per_step, seed=sd, hstep=hstep,
+                       log=lambda msg: log(f"{msg} (seed {sd}, {time.perf_counter() - t0:.1f} s)"))
         Xs.append(X)
         us.append(u)
         model.close()
@@ -200,9 +202,9 @@ if __name__ == "__main__":
     import pathlib
     ap = argparse.ArgumentParser(description="train the learned warm start for a generated workload")
     ap.add_argument("--config", default="voronoi64_250nm")
-    ap.add_argument("--seeds", type=int, nargs="+", default=[2001, 2002, 2003, 2004])
-    ap.add_argument("--steps", type=int, default=40, help="field steps per loop")
-    ap.add_argument("--hstep", type=float, default=-0.4, help="field step (T) of the training loops")
+    ap.add_argument("--seeds", type=int, nargs="+", default=[2001, 2002, 2003])
+    ap.add_argument("--steps", type=int, default=24, help="field steps per loop")
+    ap.add_argument("--hstep", type=float, default=-0.667, help="field step (T) of the training loops")
     ap.add_argument("--nodes", type=int, default=131072, help="sampled nodes per field step")
     ap.add_argument("--train-steps", type=int, default=3000)
     ap.add_argument("--batch", type=int, default=32768)
