@@ -98,7 +98,7 @@ def start_cpu_leg(cfg, seed):
 
 
 # --------------------------------------------------------------------------- GPU leg
-def gpu_run(args, precision, traffic_probe=False, learned=None):
+def gpu_run(args, precision, traffic_probe=False, learned=None, multigrid=0):
     """Warm-up and timed field steps of one form; returns throughput, counters, kernel
     timing and the per-step records (M.h, E) plus the final potential and demag field."""
     import numpy as np
@@ -114,7 +114,9 @@ def gpu_run(args, precision, traffic_probe=False, learned=None):
     t0 = time.perf_counter()
     model = Model.from_structured(g, structured=not args.general, precision=precision,
                                   inner_tol=args.inner_tol)
-    tag = precision + (" + learned" if learned else "")
+    tag = precision + (f" + multigrid ({multigrid} levels)" if multigrid else "") + (" + learned" if learned else "")
+    if multigrid:
+        model.multigrid(multigrid)
     if learned:
         from mammos_mumag_amd.learned import Network
         net = Network(model, max_batch=32, seed=0)
@@ -216,6 +218,8 @@ def main():
     ap.add_argument("--config", default="voronoi64_250nm")
     ap.add_argument("--precision", default="mixed", choices=["fp64", "mixed"],
                     help="form whose throughput is the value (mixed: FP32 inner / FP64 outer)")
+    ap.add_argument("--multigrid", type=int, default=4,
+                    help="levels of the multigrid preconditioner of the FP32 inner solve (mixed form; 0: Jacobi)")
     ap.add_argument("--learned", default="",
                     help="trained warm-start network (.npz); default: the package's network for --config "
                          "when present; 'none' disables")
@@ -247,10 +251,11 @@ def main():
             learned = str(cand)
         elif args.learned:
             raise SystemExit(f"bench.py: trained warm-start network {cand} not found")
-    res = gpu_run(args, args.precision, traffic_probe=args.traffic_probe, learned=learned)
+    mgl = args.multigrid if args.precision == "mixed" and not args.general else 0
+    res = gpu_run(args, args.precision, traffic_probe=args.traffic_probe, learned=learned, multigrid=mgl)
     ablate = None
     if learned and not args.no_compare:
-        ablate = gpu_run(args, args.precision)   # the same form without the network
+        ablate = gpu_run(args, args.precision, multigrid=mgl)   # the same form without the network
     ref = None
     if (args.precision != "fp64" or learned) and not args.no_compare:
         ref = gpu_run(args, "fp64")
@@ -292,7 +297,8 @@ def main():
                                f"{res['n_elems']} tets, field steps from {cfgd['hstart']:+g} T by {cfgd['hstep']:g} T",
                    "precision": args.precision,
                    "form": (("learned warm start + " if learned else "")
-                            + ("FP32 inner solve + FP64 outer" if args.precision == "mixed" else "FP64")),
+                            + ("FP32 inner solve" + (f" (multigrid preconditioner, {mgl} levels)" if mgl else "")
+                               + " + FP64 outer" if args.precision == "mixed" else "FP64")),
                    "learned": (None if not learned else dict(
                        network=os.path.relpath(learned, ROOT), warm_started_solves=res["warm_start"]["calls"],
                        without_network=None if ablate is None else dict(

@@ -10,8 +10,8 @@
  *    unless the name ends in _host.  Nodal 3-vectors are interleaved f64[3N]
  *    ([3i+a] = component a of node i), scalars f64[N], in mesh node order.
  *  - The context owns the operators and all workspace; no entry point allocates after
- *    mm_ctx_create, except the set-up calls of the learned warm start (mm_mlp_create,
- *    and mm_warm_start the first time it enables a network on a context).  One HIP stream per context; calls return after the stream is
+ *    mm_ctx_create, except the set-up calls of the accelerated forms (mm_multigrid,
+ *    mm_mlp_create, and mm_warm_start the first time it enables a network).  One HIP stream per context; calls return after the stream is
  *    synchronised whenever a scalar is returned to the host.  A context is not
  *    thread-safe.
  *  - mm_last_error() returns a message for the last failure on the calling thread.
@@ -174,6 +174,15 @@ int mm_apply_C(mm_ctx *ctx, const double *v, double *y);
 int mm_apply_K(mm_ctx *ctx, const double *x, double *y);
 int mm_divergence(mm_ctx *ctx, const double *m, double *b);
 int mm_gradient(mm_ctx *ctx, const double *u, double *g);
+
+/* Multigrid preconditioner for the FP32 inner solve of a structured MM_PREC_MIXED context
+ * (no counterpart in the reference, whose potential solve is Jacobi-preconditioned,
+ * solvers.py:222-231): a geometric V-cycle of up to max_levels levels (2..8; 0 restores
+ * Jacobi) replaces z = r / diag(K) inside the reliable-update PCG.  The stop test on the
+ * FP64 true residual is unchanged, so only the iteration count changes. */
+int mm_multigrid(mm_ctx *ctx, int32_t max_levels);
+/* z = M r with the multigrid V-cycle (device f32[N]; r zero on the Dirichlet faces). */
+int mm_mg_apply(mm_ctx *ctx, const float *r, float *z);
 
 /* ---- Learned warm start (no counterpart in the reference: the learned part of the
  * chosen form, for callers that run many potential solves on one structured grid).

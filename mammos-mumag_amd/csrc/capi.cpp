@@ -16,6 +16,7 @@
 #include "../../include/mumag_hip.h"
 #include "assemble.h"
 #include "common.h"
+#include "mg.h"
 #include "mlp.h"
 #include "structured.h"
 
@@ -53,6 +54,7 @@ struct mm_ctx {
     double V = 0.0;
     int pcg_batch = 16;
     double ru_delta = 0.1;   // mixed precision: residual reduction between reliable updates
+    bool ru_delta_set = false;  // given by the caller (mm_options.inner_tol)
     int64_t k_nnz = 0, k_entries = 0;
     size_t bytes = 0;
     std::vector<void *> allocs;
@@ -84,6 +86,16 @@ struct mm_ctx {
     // event timing of the PCG kernels (K apply and vector update)
     bool timing = false;
     KernelTimer t_spmv, t_upd;
+
+    // multigrid preconditioner of the FP32 inner solve (structured grids)
+    std::vector<double> axis[3];        // node coordinates per axis (host)
+    int mg_nl = 0;                      // levels (0: Jacobi)
+    int mg_batch = 4;                   // inner iterations between host checks with multigrid
+    int mg_deg = 3, mg_cdeg = 6;        // smoothing degree, coarsest-level polynomial degree
+    MgLevelDev mg[kMgMaxLevels];
+    float *mg_b[kMgMaxLevels] = {}, *mg_xa[kMgMaxLevels] = {}, *mg_xb[kMgMaxLevels] = {},
+          *mg_r[kMgMaxLevels] = {};
+    float mg_om[16] = {}, mg_omc[16] = {};
 
     // learned warm start (structured grids): network, features, predictions
     FeatGrid F;
@@ -266,7 +278,7 @@ void pcg_iteration(mm_ctx *c, int i, int mode, double tol, double atol, int maxi
     bb.parts_in = c->partsA;
     bb.nparts_in = nA;
     bb.parts_out = c->partsB;
-    *nB = launch_kb<T>(timed(c, c->t_upd, i), c->N, bb, x, r, z, pnew, q, invD);
+    *nB = launch_kb<T>(timed(c, c->t_upd, i), c->N, bb, x, r, invD ? z : nullptr, pnew, q, invD);
 }
 
 // jax.scipy.sparse.linalg.cg semantics, FP64; x is updated in place.  Batches of
@@ -295,6 +307,47 @@ int pcg64(mm_ctx *c, const double *b, double *x, double tol, double atol, int ma
     return 0;
 }
 
+// Multigrid V-cycle z = M r on the FP32 vectors of the inner solve (mg.hip); skipped
+// entirely once *skip is set.  Level 0 works in z and tmp32 (ping-pong, chosen so that
+// the result ends in z), residual in tmp32b.
+void mg_vcycle(mm_ctx *c, const float *r, float *z, const int *skip) {
+    Launch L = c->L();
+    const int nl = c->mg_nl, s = c->mg_deg, cd = c->mg_cdeg;
+    float *cur[kMgMaxLevels], *oth[kMgMaxLevels];
+    const float *B[kMgMaxLevels];
+    for (int l = 0; l < nl; ++l) {
+        B[l] = l == 0 ? r : c->mg_b[l];
+        cur[l] = c->mg_xa[l];
+        oth[l] = c->mg_xb[l];
+    }
+    // level 0 works in z and its own buffer: it makes 2 s - 1 swaps (s - 1 before and s
+    // after the coarse correction), or cd - 1 when it is the only level; start in the
+    // buffer that ends as z
+    const int swaps0 = nl > 1 ? 2 * s - 1 : cd - 1;
+    cur[0] = (swaps0 & 1) ? c->mg_xb[0] : z;
+    oth[0] = (swaps0 & 1) ? z : c->mg_xb[0];
+    auto smooth_down = [&](int l, const float *om, int deg) {
+        launch_mg_first(L, c->mg[l].P, B[l], cur[l], om[0], skip);
+        for (int k = 1; k < deg; ++k) {
+            launch_mg_smooth(L, c->mg[l].P, B[l], cur[l], oth[l], om[k], skip);
+            std::swap(cur[l], oth[l]);
+        }
+    };
+    for (int l = 0; l + 1 < nl; ++l) {
+        smooth_down(l, c->mg_om, s);
+        launch_mg_resid(L, c->mg[l].P, B[l], cur[l], c->mg_r[l], skip);
+        launch_mg_restrict(L, c->mg[l], c->mg[l + 1], c->mg_r[l], c->mg_b[l + 1], skip);
+    }
+    smooth_down(nl - 1, c->mg_omc, cd);
+    for (int l = nl - 2; l >= 0; --l) {
+        launch_mg_prolong(L, c->mg[l], c->mg[l + 1], cur[l + 1], cur[l], skip);
+        for (int k = s - 1; k >= 0; --k) {
+            launch_mg_smooth(L, c->mg[l].P, B[l], cur[l], oth[l], c->mg_om[k], skip);
+            std::swap(cur[l], oth[l]);
+        }
+    }
+}
+
 // Mixed precision with reliable updates: the Jacobi-PCG recursion (alpha, beta, r, z, p
 // and the correction dx) runs in FP32 with FP64 dot products; whenever the FP32
 // recursive residual has dropped by ru_delta since the last replacement, or claims
@@ -313,15 +366,27 @@ int pcg_mixed(mm_ctx *c, const double *b, double *x, double tol, double atol, in
     if (rc) return rc;
     float *const pb[2] = {c->p32, c->p32b};
     int i = 0, nB = 0, mode = PCG_RESTART;
+    const bool mg = c->mg_nl > 0;
+    int *const done = &c->dsc->pcg_done;
+    if (mg) {   // z = M r for the initial true residual
+        mg_vcycle(c, c->r32, c->z32, done);
+        nB = launch_dot2<float>(L, n, c->r32, c->z32, c->partsB);
+        mode = PCG_RESTART_PARTS;
+    }
     while (!c->hsc->ir_done) {
         // FP32 iterations until the recursive residual has dropped by ru_delta since the
         // last true residual (checked between batches) or claims convergence
         const double rmax2 = c->hsc->ir_rmax2;
         const int seg0 = i;
         for (;;) {
-            for (int k = 0; k < c->pcg_batch; ++k, ++i) {
+            const int batch = mg ? c->mg_batch : c->pcg_batch;
+            for (int k = 0; k < batch; ++k, ++i) {
                 pcg_iteration<float>(c, i, mode, tol, atol, maxiter, c->z32, pb, c->q32, c->dx32, c->r32,
-                                     c->invDK32, c->K32, c->P32, &nB);
+                                     mg ? nullptr : c->invDK32, c->K32, c->P32, &nB);
+                if (mg) {
+                    mg_vcycle(c, c->r32, c->z32, done);
+                    nB = launch_dot2<float>(L, n, c->r32, c->z32, c->partsB);
+                }
                 mode = PCG_NORMAL;
             }
             if ((rc = sync_scalars(c))) return rc;
@@ -341,6 +406,11 @@ int pcg_mixed(mm_ctx *c, const double *b, double *x, double tol, double atol, in
         if ((rc = sync_scalars(c))) return rc;
         if (!c->hsc->ir_done && i >= maxiter) break;
         mode = PCG_KEEP;
+        if (mg) {   // z = M r for the replaced residual; the direction continues (beta from it)
+            mg_vcycle(c, c->r32, c->z32, done);
+            nB = launch_dot2<float>(L, n, c->r32, c->z32, c->partsB);
+            mode = PCG_NORMAL;
+        }
     }
     if (iters) *iters = i;
     if (!c->hsc->ir_done) return MM_W_MAXITER;
@@ -494,7 +564,10 @@ int mm_ctx_create(const mm_mesh_desc *d, const mm_options *opts, mm_ctx **out) {
     c->V = structured ? S.V : H.V;
     c->Nm = structured ? S.n_mag_nodes : H.n_mag_nodes;
     if (o.pcg_batch > 0) c->pcg_batch = o.pcg_batch;
-    if (o.inner_tol > 0) c->ru_delta = o.inner_tol;
+    if (o.inner_tol > 0) {
+        c->ru_delta = o.inner_tol;
+        c->ru_delta_set = true;
+    }
     int rc = 0;
     auto bail = [&](int code) {
         delete c;
@@ -566,6 +639,9 @@ int mm_ctx_create(const mm_mesh_desc *d, const mm_options *opts, mm_ctx **out) {
         {
             std::vector<double> ax(d->grid_x, d->grid_x + S.nx), ay(d->grid_y, d->grid_y + S.ny),
                 az(d->grid_z, d->grid_z + S.nz);
+            c->axis[0] = ax;
+            c->axis[1] = ay;
+            c->axis[2] = az;
             double *dx, *dy, *dz;
             if ((rc = c->upload(&dx, ax)) || (rc = c->upload(&dy, ay)) || (rc = c->upload(&dz, az))) return bail(rc);
             double half = 0.0, jsmax = 0.0;
@@ -961,6 +1037,139 @@ int mm_minimize(mm_ctx *c, double *m_inout, double hext, const double hdir[3],
     c->X1 = X1;
     c->Gp = G;
     c->G1 = G1;
+    return 0;
+}
+
+// ---------------------------------------------------------------- multigrid
+
+namespace {
+// Coarse node set of an axis (even indices and the last) and its transfers: per fine
+// index the coarse interval (c0, weight t of c0 + 1); per coarse index up to 3 (fine
+// index, weight) pairs of the transposed interpolation.
+struct AxisTransfer {
+    std::vector<double> coarse;
+    std::vector<int> pc, ri;
+    std::vector<float> pt, rw;
+};
+AxisTransfer coarsen_axis(const std::vector<double> &xf) {
+    AxisTransfer t;
+    const int nf = (int)xf.size();
+    std::vector<int> keep;
+    for (int i = 0; i < nf; i += 2) keep.push_back(i);
+    if (keep.back() != nf - 1) keep.push_back(nf - 1);
+    const int nc = (int)keep.size();
+    for (int k : keep) t.coarse.push_back(xf[k]);
+    t.pc.assign(nf, 0);
+    t.pt.assign(nf, 0.f);
+    t.ri.assign(3 * nc, 0);
+    t.rw.assign(3 * nc, 0.f);
+    std::vector<int> fill(nc, 0);
+    int c0 = 0;
+    for (int i = 0; i < nf; ++i) {
+        while (c0 + 1 < nc && keep[c0 + 1] <= i) ++c0;
+        const double tt = keep[c0] == i ? 0.0 : (xf[i] - t.coarse[c0]) / (t.coarse[c0 + 1] - t.coarse[c0]);
+        t.pc[i] = c0;
+        t.pt[i] = (float)tt;
+        // restriction: fine i feeds coarse c0 (1 - t) and c0 + 1 (t)
+        const double w[2] = {1.0 - tt, tt};
+        for (int q = 0; q < 2; ++q) {
+            if (w[q] == 0.0) continue;
+            const int C = c0 + q;
+            t.ri[3 * C + fill[C]] = i;
+            t.rw[3 * C + fill[C]] = (float)w[q];
+            ++fill[C];
+        }
+    }
+    return t;
+}
+}  // namespace
+
+int mm_multigrid(mm_ctx *c, int32_t max_levels) {
+    if (!c) return fail(MM_E_ARG, "null context");
+    if (max_levels == 0) {
+        c->mg_nl = 0;
+        return 0;
+    }
+    if (!c->structured || c->precision != MM_PREC_MIXED)
+        return fail(MM_E_STATE, "the multigrid preconditioner needs a structured mixed-precision context");
+    if (max_levels < 2 || max_levels > kMgMaxLevels) return fail(MM_E_ARG, "levels outside [2, 8]");
+    int rc = 0;
+    std::vector<double> ax[3] = {c->axis[0], c->axis[1], c->axis[2]};
+    int nl = 0;
+    c->mg[0].P = c->P32;
+    for (int l = 0;; ++l) {
+        const int64_t n = (int64_t)ax[0].size() * ax[1].size() * ax[2].size();
+        if (l > 0) {
+            if ((rc = c->alloc(&c->mg_b[l], (size_t)n)) || (rc = c->alloc(&c->mg_xa[l], (size_t)n)) ||
+                (rc = c->alloc(&c->mg_xb[l], (size_t)n)))
+                return rc;
+        } else if (!c->mg_xb[0]) {
+            if ((rc = c->alloc(&c->mg_xb[0], (size_t)n))) return rc;
+        }
+        nl = l + 1;
+        const size_t mx = std::max(ax[0].size(), std::max(ax[1].size(), ax[2].size()));
+        if (mx <= 24 || nl == max_levels) break;
+        // transfers to level l + 1 and its operator
+        AxisTransfer t[3] = {coarsen_axis(ax[0]), coarsen_axis(ax[1]), coarsen_axis(ax[2])};
+        for (int a = 0; a < 3; ++a) {   // host-side bounds of every index the kernels use
+            const int nf = (int)ax[a].size(), nc = (int)t[a].coarse.size();
+            for (int i = 0; i < nf; ++i)
+                if (t[a].pc[i] < 0 || t[a].pc[i] >= nc || (t[a].pt[i] != 0.f && t[a].pc[i] + 1 >= nc))
+                    return fail(MM_E_STATE, "multigrid: prolongation index out of range");
+            for (int q = 0; q < 3 * nc; ++q)
+                if (t[a].ri[q] < 0 || t[a].ri[q] >= nf) return fail(MM_E_STATE, "multigrid: restriction index out of range");
+        }
+        MgLevelDev &f = c->mg[l];
+        int *pci[3], *rii[3];
+        float *ptf[3], *rwf[3];
+        for (int a = 0; a < 3; ++a)
+            if ((rc = c->upload(&pci[a], t[a].pc)) || (rc = c->upload(&ptf[a], t[a].pt)) ||
+                (rc = c->upload(&rii[a], t[a].ri)) || (rc = c->upload(&rwf[a], t[a].rw)))
+                return rc;
+        f.pxc = pci[0]; f.pyc = pci[1]; f.pzc = pci[2];
+        f.pxt = ptf[0]; f.pyt = ptf[1]; f.pzt = ptf[2];
+        f.rxi = rii[0]; f.ryi = rii[1]; f.rzi = rii[2];
+        f.rxw = rwf[0]; f.ryw = rwf[1]; f.rzw = rwf[2];
+        if ((rc = c->alloc(&c->mg_r[l], (size_t)n))) return rc;
+        for (int a = 0; a < 3; ++a) ax[a] = t[a].coarse;
+        PoissonGrid<float> &P = c->mg[l + 1].P;
+        P.nx = (int)ax[0].size();
+        P.ny = (int)ax[1].size();
+        P.nz = (int)ax[2].size();
+        const float **hp[3] = {&P.hxp, &P.hyp, &P.hzp}, **ip[3] = {&P.ixp, &P.iyp, &P.izp};
+        for (int a = 0; a < 3; ++a) {
+            std::vector<float> h(ax[a].size() + 1, 0.f), inv(ax[a].size() + 1, 0.f);
+            for (size_t i = 1; i < ax[a].size(); ++i) {
+                const double d = ax[a][i] - ax[a][i - 1];
+                h[i] = (float)d;
+                inv[i] = (float)(1.0 / (6.0 * d));
+            }
+            float *dh, *di;
+            if ((rc = c->upload(&dh, h)) || (rc = c->upload(&di, inv))) return rc;
+            *hp[a] = dh;
+            *ip[a] = di;
+        }
+    }
+    // Chebyshev-Richardson steps: reciprocal Chebyshev nodes of [lo, 2]
+    auto omegas = [](float *om, int deg, double lo) {
+        const double hi = 2.0;
+        for (int k = 0; k < deg; ++k)
+            om[k] = (float)(1.0 / ((hi + lo) / 2 + (hi - lo) / 2 * std::cos(M_PI * (2 * k + 1) / (2.0 * deg))));
+    };
+    omegas(c->mg_om, c->mg_deg, 2.0 / 30.0);
+    omegas(c->mg_omc, c->mg_cdeg, 2.0 / 300.0);
+    c->mg_nl = nl;
+    // a multigrid-preconditioned FP32 recursion stays accurate over a longer stretch:
+    // replace the residual every 1e-3 of reduction unless the caller chose otherwise
+    if (!c->ru_delta_set) c->ru_delta = 1e-3;
+    return 0;
+}
+
+int mm_mg_apply(mm_ctx *c, const float *r, float *z) {
+    if (!c || !r || !z) return fail(MM_E_ARG, "null argument");
+    if (c->mg_nl == 0) return fail(MM_E_STATE, "no multigrid preconditioner (mm_multigrid)");
+    mg_vcycle(c, r, z, nullptr);
+    HIPC(hipStreamSynchronize(c->s));
     return 0;
 }
 

@@ -113,7 +113,9 @@ struct FinArgs {
 };
 
 // Arguments of one fused PCG iteration (pcg_fused.hip).
-enum PcgMode { PCG_FIRST = 0, PCG_NORMAL = 1, PCG_RESTART = 2, PCG_KEEP = 3 };
+// RESTART_PARTS: p = z with gamma and r.r from the partials (a restart whose z came from a
+// preconditioner other than Jacobi); the stop bound is kept.
+enum PcgMode { PCG_FIRST = 0, PCG_NORMAL = 1, PCG_RESTART = 2, PCG_KEEP = 3, PCG_RESTART_PARTS = 4 };
 struct PcgIterArgs {
     const double *parts_in = nullptr;  // partials of the previous kernel (slots of kMaxPartials)
     int nparts_in = 0;
@@ -159,9 +161,13 @@ int launch_ka_grid(const Launch &L, const PoissonGrid<T> &P, const PcgIterArgs &
 template <typename T>
 int launch_ka_sell(const Launch &L, const SellDev<T> &A, const PcgIterArgs &a, const T *z,
                    const T *pold, T *pnew, T *q);
+// z == nullptr: x and r only (the preconditioner runs after the kernel), no partials.
 template <typename T>
 int launch_kb(const Launch &L, int64_t n, const PcgIterArgs &a, T *x, T *r, T *z, const T *p,
               const T *q, const T *invD);
+// partials (r.z, r.r) of the preconditioned residual, for the next KA; skipped when done.
+template <typename T>
+int launch_dot2(const Launch &L, int64_t n, const T *r, const T *z, double *parts_out);
 void launch_ru_residual(const Launch &L, int64_t n, const double *b, const double *Ax, float *r32,
                         float *z32, float *p32, const float *invD32, double tol, double atol,
                         int maxiter, bool init);

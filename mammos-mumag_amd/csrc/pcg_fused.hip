@@ -93,7 +93,7 @@ template <int NT>
 __device__ double ka_prologue(const PcgIterArgs &a, DevScalars *dsc, double *sh, int *flag, bool *stop) {
     constexpr int NW = NT / kWave;
     if (threadIdx.x == 0) *flag = dsc->pcg_done;
-    const bool from_parts = !(a.mode == PCG_KEEP || a.mode == PCG_RESTART);
+    const bool from_parts = !(a.mode == PCG_KEEP || a.mode == PCG_RESTART);   // FIRST, NORMAL, RESTART_PARTS
     const bool first = a.mode == PCG_FIRST;
     const double gprev = dsc->pcg_gam[(a.iter + 1) & 1];
     const double bound_prev = dsc->pcg_bound;
@@ -225,7 +225,7 @@ void k_ka_grid(PoissonGrid<T> P, PcgIterArgs a, int gx, int gz, const T *__restr
     const uint32_t sz = (uint32_t)nx * (uint32_t)ny * es;  // plane stride in bytes
     const bool in = i < nx && j < ny;
     const bool run = k0 < k1;
-    const bool pz = a.mode == PCG_FIRST || a.mode == PCG_RESTART;  // p = z, else z + beta p_old
+    const bool pz = a.mode == PCG_FIRST || a.mode == PCG_RESTART || a.mode == PCG_RESTART_PARTS;  // p = z, else z + beta p_old
     // halo node of this thread (at most one): south row (ty == 0), north row
     // (ty == TY - 1), west / east columns (ty == 1, lanes [0, TY) and [32, 32 + TY))
     int hi = 0, hj = 0, hly = -1, hlx = 0;
@@ -345,7 +345,7 @@ __global__ __launch_bounds__(kFB) void k_ka_sell(SellDev<T> A, PcgIterArgs a, co
     const double bd = ka_prologue<kFB>(a, dsc, sh, &flag, &stop);
     if (stop) return;
     const T beta = (T)bd;
-    const bool pz = a.mode == PCG_FIRST || a.mode == PCG_RESTART;
+    const bool pz = a.mode == PCG_FIRST || a.mode == PCG_RESTART || a.mode == PCG_RESTART_PARTS;
     auto pv = [&](int c) -> T { return pz ? z[c] : z[c] + beta * pold[c]; };
     const int lane = threadIdx.x & (kWave - 1);
     double dot = 0.0;
@@ -369,8 +369,9 @@ __global__ __launch_bounds__(kFB) void k_ka_sell(SellDev<T> A, PcgIterArgs a, co
     fblock_partials<1, kFB>(v, sh, a.parts_out);
 }
 
-// KB: alpha = gamma_i / (p.q); x += alpha p; r -= alpha q; z = r * invD; (r.z, r.r).
-template <typename T>
+// KB: alpha = gamma_i / (p.q); x += alpha p; r -= alpha q; WZ: z = r * invD and partials
+// (r.z, r.r); !WZ: x and r only (another preconditioner forms z afterwards).
+template <typename T, bool WZ>
 __global__ __launch_bounds__(kFB) void k_kb(int64_t n, PcgIterArgs a, T *__restrict__ x,
                                             T *__restrict__ r, T *__restrict__ z,
                                             const T *__restrict__ p, const T *__restrict__ q,
@@ -383,13 +384,30 @@ __global__ __launch_bounds__(kFB) void k_kb(int64_t n, PcgIterArgs a, T *__restr
     for (int64_t i = blockIdx.x * (int64_t)kFB + threadIdx.x; i < n; i += (int64_t)gridDim.x * kFB) {
         x[i] = x[i] + alpha * p[i];
         const T ri = r[i] - alpha * q[i];
-        const T zi = ri * invD[i];
         r[i] = ri;
-        z[i] = zi;
-        s[0] += (double)ri * (double)zi;
-        s[1] += (double)ri * (double)ri;
+        if (WZ) {
+            const T zi = ri * invD[i];
+            z[i] = zi;
+            s[0] += (double)ri * (double)zi;
+            s[1] += (double)ri * (double)ri;
+        }
     }
-    fblock_partials<2, kFB>(s, sh, a.parts_out);
+    if (WZ) fblock_partials<2, kFB>(s, sh, a.parts_out);
+}
+
+// Partials (r.z, r.r) for the next KA when z came from another preconditioner.
+template <typename T>
+__global__ __launch_bounds__(kFB) void k_dot2(int64_t n, const T *__restrict__ r, const T *__restrict__ z,
+                                              double *parts_out, const DevScalars *dsc) {
+    __shared__ double sh[kFB / kWave];
+    if (dsc->pcg_done) return;
+    double s[2] = {0.0, 0.0};
+    for (int64_t i = blockIdx.x * (int64_t)kFB + threadIdx.x; i < n; i += (int64_t)gridDim.x * kFB) {
+        const double ri = r[i];
+        s[0] += ri * (double)z[i];
+        s[1] += ri * ri;
+    }
+    fblock_partials<2, kFB>(s, sh, parts_out);
 }
 
 // Init: r = b - A x0, z = r * invD; partials (r.z, r.r, b.b) for KA(0) in FIRST mode.
@@ -484,7 +502,17 @@ template <typename T>
 int launch_kb(const Launch &L, int64_t n, const PcgIterArgs &a, T *x, T *r, T *z, const T *p,
               const T *q, const T *invD) {
     const int nb = streaming_blocks(n);
-    launch_timed(L, k_kb<T>, dim3(nb), dim3(kFB), n, a, x, r, z, p, q, invD, L.dsc);
+    if (z)
+        launch_timed(L, k_kb<T, true>, dim3(nb), dim3(kFB), n, a, x, r, z, p, q, invD, L.dsc);
+    else
+        launch_timed(L, k_kb<T, false>, dim3(nb), dim3(kFB), n, a, x, r, z, p, q, invD, L.dsc);
+    return nb;
+}
+
+template <typename T>
+int launch_dot2(const Launch &L, int64_t n, const T *r, const T *z, double *parts_out) {
+    const int nb = streaming_blocks(n);
+    hipLaunchKernelGGL(k_dot2<T>, dim3(nb), dim3(kFB), 0, L.s, n, r, z, parts_out, L.dsc);
     return nb;
 }
 
@@ -500,6 +528,8 @@ template int launch_ka_sell<double>(const Launch &, const SellDev<double> &, con
                                     const double *, const double *, double *, double *);
 template int launch_ka_sell<float>(const Launch &, const SellDev<float> &, const PcgIterArgs &,
                                    const float *, const float *, float *, float *);
+template int launch_dot2<float>(const Launch &, int64_t, const float *, const float *, double *);
+template int launch_dot2<double>(const Launch &, int64_t, const double *, const double *, double *);
 template int launch_kb<double>(const Launch &, int64_t, const PcgIterArgs &, double *, double *, double *,
                                const double *, const double *, const double *);
 template int launch_kb<float>(const Launch &, int64_t, const PcgIterArgs &, float *, float *, float *,

@@ -111,6 +111,8 @@ SIGNATURES = {
     "mm_mlp_grads": [VP, ctypes.c_int64, VP, VP, c_float_p, c_double_p],
     "mm_mlp_train_step": [VP, ctypes.c_int64, VP, VP, ctypes.c_float, c_double_p],
     "mm_warm_start": [VP, VP, ctypes.c_double],
+    "mm_multigrid": [VP, ctypes.c_int32],
+    "mm_mg_apply": [VP, VP, VP],
     "mm_warm_start_stats": [VP, ctypes.POINTER(ctypes.c_int64), c_double_p],
 }
 

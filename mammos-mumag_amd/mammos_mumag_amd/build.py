@@ -14,7 +14,7 @@ INCLUDE = PKG.parent.parent / "include"
 LIBDIR = PKG / "_lib"
 LIB = LIBDIR / "libmumag_hip.so"
 
-SOURCES = ["kernels.hip", "pcg_fused.hip", "mlp.hip", "assemble.cpp", "structured.cpp", "capi.cpp"]
+SOURCES = ["kernels.hip", "pcg_fused.hip", "mlp.hip", "mg.hip", "assemble.cpp", "structured.cpp", "capi.cpp"]
 ARCH = os.environ.get("MUMAG_OFFLOAD_ARCH", "gfx950")
 
 

@@ -367,6 +367,20 @@ class Model:
         _lib.check(self._lib.mm_gradient(self._ctx, _ptr(u, self.N), _ptr(g, 3 * self.N)), "mm_gradient")
         return g
 
+    def multigrid(self, levels=4):
+        """Geometric multigrid preconditioner for the FP32 inner solve (structured mixed
+        contexts); 0 restores Jacobi."""
+        _lib.check(self._lib.mm_multigrid(self._ctx, int(levels)), "mm_multigrid")
+
+    def mg_apply(self, r, z=None):
+        """z = M r with the multigrid V-cycle (float32 device vectors)."""
+        torch = _torch()
+        z = torch.empty(self.N, dtype=torch.float32, device=r.device) if z is None else z
+        self._sync()
+        _lib.check(self._lib.mm_mg_apply(self._ctx, _ptr(r, self.N, "float32"), _ptr(z, self.N, "float32")),
+                   "mm_mg_apply")
+        return z
+
     def warm_start(self, net=None):
         """Enable the learned warm start of every potential solve with ``net``
         (``learned.Network``), or disable it (None)."""

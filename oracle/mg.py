@@ -1,0 +1,123 @@
+"""Multigrid V-cycle of the mixed-precision form's inner solve: CPU restatement of
+mammos-mumag_amd/csrc/mg.hip (FP64) for the GPU checks.  Test infrastructure only.
+
+No counterpart in the reference (its potential solve is Jacobi-preconditioned).  The
+restatement follows the kernels: per-axis coarsening (even node indices and the last),
+the 7-point operator of each level's rectilinear grid with zero on the faces, linear
+interpolation and its transpose, Chebyshev-Richardson smoothing x += w_k (b - A x) / diag
+with w_k the reciprocal Chebyshev nodes of [2/30, 2] (reverse order after the coarse
+correction), the coarsest level a degree-6 polynomial on [2/300, 2].
+
+Pinned by: its own operator against the oracle's assembled K (interior rows), its
+symmetry, and V-cycle properties (exact on the coarsest level's polynomial, linearity);
+tests/test_oracle_mg.py.
+"""
+import numpy as np
+import scipy.sparse as sp
+
+
+def _pad(c):
+    return np.concatenate([[0.0], np.diff(c), [0.0]])
+
+
+def _edge(h1p, h2p, a, b):
+    l1, u1, l2, u2 = h1p[a], h1p[a + 1], h2p[b], h2p[b + 1]
+    return 2 * l1 * l2 + l1 * u2 + u1 * l2 + 2 * u1 * u2
+
+
+def operator(xs, ys, zs):
+    """7-point operator on the interior (faces: identity rows, no couplings), node
+    (i, j, k) = i + nx (j + ny k)."""
+    nx, ny, nz = len(xs), len(ys), len(zs)
+    hx, hy, hz = _pad(xs), _pad(ys), _pad(zs)
+    K_, J_, I_ = np.meshgrid(np.arange(nz), np.arange(ny), np.arange(nx), indexing="ij")
+    face = (I_ == 0) | (J_ == 0) | (K_ == 0) | (I_ == nx - 1) | (J_ == ny - 1) | (K_ == nz - 1)
+    sx = _edge(hy, hz, J_, K_)
+    syy = _edge(hz, hx, K_, I_)
+    szz = _edge(hx, hy, I_, J_)
+    ix = np.where(hx > 0, 1.0 / (6.0 * np.where(hx > 0, hx, 1.0)), 0.0)
+    iy = np.where(hy > 0, 1.0 / (6.0 * np.where(hy > 0, hy, 1.0)), 0.0)
+    iz = np.where(hz > 0, 1.0 / (6.0 * np.where(hz > 0, hz, 1.0)), 0.0)
+    cxm, cxp = sx * ix[I_], sx * ix[I_ + 1]
+    cym, cyp = syy * iy[J_], syy * iy[J_ + 1]
+    czm, czp = szz * iz[K_], szz * iz[K_ + 1]
+    diag = cxm + cxp + cym + cyp + czm + czp
+    idx = np.arange(nx * ny * nz).reshape(nz, ny, nx)
+    rows, cols, vals = [], [], []
+    inner = ~face
+    for c, (dk, dj, di) in ((cxm, (0, 0, -1)), (cxp, (0, 0, 1)), (cym, (0, -1, 0)), (cyp, (0, 1, 0)),
+                            (czm, (-1, 0, 0)), (czp, (1, 0, 0))):
+        kk, jj, ii = K_[inner] + dk, J_[inner] + dj, I_[inner] + di
+        nb = idx[kk, jj, ii]
+        keep = ~face[kk, jj, ii]
+        rows.append(idx[inner][keep])
+        cols.append(nb[keep])
+        vals.append(-c[inner][keep])
+    A = sp.csr_matrix((np.concatenate(vals), (np.concatenate(rows), np.concatenate(cols))),
+                      shape=(idx.size, idx.size))
+    d = np.where(face, 1.0, diag).ravel()
+    return (A + sp.diags(d)).tocsr(), face.ravel()
+
+
+def coarsen(x):
+    keep = list(range(0, len(x), 2))
+    if keep[-1] != len(x) - 1:
+        keep.append(len(x) - 1)
+    return np.asarray(keep)
+
+
+def interp(xf, keep):
+    xc = xf[keep]
+    P = sp.lil_matrix((len(xf), len(xc)))
+    c0 = 0
+    for i in range(len(xf)):
+        while c0 + 1 < len(keep) and keep[c0 + 1] <= i:
+            c0 += 1
+        if keep[c0] == i:
+            P[i, c0] = 1.0
+        else:
+            t = (xf[i] - xc[c0]) / (xc[c0 + 1] - xc[c0])
+            P[i, c0] = 1.0 - t
+            P[i, c0 + 1] = t
+    return P.tocsr()
+
+
+def omegas(deg, lo, hi=2.0):
+    k = np.arange(deg)
+    return 1.0 / ((hi + lo) / 2 + (hi - lo) / 2 * np.cos(np.pi * (2 * k + 1) / (2 * deg)))
+
+
+class VCycle:
+    def __init__(self, xs, ys, zs, max_levels=4, deg=3, cdeg=6):
+        self.levels = []
+        axes = [np.asarray(xs, float), np.asarray(ys, float), np.asarray(zs, float)]
+        while True:
+            A, face = operator(*axes)
+            lev = dict(A=A, face=face, dinv=1.0 / A.diagonal())
+            self.levels.append(lev)
+            if max(len(a) for a in axes) <= 24 or len(self.levels) == max_levels:
+                break
+            keeps = [coarsen(a) for a in axes]
+            lev["P"] = sp.kron(sp.kron(interp(axes[2], keeps[2]), interp(axes[1], keeps[1])),
+                               interp(axes[0], keeps[0])).tocsr()
+            axes = [a[k] for a, k in zip(axes, keeps)]
+        self.om = omegas(deg, 2.0 / 30.0)
+        self.omc = omegas(cdeg, 2.0 / 300.0)
+
+    def _smooth(self, lev, x, b, oms):
+        for w in oms:
+            x = x + w * lev["dinv"] * (b - lev["A"] @ x)
+            x[lev["face"]] = 0.0
+        return x
+
+    def apply(self, r, l=0):
+        lev = self.levels[l]
+        r = np.where(lev["face"], 0.0, r)
+        if l == len(self.levels) - 1:
+            return self._smooth(lev, np.zeros_like(r), r, self.omc)
+        x = self._smooth(lev, np.zeros_like(r), r, self.om)
+        rc = lev["P"].T @ (r - lev["A"] @ x)
+        rc[self.levels[l + 1]["face"]] = 0.0
+        x = x + lev["P"] @ self.apply(rc, l + 1)
+        x[lev["face"]] = 0.0
+        return self._smooth(lev, x, r, self.om[::-1])
