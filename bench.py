@@ -114,9 +114,11 @@ def gpu_run(args, precision, traffic_probe=False, learned=None, multigrid=0):
     t0 = time.perf_counter()
     model = Model.from_structured(g, structured=not args.general, precision=precision,
                                   inner_tol=args.inner_tol)
-    tag = precision + (f" + multigrid ({multigrid} levels)" if multigrid else "") + (" + learned" if learned else "")
+    levels = []
     if multigrid:
         model.multigrid(multigrid)
+        levels = model.mg_levels()
+    tag = precision + (f" + multigrid ({len(levels)} levels)" if multigrid else "") + (" + learned" if learned else "")
     if learned:
         from mammos_mumag_amd.learned import Network
         net = Network(model, max_batch=32, seed=0)
@@ -162,6 +164,7 @@ def gpu_run(args, precision, traffic_probe=False, learned=None, multigrid=0):
         log("[bench] traffic probe: copied 512 MiB")
     ws = model.warm_start_stats() if learned else None
     res = dict(elapsed=elapsed, stats=st.as_dict(), timing=timing, info=model.info, warm_start=ws,
+               mg_levels=" -> ".join("x".join(map(str, d)) for d in levels),
                n_nodes=n_nodes, n_elems=n_elems, n_mag_nodes=int(np.sum(model.meas > 0)),
                mh=mh, energies=energies, u=u.cpu().numpy(), hd=model.gradient(u).cpu().numpy())
     model.close()
@@ -218,8 +221,9 @@ def main():
     ap.add_argument("--config", default="voronoi64_250nm")
     ap.add_argument("--precision", default="mixed", choices=["fp64", "mixed"],
                     help="form whose throughput is the value (mixed: FP32 inner / FP64 outer)")
-    ap.add_argument("--multigrid", type=int, default=4,
-                    help="levels of the multigrid preconditioner of the FP32 inner solve (mixed form; 0: Jacobi)")
+    ap.add_argument("--multigrid", type=int, default=8,
+                    help="most levels of the multigrid preconditioner of the FP32 inner solve (mixed form; "
+                         "0: Jacobi)")
     ap.add_argument("--learned", default="",
                     help="trained warm-start network (.npz); default: the package's network for --config "
                          "when present; 'none' disables")
@@ -297,7 +301,7 @@ def main():
                                f"{res['n_elems']} tets, field steps from {cfgd['hstart']:+g} T by {cfgd['hstep']:g} T",
                    "precision": args.precision,
                    "form": (("learned warm start + " if learned else "")
-                            + ("FP32 inner solve" + (f" (multigrid preconditioner, {mgl} levels)" if mgl else "")
+                            + ("FP32 inner solve" + (f" (multigrid preconditioner, levels {res['mg_levels']})" if mgl else "")
                                + " + FP64 outer" if args.precision == "mixed" else "FP64")),
                    "learned": (None if not learned else dict(
                        network=os.path.relpath(learned, ROOT), warm_started_solves=res["warm_start"]["calls"],

@@ -178,9 +178,15 @@ int mm_gradient(mm_ctx *ctx, const double *u, double *g);
 /* Multigrid preconditioner for the FP32 inner solve of a structured MM_PREC_MIXED context
  * (no counterpart in the reference, whose potential solve is Jacobi-preconditioned,
  * solvers.py:222-231): a geometric V-cycle of up to max_levels levels (2..8; 0 restores
- * Jacobi) replaces z = r / diag(K) inside the reliable-update PCG.  The stop test on the
- * FP64 true residual is unchanged, so only the iteration count changes. */
+ * Jacobi) replaces z = r / diag(K) inside the reliable-update PCG.  Coarsening is
+ * spacing-aware (the fine cells of a graded air box merge first) and stops at 16 nodes
+ * per axis.  The stop test on the FP64 true residual is unchanged, so only the
+ * iteration count changes. */
+#define MM_MG_MAX_LEVELS 8
 int mm_multigrid(mm_ctx *ctx, int32_t max_levels);
+/* Levels of the V-cycle (0: none) and, if dims is not null, the node counts per axis of
+ * each level, dims[3 l + a] for l < *levels (dims holds 3 * MM_MG_MAX_LEVELS). */
+int mm_mg_levels(mm_ctx *ctx, int32_t *levels, int32_t *dims);
 /* z = M r with the multigrid V-cycle (device f32[N]; r zero on the Dirichlet faces). */
 int mm_mg_apply(mm_ctx *ctx, const float *r, float *z);
 

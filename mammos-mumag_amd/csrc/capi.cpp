@@ -124,7 +124,10 @@ struct mm_ctx {
         allocs.push_back(p_);
         bytes += count * sizeof(T);
         *ptr = (T *)p_;
+        // the context's stream does not wait for the null stream: the zeroing completes
+        // here, before any kernel of a later call can write the buffer
         e = hipMemset(p_, 0, std::max<size_t>(count, 1) * sizeof(T));
+        if (e == hipSuccess) e = hipStreamSynchronize(nullptr);
         if (e != hipSuccess) return fail(MM_E_HIP, std::string("hipMemset: ") + hipGetErrorString(e));
         return 0;
     }
@@ -179,6 +182,7 @@ struct mm_mlp {
         allocs.push_back(q);
         *ptr = (T *)q;
         e = hipMemset(q, 0, std::max<size_t>(count, 1) * sizeof(T));
+        if (e == hipSuccess) e = hipStreamSynchronize(nullptr);   // see mm_ctx::alloc
         if (e != hipSuccess) return fail(MM_E_HIP, std::string("hipMemset: ") + hipGetErrorString(e));
         return 0;
     }
@@ -307,70 +311,78 @@ int pcg64(mm_ctx *c, const double *b, double *x, double tol, double atol, int ma
     return 0;
 }
 
-// Multigrid V-cycle z = M r on the FP32 vectors of the inner solve (mg.hip); skipped
-// entirely once *skip is set.  Level 0 works in z and tmp32 (ping-pong, chosen so that
-// the result ends in z), residual in tmp32b.
-void mg_vcycle(mm_ctx *c, const float *r, float *z, const int *skip) {
+// Multigrid V-cycle M r on the FP32 vectors of the inner solve (mg.hip); skipped entirely
+// once *skip is set.  Level 0 ping-pongs between z and alt; the returned buffer (one of
+// the two) holds M r.  first_done: z already holds the first smoothing step
+// mg[0].invw * r (the kernel that produced r wrote it).  With parts, the last level-0
+// smoothing step also writes the partials (r.Mr, r.r) there; *nparts is their count.
+float *mg_vcycle(mm_ctx *c, const float *r, float *z, float *alt, const int *skip, bool first_done,
+                 double *parts = nullptr, int *nparts = nullptr) {
     Launch L = c->L();
     const int nl = c->mg_nl, s = c->mg_deg, cd = c->mg_cdeg;
     float *cur[kMgMaxLevels], *oth[kMgMaxLevels];
     const float *B[kMgMaxLevels];
     for (int l = 0; l < nl; ++l) {
         B[l] = l == 0 ? r : c->mg_b[l];
-        cur[l] = c->mg_xa[l];
-        oth[l] = c->mg_xb[l];
+        cur[l] = l == 0 ? z : c->mg_xa[l];
+        oth[l] = l == 0 ? alt : c->mg_xb[l];
     }
-    // level 0 works in z and its own buffer: it makes 2 s - 1 swaps (s - 1 before and s
-    // after the coarse correction), or cd - 1 when it is the only level; start in the
-    // buffer that ends as z
-    const int swaps0 = nl > 1 ? 2 * s - 1 : cd - 1;
-    cur[0] = (swaps0 & 1) ? c->mg_xb[0] : z;
-    oth[0] = (swaps0 & 1) ? z : c->mg_xb[0];
-    auto smooth_down = [&](int l, const float *om, int deg) {
-        launch_mg_first(L, c->mg[l].P, B[l], cur[l], om[0], skip);
-        for (int k = 1; k < deg; ++k) {
-            launch_mg_smooth(L, c->mg[l].P, B[l], cur[l], oth[l], om[k], skip);
-            std::swap(cur[l], oth[l]);
-        }
+    if (!first_done) launch_mg_first(L, c->mg[0].P, r, z, nl > 1 ? c->mg_om[0] : c->mg_omc[0], skip);
+    // the last level-0 smoothing step carries the dot when the stencil grid has a
+    // partial slot per block
+    const bool fuse = parts && mg_stencil_fits(c->mg[0].P);
+    int np = 0;
+    auto smooth = [&](int l, float w, bool last0) {
+        const int nb = launch_mg_smooth(L, c->mg[l].P, B[l], cur[l], oth[l], w, skip,
+                                        last0 && fuse ? parts : nullptr);
+        if (last0 && fuse) np = nb;
+        std::swap(cur[l], oth[l]);
     };
+    // pre-smoothing (the first step of every level is already in cur[l]), residual, restriction
     for (int l = 0; l + 1 < nl; ++l) {
-        smooth_down(l, c->mg_om, s);
+        for (int k = 1; k < s; ++k) smooth(l, c->mg_om[k], false);
         launch_mg_resid(L, c->mg[l].P, B[l], cur[l], c->mg_r[l], skip);
-        launch_mg_restrict(L, c->mg[l], c->mg[l + 1], c->mg_r[l], c->mg_b[l + 1], skip);
+        launch_mg_restrict(L, c->mg[l], c->mg[l + 1], c->mg_r[l], c->mg_b[l + 1], cur[l + 1], skip);
     }
-    smooth_down(nl - 1, c->mg_omc, cd);
+    for (int k = 1; k < cd; ++k) smooth(nl - 1, c->mg_omc[k], nl == 1 && k == cd - 1);
     for (int l = nl - 2; l >= 0; --l) {
         launch_mg_prolong(L, c->mg[l], c->mg[l + 1], cur[l + 1], cur[l], skip);
-        for (int k = s - 1; k >= 0; --k) {
-            launch_mg_smooth(L, c->mg[l].P, B[l], cur[l], oth[l], c->mg_om[k], skip);
-            std::swap(cur[l], oth[l]);
-        }
+        for (int k = s - 1; k >= 0; --k) smooth(l, c->mg_om[k], l == 0 && k == 0);
     }
+    if (parts && np == 0) np = launch_dot2<float>(L, c->N, r, cur[0], parts);
+    if (nparts) *nparts = np;
+    return cur[0];
 }
 
 // Mixed precision with reliable updates: the Jacobi-PCG recursion (alpha, beta, r, z, p
 // and the correction dx) runs in FP32 with FP64 dot products; whenever the FP32
 // recursive residual has dropped by ru_delta since the last replacement, or claims
 // convergence, x += dx in FP64 and the residual is replaced by the true FP64 residual
-// b - K x; the search direction continues as p = z + beta p with gamma from the true
-// residual.  Stops when the true residual satisfies
-// r.r <= max(tol^2 b.b, atol^2), the FP64 path's bound.
+// b - K x; with Jacobi the search direction continues as p = z + beta p with gamma from
+// the true residual, with multigrid the recursion restarts from p = M r.  Stops when the
+// true residual satisfies r.r <= max(tol^2 b.b, atol^2), the FP64 path's bound.
 int pcg_mixed(mm_ctx *c, const double *b, double *x, double tol, double atol, int maxiter, int *iters) {
     Launch L = c->L();
     const int64_t n = c->N;
     const double delta2 = c->ru_delta * c->ru_delta;
     op_K(c, x, c->q, nullptr, nullptr);
-    launch_ru_residual(L, n, b, c->q, c->r32, c->z32, c->p32, c->invDK32, tol, atol, maxiter, true);
+    const bool mg = c->mg_nl > 0;
+    // with multigrid, every kernel that forms z from r writes the V-cycle's first step
+    const float *invD = mg ? c->mg[0].invw : c->invDK32;
+    launch_ru_residual(L, n, b, c->q, c->r32, c->z32, c->p32, invD, tol, atol, maxiter, true);
     launch_fill_zero32(L, n, c->dx32, nullptr);
     int rc = sync_scalars(c);
     if (rc) return rc;
     float *const pb[2] = {c->p32, c->p32b};
     int i = 0, nB = 0, mode = PCG_RESTART;
-    const bool mg = c->mg_nl > 0;
     int *const done = &c->dsc->pcg_done;
-    if (mg) {   // z = M r for the initial true residual
-        mg_vcycle(c, c->r32, c->z32, done);
-        nB = launch_dot2<float>(L, n, c->r32, c->z32, c->partsB);
+    // z = M r; the V-cycle's result buffer becomes z32 (the other one its scratch)
+    auto precondition = [&]() {
+        float *res = mg_vcycle(c, c->r32, c->z32, c->mg_xb[0], done, true, c->partsB, &nB);
+        if (res != c->z32) std::swap(c->z32, c->mg_xb[0]);
+    };
+    if (mg) {   // for the initial true residual
+        precondition();
         mode = PCG_RESTART_PARTS;
     }
     while (!c->hsc->ir_done) {
@@ -381,12 +393,9 @@ int pcg_mixed(mm_ctx *c, const double *b, double *x, double tol, double atol, in
         for (;;) {
             const int batch = mg ? c->mg_batch : c->pcg_batch;
             for (int k = 0; k < batch; ++k, ++i) {
-                pcg_iteration<float>(c, i, mode, tol, atol, maxiter, c->z32, pb, c->q32, c->dx32, c->r32,
-                                     mg ? nullptr : c->invDK32, c->K32, c->P32, &nB);
-                if (mg) {
-                    mg_vcycle(c, c->r32, c->z32, done);
-                    nB = launch_dot2<float>(L, n, c->r32, c->z32, c->partsB);
-                }
+                pcg_iteration<float>(c, i, mode, tol, atol, maxiter, c->z32, pb, c->q32, c->dx32, c->r32, invD,
+                                     c->K32, c->P32, &nB);
+                if (mg) precondition();
                 mode = PCG_NORMAL;
             }
             if ((rc = sync_scalars(c))) return rc;
@@ -402,14 +411,18 @@ int pcg_mixed(mm_ctx *c, const double *b, double *x, double tol, double atol, in
         // reliable update: fold dx into x, replace r by the true residual
         launch_add_correction(L, n, x, c->dx32);
         op_K(c, x, c->q, nullptr, nullptr);
-        launch_ru_residual(L, n, b, c->q, c->r32, c->z32, c->p32, c->invDK32, tol, atol, maxiter, false);
+        launch_ru_residual(L, n, b, c->q, c->r32, c->z32, c->p32, invD, tol, atol, maxiter, false);
         if ((rc = sync_scalars(c))) return rc;
         if (!c->hsc->ir_done && i >= maxiter) break;
         mode = PCG_KEEP;
-        if (mg) {   // z = M r for the replaced residual; the direction continues (beta from it)
-            mg_vcycle(c, c->r32, c->z32, done);
-            nB = launch_dot2<float>(L, n, c->r32, c->z32, c->partsB);
-            mode = PCG_NORMAL;
+        if (mg) {
+            // z = M r for the replaced residual and a restart (p = z).  The multigrid
+            // recursion runs far below the FP32 residual's attainable accuracy between
+            // replacements, so the true residual's r.z can exceed the recursive one by
+            // orders of magnitude: continuing the direction (beta = gamma_true /
+            // gamma_recursive) would wreck the conjugacy.
+            precondition();
+            mode = PCG_RESTART_PARTS;
         }
     }
     if (iters) *iters = i;
@@ -518,6 +531,63 @@ std::vector<T> convert(const std::vector<double> &v) {
     return std::vector<T>(v.begin(), v.end());
 }
 
+}  // namespace
+
+namespace {
+// Coarse node set of an axis and its transfers.  Spacing-aware coarsening: walking the
+// intervals from the first node, two adjacent intervals merge (their middle node drops)
+// when both are at most kMgMerge * hmin, hmin the smallest spacing of the level over all
+// three axes; wider intervals stay.  On a graded air box the fine cells coarsen first and
+// the coarse levels approach uniform spacing, instead of stretching the already wide
+// cells (which point smoothers cannot handle).  Transfers: per fine index the coarse
+// interval (c0, weight t of c0 + 1); per coarse index up to 3 (fine index, weight) pairs
+// of the transposed interpolation (at most one dropped node on each side).
+struct AxisTransfer {
+    std::vector<double> coarse;
+    std::vector<int> pc, ri;
+    std::vector<float> pt, rw;
+};
+constexpr double kMgMerge = 3.0;
+constexpr int kMgCoarsest = 16;   // stop once every axis has at most this many nodes
+AxisTransfer coarsen_axis(const std::vector<double> &xf, double hmin) {
+    AxisTransfer t;
+    const int nf = (int)xf.size();
+    std::vector<int> keep{0};
+    for (int i = 0; i < nf - 1;) {
+        const bool merge = i + 2 <= nf - 1 && xf[i + 1] - xf[i] <= kMgMerge * hmin &&
+                           xf[i + 2] - xf[i + 1] <= kMgMerge * hmin;
+        i += merge ? 2 : 1;
+        keep.push_back(i);
+    }
+    const int nc = (int)keep.size();
+    for (int k : keep) t.coarse.push_back(xf[k]);
+    t.pc.assign(nf, 0);
+    t.pt.assign(nf, 0.f);
+    t.ri.assign(3 * nc, 0);
+    t.rw.assign(3 * nc, 0.f);
+    std::vector<int> fill(nc, 0);
+    int c0 = 0;
+    for (int i = 0; i < nf; ++i) {
+        while (c0 + 1 < nc && keep[c0 + 1] <= i) ++c0;
+        const double tt = keep[c0] == i ? 0.0 : (xf[i] - t.coarse[c0]) / (t.coarse[c0 + 1] - t.coarse[c0]);
+        t.pc[i] = c0;
+        t.pt[i] = (float)tt;
+        // restriction: fine i feeds coarse c0 (1 - t) and c0 + 1 (t)
+        const double w[2] = {1.0 - tt, tt};
+        for (int q = 0; q < 2; ++q) {
+            if (w[q] == 0.0) continue;
+            const int C = c0 + q;
+            if (fill[C] == 3) {   // cannot happen with merges of two intervals
+                t.coarse.clear();
+                return t;
+            }
+            t.ri[3 * C + fill[C]] = i;
+            t.rw[3 * C + fill[C]] = (float)w[q];
+            ++fill[C];
+        }
+    }
+    return t;
+}
 }  // namespace
 
 extern "C" {
@@ -1042,47 +1112,6 @@ int mm_minimize(mm_ctx *c, double *m_inout, double hext, const double hdir[3],
 
 // ---------------------------------------------------------------- multigrid
 
-namespace {
-// Coarse node set of an axis (even indices and the last) and its transfers: per fine
-// index the coarse interval (c0, weight t of c0 + 1); per coarse index up to 3 (fine
-// index, weight) pairs of the transposed interpolation.
-struct AxisTransfer {
-    std::vector<double> coarse;
-    std::vector<int> pc, ri;
-    std::vector<float> pt, rw;
-};
-AxisTransfer coarsen_axis(const std::vector<double> &xf) {
-    AxisTransfer t;
-    const int nf = (int)xf.size();
-    std::vector<int> keep;
-    for (int i = 0; i < nf; i += 2) keep.push_back(i);
-    if (keep.back() != nf - 1) keep.push_back(nf - 1);
-    const int nc = (int)keep.size();
-    for (int k : keep) t.coarse.push_back(xf[k]);
-    t.pc.assign(nf, 0);
-    t.pt.assign(nf, 0.f);
-    t.ri.assign(3 * nc, 0);
-    t.rw.assign(3 * nc, 0.f);
-    std::vector<int> fill(nc, 0);
-    int c0 = 0;
-    for (int i = 0; i < nf; ++i) {
-        while (c0 + 1 < nc && keep[c0 + 1] <= i) ++c0;
-        const double tt = keep[c0] == i ? 0.0 : (xf[i] - t.coarse[c0]) / (t.coarse[c0 + 1] - t.coarse[c0]);
-        t.pc[i] = c0;
-        t.pt[i] = (float)tt;
-        // restriction: fine i feeds coarse c0 (1 - t) and c0 + 1 (t)
-        const double w[2] = {1.0 - tt, tt};
-        for (int q = 0; q < 2; ++q) {
-            if (w[q] == 0.0) continue;
-            const int C = c0 + q;
-            t.ri[3 * C + fill[C]] = i;
-            t.rw[3 * C + fill[C]] = (float)w[q];
-            ++fill[C];
-        }
-    }
-    return t;
-}
-}  // namespace
 
 int mm_multigrid(mm_ctx *c, int32_t max_levels) {
     if (!c) return fail(MM_E_ARG, "null context");
@@ -1093,6 +1122,7 @@ int mm_multigrid(mm_ctx *c, int32_t max_levels) {
     if (!c->structured || c->precision != MM_PREC_MIXED)
         return fail(MM_E_STATE, "the multigrid preconditioner needs a structured mixed-precision context");
     if (max_levels < 2 || max_levels > kMgMaxLevels) return fail(MM_E_ARG, "levels outside [2, 8]");
+    static_assert(kMgMaxLevels == MM_MG_MAX_LEVELS, "level bound of the header");
     int rc = 0;
     std::vector<double> ax[3] = {c->axis[0], c->axis[1], c->axis[2]};
     int nl = 0;
@@ -1108,10 +1138,17 @@ int mm_multigrid(mm_ctx *c, int32_t max_levels) {
         }
         nl = l + 1;
         const size_t mx = std::max(ax[0].size(), std::max(ax[1].size(), ax[2].size()));
-        if (mx <= 24 || nl == max_levels) break;
+        if (mx <= (size_t)kMgCoarsest || nl == max_levels) break;
         // transfers to level l + 1 and its operator
-        AxisTransfer t[3] = {coarsen_axis(ax[0]), coarsen_axis(ax[1]), coarsen_axis(ax[2])};
+        double hmin = HUGE_VAL;
+        for (int a = 0; a < 3; ++a)
+            for (size_t i = 1; i < ax[a].size(); ++i) hmin = std::min(hmin, ax[a][i] - ax[a][i - 1]);
+        AxisTransfer t[3] = {coarsen_axis(ax[0], hmin), coarsen_axis(ax[1], hmin), coarsen_axis(ax[2], hmin)};
+        if (t[0].coarse.size() == ax[0].size() && t[1].coarse.size() == ax[1].size() &&
+            t[2].coarse.size() == ax[2].size())
+            break;   // nothing merges any more
         for (int a = 0; a < 3; ++a) {   // host-side bounds of every index the kernels use
+            if (t[a].coarse.size() < 3) return fail(MM_E_STATE, "multigrid: degenerate coarse axis");
             const int nf = (int)ax[a].size(), nc = (int)t[a].coarse.size();
             for (int i = 0; i < nf; ++i)
                 if (t[a].pc[i] < 0 || t[a].pc[i] >= nc || (t[a].pt[i] != 0.f && t[a].pc[i] + 1 >= nc))
@@ -1158,6 +1195,20 @@ int mm_multigrid(mm_ctx *c, int32_t max_levels) {
     };
     omegas(c->mg_om, c->mg_deg, 2.0 / 30.0);
     omegas(c->mg_omc, c->mg_cdeg, 2.0 / 300.0);
+    // invw of every level: its first step's weight over the stencil's own diagonal (the
+    // first-step kernel applied to ones), 0 on the faces
+    {
+        float *ones = nullptr;
+        const std::vector<float> h1((size_t)c->N, 1.f);
+        if ((rc = c->upload(&ones, h1))) return rc;
+        Launch L = c->L();
+        for (int l = 0; l < nl; ++l) {
+            const PoissonGrid<float> &P = c->mg[l].P;
+            if ((rc = c->alloc(&c->mg[l].invw, (size_t)P.nx * P.ny * P.nz))) return rc;
+            launch_mg_first(L, P, ones, c->mg[l].invw, l + 1 == nl ? c->mg_omc[0] : c->mg_om[0], nullptr);
+        }
+        HIPC(hipStreamSynchronize(c->s));
+    }
     c->mg_nl = nl;
     // a multigrid-preconditioned FP32 recursion stays accurate over a longer stretch:
     // replace the residual every 1e-3 of reduction unless the caller chose otherwise
@@ -1165,10 +1216,23 @@ int mm_multigrid(mm_ctx *c, int32_t max_levels) {
     return 0;
 }
 
+int mm_mg_levels(mm_ctx *c, int32_t *levels, int32_t *dims) {
+    if (!c || !levels) return fail(MM_E_ARG, "null argument");
+    *levels = c->mg_nl;
+    if (dims)
+        for (int l = 0; l < c->mg_nl; ++l) {
+            dims[3 * l] = c->mg[l].P.nx;
+            dims[3 * l + 1] = c->mg[l].P.ny;
+            dims[3 * l + 2] = c->mg[l].P.nz;
+        }
+    return 0;
+}
+
 int mm_mg_apply(mm_ctx *c, const float *r, float *z) {
     if (!c || !r || !z) return fail(MM_E_ARG, "null argument");
     if (c->mg_nl == 0) return fail(MM_E_STATE, "no multigrid preconditioner (mm_multigrid)");
-    mg_vcycle(c, r, z, nullptr);
+    float *res = mg_vcycle(c, r, z, c->mg_xb[0], nullptr, false);
+    if (res != z) HIPC(hipMemcpyAsync(z, res, (size_t)c->N * sizeof(float), hipMemcpyDeviceToDevice, c->s));
     HIPC(hipStreamSynchronize(c->s));
     return 0;
 }

@@ -113,6 +113,7 @@ SIGNATURES = {
     "mm_warm_start": [VP, VP, ctypes.c_double],
     "mm_multigrid": [VP, ctypes.c_int32],
     "mm_mg_apply": [VP, VP, VP],
+    "mm_mg_levels": [VP, VP, VP],
     "mm_warm_start_stats": [VP, ctypes.POINTER(ctypes.c_int64), c_double_p],
 }
 

@@ -367,10 +367,17 @@ class Model:
         _lib.check(self._lib.mm_gradient(self._ctx, _ptr(u, self.N), _ptr(g, 3 * self.N)), "mm_gradient")
         return g
 
-    def multigrid(self, levels=4):
+    def multigrid(self, levels=8):
         """Geometric multigrid preconditioner for the FP32 inner solve (structured mixed
-        contexts); 0 restores Jacobi."""
+        contexts) with at most ``levels`` levels; 0 restores Jacobi."""
         _lib.check(self._lib.mm_multigrid(self._ctx, int(levels)), "mm_multigrid")
+
+    def mg_levels(self):
+        """Node counts per axis (nx, ny, nz) of every V-cycle level, finest first."""
+        nl = ctypes.c_int32()
+        dims = (ctypes.c_int32 * (3 * 8))()
+        _lib.check(self._lib.mm_mg_levels(self._ctx, ctypes.byref(nl), dims), "mm_mg_levels")
+        return [tuple(dims[3 * l:3 * l + 3]) for l in range(nl.value)]
 
     def mg_apply(self, r, z=None):
         """z = M r with the multigrid V-cycle (float32 device vectors)."""

@@ -2,7 +2,9 @@
 mammos-mumag_amd/csrc/mg.hip (FP64) for the GPU checks.  Test infrastructure only.
 
 No counterpart in the reference (its potential solve is Jacobi-preconditioned).  The
-restatement follows the kernels: per-axis coarsening (even node indices and the last),
+restatement follows the kernels: per-axis spacing-aware coarsening (two adjacent
+intervals merge when both are at most 3 hmin, hmin the level's smallest spacing over the
+three axes; down to at most 16 nodes per axis),
 the 7-point operator of each level's rectilinear grid with zero on the faces, linear
 interpolation and its transpose, Chebyshev-Richardson smoothing x += w_k (b - A x) / diag
 with w_k the reciprocal Chebyshev nodes of [2/30, 2] (reverse order after the coarse
@@ -59,10 +61,18 @@ def operator(xs, ys, zs):
     return (A + sp.diags(d)).tocsr(), face.ravel()
 
 
-def coarsen(x):
-    keep = list(range(0, len(x), 2))
-    if keep[-1] != len(x) - 1:
-        keep.append(len(x) - 1)
+MERGE = 3.0
+COARSEST = 16
+
+
+def coarsen(x, hmin):
+    """Kept node indices of axis x: walking from the first node, two adjacent intervals
+    merge (the middle node drops) when both are at most MERGE * hmin."""
+    d = np.diff(x)
+    keep, i = [0], 0
+    while i < len(x) - 1:
+        i += 2 if (i + 2 <= len(x) - 1 and d[i] <= MERGE * hmin and d[i + 1] <= MERGE * hmin) else 1
+        keep.append(i)
     return np.asarray(keep)
 
 
@@ -88,16 +98,20 @@ def omegas(deg, lo, hi=2.0):
 
 
 class VCycle:
-    def __init__(self, xs, ys, zs, max_levels=4, deg=3, cdeg=6):
-        self.levels = []
+    def __init__(self, xs, ys, zs, max_levels=8, deg=3, cdeg=6):
+        self.levels, self.axes = [], []
         axes = [np.asarray(xs, float), np.asarray(ys, float), np.asarray(zs, float)]
         while True:
             A, face = operator(*axes)
             lev = dict(A=A, face=face, dinv=1.0 / A.diagonal())
             self.levels.append(lev)
-            if max(len(a) for a in axes) <= 24 or len(self.levels) == max_levels:
+            self.axes.append(tuple(axes))
+            if max(len(a) for a in axes) <= COARSEST or len(self.levels) == max_levels:
                 break
-            keeps = [coarsen(a) for a in axes]
+            hmin = min(np.diff(a).min() for a in axes)
+            keeps = [coarsen(a, hmin) for a in axes]
+            if all(len(k) == len(a) for k, a in zip(keeps, axes)):
+                break
             lev["P"] = sp.kron(sp.kron(interp(axes[2], keeps[2]), interp(axes[1], keeps[1])),
                                interp(axes[0], keeps[0])).tocsr()
             axes = [a[k] for a, k in zip(axes, keeps)]

@@ -146,26 +146,33 @@ def test_poisson_stencil_chunked_grid():
     assert rel(model.apply_K(model.to_device(x)).cpu().numpy(), O.K @ x) < 1e-12
 
 
-@pytest.mark.parametrize("n,layers,levels", [(8, 3, 1), (39, 13, 3)])
+@pytest.mark.parametrize("n,layers,levels", [(8, 3, 1), (39, 13, 4), (125, 0, 6)])
 def test_multigrid_vcycle_matches_oracle(n, layers, levels):
-    """The GPU V-cycle (FP32) against its FP64 restatement: one level (15^3 nodes) and
-    three (66^3 -> 34^3 -> 18^3)."""
+    """The GPU V-cycle (FP32) against its FP64 restatement: one level (15^3 nodes), four
+    (66^3 -> 40^3 -> 22^3 -> 12^3, the graded air box coarsening last) and the six of the
+    64-grain 125^3-voxel benchmark grid (154^3 -> ... -> 9^3), on a fresh context (the
+    set-up kernels of mm_multigrid must not race the zeroing of new buffers)."""
     import torch
     from mammos_mumag_amd import Model
     from oracle import mg
-    g = generate.build(edge=39.0 if n == 39 else 24.0, n=n, layers=layers, grains=2, cone=15.0, seed=3)
+    if n == 125:
+        g = generate.build_config("voronoi64_250nm", seed=1)
+    else:
+        g = generate.build(edge=39.0 if n == 39 else 24.0, n=n, layers=layers, grains=2, cone=15.0, seed=3)
     print(f"[gpu] multigrid V-cycle, {len(g.xs)}^3 nodes", flush=True)
     model = Model.from_structured(g, precision="mixed")
-    model.multigrid(4)
-    V = mg.VCycle(g.xs, g.ys, g.zs, max_levels=4)
+    model.multigrid()
+    V = mg.VCycle(g.xs, g.ys, g.zs)
     assert len(V.levels) == levels
+    assert model.mg_levels() == [(len(a), len(b), len(c)) for a, b, c in V.axes]
     rng = np.random.default_rng(8)
     _, face = mg.operator(g.xs, g.ys, g.zs)
     r = rng.normal(size=model.N)
     r[face] = 0.0
     z = model.mg_apply(torch.tensor(r, dtype=torch.float32, device="cuda")).cpu().numpy().astype(np.float64)
     zr = V.apply(r)
-    assert rel(z, zr) < 1e-4
+    print(f"[gpu] V-cycle levels {model.mg_levels()}: relative difference {rel(z, zr):.2e}", flush=True)
+    assert rel(z, zr) < 1e-5
     model.close()
 
 
@@ -180,7 +187,7 @@ def test_mixed_multigrid_loop():
         print(f"[gpu] 66^3 loop, {form}", flush=True)
         model = Model.from_structured(g, precision="fp64" if form == "fp64" else "mixed")
         if form == "mixed+mg":
-            model.multigrid(4)
+            model.multigrid()
         m = model.to_device(model.uniform_m(p["m"]))
         m_ref, u, gF = m.clone(), model.empty1(), model.empty3()
         st = _lib.Stats()
@@ -197,4 +204,4 @@ def test_mixed_multigrid_loop():
         r1 = out[form][0]
         assert np.max(np.abs(r1[:, 1] - r0[:, 1])) < 1e-6, form
         assert np.max(np.abs(r1[:, 0] / r0[:, 0] - 1)) < 1e-6, form
-    assert out["mixed+mg"][1]["pcg_iters"] < out["mixed"][1]["pcg_iters"] / 4
+    assert out["mixed+mg"][1]["pcg_iters"] < out["mixed"][1]["pcg_iters"] / 10

@@ -27,11 +27,11 @@ def test_level_operator_is_poisson_k(grid):
 
 @pytest.mark.parametrize("n,layers", [(8, 3), (30, 4)])
 def test_vcycle_symmetric_positive_and_effective(n, layers):
-    """One level (15 nodes per axis) and two (38 -> 20)."""
+    """One level (15 nodes per axis) and three (39 -> 23 -> 14)."""
     from mammos_mumag_amd import generate
     grid = generate.build(edge=24.0, n=n, layers=layers, grains=2, cone=15.0, seed=3)
-    V = mg.VCycle(grid.xs, grid.ys, grid.zs, max_levels=4)
-    assert len(V.levels) == (1 if n == 8 else 2)
+    V = mg.VCycle(grid.xs, grid.ys, grid.zs)
+    assert [len(a) for a, _, _ in V.axes] == ([15] if n == 8 else [39, 23, 14])
     A, face = mg.operator(grid.xs, grid.ys, grid.zs)
     rng = np.random.default_rng(0)
     u, v = rng.normal(size=(2, A.shape[0]))
