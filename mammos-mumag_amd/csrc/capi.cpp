@@ -65,6 +65,8 @@ struct mm_ctx {
     SellDev<float> K32;
     ElemDev E;
     GridDev G;
+    TcgBlock<float> tb32;    // truncated CG on the magnet block (structured grids)
+    TcgBlock<double> tb64;
     PoissonGrid<double> P;
     PoissonGrid<float> P32;
     MatCoef *mats = nullptr;
@@ -489,6 +491,13 @@ int precond(mm_ctx *c, const double *m_ref, const double *gF, const double *g, i
             double *zout) {
     Launch L = c->L();
     const int64_t n = c->N, n3 = 3 * c->N;
+    if (c->structured) {   // on the magnet block only (FP32 vectors in the mixed form)
+        if (c->precision == MM_PREC_MIXED)
+            tcgc_run<float>(L, c->G, n, c->tb32, m_ref, gF, g, c->Dex, max_iter, zout);
+        else
+            tcgc_run<double>(L, c->G, n, c->tb64, m_ref, gF, g, c->Dex, max_iter, zout);
+        return 0;
+    }
     launch_tcg_init(L, n3, g, c->Dex, c->tp, c->tr, c->tz, c->td);
     for (int i = 0; i < max_iter; ++i) {
         op_C(c, c->td, c->tCd, nullptr, &c->dsc->tcg_done);
@@ -802,10 +811,24 @@ int mm_ctx_create(const mm_mesh_desc *d, const mm_options *opts, mm_ctx **out) {
         for (float **v : vF)
             if ((rc = c->alloc(v, n))) return bail(rc);
     }
-    double **v3[] = {&c->gC, &c->gd, &c->tp, &c->tr, &c->tz, &c->td, &c->tCd, &c->tHd,
-                     &c->X,  &c->X0, &c->X1, &c->Gp, &c->G1, &c->Dd, &c->Z1, &c->gtmp};
+    double **v3[] = {&c->gC, &c->gd, &c->X, &c->X0, &c->X1, &c->Gp, &c->G1, &c->Dd, &c->Z1, &c->gtmp};
     for (double **v : v3)
         if ((rc = c->alloc(v, n3))) return bail(rc);
+    if (structured) {   // truncated CG on the magnet block
+        const size_t n1 = (size_t)c->G.mn + 1, nb = n1 * n1 * n1;
+        auto block = [&](auto &B) -> int {
+            int r_;
+            if ((r_ = c->alloc(&B.mg, nb))) return r_;
+            for (auto **v : {&B.m0, &B.D, &B.p, &B.r, &B.z, &B.Hd, &B.d[0], &B.d[1]})
+                if ((r_ = c->alloc(v, 3 * nb))) return r_;
+            return 0;
+        };
+        if ((rc = c->precision == MM_PREC_MIXED ? block(c->tb32) : block(c->tb64))) return bail(rc);
+    } else {
+        double **vt[] = {&c->tp, &c->tr, &c->tz, &c->td, &c->tCd, &c->tHd};
+        for (double **v : vt)
+            if ((rc = c->alloc(v, n3))) return bail(rc);
+    }
     HIPC(hipDeviceSynchronize());
     *out = c;
     return 0;

@@ -197,6 +197,16 @@ void launch_tcg_update(const Launch &L, int64_t n3, double *p, double *r, double
                        const double *Hd, const double *D);
 void launch_tcg_dupdate(const Launch &L, int64_t n3, double *d, const double *z);
 void launch_tcg_final(const Launch &L, int64_t n3, const double *p, const double *g, double *zout);
+// Truncated CG on the magnet block of a structured grid (vectors over the (mn+1)^3 block
+// nodes, [3 t + a]); d ping-pongs between d[0] and d[1].
+template <typename T>
+struct TcgBlock {
+    T *m0 = nullptr, *mg = nullptr, *D = nullptr, *p = nullptr, *r = nullptr, *z = nullptr, *Hd = nullptr;
+    T *d[2] = {nullptr, nullptr};
+};
+template <typename T>
+void tcgc_run(const Launch &L, const GridDev &G, int64_t n, const TcgBlock<T> &B, const double *m0,
+              const double *gF, const double *g, const double *D, int max_iter, double *zout);
 void launch_update_m(const Launch &L, int64_t n, const double *x, double alpha, const double *d,
                      double *out);
 void launch_axpby(const Launch &L, int64_t n, double a, const double *x, double b, const double *y,

@@ -470,32 +470,12 @@ __device__ __forceinline__ int64_t gnode(const GridDev &G, int i, int j, int k) 
     return (int64_t)(G.m0x + i) + (int64_t)G.nx * ((G.m0y + j) + (int64_t)G.ny * (G.m0z + k));
 }
 
-// y = C v at magnet nodes; with DIV also b = Dx vx + Dy vy + Dz vz at magnet nodes.
+// (C v)_node from the values nv[(dx+1) + 3 (dy+1) + 9 (dz+1)][a] of v at the node's 27
+// neighbours (clamped to the block: clamped entries only meet zero-material voxels);
+// with DIV also the node's divergence term sum Js (B v).  Unscaled: y = fS ex + fM an.
 template <typename TC, bool DIV>
-__global__ __launch_bounds__(kBlock) void k_grid_exani(GridDev G, const MatCoef *__restrict__ mats,
-                                                       const double *__restrict__ v,
-                                                       double *__restrict__ y, double *__restrict__ b,
-                                                       const int *skip) {
-    if (skip && *skip) return;
-    const int n1 = G.mn + 1;
-    const int64_t nmag = (int64_t)n1 * n1 * n1;
-    const int64_t t = blockIdx.x * (int64_t)kBlock + threadIdx.x;
-    if (t >= nmag) return;
-    const int i = (int)(t % n1), j = (int)((t / n1) % n1), k = (int)(t / ((int64_t)n1 * n1));
-    TC nv[27][3];
-#pragma unroll
-    for (int dz = -1; dz <= 1; ++dz)
-#pragma unroll
-        for (int dy = -1; dy <= 1; ++dy)
-#pragma unroll
-            for (int dx = -1; dx <= 1; ++dx) {
-                const int ii = min(max(i + dx, 0), G.mn), jj = min(max(j + dy, 0), G.mn),
-                          kk = min(max(k + dz, 0), G.mn);
-                const int64_t g = gnode(G, ii, jj, kk);
-                const int s = (dx + 1) + 3 * (dy + 1) + 9 * (dz + 1);
-#pragma unroll
-                for (int a = 0; a < 3; ++a) nv[s][a] = (TC)v[3 * g + a];
-            }
+__device__ __forceinline__ void grid_c_node(const GridDev &G, const MatCoef *__restrict__ mats,
+                                            const TC (&nv)[27][3], int i, int j, int k, TC (&y)[3], TC &bdiv) {
     TC ex[3] = {0, 0, 0}, an[3] = {0, 0, 0}, bd = 0;
 #pragma unroll
     for (int w = 0; w < 8; ++w) {
@@ -532,11 +512,44 @@ __global__ __launch_bounds__(kBlock) void k_grid_exani(GridDev G, const MatCoef 
         an[2] += cK * sM * u2;
         if (DIV) bd += (TC)mc.Js * sB;
     }
-    const int64_t g = gnode(G, i, j, k);
     const TC fS = (TC)G.fS, fM = (TC)G.fM;
 #pragma unroll
-    for (int a = 0; a < 3; ++a) y[3 * g + a] = (double)(fS * ex[a] + fM * an[a]);
-    if (DIV) b[g] = (double)((TC)G.fB * bd);
+    for (int a = 0; a < 3; ++a) y[a] = fS * ex[a] + fM * an[a];
+    if (DIV) bdiv = (TC)G.fB * bd;
+}
+
+// y = C v at magnet nodes; with DIV also b = Dx vx + Dy vy + Dz vz at magnet nodes.
+template <typename TC, bool DIV>
+__global__ __launch_bounds__(kBlock) void k_grid_exani(GridDev G, const MatCoef *__restrict__ mats,
+                                                       const double *__restrict__ v,
+                                                       double *__restrict__ y, double *__restrict__ b,
+                                                       const int *skip) {
+    if (skip && *skip) return;
+    const int n1 = G.mn + 1;
+    const int64_t nmag = (int64_t)n1 * n1 * n1;
+    const int64_t t = blockIdx.x * (int64_t)kBlock + threadIdx.x;
+    if (t >= nmag) return;
+    const int i = (int)(t % n1), j = (int)((t / n1) % n1), k = (int)(t / ((int64_t)n1 * n1));
+    TC nv[27][3];
+#pragma unroll
+    for (int dz = -1; dz <= 1; ++dz)
+#pragma unroll
+        for (int dy = -1; dy <= 1; ++dy)
+#pragma unroll
+            for (int dx = -1; dx <= 1; ++dx) {
+                const int ii = min(max(i + dx, 0), G.mn), jj = min(max(j + dy, 0), G.mn),
+                          kk = min(max(k + dz, 0), G.mn);
+                const int64_t g = gnode(G, ii, jj, kk);
+                const int s = (dx + 1) + 3 * (dy + 1) + 9 * (dz + 1);
+#pragma unroll
+                for (int a = 0; a < 3; ++a) nv[s][a] = (TC)v[3 * g + a];
+            }
+    TC yv[3], bd = 0;
+    grid_c_node<TC, DIV>(G, mats, nv, i, j, k, yv, bd);
+    const int64_t g = gnode(G, i, j, k);
+#pragma unroll
+    for (int a = 0; a < 3; ++a) y[3 * g + a] = (double)yv[a];
+    if (DIV) b[g] = (double)bd;
 }
 
 // g = interleave(Gx u, Gy u, Gz u) at magnet nodes.
@@ -827,6 +840,206 @@ void launch_tcg_dupdate(const Launch &L, int64_t n3, double *d, const double *z)
 void launch_tcg_final(const Launch &L, int64_t n3, const double *p, const double *g, double *zout) {
     hipLaunchKernelGGL(k_tcg_final, dim3(grid_for(n3, 4)), dim3(kBlock), 0, L.s, n3, p, g, zout, L.dsc);
 }
+
+// ------------------------------------------------------------------ truncated CG on the magnet block
+// Structured grids: every node outside the magnet block has m = 0 and a zero gradient
+// (all its voxels carry the zero material), so the truncated CG above leaves it at zero
+// and adds nothing to its dot products.  These kernels run the same recursion on the
+// (mn+1)^3 block nodes only, magnet index t = i + n1 (j + n1 k), vectors [3 t + a] in the
+// storage type T (FP32 for the mixed form, FP64 otherwise; dot products in FP64), with
+// d = -z + beta d, C d and H d fused into one kernel:
+//   k_tcgc_init   r = -g, z = r / D, d = -z, p = 0 (block copies of m0, m0.gF, D); (r.r, r.z)
+//   k_tcgc_dh     d' = -z + beta d (27-neighbour gather), C d', H d'; d'.H d'
+//   k_tcgc_upd    p += alpha d', r += alpha H d', z = r / D; (r.r, r.z)
+//   k_tcgc_final  zout = (max|p| == 0) ? g : p scattered to the grid (0 off the block)
+
+__device__ __forceinline__ void block_ijk(int n1, int64_t t, int &i, int &j, int &k) {
+    i = (int)(t % n1);
+    j = (int)((t / n1) % n1);
+    k = (int)(t / ((int64_t)n1 * n1));
+}
+
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_tcgc_init(GridDev G, const double *__restrict__ g,
+                                                      const double *__restrict__ gF,
+                                                      const double *__restrict__ m0,
+                                                      const double *__restrict__ D, T *__restrict__ m0c,
+                                                      T *__restrict__ mgc, T *__restrict__ Dc, T *__restrict__ p,
+                                                      T *__restrict__ r, T *__restrict__ z, T *__restrict__ d,
+                                                      double *partials) {
+    const int n1 = G.mn + 1;
+    const int64_t nb = (int64_t)n1 * n1 * n1;
+    double s[2] = {0.0, 0.0};
+    for (int64_t t = blockIdx.x * (int64_t)kBlock + threadIdx.x; t < nb; t += (int64_t)gridDim.x * kBlock) {
+        int i, j, k;
+        block_ijk(n1, t, i, j, k);
+        const int64_t gn = gnode(G, i, j, k);
+        double mg = 0.0;
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            const double ma = m0[3 * gn + a];
+            mg += ma * gF[3 * gn + a];
+            m0c[3 * t + a] = (T)ma;
+            const T Da = (T)D[3 * gn + a];
+            const T ri = (T)(-g[3 * gn + a]);
+            const T zi = (T)((double)ri / (double)Da);
+            Dc[3 * t + a] = Da;
+            p[3 * t + a] = (T)0;
+            r[3 * t + a] = ri;
+            z[3 * t + a] = zi;
+            d[3 * t + a] = -zi;
+            s[0] += (double)ri * (double)ri;
+            s[1] += (double)ri * (double)zi;
+        }
+        mgc[t] = (T)mg;
+    }
+    block_partials<2>(s, 0u, partials);
+}
+
+// d' = FIRST ? d : -z + beta d at the node and its neighbours, dout = d', Hd = H d';
+// partial d'.H d'.  H v = m0 x ((C v) x m0) - (m0 . gF) v.
+template <typename T, typename TC, bool FIRST>
+__global__ __launch_bounds__(kBlock) void k_tcgc_dh(GridDev G, const MatCoef *__restrict__ mats,
+                                                    const T *__restrict__ z, const T *__restrict__ din,
+                                                    T *__restrict__ dout, const T *__restrict__ m0c,
+                                                    const T *__restrict__ mgc, T *__restrict__ Hd,
+                                                    const DevScalars *dsc, double *partials) {
+    if (dsc->tcg_done) return;
+    const int n1 = G.mn + 1;
+    const int64_t nb = (int64_t)n1 * n1 * n1;
+    const T beta = (T)dsc->tcg_beta;
+    double den = 0.0;
+    for (int64_t t = blockIdx.x * (int64_t)kBlock + threadIdx.x; t < nb; t += (int64_t)gridDim.x * kBlock) {
+        int i, j, k;
+        block_ijk(n1, t, i, j, k);
+        TC nv[27][3];
+#pragma unroll
+        for (int dz = -1; dz <= 1; ++dz)
+#pragma unroll
+            for (int dy = -1; dy <= 1; ++dy)
+#pragma unroll
+                for (int dx = -1; dx <= 1; ++dx) {
+                    const int ii = min(max(i + dx, 0), G.mn), jj = min(max(j + dy, 0), G.mn),
+                              kk = min(max(k + dz, 0), G.mn);
+                    const int64_t tn = ii + (int64_t)n1 * (jj + (int64_t)n1 * kk);
+                    const int s = (dx + 1) + 3 * (dy + 1) + 9 * (dz + 1);
+#pragma unroll
+                    for (int a = 0; a < 3; ++a)
+                        nv[s][a] = (TC)(FIRST ? din[3 * tn + a] : -z[3 * tn + a] + beta * din[3 * tn + a]);
+                }
+        TC cv[3], unused = 0;
+        grid_c_node<TC, false>(G, mats, nv, i, j, k, cv, unused);
+        double mm[3], cd[3], dd[3], c[3], pr[3];
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            mm[a] = (double)m0c[3 * t + a];
+            cd[a] = (double)cv[a];
+            dd[a] = (double)nv[13][a];
+            dout[3 * t + a] = (T)nv[13][a];
+        }
+        const double mg = (double)mgc[t];
+        cross3(cd, mm, c);
+        cross3(mm, c, pr);
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            const T h = (T)(pr[a] - mg * dd[a]);
+            Hd[3 * t + a] = h;
+            den += dd[a] * (double)h;
+        }
+    }
+    double v[1] = {den};
+    block_partials<1>(v, 0u, partials);
+}
+
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_tcgc_upd(int64_t n3, T *__restrict__ p, T *__restrict__ r,
+                                                     T *__restrict__ z, const T *__restrict__ d,
+                                                     const T *__restrict__ Hd, const T *__restrict__ Dc,
+                                                     const DevScalars *dsc, double *partials) {
+    if (dsc->tcg_done) return;
+    const double den = dsc->tcg_den;
+    const bool neg = den <= 0.0;
+    const double alpha = neg ? 0.0 : dsc->tcg_rz / den;
+    double s[2] = {0.0, 0.0};
+    for (int64_t i = blockIdx.x * (int64_t)kBlock + threadIdx.x; i < n3; i += (int64_t)gridDim.x * kBlock) {
+        const T rn = (T)((double)r[i] + alpha * (double)Hd[i]);
+        const T zn = (T)((double)rn / (double)Dc[i]);
+        if (!neg) {
+            p[i] = (T)((double)p[i] + alpha * (double)d[i]);
+            r[i] = rn;
+            z[i] = zn;
+        }
+        s[0] += (double)rn * (double)rn;
+        s[1] += (double)rn * (double)zn;
+    }
+    block_partials<2>(s, 0u, partials);
+}
+
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_tcgc_pmax(int64_t n3, const T *__restrict__ p, double *partials) {
+    double s[1] = {0.0};
+    for (int64_t i = blockIdx.x * (int64_t)kBlock + threadIdx.x; i < n3; i += (int64_t)gridDim.x * kBlock)
+        s[0] = fmax(s[0], fabs((double)p[i]));
+    block_partials<1>(s, 1u, partials);
+}
+
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_tcgc_final(GridDev G, int64_t n, const T *__restrict__ p,
+                                                       const double *__restrict__ g, double *__restrict__ zout,
+                                                       const DevScalars *dsc) {
+    const bool zero = dsc->tcg_pmax == 0.0;
+    const int64_t sxy = (int64_t)G.nx * G.ny;
+    const int n1 = G.mn + 1;
+    for (int64_t gn = blockIdx.x * (int64_t)kBlock + threadIdx.x; gn < n; gn += (int64_t)gridDim.x * kBlock) {
+        const int i = (int)(gn % G.nx) - G.m0x, j = (int)((gn / G.nx) % G.ny) - G.m0y, k = (int)(gn / sxy) - G.m0z;
+        const bool in = i >= 0 && j >= 0 && k >= 0 && i < n1 && j < n1 && k < n1;
+        const int64_t t = in ? i + (int64_t)n1 * (j + (int64_t)n1 * k) : 0;
+#pragma unroll
+        for (int a = 0; a < 3; ++a)
+            zout[3 * gn + a] = zero ? g[3 * gn + a] : (in ? (double)p[3 * t + a] : 0.0);
+    }
+}
+
+template <typename T>
+void tcgc_run(const Launch &L, const GridDev &G, int64_t n, const TcgBlock<T> &B, const double *m0,
+              const double *gF, const double *g, const double *D, int max_iter, double *zout) {
+    const int64_t n1 = G.mn + 1, nb = n1 * n1 * n1, n3 = 3 * nb;
+    const int gi = grid_for(nb, 2), gu = grid_for(n3, 4);
+    const int gd = grid_for(nb, 1);
+    hipLaunchKernelGGL(k_tcgc_init<T>, dim3(gi), dim3(kBlock), 0, L.s, G, g, gF, m0, D, B.m0, B.mg, B.D, B.p, B.r,
+                       B.z, B.d[0], L.red.partials);
+    FinArgs ai{};
+    ai.op = FIN_TCG_INIT;
+    ai.nv = 2;
+    ai.nparts = gi;
+    launch_fin(L, ai);
+    for (int it = 0; it < max_iter; ++it) {
+        const T *din = B.d[it & 1];
+        T *dout = B.d[(it + 1) & 1];
+        if (it == 0)
+            hipLaunchKernelGGL((k_tcgc_dh<T, T, true>), dim3(gd), dim3(kBlock), 0, L.s, G, L.mats, B.z, din, dout,
+                               B.m0, B.mg, B.Hd, L.dsc, L.red.partials);
+        else
+            hipLaunchKernelGGL((k_tcgc_dh<T, T, false>), dim3(gd), dim3(kBlock), 0, L.s, G, L.mats, B.z, din, dout,
+                               B.m0, B.mg, B.Hd, L.dsc, L.red.partials);
+        launch_fin(L, FinArgs::store(&L.dsc->tcg_den, 1, 0u, gd, &L.dsc->tcg_done));
+        hipLaunchKernelGGL(k_tcgc_upd<T>, dim3(gu), dim3(kBlock), 0, L.s, n3, B.p, B.r, B.z, dout, B.Hd, B.D, L.dsc,
+                           L.red.partials);
+        FinArgs a{};
+        a.op = FIN_TCG_UPDATE;
+        a.nv = 2;
+        a.nparts = gu;
+        a.skip = &L.dsc->tcg_done;
+        launch_fin(L, a);
+    }
+    hipLaunchKernelGGL(k_tcgc_pmax<T>, dim3(gu), dim3(kBlock), 0, L.s, n3, B.p, L.red.partials);
+    launch_fin(L, FinArgs::store(&L.dsc->tcg_pmax, 1, 1u, gu, nullptr));
+    hipLaunchKernelGGL(k_tcgc_final<T>, dim3(grid_for(n, 2)), dim3(kBlock), 0, L.s, G, n, B.p, g, zout, L.dsc);
+}
+template void tcgc_run<float>(const Launch &, const GridDev &, int64_t, const TcgBlock<float> &, const double *,
+                              const double *, const double *, const double *, int, double *);
+template void tcgc_run<double>(const Launch &, const GridDev &, int64_t, const TcgBlock<double> &, const double *,
+                               const double *, const double *, const double *, int, double *);
 
 // ------------------------------------------------------------------ vector kernels
 
