@@ -124,6 +124,7 @@ def gpu_run(args, precision, traffic_probe=False, learned=None, multigrid=0):
         net = Network(model, max_batch=32, seed=0)
         net.load(learned)
         model.warm_start(net)
+        model.warm_start_policy(args.ws_policy)
     log(f"[bench] {tag}: context {model.info} ({time.perf_counter() - t0:.1f} s)")
     p = loop_params(args.config)
     m = model.to_device(model.uniform_m(p["m"]))
@@ -162,7 +163,7 @@ def gpu_run(args, precision, traffic_probe=False, learned=None, multigrid=0):
         dst.copy_(src)
         torch.cuda.synchronize()
         log("[bench] traffic probe: copied 512 MiB")
-    ws = model.warm_start_stats() if learned else None
+    ws = dict(model.warm_start_stats(), **model.warm_start_report()) if learned else None
     res = dict(elapsed=elapsed, stats=st.as_dict(), timing=timing, info=model.info, warm_start=ws,
                mg_levels=" -> ".join("x".join(map(str, d)) for d in levels),
                n_nodes=n_nodes, n_elems=n_elems, n_mag_nodes=int(np.sum(model.meas > 0)),
@@ -224,6 +225,9 @@ def main():
     ap.add_argument("--multigrid", type=int, default=8,
                     help="most levels of the multigrid preconditioner of the FP32 inner solve (mixed form; "
                          "0: Jacobi)")
+    ap.add_argument("--ws-policy", default="measured", choices=["every", "measured"],
+                    help="when the network warm-starts a potential solve: every solve, or per solve the "
+                         "faster form by measured solve time")
     ap.add_argument("--learned", default="",
                     help="trained warm-start network (.npz); default: the package's network for --config "
                          "when present; 'none' disables")
@@ -304,7 +308,10 @@ def main():
                             + ("FP32 inner solve" + (f" (multigrid preconditioner, levels {res['mg_levels']})" if mgl else "")
                                + " + FP64 outer" if args.precision == "mixed" else "FP64")),
                    "learned": (None if not learned else dict(
-                       network=os.path.relpath(learned, ROOT), warm_started_solves=res["warm_start"]["calls"],
+                       network=os.path.relpath(learned, ROOT), policy=args.ws_policy,
+                       solves=res["warm_start"]["solves"], warm_started_solves=res["warm_start"]["calls"],
+                       mean_ms_per_solve_plain=round(1e3 * res["warm_start"]["mean_s_plain"], 3),
+                       mean_ms_per_solve_warm=round(1e3 * res["warm_start"]["mean_s_warm"], 3),
                        without_network=None if ablate is None else dict(
                            value=round(steps / ablate["elapsed"], 6),
                            pcg_iters_per_step=round(ablate["stats"]["pcg_iters"] / steps, 2),

@@ -230,6 +230,18 @@ int mm_mlp_train_step(mm_mlp *net, int64_t n, const uint16_t *X, const float *ta
 int mm_warm_start(mm_ctx *ctx, mm_mlp *net, double scale);
 /* Warm-started solves since mm_warm_start and the last Galerkin coefficient. */
 int mm_warm_start_stats(mm_ctx *ctx, int64_t *calls_host, double *last_gamma_host);
+/* When the network is applied: MM_WS_EVERY (default) before every potential solve;
+ * MM_WS_MEASURED per solve in the form, warm-started or plain, with the lower mean wall
+ * time (warm start plus PCG, timed on the host) over the solves that used it; the first
+ * 3 solves of each form alternate and every 16th solve takes the other form.  The
+ * answer is the plain method's either way. */
+#define MM_WS_EVERY 0
+#define MM_WS_MEASURED 1
+int mm_warm_start_policy(mm_ctx *ctx, int32_t policy);
+/* Potential solves and warm-started ones since the policy was set, with the mean wall
+ * seconds of plain and warm-started solves (0 for a form no solve has used). */
+int mm_warm_start_report(mm_ctx *ctx, int64_t *solves, int64_t *warm_solves, double *mean_s_plain,
+                         double *mean_s_warm);
 
 /* HIP-event timing of the two fused PCG kernels since the last mm_timing_enable: the
  * K apply with the search-direction update (SELL SpMV or 7-point stencil, FP64 or the

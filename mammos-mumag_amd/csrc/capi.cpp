@@ -7,6 +7,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <chrono>
 #include <cfloat>
 #include <cmath>
 #include <cstring>
@@ -108,6 +109,10 @@ struct mm_ctx {
     bool ws_have_prev = false;
     double *ws_red = nullptr;        // [0..2] mbar sums, [4] num, [5] unused, [6] den, [7] gamma
     int64_t ws_calls = 0;
+    // MM_WS_MEASURED: host-timed solves with and without the warm start (running means)
+    int ws_policy = 0;
+    int64_t ws_solves = 0, ws_n[2] = {0, 0};
+    double ws_mean[2] = {0.0, 0.0};
 
     Launch L() const {
         Launch l;
@@ -463,6 +468,38 @@ void learned_warm_start(mm_ctx *c, const double *m, const double *b, double *u) 
     c->ws_calls++;
 }
 
+// MM_WS_EVERY: every solve is warm-started.  MM_WS_MEASURED: each solve takes the form
+// (warm-started or plain) with the lower mean wall time of the solves that used it, the
+// warm start plus the PCG, measured on the host; the first 3 solves of each form
+// alternate, and every 16th solve takes the other form so that both means stay current.
+bool ws_choose(mm_ctx *c) {
+    if (c->ws_policy != MM_WS_MEASURED) return true;
+    const int64_t k = c->ws_solves;
+    if (c->ws_n[0] < 3 || c->ws_n[1] < 3) return (k & 1) == 0;
+    const bool best = c->ws_mean[1] < c->ws_mean[0];
+    return (k % 16 == 15) ? !best : best;
+}
+void ws_record(mm_ctx *c, bool ws, double seconds) {
+    const int a = ws ? 1 : 0;
+    c->ws_n[a]++;
+    const double w = c->ws_n[a] < 8 ? 1.0 / c->ws_n[a] : 0.125;   // mean, then a running mean
+    c->ws_mean[a] += w * (seconds - c->ws_mean[a]);
+    c->ws_solves++;
+}
+
+// u = K^-1 b for b = D m (already in c->b), from the caller's u, with the learned warm
+// start when a network is enabled and the policy picks it.
+int potential_solve(mm_ctx *c, const double *m, double *u, double tol, int *it) {
+    const bool ws = c->ws_net && ws_choose(c);
+    const auto t0 = std::chrono::steady_clock::now();
+    if (ws) learned_warm_start(c, m, c->b, u);
+    else c->ws_have_prev = false;   // the next direction starts from the prediction
+    const int rc = pcg(c, c->b, u, tol, 0.0, 10000, it);   // returns after the solve has ended
+    if (c->ws_net && rc >= 0)
+        ws_record(c, ws, std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count());
+    return rc;
+}
+
 // E, g_proj, g_full for magnetisation m (energies.py:67-99).
 int total_eg(mm_ctx *c, const double *m, double hext, const double *hdir, double tol_u,
              int hmag_on, double *u, double *E, double *gproj, double *gfull, int *pcg_it) {
@@ -470,8 +507,7 @@ int total_eg(mm_ctx *c, const double *m, double hext, const double *hdir, double
     op_C(c, m, c->gC, hmag_on > 0 ? c->b : nullptr, nullptr);
     int it = 0, rc = 0;
     if (hmag_on > 0) {
-        if (c->ws_net) learned_warm_start(c, m, c->b, u);
-        rc = pcg(c, c->b, u, tol_u, 0.0, 10000, &it);
+        rc = potential_solve(c, m, u, tol_u, &it);
         if (rc < 0) return rc;
         op_grad(c, u, c->gd);
     }
@@ -898,9 +934,8 @@ int mm_hmag_eg(mm_ctx *c, const double *m, double *u, double tol, double *E_host
                int32_t *pcg_iters_host) {
     if (!c || !m || !u || !E_host || !g) return fail(MM_E_ARG, "null argument");
     op_C(c, m, c->gC, c->b, nullptr);
-    if (c->ws_net) learned_warm_start(c, m, c->b, u);
     int it = 0;
-    int rc = pcg(c, c->b, u, tol, 0.0, 10000, &it);
+    int rc = potential_solve(c, m, u, tol, &it);
     if (rc < 0) return rc;
     HIPC(hipMemsetAsync(g, 0, 3 * (size_t)c->N * sizeof(double), c->s));
     op_grad(c, u, g);
@@ -1415,6 +1450,25 @@ int mm_warm_start(mm_ctx *c, mm_mlp *w, double scale) {
     c->ws_scale = scale;
     c->ws_have_prev = false;
     c->ws_calls = 0;
+    return 0;
+}
+
+int mm_warm_start_policy(mm_ctx *c, int32_t policy) {
+    if (!c) return fail(MM_E_ARG, "null context");
+    if (policy != MM_WS_EVERY && policy != MM_WS_MEASURED) return fail(MM_E_ARG, "unknown warm-start policy");
+    c->ws_policy = policy;
+    c->ws_solves = c->ws_n[0] = c->ws_n[1] = 0;
+    c->ws_mean[0] = c->ws_mean[1] = 0.0;
+    return 0;
+}
+
+int mm_warm_start_report(mm_ctx *c, int64_t *solves, int64_t *warm_solves, double *mean_s_plain,
+                         double *mean_s_warm) {
+    if (!c) return fail(MM_E_ARG, "null context");
+    if (solves) *solves = c->ws_solves;
+    if (warm_solves) *warm_solves = c->ws_n[1];
+    if (mean_s_plain) *mean_s_plain = c->ws_mean[0];
+    if (mean_s_warm) *mean_s_warm = c->ws_mean[1];
     return 0;
 }
 

@@ -115,10 +115,13 @@ SIGNATURES = {
     "mm_mg_apply": [VP, VP, VP],
     "mm_mg_levels": [VP, VP, VP],
     "mm_warm_start_stats": [VP, ctypes.POINTER(ctypes.c_int64), c_double_p],
+    "mm_warm_start_policy": [VP, ctypes.c_int32],
+    "mm_warm_start_report": [VP, VP, VP, VP, VP],
 }
 
 MLP_IN = 32
 MLP_PARAMS = 37377
+WS_EVERY, WS_MEASURED = 0, 1
 
 _lib = None
 

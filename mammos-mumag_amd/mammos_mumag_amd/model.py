@@ -401,6 +401,21 @@ class Model:
                    "mm_warm_start_stats")
         return dict(calls=calls.value, last_gamma=gamma.value)
 
+    def warm_start_policy(self, policy="every"):
+        """``"every"``: the network warm-starts every potential solve; ``"measured"``: each
+        solve takes the faster form (warm-started or plain) by measured solve time."""
+        code = {"every": _lib.WS_EVERY, "measured": _lib.WS_MEASURED}[policy]
+        _lib.check(self._lib.mm_warm_start_policy(self._ctx, code), "mm_warm_start_policy")
+
+    def warm_start_report(self):
+        """Solves and warm-started solves since the policy was set, with the mean wall
+        seconds of each form."""
+        n, nw = ctypes.c_int64(), ctypes.c_int64()
+        tp, tw = ctypes.c_double(), ctypes.c_double()
+        _lib.check(self._lib.mm_warm_start_report(self._ctx, ctypes.byref(n), ctypes.byref(nw), ctypes.byref(tp),
+                                                  ctypes.byref(tw)), "mm_warm_start_report")
+        return dict(solves=n.value, warm_solves=nw.value, mean_s_plain=tp.value, mean_s_warm=tw.value)
+
     def timing(self, on=True):
         _lib.check(self._lib.mm_timing_enable(self._ctx, int(on)), "mm_timing_enable")
 

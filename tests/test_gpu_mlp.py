@@ -129,8 +129,9 @@ def test_training_reduces_loss(model):
 
 def test_warm_start_ablation(mesh):
     """Train on (m, u) pairs of another microstructure's loop, then run the same field
-    steps on this one with and without the learned warm start: the answers agree to the
-    solver tolerance; the iteration counts show what the network gains."""
+    steps on this one without the learned warm start, with it before every solve, and
+    with the measured policy (per solve the faster form): the answers agree to the solver
+    tolerance; the iteration counts and solve times show what the network gains."""
     import torch
     from mammos_mumag_amd import Model, _lib, generate
     from mammos_mumag_amd.learned import Network, collect, train
@@ -146,13 +147,15 @@ def test_warm_start_ablation(mesh):
     src.close()
 
     results = {}
-    for learned in (False, True):
+    for learned in (False, True, "measured"):
         model = Model.from_structured(mesh)
         if learned:
             n2 = Network(model, max_batch=32, seed=0)
             n2.set_params(params)
             n2.scale = scale
             model.warm_start(n2)
+            if learned == "measured":
+                model.warm_start_policy("measured")
         m = model.to_device(model.uniform_m(p["m"]))
         m_ref, uu, gF = m.clone(), model.empty1(), model.empty3()
         st = _lib.Stats()
@@ -161,9 +164,18 @@ def test_warm_start_ablation(mesh):
             E = model.minimize(m, hext, p["h"], p, m_ref, uu, gF, st)
             rec.append((E, model.compute_mh(m, p["h"])))
             hext += -0.4
-        results[learned] = (np.array(rec), st.as_dict(), model.warm_start_stats() if learned else None)
+        results[learned] = (np.array(rec), st.as_dict(),
+                            dict(model.warm_start_stats(), **model.warm_start_report()) if learned else None)
         model.close()
     (r0, s0, _), (r1, s1, ws) = results[False], results[True]
+    (r2, s2, wm) = results["measured"]
+    print(f"[gpu] measured policy: {wm['calls']} of {wm['solves']} solves warm-started, mean "
+          f"{1e3 * wm['mean_s_plain']:.2f} ms plain / {1e3 * wm['mean_s_warm']:.2f} ms warm-started", flush=True)
+    # every solve is timed; both forms are measured at least 3 times
+    assert wm["solves"] == s2["pcg_calls"] and wm["calls"] == wm["warm_solves"]
+    assert 3 <= wm["warm_solves"] <= wm["solves"] - 3
+    assert np.max(np.abs(r2[:, 1] - r0[:, 1])) < 1e-6
+    assert np.max(np.abs(r2[:, 0] / r0[:, 0] - 1)) < 1e-8
     print(f"[gpu] plain PCG iterations {s0['pcg_iters']}, learned warm start {s1['pcg_iters']} "
           f"({ws['calls']} warm-started solves); NCG {s0['ncg_iters']} / {s1['ncg_iters']}", flush=True)
     assert ws["calls"] == s1["pcg_calls"]
