@@ -93,12 +93,15 @@ struct mm_ctx {
     // multigrid preconditioner of the FP32 inner solve (structured grids)
     std::vector<double> axis[3];        // node coordinates per axis (host)
     int mg_nl = 0;                      // levels (0: Jacobi)
-    int mg_batch = 4;                   // inner iterations between host checks with multigrid
+    int mg_batch = 8;                   // most inner iterations between host checks with multigrid
+    double mg_rate2 = 0.05;             // running r.r reduction per multigrid-PCG iteration
     int mg_deg = 3, mg_cdeg = 6;        // smoothing degree, coarsest-level polynomial degree
     MgLevelDev mg[kMgMaxLevels];
     float *mg_b[kMgMaxLevels] = {}, *mg_xa[kMgMaxLevels] = {}, *mg_xb[kMgMaxLevels] = {},
           *mg_r[kMgMaxLevels] = {};
     float mg_om[16] = {}, mg_omc[16] = {};
+    int mg_tail = 0;                    // levels >= mg_tail run in the one-workgroup tail kernel
+    MgTailArgs tail;
 
     // learned warm start (structured grids): network, features, predictions
     FeatGrid F;
@@ -345,14 +348,23 @@ float *mg_vcycle(mm_ctx *c, const float *r, float *z, float *alt, const int *ski
         if (last0 && fuse) np = nb;
         std::swap(cur[l], oth[l]);
     };
+    // levels below top run one kernel per step; from top on, the tail kernel (or, with no
+    // tail, the coarsest level's polynomial)
+    const bool tail = c->mg_tail < nl;
+    const int top = tail ? c->mg_tail : nl - 1;
     // pre-smoothing (the first step of every level is already in cur[l]), residual, restriction
-    for (int l = 0; l + 1 < nl; ++l) {
+    for (int l = 0; l < top; ++l) {
         for (int k = 1; k < s; ++k) smooth(l, c->mg_om[k], false);
         launch_mg_resid(L, c->mg[l].P, B[l], cur[l], c->mg_r[l], skip);
         launch_mg_restrict(L, c->mg[l], c->mg[l + 1], c->mg_r[l], c->mg_b[l + 1], cur[l + 1], skip);
     }
-    for (int k = 1; k < cd; ++k) smooth(nl - 1, c->mg_omc[k], nl == 1 && k == cd - 1);
-    for (int l = nl - 2; l >= 0; --l) {
+    if (tail) {
+        launch_mg_tail(L, c->tail, skip);   // reads mg_b[top], writes mg_xa[top]
+        cur[top] = c->mg_xa[top];
+    } else {
+        for (int k = 1; k < cd; ++k) smooth(nl - 1, c->mg_omc[k], nl == 1 && k == cd - 1);
+    }
+    for (int l = top - 1; l >= 0; --l) {
         launch_mg_prolong(L, c->mg[l], c->mg[l + 1], cur[l + 1], cur[l], skip);
         for (int k = s - 1; k >= 0; --k) smooth(l, c->mg_om[k], l == 0 && k == 0);
     }
@@ -397,8 +409,19 @@ int pcg_mixed(mm_ctx *c, const double *b, double *x, double tol, double atol, in
         // last true residual (checked between batches) or claims convergence
         const double rmax2 = c->hsc->ir_rmax2;
         const int seg0 = i;
+        // multigrid: each batch is sized to reach the segment's target at the measured
+        // convergence rate (rr per iteration, a running value over segments), so that few
+        // iterations are queued past the stop and few host checks are needed
+        const double target2 = std::max(delta2 * rmax2, c->hsc->ir_bound);
+        double rr_ref = rmax2;   // r.r at iteration j_ref (the replaced residual at seg0)
+        int j_ref = seg0;
         for (;;) {
-            const int batch = mg ? c->mg_batch : c->pcg_batch;
+            int batch = c->pcg_batch;
+            if (mg) {
+                const double need = std::log(target2 / rr_ref) / std::log(c->mg_rate2);
+                batch = (int)std::ceil(std::max(need, 0.0)) + (i == seg0 ? 1 : 0);
+                batch = std::min(std::max(batch, 1), c->mg_batch);
+            }
             for (int k = 0; k < batch; ++k, ++i) {
                 pcg_iteration<float>(c, i, mode, tol, atol, maxiter, c->z32, pb, c->q32, c->dx32, c->r32, invD,
                                      c->K32, c->P32, &nB);
@@ -406,6 +429,12 @@ int pcg_mixed(mm_ctx *c, const double *b, double *x, double tol, double atol, in
                 mode = PCG_NORMAL;
             }
             if ((rc = sync_scalars(c))) return rc;
+            if (mg && !c->hsc->pcg_done && i - 1 > j_ref && c->hsc->pcg_rr > 0.0 && rr_ref > 0.0) {
+                const double r2 = std::pow(c->hsc->pcg_rr / rr_ref, 1.0 / (i - 1 - j_ref));
+                c->mg_rate2 = std::min(std::max(0.5 * c->mg_rate2 + 0.5 * r2, 1e-6), 0.9);
+                rr_ref = c->hsc->pcg_rr;
+                j_ref = i - 1;
+            }
             if (c->hsc->pcg_done || c->hsc->pcg_rr < delta2 * rmax2) break;
         }
         // the stopped iteration index; the next segment continues from it with p kept
@@ -1268,6 +1297,37 @@ int mm_multigrid(mm_ctx *c, int32_t max_levels) {
         HIPC(hipStreamSynchronize(c->s));
     }
     c->mg_nl = nl;
+    // the coarse tail: the most levels (at least one, not level 0) whose vectors fit the
+    // tail kernel's LDS
+    c->mg_tail = nl;
+    auto nodes = [&](int l) { return c->mg[l].P.nx * c->mg[l].P.ny * c->mg[l].P.nz; };
+    for (int t = nl - 1; t >= 1 && nl - t <= kMgTailMax; --t) {
+        int64_t bytes = 0;
+        for (int l = t; l < nl; ++l) bytes += (l == t ? 2 : 3) * (int64_t)nodes(l) * (int64_t)sizeof(float);
+        if (bytes > kMgTailLdsBytes) break;
+        c->mg_tail = t;
+    }
+    if (c->mg_tail < nl) {
+        MgTailArgs &a = c->tail;
+        a = MgTailArgs{};
+        a.nt = nl - c->mg_tail;
+        a.s = c->mg_deg;
+        a.cd = c->mg_cdeg;
+        for (int k = 0; k < 8; ++k) {
+            a.om[k] = c->mg_om[k];
+            a.omc[k] = c->mg_omc[k];
+        }
+        int off = 0;
+        for (int q = 0; q < a.nt; ++q) {
+            a.lev[q] = c->mg[c->mg_tail + q];
+            a.n[q] = nodes(c->mg_tail + q);
+            a.off[q] = off;
+            off += (q == 0 ? 2 : 3) * a.n[q];
+        }
+        a.b0 = c->mg_b[c->mg_tail];
+        a.x0 = c->mg_xa[c->mg_tail];
+        if (mg_tail_lds_bytes(a) > kMgTailLdsBytes) return fail(MM_E_STATE, "multigrid: tail exceeds its LDS");
+    }
     // a multigrid-preconditioned FP32 recursion stays accurate over a longer stretch:
     // replace the residual every 1e-3 of reduction unless the caller chose otherwise
     if (!c->ru_delta_set) c->ru_delta = 1e-3;

@@ -22,6 +22,23 @@ struct MgLevelDev {
     const float *pxt = nullptr, *pyt = nullptr, *pzt = nullptr;
 };
 
+// The last levels of the V-cycle in one workgroup with their vectors in LDS: b0 is the
+// right-hand side of the first tail level (global), x0 receives its result (all of it,
+// zero on the faces); off[q] are the LDS offsets (floats) of tail level q's vectors
+// (x, y, and b for q > 0), n[q] its node count.
+constexpr int kMgTailMax = 4;
+constexpr int kMgTailLdsBytes = 64 * 1024;
+struct MgTailArgs {
+    int nt = 0, s = 0, cd = 0;
+    int n[kMgTailMax] = {}, off[kMgTailMax] = {};
+    MgLevelDev lev[kMgTailMax];
+    const float *b0 = nullptr;
+    float *x0 = nullptr;
+    float om[8] = {}, omc[8] = {};
+};
+int mg_tail_lds_bytes(const MgTailArgs &a);
+void launch_mg_tail(const Launch &L, const MgTailArgs &a, const int *skip);
+
 void launch_mg_first(const Launch &L, const PoissonGrid<float> &P, const float *b, float *out, float w,
                      const int *skip);
 // returns the number of blocks (= partials written when dot_parts != null)

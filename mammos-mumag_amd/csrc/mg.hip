@@ -2,7 +2,8 @@
 // grid: the preconditioner of the FP32 inner solve of the mixed-precision PCG (no
 // counterpart in the reference, whose potential solve is Jacobi-preconditioned).
 //
-// Levels: the node set of every axis keeps its even indices and its last index; each
+// Levels: spacing-aware coarsening per axis (two adjacent intervals merge while both are
+// at most 3 times the level's smallest spacing), down to at most 16 nodes per axis; each
 // level has the 7-point operator of its own rectilinear grid (same coefficients as
 // k_grid_poisson), Dirichlet (zero) on the six faces.  Transfers are per-axis linear
 // interpolation (prolongation) and its transpose (restriction).  Smoothing: Chebyshev-
@@ -125,9 +126,11 @@ __global__ __launch_bounds__(256) void k_mg_stencil(PoissonGrid<float> P, const 
 }
 
 // b_c = R r_f (transpose of the per-axis linear interpolation); zero on the coarse faces.
-// Per coarse index I of an axis, up to 3 (fine index, weight) pairs at [3 I + q].
-// Also x1c = invw_c b_c, the coarse level's first smoothing step from zero (invw_c: the
-// level's first Chebyshev weight over the diagonal, 0 on the faces).
+// Per coarse index I of an axis, up to 3 (fine index, weight) pairs at [3 I + q] (unused
+// pairs: index 0, weight 0).  Also x1c = invw_c b_c, the coarse level's first smoothing
+// step from zero (invw_c: the level's first Chebyshev weight over the diagonal, 0 on the
+// faces).  Tiles of 64 x 4 coarse (x, y) columns, each thread marching a z chunk; the
+// z and y pairs are uniform over a wave, the x pairs per lane.
 __global__ __launch_bounds__(256) void k_mg_restrict(int nxc, int nyc, int nzc, int nxf, int nyf,
                                                      const int *__restrict__ rxi, const float *__restrict__ rxw,
                                                      const int *__restrict__ ryi, const float *__restrict__ ryw,
@@ -136,29 +139,39 @@ __global__ __launch_bounds__(256) void k_mg_restrict(int nxc, int nyc, int nzc, 
                                                      const float *__restrict__ invwc, float *__restrict__ x1c,
                                                      const int *skip) {
     if (skip && *skip) return;
-    const int64_t nc = (int64_t)nxc * nyc * nzc;
-    const int64_t sf = (int64_t)nxf * nyf;
-    for (int64_t idx = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; idx < nc;
-         idx += (int64_t)gridDim.x * blockDim.x) {
-        const int I = (int)(idx % nxc), J = (int)((idx / nxc) % nyc), K = (int)(idx / ((int64_t)nxc * nyc));
+    const int I = blockIdx.x * 64 + (threadIdx.x & 63);
+    const int J = blockIdx.y * 4 + (threadIdx.x >> 6);
+    const int chunk = (nzc + gridDim.z - 1) / gridDim.z;
+    const int k0 = blockIdx.z * chunk, k1 = min(nzc, k0 + chunk);
+    if (I >= nxc || J >= nyc) return;
+    const bool edge = I == 0 || J == 0 || I == nxc - 1 || J == nyc - 1;
+    const int64_t sf = (int64_t)nxf * nyf, sc = (int64_t)nxc * nyc;
+    int xi[3], yo[3];
+    float xw[3], yw[3];
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+        xi[q] = rxi[3 * I + q];
+        xw[q] = rxw[3 * I + q];
+        yo[q] = nxf * ryi[3 * J + q];
+        yw[q] = ryw[3 * J + q];
+    }
+    for (int K = k0; K < k1; ++K) {
+        const int64_t idx = I + (int64_t)nxc * J + sc * K;
         float s = 0.f;
-        if (!(I == 0 || J == 0 || K == 0 || I == nxc - 1 || J == nyc - 1 || K == nzc - 1)) {
+        if (!(edge || K == 0 || K == nzc - 1)) {
 #pragma unroll
             for (int c = 0; c < 3; ++c) {
                 const float wz = rzw[3 * K + c];
                 if (wz == 0.f) continue;
-                const int64_t oz = sf * rzi[3 * K + c];
+                const float *pz = rf + sf * rzi[3 * K + c];
+                float sz = 0.f;
 #pragma unroll
                 for (int bq = 0; bq < 3; ++bq) {
-                    const float wy = ryw[3 * J + bq];
-                    if (wy == 0.f) continue;
-                    const int64_t oy = oz + (int64_t)nxf * ryi[3 * J + bq];
-#pragma unroll
-                    for (int a = 0; a < 3; ++a) {
-                        const float wx = rxw[3 * I + a];
-                        if (wx != 0.f) s += wz * wy * wx * rf[oy + rxi[3 * I + a]];
-                    }
+                    if (yw[bq] == 0.f) continue;
+                    const float *py = pz + yo[bq];
+                    sz += yw[bq] * (xw[0] * py[xi[0]] + xw[1] * py[xi[1]] + xw[2] * py[xi[2]]);
                 }
+                s += wz * sz;
             }
         }
         bc[idx] = s;
@@ -166,8 +179,10 @@ __global__ __launch_bounds__(256) void k_mg_restrict(int nxc, int nyc, int nzc, 
     }
 }
 
-// x_f += P x_c: per fine index i of an axis, coarse c0[i] with weight 1 - t[i] and
-// c0[i] + 1 with weight t[i] (t = 0 on coarse nodes).  Fine faces stay zero.
+// x_f += P x_c on the interior: per fine index i of an axis, coarse c0[i] with weight
+// 1 - t[i] and c0[i] + 1 with weight t[i] (t = 0 on coarse nodes; an interior fine node
+// has c0 + 1 inside the coarse axis).  Fine faces stay zero.  Tiles of 64 x 4 fine
+// columns marching z chunks.
 __global__ __launch_bounds__(256) void k_mg_prolong(int nxf, int nyf, int nzf, int nxc, int nyc,
                                                     const int *__restrict__ pxc, const float *__restrict__ pxt,
                                                     const int *__restrict__ pyc, const float *__restrict__ pyt,
@@ -175,33 +190,150 @@ __global__ __launch_bounds__(256) void k_mg_prolong(int nxf, int nyf, int nzf, i
                                                     const float *__restrict__ xc, float *__restrict__ xf,
                                                     const int *skip) {
     if (skip && *skip) return;
-    const int64_t nf = (int64_t)nxf * nyf * nzf;
-    const int64_t sc = (int64_t)nxc * nyc;
-    for (int64_t idx = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; idx < nf;
-         idx += (int64_t)gridDim.x * blockDim.x) {
-        const int i = (int)(idx % nxf), j = (int)((idx / nxf) % nyf), k = (int)(idx / ((int64_t)nxf * nyf));
-        if (i == 0 || j == 0 || k == 0 || i == nxf - 1 || j == nyf - 1 || k == nzf - 1) continue;
-        const int cx = pxc[i], cy = pyc[j], cz = pzc[k];
-        const float tx = pxt[i], ty = pyt[j], tz = pzt[k];
-        float s = 0.f;
-#pragma unroll
-        for (int dz = 0; dz < 2; ++dz) {
-            const float wz = dz ? tz : 1.f - tz;
-            if (wz == 0.f) continue;
-#pragma unroll
-            for (int dy = 0; dy < 2; ++dy) {
-                const float wy = dy ? ty : 1.f - ty;
-                if (wy == 0.f) continue;
-#pragma unroll
-                for (int dx = 0; dx < 2; ++dx) {
-                    const float wx = dx ? tx : 1.f - tx;
-                    if (wx == 0.f) continue;
-                    s += wz * wy * wx * xc[(cx + dx) + (int64_t)nxc * (cy + dy) + sc * (cz + dz)];
-                }
-            }
+    const int i = blockIdx.x * 64 + (threadIdx.x & 63);
+    const int j = blockIdx.y * 4 + (threadIdx.x >> 6);
+    const int chunk = (nzf + gridDim.z - 1) / gridDim.z;
+    const int k0 = max(1, blockIdx.z * chunk), k1 = min(nzf - 1, blockIdx.z * chunk + chunk);
+    if (i <= 0 || j <= 0 || i >= nxf - 1 || j >= nyf - 1) return;
+    const int64_t sf = (int64_t)nxf * nyf, sc = (int64_t)nxc * nyc;
+    const float tx = pxt[i], ty = pyt[j];
+    const int64_t oxy = pxc[i] + (int64_t)nxc * pyc[j];
+    for (int k = k0; k < k1; ++k) {
+        const float tz = pzt[k];
+        const float *c0 = xc + oxy + sc * pzc[k];
+        const float *c1 = c0 + sc;
+        const float a0 = (1.f - ty) * ((1.f - tx) * c0[0] + tx * c0[1]) + ty * ((1.f - tx) * c0[nxc] + tx * c0[nxc + 1]);
+        float s = (1.f - tz) * a0;
+        if (tz != 0.f) {   // uniform over the block
+            const float a1 =
+                (1.f - ty) * ((1.f - tx) * c1[0] + tx * c1[1]) + ty * ((1.f - tx) * c1[nxc] + tx * c1[nxc + 1]);
+            s += tz * a1;
         }
+        const int64_t idx = i + (int64_t)nxf * j + sf * k;
         xf[idx] += s;
     }
+}
+
+// ---------------------------------------------------------------- coarse tail in LDS
+// The last levels of the V-cycle in one workgroup: each level's vectors live in LDS
+// (x and y ping-pong, the right-hand side for all but the first tail level, whose b is
+// read from global memory), and every smoothing step, residual and transfer is one
+// pass of the workgroup over the level's nodes, separated by barriers.  Same arithmetic
+// as the per-level kernels above.
+constexpr int kTailThreads = 1024;
+
+__device__ __forceinline__ void tail_stencil(const PoissonGrid<float> &P, const float *b, const float *x,
+                                             float *out, float w, int op) {
+    const int nx = P.nx, ny = P.ny, nz = P.nz, nxy = nx * ny, n = nxy * nz;
+    for (int t = threadIdx.x; t < n; t += kTailThreads) {
+        const int i = t % nx, j = (t / nx) % ny, k = t / nxy;
+        float o = 0.f;
+        if (!(i == 0 || j == 0 || k == 0 || i == nx - 1 || j == ny - 1 || k == nz - 1)) {
+            const float sx = edge_sum_mg(P.hyp, P.hzp, j, k);
+            const float syy = edge_sum_mg(P.hzp, P.hxp, k, i);
+            const float szz = edge_sum_mg(P.hxp, P.hyp, i, j);
+            const float ixm = P.ixp[i], ixq = P.ixp[i + 1], iym = P.iyp[j], iyq = P.iyp[j + 1];
+            const float izm = P.izp[k], izq = P.izp[k + 1];
+            const float diag = sx * (ixm + ixq) + syy * (iym + iyq) + szz * (izm + izq);
+            const float bi = b[t];
+            if (op == OP_FIRST) {
+                o = w * bi / diag;
+            } else {
+                const float ax = diag * x[t] - sx * (ixm * x[t - 1] + ixq * x[t + 1]) -
+                                 syy * (iym * x[t - nx] + iyq * x[t + nx]) - szz * (izm * x[t - nxy] + izq * x[t + nxy]);
+                o = op == OP_RESID ? bi - ax : x[t] + w * (bi - ax) / diag;
+            }
+        }
+        out[t] = o;
+    }
+    __syncthreads();
+}
+
+// bc = R rf (f: the fine level with its transfer lists), zero on the coarse faces
+__device__ __forceinline__ void tail_restrict(const MgLevelDev &f, const PoissonGrid<float> &C, const float *rf,
+                                              float *bc) {
+    const int nxc = C.nx, nyc = C.ny, nzc = C.nz, nxyc = nxc * nyc, nc = nxyc * nzc;
+    const int nxf = f.P.nx, nxyf = f.P.nx * f.P.ny;
+    for (int t = threadIdx.x; t < nc; t += kTailThreads) {
+        const int I = t % nxc, J = (t / nxc) % nyc, K = t / nxyc;
+        float s = 0.f;
+        if (!(I == 0 || J == 0 || K == 0 || I == nxc - 1 || J == nyc - 1 || K == nzc - 1)) {
+            for (int c = 0; c < 3; ++c) {
+                const float wz = f.rzw[3 * K + c];
+                if (wz == 0.f) continue;
+                const float *pz = rf + nxyf * f.rzi[3 * K + c];
+                float sz = 0.f;
+                for (int q = 0; q < 3; ++q) {
+                    const float wy = f.ryw[3 * J + q];
+                    if (wy == 0.f) continue;
+                    const float *py = pz + nxf * f.ryi[3 * J + q];
+                    sz += wy * (f.rxw[3 * I] * py[f.rxi[3 * I]] + f.rxw[3 * I + 1] * py[f.rxi[3 * I + 1]] +
+                                f.rxw[3 * I + 2] * py[f.rxi[3 * I + 2]]);
+                }
+                s += wz * sz;
+            }
+        }
+        bc[t] = s;
+    }
+    __syncthreads();
+}
+
+// xf += P xc on the interior of the fine level f
+__device__ __forceinline__ void tail_prolong(const MgLevelDev &f, const PoissonGrid<float> &C, const float *xc,
+                                             float *xf) {
+    const int nx = f.P.nx, ny = f.P.ny, nz = f.P.nz, nxy = nx * ny, n = nxy * nz;
+    const int nxc = C.nx, nxyc = C.nx * C.ny;
+    for (int t = threadIdx.x; t < n; t += kTailThreads) {
+        const int i = t % nx, j = (t / nx) % ny, k = t / nxy;
+        if (i == 0 || j == 0 || k == 0 || i == nx - 1 || j == ny - 1 || k == nz - 1) continue;
+        const float tx = f.pxt[i], ty = f.pyt[j], tz = f.pzt[k];
+        const float *c0 = xc + f.pxc[i] + nxc * f.pyc[j] + nxyc * f.pzc[k];
+        const float *c1 = c0 + nxyc;
+        const float a0 = (1.f - ty) * ((1.f - tx) * c0[0] + tx * c0[1]) + ty * ((1.f - tx) * c0[nxc] + tx * c0[nxc + 1]);
+        const float a1 = (1.f - ty) * ((1.f - tx) * c1[0] + tx * c1[1]) + ty * ((1.f - tx) * c1[nxc] + tx * c1[nxc + 1]);
+        xf[t] += (1.f - tz) * a0 + tz * a1;
+    }
+    __syncthreads();
+}
+
+__global__ __launch_bounds__(kTailThreads) void k_mg_tail(MgTailArgs a, const int *skip) {
+    if (skip && *skip) return;
+    extern __shared__ float lds[];
+    float *cur[kMgTailMax], *oth[kMgTailMax];
+    const float *B[kMgTailMax];
+    for (int q = 0; q < a.nt; ++q) {
+        cur[q] = lds + a.off[q];
+        oth[q] = lds + a.off[q] + a.n[q];
+        B[q] = q == 0 ? a.b0 : lds + a.off[q] + 2 * a.n[q];
+    }
+    for (int q = 0; q + 1 < a.nt; ++q) {
+        const PoissonGrid<float> &P = a.lev[q].P;
+        tail_stencil(P, B[q], nullptr, cur[q], a.om[0], OP_FIRST);
+        for (int k = 1; k < a.s; ++k) {
+            tail_stencil(P, B[q], cur[q], oth[q], a.om[k], OP_SMOOTH);
+            float *t = cur[q]; cur[q] = oth[q]; oth[q] = t;
+        }
+        tail_stencil(P, B[q], cur[q], oth[q], 0.f, OP_RESID);
+        tail_restrict(a.lev[q], a.lev[q + 1].P, oth[q], (float *)B[q + 1]);
+    }
+    {
+        const int q = a.nt - 1;
+        const PoissonGrid<float> &P = a.lev[q].P;
+        tail_stencil(P, B[q], nullptr, cur[q], a.omc[0], OP_FIRST);
+        for (int k = 1; k < a.cd; ++k) {
+            tail_stencil(P, B[q], cur[q], oth[q], a.omc[k], OP_SMOOTH);
+            float *t = cur[q]; cur[q] = oth[q]; oth[q] = t;
+        }
+    }
+    for (int q = a.nt - 2; q >= 0; --q) {
+        const PoissonGrid<float> &P = a.lev[q].P;
+        tail_prolong(a.lev[q], a.lev[q + 1].P, cur[q + 1], cur[q]);
+        for (int k = a.s - 1; k >= 0; --k) {
+            tail_stencil(P, B[q], cur[q], oth[q], a.om[k], OP_SMOOTH);
+            float *t = cur[q]; cur[q] = oth[q]; oth[q] = t;
+        }
+    }
+    for (int t = threadIdx.x; t < a.n[0]; t += kTailThreads) a.x0[t] = cur[0][t];
 }
 
 // At most kMaxPartials blocks (the fused dot writes one partial per block); the setup
@@ -213,10 +345,6 @@ dim3 stencil_grid(const PoissonGrid<float> &P) {
     return dim3(gx, gy, gz);
 }
 
-int blocks_for(int64_t n) {
-    const int64_t b = (n + 255) / 256;
-    return (int)(b < 1 ? 1 : b > 8192 ? 8192 : b);
-}
 
 }  // namespace
 
@@ -240,19 +368,25 @@ void launch_mg_resid(const Launch &L, const PoissonGrid<float> &P, const float *
     hipLaunchKernelGGL((k_mg_stencil<OP_RESID, false>), stencil_grid(P), dim3(256), 0, L.s, P, b, x, out, 0.f, skip,
                        (double *)nullptr);
 }
+int mg_tail_lds_bytes(const MgTailArgs &a) {
+    int f = 0;
+    for (int q = 0; q < a.nt; ++q) f += (q == 0 ? 2 : 3) * a.n[q];
+    return f * (int)sizeof(float);
+}
+void launch_mg_tail(const Launch &L, const MgTailArgs &a, const int *skip) {
+    hipLaunchKernelGGL(k_mg_tail, dim3(1), dim3(kTailThreads), mg_tail_lds_bytes(a), L.s, a, skip);
+}
 bool mg_stencil_fits(const PoissonGrid<float> &P) {
     return (int64_t)((P.nx + 63) / 64) * ((P.ny + 3) / 4) <= kMaxPartials;
 }
 void launch_mg_restrict(const Launch &L, const MgLevelDev &f, const MgLevelDev &c, const float *rf, float *bc,
                         float *x1c, const int *skip) {
-    const int64_t nc = (int64_t)c.P.nx * c.P.ny * c.P.nz;
-    hipLaunchKernelGGL(k_mg_restrict, dim3(blocks_for(nc)), dim3(256), 0, L.s, c.P.nx, c.P.ny, c.P.nz, f.P.nx,
+    hipLaunchKernelGGL(k_mg_restrict, stencil_grid(c.P), dim3(256), 0, L.s, c.P.nx, c.P.ny, c.P.nz, f.P.nx,
                        f.P.ny, f.rxi, f.rxw, f.ryi, f.ryw, f.rzi, f.rzw, rf, bc, c.invw, x1c, skip);
 }
 void launch_mg_prolong(const Launch &L, const MgLevelDev &f, const MgLevelDev &c, const float *xc, float *xf,
                        const int *skip) {
-    const int64_t nf = (int64_t)f.P.nx * f.P.ny * f.P.nz;
-    hipLaunchKernelGGL(k_mg_prolong, dim3(blocks_for(nf)), dim3(256), 0, L.s, f.P.nx, f.P.ny, f.P.nz, c.P.nx,
+    hipLaunchKernelGGL(k_mg_prolong, stencil_grid(f.P), dim3(256), 0, L.s, f.P.nx, f.P.ny, f.P.nz, c.P.nx,
                        c.P.ny, f.pxc, f.pxt, f.pyc, f.pyt, f.pzc, f.pzt, xc, xf, skip);
 }
 
