@@ -1300,33 +1300,29 @@ int mm_multigrid(mm_ctx *c, int32_t max_levels) {
     // the coarse tail: the most levels (at least one, not level 0) whose vectors fit the
     // tail kernel's LDS
     c->mg_tail = nl;
-    auto nodes = [&](int l) { return c->mg[l].P.nx * c->mg[l].P.ny * c->mg[l].P.nz; };
-    for (int t = nl - 1; t >= 1 && nl - t <= kMgTailMax; --t) {
-        int64_t bytes = 0;
-        for (int l = t; l < nl; ++l) bytes += (l == t ? 2 : 3) * (int64_t)nodes(l) * (int64_t)sizeof(float);
-        if (bytes > kMgTailLdsBytes) break;
-        c->mg_tail = t;
-    }
-    if (c->mg_tail < nl) {
-        MgTailArgs &a = c->tail;
-        a = MgTailArgs{};
-        a.nt = nl - c->mg_tail;
+    auto tail_args = [&](int t) {
+        MgTailArgs a;
+        a.nt = nl - t;
         a.s = c->mg_deg;
         a.cd = c->mg_cdeg;
         for (int k = 0; k < 8; ++k) {
             a.om[k] = c->mg_om[k];
             a.omc[k] = c->mg_omc[k];
         }
-        int off = 0;
         for (int q = 0; q < a.nt; ++q) {
-            a.lev[q] = c->mg[c->mg_tail + q];
-            a.n[q] = nodes(c->mg_tail + q);
-            a.off[q] = off;
-            off += (q == 0 ? 2 : 3) * a.n[q];
+            a.lev[q] = c->mg[t + q];
+            a.n[q] = c->mg[t + q].P.nx * c->mg[t + q].P.ny * c->mg[t + q].P.nz;
         }
-        a.b0 = c->mg_b[c->mg_tail];
-        a.x0 = c->mg_xa[c->mg_tail];
-        if (mg_tail_lds_bytes(a) > kMgTailLdsBytes) return fail(MM_E_STATE, "multigrid: tail exceeds its LDS");
+        a.b0 = c->mg_b[t];
+        a.x0 = c->mg_xa[t];
+        mg_tail_layout(a);
+        return a;
+    };
+    for (int t = nl - 1; t >= 1 && nl - t <= kMgTailMax; --t) {
+        const MgTailArgs a = tail_args(t);
+        if (a.lds_bytes > kMgTailLdsBytes) break;
+        c->mg_tail = t;
+        c->tail = a;
     }
     // a multigrid-preconditioned FP32 recursion stays accurate over a longer stretch:
     // replace the residual every 1e-3 of reduction unless the caller chose otherwise

@@ -24,19 +24,21 @@ struct MgLevelDev {
 
 // The last levels of the V-cycle in one workgroup with their vectors in LDS: b0 is the
 // right-hand side of the first tail level (global), x0 receives its result (all of it,
-// zero on the faces); off[q] are the LDS offsets (floats) of tail level q's vectors
-// (x, y, and b for q > 0), n[q] its node count.
+// zero on the faces); n[q] is tail level q's node count, off[q] the LDS offset (floats)
+// of its vectors (x, y, and b for q > 0), coef_off[q] of its stencil factors and
+// tr_off[q] of its transfer lists (mg_tail_layout fills the offsets and lds_bytes).
 constexpr int kMgTailMax = 4;
 constexpr int kMgTailLdsBytes = 64 * 1024;
 struct MgTailArgs {
     int nt = 0, s = 0, cd = 0;
-    int n[kMgTailMax] = {}, off[kMgTailMax] = {};
+    int n[kMgTailMax] = {}, off[kMgTailMax] = {}, coef_off[kMgTailMax] = {}, tr_off[kMgTailMax] = {};
+    int lds_bytes = 0;
     MgLevelDev lev[kMgTailMax];
     const float *b0 = nullptr;
     float *x0 = nullptr;
     float om[8] = {}, omc[8] = {};
 };
-int mg_tail_lds_bytes(const MgTailArgs &a);
+int mg_tail_layout(MgTailArgs &a);
 void launch_mg_tail(const Launch &L, const MgTailArgs &a, const int *skip);
 
 void launch_mg_first(const Launch &L, const PoissonGrid<float> &P, const float *b, float *out, float w,

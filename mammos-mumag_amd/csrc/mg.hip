@@ -217,123 +217,250 @@ __global__ __launch_bounds__(256) void k_mg_prolong(int nxf, int nyf, int nzf, i
 // ---------------------------------------------------------------- coarse tail in LDS
 // The last levels of the V-cycle in one workgroup: each level's vectors live in LDS
 // (x and y ping-pong, the right-hand side for all but the first tail level, whose b is
-// read from global memory), and every smoothing step, residual and transfer is one
-// pass of the workgroup over the level's nodes, separated by barriers.  Same arithmetic
-// as the per-level kernels above.
+// read from global memory), with the level's stencil factors (the three edge-sum planes
+// and the 1/(6h) rows) and the transfer lists copied to LDS once; every smoothing step,
+// residual and transfer is one pass of the workgroup over the level's nodes, separated
+// by barriers.  Same arithmetic as the per-level kernels above.
 constexpr int kTailThreads = 1024;
 
-__device__ __forceinline__ void tail_stencil(const PoissonGrid<float> &P, const float *b, const float *x,
-                                             float *out, float w, int op) {
-    const int nx = P.nx, ny = P.ny, nz = P.nz, nxy = nx * ny, n = nxy * nz;
+// The tail's dynamic LDS; every tail array is an offset into it (so that the accesses
+// are LDS instructions; integers are stored bit-cast).
+extern __shared__ float tail_lds[];
+
+// A level's stencil factors (offsets): SX[j + ny k] = edge_sum(hy, hz; j, k),
+// SY[k + nz i] = edge_sum(hz, hx; k, i), SZ[i + nx j] = edge_sum(hx, hy; i, j),
+// IX[0..nx] = 1/(6 hx) (0 on the pads), likewise IY, IZ.
+struct TailLevel {
+    int nx, ny, nz;
+    int SX, SY, SZ, IX, IY, IZ;
+};
+// Transfer lists of a fine tail level (offsets; layout of MgLevelDev's lists)
+struct TailTransfer {
+    int rxi, ryi, rzi, rxw, ryw, rzw, pxc, pyc, pzc, pxt, pyt, pzt;
+};
+
+__host__ __device__ inline int tail_coef_floats(int nx, int ny, int nz) {
+    return ny * nz + nz * nx + nx * ny + (nx + 1) + (ny + 1) + (nz + 1);
+}
+__host__ __device__ inline int tail_transfer_words(int nxf, int nyf, int nzf, int nxc, int nyc, int nzc) {
+    return 6 * (nxc + nyc + nzc) + 2 * (nxf + nyf + nzf);
+}
+
+// Node coordinates of t = threadIdx.x, threadIdx.x + kTailThreads, ... on an nx x ny x nz
+// level, advanced without divisions.
+struct TailIter {
+    int i, j, k, si, sj, sk, nx, ny;
+    __device__ TailIter(int nx_, int ny_) : nx(nx_), ny(ny_) {
+        const int r = threadIdx.x / nx;
+        i = threadIdx.x - r * nx;
+        j = r % ny;
+        k = r / ny;
+        const int s = kTailThreads / nx;
+        si = kTailThreads - s * nx;
+        sj = s % ny;
+        sk = s / ny;
+    }
+    __device__ void next() {
+        i += si;
+        j += sj;
+        k += sk;
+        if (i >= nx) {
+            i -= nx;
+            ++j;
+        }
+        if (j >= ny) {
+            j -= ny;
+            ++k;
+        }
+    }
+};
+
+__device__ __forceinline__ float lf(int o) { return tail_lds[o]; }
+__device__ __forceinline__ int li(int o) { return __float_as_int(tail_lds[o]); }
+
+__device__ TailLevel tail_level(int base, int nx, int ny, int nz) {
+    TailLevel t;
+    t.nx = nx;
+    t.ny = ny;
+    t.nz = nz;
+    t.SX = base;
+    t.SY = t.SX + ny * nz;
+    t.SZ = t.SY + nz * nx;
+    t.IX = t.SZ + nx * ny;
+    t.IY = t.IX + nx + 1;
+    t.IZ = t.IY + ny + 1;
+    return t;
+}
+__device__ TailTransfer tail_transfer(int base, int nxf, int nyf, int nzf, int nxc, int nyc, int nzc) {
+    TailTransfer t;
+    t.rxi = base;
+    t.ryi = t.rxi + 3 * nxc;
+    t.rzi = t.ryi + 3 * nyc;
+    t.rxw = t.rzi + 3 * nzc;
+    t.ryw = t.rxw + 3 * nxc;
+    t.rzw = t.ryw + 3 * nyc;
+    t.pxc = t.rzw + 3 * nzc;
+    t.pyc = t.pxc + nxf;
+    t.pzc = t.pyc + nyf;
+    t.pxt = t.pzc + nzf;
+    t.pyt = t.pxt + nxf;
+    t.pzt = t.pyt + nyf;
+    return t;
+}
+
+__device__ void tail_fill_level(const PoissonGrid<float> &P, const TailLevel &T) {
+    const int nx = P.nx, ny = P.ny, nz = P.nz;
+    for (int t = threadIdx.x; t < ny * nz; t += kTailThreads) tail_lds[T.SX + t] = edge_sum_mg(P.hyp, P.hzp, t % ny, t / ny);
+    for (int t = threadIdx.x; t < nz * nx; t += kTailThreads) tail_lds[T.SY + t] = edge_sum_mg(P.hzp, P.hxp, t % nz, t / nz);
+    for (int t = threadIdx.x; t < nx * ny; t += kTailThreads) tail_lds[T.SZ + t] = edge_sum_mg(P.hxp, P.hyp, t % nx, t / nx);
+    for (int t = threadIdx.x; t <= nx; t += kTailThreads) tail_lds[T.IX + t] = P.ixp[t];
+    for (int t = threadIdx.x; t <= ny; t += kTailThreads) tail_lds[T.IY + t] = P.iyp[t];
+    for (int t = threadIdx.x; t <= nz; t += kTailThreads) tail_lds[T.IZ + t] = P.izp[t];
+}
+__device__ void tail_fill_pairs(int n, const int *idx, const float *w, int oi, int ow) {
     for (int t = threadIdx.x; t < n; t += kTailThreads) {
-        const int i = t % nx, j = (t / nx) % ny, k = t / nxy;
+        tail_lds[oi + t] = __int_as_float(idx[t]);
+        tail_lds[ow + t] = w[t];
+    }
+}
+__device__ void tail_fill_transfer(const MgLevelDev &f, const PoissonGrid<float> &C, const TailTransfer &T) {
+    tail_fill_pairs(3 * C.nx, f.rxi, f.rxw, T.rxi, T.rxw);
+    tail_fill_pairs(3 * C.ny, f.ryi, f.ryw, T.ryi, T.ryw);
+    tail_fill_pairs(3 * C.nz, f.rzi, f.rzw, T.rzi, T.rzw);
+    tail_fill_pairs(f.P.nx, f.pxc, f.pxt, T.pxc, T.pxt);
+    tail_fill_pairs(f.P.ny, f.pyc, f.pyt, T.pyc, T.pyt);
+    tail_fill_pairs(f.P.nz, f.pzc, f.pzt, T.pzc, T.pzt);
+}
+
+// out = op(b, x) on level P; b at LDS offset ob, or the global bg when ob < 0
+__device__ __forceinline__ void tail_stencil(const TailLevel &P, int ob, const float *bg, int ox, int oo, float w,
+                                             int op) {
+    const int nx = P.nx, ny = P.ny, nz = P.nz, nxy = nx * ny, n = nxy * nz;
+    TailIter it(nx, ny);
+    for (int t = threadIdx.x; t < n; t += kTailThreads, it.next()) {
+        const int i = it.i, j = it.j, k = it.k;
         float o = 0.f;
         if (!(i == 0 || j == 0 || k == 0 || i == nx - 1 || j == ny - 1 || k == nz - 1)) {
-            const float sx = edge_sum_mg(P.hyp, P.hzp, j, k);
-            const float syy = edge_sum_mg(P.hzp, P.hxp, k, i);
-            const float szz = edge_sum_mg(P.hxp, P.hyp, i, j);
-            const float ixm = P.ixp[i], ixq = P.ixp[i + 1], iym = P.iyp[j], iyq = P.iyp[j + 1];
-            const float izm = P.izp[k], izq = P.izp[k + 1];
+            const float sx = lf(P.SX + j + ny * k), syy = lf(P.SY + k + nz * i), szz = lf(P.SZ + i + nx * j);
+            const float ixm = lf(P.IX + i), ixq = lf(P.IX + i + 1), iym = lf(P.IY + j), iyq = lf(P.IY + j + 1);
+            const float izm = lf(P.IZ + k), izq = lf(P.IZ + k + 1);
             const float diag = sx * (ixm + ixq) + syy * (iym + iyq) + szz * (izm + izq);
-            const float bi = b[t];
+            const float bi = ob < 0 ? bg[t] : lf(ob + t);
             if (op == OP_FIRST) {
                 o = w * bi / diag;
             } else {
-                const float ax = diag * x[t] - sx * (ixm * x[t - 1] + ixq * x[t + 1]) -
-                                 syy * (iym * x[t - nx] + iyq * x[t + nx]) - szz * (izm * x[t - nxy] + izq * x[t + nxy]);
-                o = op == OP_RESID ? bi - ax : x[t] + w * (bi - ax) / diag;
+                const int x = ox + t;
+                const float ax = diag * lf(x) - sx * (ixm * lf(x - 1) + ixq * lf(x + 1)) -
+                                 syy * (iym * lf(x - nx) + iyq * lf(x + nx)) -
+                                 szz * (izm * lf(x - nxy) + izq * lf(x + nxy));
+                o = op == OP_RESID ? bi - ax : lf(x) + w * (bi - ax) / diag;
             }
         }
-        out[t] = o;
+        tail_lds[oo + t] = o;
     }
     __syncthreads();
 }
 
-// bc = R rf (f: the fine level with its transfer lists), zero on the coarse faces
-__device__ __forceinline__ void tail_restrict(const MgLevelDev &f, const PoissonGrid<float> &C, const float *rf,
-                                              float *bc) {
+// bc = R rf, zero on the coarse faces (fine level F with its lists T, coarse level C)
+__device__ __forceinline__ void tail_restrict(const TailLevel &F, const TailTransfer &T, const TailLevel &C, int orf,
+                                              int obc) {
     const int nxc = C.nx, nyc = C.ny, nzc = C.nz, nxyc = nxc * nyc, nc = nxyc * nzc;
-    const int nxf = f.P.nx, nxyf = f.P.nx * f.P.ny;
-    for (int t = threadIdx.x; t < nc; t += kTailThreads) {
-        const int I = t % nxc, J = (t / nxc) % nyc, K = t / nxyc;
+    const int nxf = F.nx, nxyf = F.nx * F.ny;
+    TailIter it(nxc, nyc);
+    for (int t = threadIdx.x; t < nc; t += kTailThreads, it.next()) {
+        const int I = it.i, J = it.j, K = it.k;
         float s = 0.f;
         if (!(I == 0 || J == 0 || K == 0 || I == nxc - 1 || J == nyc - 1 || K == nzc - 1)) {
+            const int x0 = li(T.rxi + 3 * I), x1 = li(T.rxi + 3 * I + 1), x2 = li(T.rxi + 3 * I + 2);
+            const float w0 = lf(T.rxw + 3 * I), w1 = lf(T.rxw + 3 * I + 1), w2 = lf(T.rxw + 3 * I + 2);
             for (int c = 0; c < 3; ++c) {
-                const float wz = f.rzw[3 * K + c];
+                const float wz = lf(T.rzw + 3 * K + c);
                 if (wz == 0.f) continue;
-                const float *pz = rf + nxyf * f.rzi[3 * K + c];
+                const int pz = orf + nxyf * li(T.rzi + 3 * K + c);
                 float sz = 0.f;
                 for (int q = 0; q < 3; ++q) {
-                    const float wy = f.ryw[3 * J + q];
+                    const float wy = lf(T.ryw + 3 * J + q);
                     if (wy == 0.f) continue;
-                    const float *py = pz + nxf * f.ryi[3 * J + q];
-                    sz += wy * (f.rxw[3 * I] * py[f.rxi[3 * I]] + f.rxw[3 * I + 1] * py[f.rxi[3 * I + 1]] +
-                                f.rxw[3 * I + 2] * py[f.rxi[3 * I + 2]]);
+                    const int py = pz + nxf * li(T.ryi + 3 * J + q);
+                    sz += wy * (w0 * lf(py + x0) + w1 * lf(py + x1) + w2 * lf(py + x2));
                 }
                 s += wz * sz;
             }
         }
-        bc[t] = s;
+        tail_lds[obc + t] = s;
     }
     __syncthreads();
 }
 
-// xf += P xc on the interior of the fine level f
-__device__ __forceinline__ void tail_prolong(const MgLevelDev &f, const PoissonGrid<float> &C, const float *xc,
-                                             float *xf) {
-    const int nx = f.P.nx, ny = f.P.ny, nz = f.P.nz, nxy = nx * ny, n = nxy * nz;
+// xf += P xc on the interior of the fine level F
+__device__ __forceinline__ void tail_prolong(const TailLevel &F, const TailTransfer &T, const TailLevel &C, int oxc,
+                                             int oxf) {
+    const int nx = F.nx, ny = F.ny, nz = F.nz, nxy = nx * ny, n = nxy * nz;
     const int nxc = C.nx, nxyc = C.nx * C.ny;
-    for (int t = threadIdx.x; t < n; t += kTailThreads) {
-        const int i = t % nx, j = (t / nx) % ny, k = t / nxy;
+    TailIter it(nx, ny);
+    for (int t = threadIdx.x; t < n; t += kTailThreads, it.next()) {
+        const int i = it.i, j = it.j, k = it.k;
         if (i == 0 || j == 0 || k == 0 || i == nx - 1 || j == ny - 1 || k == nz - 1) continue;
-        const float tx = f.pxt[i], ty = f.pyt[j], tz = f.pzt[k];
-        const float *c0 = xc + f.pxc[i] + nxc * f.pyc[j] + nxyc * f.pzc[k];
-        const float *c1 = c0 + nxyc;
-        const float a0 = (1.f - ty) * ((1.f - tx) * c0[0] + tx * c0[1]) + ty * ((1.f - tx) * c0[nxc] + tx * c0[nxc + 1]);
-        const float a1 = (1.f - ty) * ((1.f - tx) * c1[0] + tx * c1[1]) + ty * ((1.f - tx) * c1[nxc] + tx * c1[nxc + 1]);
-        xf[t] += (1.f - tz) * a0 + tz * a1;
+        const float tx = lf(T.pxt + i), ty = lf(T.pyt + j), tz = lf(T.pzt + k);
+        const int c0 = oxc + li(T.pxc + i) + nxc * li(T.pyc + j) + nxyc * li(T.pzc + k), c1 = c0 + nxyc;
+        const float a0 = (1.f - ty) * ((1.f - tx) * lf(c0) + tx * lf(c0 + 1)) +
+                         ty * ((1.f - tx) * lf(c0 + nxc) + tx * lf(c0 + nxc + 1));
+        const float a1 = (1.f - ty) * ((1.f - tx) * lf(c1) + tx * lf(c1 + 1)) +
+                         ty * ((1.f - tx) * lf(c1 + nxc) + tx * lf(c1 + nxc + 1));
+        tail_lds[oxf + t] += (1.f - tz) * a0 + tz * a1;
     }
     __syncthreads();
 }
 
+// NT tail levels (compile-time, so that the per-level state stays in registers)
+template <int NT>
 __global__ __launch_bounds__(kTailThreads) void k_mg_tail(MgTailArgs a, const int *skip) {
     if (skip && *skip) return;
-    extern __shared__ float lds[];
-    float *cur[kMgTailMax], *oth[kMgTailMax];
-    const float *B[kMgTailMax];
-    for (int q = 0; q < a.nt; ++q) {
-        cur[q] = lds + a.off[q];
-        oth[q] = lds + a.off[q] + a.n[q];
-        B[q] = q == 0 ? a.b0 : lds + a.off[q] + 2 * a.n[q];
-    }
-    for (int q = 0; q + 1 < a.nt; ++q) {
+    int cur[NT], oth[NT], ob[NT];
+    TailLevel TL[NT];
+    TailTransfer TT[NT];
+#pragma unroll
+    for (int q = 0; q < NT; ++q) {
         const PoissonGrid<float> &P = a.lev[q].P;
-        tail_stencil(P, B[q], nullptr, cur[q], a.om[0], OP_FIRST);
-        for (int k = 1; k < a.s; ++k) {
-            tail_stencil(P, B[q], cur[q], oth[q], a.om[k], OP_SMOOTH);
-            float *t = cur[q]; cur[q] = oth[q]; oth[q] = t;
+        cur[q] = a.off[q];
+        oth[q] = a.off[q] + a.n[q];
+        ob[q] = q == 0 ? -1 : a.off[q] + 2 * a.n[q];
+        TL[q] = tail_level(a.coef_off[q], P.nx, P.ny, P.nz);
+        tail_fill_level(P, TL[q]);
+        if (q + 1 < NT) {
+            const PoissonGrid<float> &C = a.lev[q + 1].P;
+            TT[q] = tail_transfer(a.tr_off[q], P.nx, P.ny, P.nz, C.nx, C.ny, C.nz);
+            tail_fill_transfer(a.lev[q], C, TT[q]);
         }
-        tail_stencil(P, B[q], cur[q], oth[q], 0.f, OP_RESID);
-        tail_restrict(a.lev[q], a.lev[q + 1].P, oth[q], (float *)B[q + 1]);
+    }
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q + 1 < NT; ++q) {
+        tail_stencil(TL[q], ob[q], a.b0, 0, cur[q], a.om[0], OP_FIRST);
+        for (int k = 1; k < a.s; ++k) {
+            tail_stencil(TL[q], ob[q], a.b0, cur[q], oth[q], a.om[k], OP_SMOOTH);
+            const int t = cur[q]; cur[q] = oth[q]; oth[q] = t;
+        }
+        tail_stencil(TL[q], ob[q], a.b0, cur[q], oth[q], 0.f, OP_RESID);
+        tail_restrict(TL[q], TT[q], TL[q + 1], oth[q], ob[q + 1]);
     }
     {
-        const int q = a.nt - 1;
-        const PoissonGrid<float> &P = a.lev[q].P;
-        tail_stencil(P, B[q], nullptr, cur[q], a.omc[0], OP_FIRST);
+        constexpr int q = NT - 1;
+        tail_stencil(TL[q], ob[q], a.b0, 0, cur[q], a.omc[0], OP_FIRST);
         for (int k = 1; k < a.cd; ++k) {
-            tail_stencil(P, B[q], cur[q], oth[q], a.omc[k], OP_SMOOTH);
-            float *t = cur[q]; cur[q] = oth[q]; oth[q] = t;
+            tail_stencil(TL[q], ob[q], a.b0, cur[q], oth[q], a.omc[k], OP_SMOOTH);
+            const int t = cur[q]; cur[q] = oth[q]; oth[q] = t;
         }
     }
-    for (int q = a.nt - 2; q >= 0; --q) {
-        const PoissonGrid<float> &P = a.lev[q].P;
-        tail_prolong(a.lev[q], a.lev[q + 1].P, cur[q + 1], cur[q]);
+#pragma unroll
+    for (int q = NT - 2; q >= 0; --q) {
+        tail_prolong(TL[q], TT[q], TL[q + 1], cur[q + 1], cur[q]);
         for (int k = a.s - 1; k >= 0; --k) {
-            tail_stencil(P, B[q], cur[q], oth[q], a.om[k], OP_SMOOTH);
-            float *t = cur[q]; cur[q] = oth[q]; oth[q] = t;
+            tail_stencil(TL[q], ob[q], a.b0, cur[q], oth[q], a.om[k], OP_SMOOTH);
+            const int t = cur[q]; cur[q] = oth[q]; oth[q] = t;
         }
     }
-    for (int t = threadIdx.x; t < a.n[0]; t += kTailThreads) a.x0[t] = cur[0][t];
+    for (int t = threadIdx.x; t < a.n[0]; t += kTailThreads) a.x0[t] = tail_lds[cur[0] + t];
 }
 
 // At most kMaxPartials blocks (the fused dot writes one partial per block); the setup
@@ -368,13 +495,32 @@ void launch_mg_resid(const Launch &L, const PoissonGrid<float> &P, const float *
     hipLaunchKernelGGL((k_mg_stencil<OP_RESID, false>), stencil_grid(P), dim3(256), 0, L.s, P, b, x, out, 0.f, skip,
                        (double *)nullptr);
 }
-int mg_tail_lds_bytes(const MgTailArgs &a) {
+int mg_tail_layout(MgTailArgs &a) {
     int f = 0;
-    for (int q = 0; q < a.nt; ++q) f += (q == 0 ? 2 : 3) * a.n[q];
-    return f * (int)sizeof(float);
+    for (int q = 0; q < a.nt; ++q) {
+        a.off[q] = f;
+        f += (q == 0 ? 2 : 3) * a.n[q];
+    }
+    for (int q = 0; q < a.nt; ++q) {
+        const PoissonGrid<float> &P = a.lev[q].P;
+        a.coef_off[q] = f;
+        f += tail_coef_floats(P.nx, P.ny, P.nz);
+        if (q + 1 < a.nt) {
+            const PoissonGrid<float> &C = a.lev[q + 1].P;
+            a.tr_off[q] = f;
+            f += tail_transfer_words(P.nx, P.ny, P.nz, C.nx, C.ny, C.nz);
+        }
+    }
+    a.lds_bytes = f * (int)sizeof(float);
+    return a.lds_bytes;
 }
 void launch_mg_tail(const Launch &L, const MgTailArgs &a, const int *skip) {
-    hipLaunchKernelGGL(k_mg_tail, dim3(1), dim3(kTailThreads), mg_tail_lds_bytes(a), L.s, a, skip);
+    switch (a.nt) {
+        case 1: hipLaunchKernelGGL(k_mg_tail<1>, dim3(1), dim3(kTailThreads), a.lds_bytes, L.s, a, skip); break;
+        case 2: hipLaunchKernelGGL(k_mg_tail<2>, dim3(1), dim3(kTailThreads), a.lds_bytes, L.s, a, skip); break;
+        case 3: hipLaunchKernelGGL(k_mg_tail<3>, dim3(1), dim3(kTailThreads), a.lds_bytes, L.s, a, skip); break;
+        default: hipLaunchKernelGGL(k_mg_tail<4>, dim3(1), dim3(kTailThreads), a.lds_bytes, L.s, a, skip); break;
+    }
 }
 bool mg_stencil_fits(const PoissonGrid<float> &P) {
     return (int64_t)((P.nx + 63) / 64) * ((P.ny + 3) / 4) <= kMaxPartials;
