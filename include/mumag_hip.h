@@ -245,9 +245,10 @@ int mm_warm_start_report(mm_ctx *ctx, int64_t *solves, int64_t *warm_solves, dou
 
 /* HIP-event timing of the two fused PCG kernels since the last mm_timing_enable: the
  * K apply with the search-direction update (SELL SpMV or 7-point stencil, FP64 or the
- * FP32 inner solve) and the vector update.  One iteration in 16 is timed (the counts
- * are of the sampled launches); only launches that did work (before convergence) are
- * counted. */
+ * FP32 inner solve) and the vector update, and with the multigrid preconditioner one
+ * level-0 smoothing step of the iteration's V-cycle.  One iteration in 16 is timed (the
+ * counts are of the sampled launches); only launches that did work (before
+ * convergence) are counted. */
 typedef struct {
     int64_t spmv_launches;
     double spmv_ms;           /* summed device time of those launches */
@@ -255,6 +256,9 @@ typedef struct {
     int64_t update_launches;
     double update_ms;
     int64_t update_bytes;
+    int64_t mg_smooth_launches;
+    double mg_smooth_ms;
+    int64_t mg_smooth_bytes;
 } mm_kernel_timing;
 int mm_timing_enable(mm_ctx *ctx, int32_t on);
 int mm_timing_get(mm_ctx *ctx, mm_kernel_timing *out);

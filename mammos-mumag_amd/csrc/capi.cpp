@@ -88,7 +88,7 @@ struct mm_ctx {
 
     // event timing of the PCG kernels (K apply and vector update)
     bool timing = false;
-    KernelTimer t_spmv, t_upd;
+    KernelTimer t_spmv, t_upd, t_mg;   // t_mg: a level-0 smoothing step of the V-cycle
 
     // multigrid preconditioner of the FP32 inner solve (structured grids)
     std::vector<double> axis[3];        // node coordinates per axis (host)
@@ -154,6 +154,7 @@ struct mm_ctx {
         if (hsc) (void)hipHostFree(hsc);
         for (hipEvent_t e : t_spmv.ev) (void)hipEventDestroy(e);
         for (hipEvent_t e : t_upd.ev) (void)hipEventDestroy(e);
+        for (hipEvent_t e : t_mg.ev) (void)hipEventDestroy(e);
         if (s) (void)hipStreamDestroy(s);
     }
 };
@@ -241,15 +242,20 @@ void op_grad(mm_ctx *c, const double *u, double *g) {
 }
 
 // The PCG kernels are timed on a sample: iteration i is bracketed by an event pair when
-// i % kTimerStride == kTimerPhase (mid-batch, away from the host synchronisation), so
-// that the events barely perturb the timed solve.  Pair index i / kTimerStride; pairs
+// i % stride == phase (mid-batch, away from the host synchronisation), so that the
+// events barely perturb the timed solve: 1 in 16 with Jacobi, every other iteration with
+// multigrid (whose solves take a few iterations each).  Pair index i / stride; pairs
 // past the pool are not recorded.
-constexpr int kTimerStride = 16, kTimerPhase = 7;
+struct TimerSample {
+    int stride, phase;
+};
+TimerSample timer_sample(const mm_ctx *c) { return c->mg_nl > 0 ? TimerSample{2, 1} : TimerSample{16, 7}; }
 // the launch of iteration i, with the timer's event pair attached when i is sampled
 Launch timed(mm_ctx *c, KernelTimer &t, int i) {
     Launch l = c->L();
-    if (!c->timing || i % kTimerStride != kTimerPhase) return l;
-    const int idx = i / kTimerStride;
+    const TimerSample ts = timer_sample(c);
+    if (!c->timing || i % ts.stride != ts.phase) return l;
+    const int idx = i / ts.stride;
     if (idx >= (int)t.ev.size() / 2) return l;
     l.t0 = t.ev[2 * idx];
     l.t1 = t.ev[2 * idx + 1];
@@ -257,11 +263,12 @@ Launch timed(mm_ctx *c, KernelTimer &t, int i) {
 }
 // add the durations of the sampled iterations in [from, from + count) (after the stream
 // has synchronised)
-int collect(KernelTimer &t, int from, int count) {
+int collect(const mm_ctx *c, KernelTimer &t, int from, int count) {
     const int pool = (int)t.ev.size() / 2;
+    const TimerSample ts = timer_sample(c);
     for (int i = from; i < from + count; ++i) {
-        if (i % kTimerStride != kTimerPhase || i / kTimerStride >= pool) continue;
-        const int j = i / kTimerStride;
+        if (i % ts.stride != ts.phase || i / ts.stride >= pool) continue;
+        const int j = i / ts.stride;
         float ms = 0.f;
         HIPC(hipEventElapsedTime(&ms, t.ev[2 * j], t.ev[2 * j + 1]));
         t.ms += ms;
@@ -314,7 +321,7 @@ int pcg64(mm_ctx *c, const double *b, double *x, double tol, double atol, int ma
     }
     const int it = c->hsc->pcg_iter;
     if (c->timing) {
-        if ((rc = collect(c->t_spmv, 0, it)) || (rc = collect(c->t_upd, 0, it))) return rc;
+        if ((rc = collect(c, c->t_spmv, 0, it)) || (rc = collect(c, c->t_upd, 0, it))) return rc;
     }
     if (iters) *iters = it;
     if (it >= maxiter && c->hsc->pcg_rr > c->hsc->pcg_bound) return MM_W_MAXITER;
@@ -327,7 +334,7 @@ int pcg64(mm_ctx *c, const double *b, double *x, double tol, double atol, int ma
 // mg[0].invw * r (the kernel that produced r wrote it).  With parts, the last level-0
 // smoothing step also writes the partials (r.Mr, r.r) there; *nparts is their count.
 float *mg_vcycle(mm_ctx *c, const float *r, float *z, float *alt, const int *skip, bool first_done,
-                 double *parts = nullptr, int *nparts = nullptr) {
+                 double *parts = nullptr, int *nparts = nullptr, int it = -1) {
     Launch L = c->L();
     const int nl = c->mg_nl, s = c->mg_deg, cd = c->mg_cdeg;
     float *cur[kMgMaxLevels], *oth[kMgMaxLevels];
@@ -342,8 +349,10 @@ float *mg_vcycle(mm_ctx *c, const float *r, float *z, float *alt, const int *ski
     // partial slot per block
     const bool fuse = parts && mg_stencil_fits(c->mg[0].P);
     int np = 0;
-    auto smooth = [&](int l, float w, bool last0) {
-        const int nb = launch_mg_smooth(L, c->mg[l].P, B[l], cur[l], oth[l], w, skip,
+    auto smooth = [&](int l, float w, bool last0, bool sample = false) {
+        // sample: the PCG iteration's timed level-0 step (HIP events on its dispatch)
+        const Launch LS = sample && it >= 0 ? timed(c, c->t_mg, it) : L;
+        const int nb = launch_mg_smooth(LS, c->mg[l].P, B[l], cur[l], oth[l], w, skip,
                                         last0 && fuse ? parts : nullptr);
         if (last0 && fuse) np = nb;
         std::swap(cur[l], oth[l]);
@@ -366,7 +375,7 @@ float *mg_vcycle(mm_ctx *c, const float *r, float *z, float *alt, const int *ski
     }
     for (int l = top - 1; l >= 0; --l) {
         launch_mg_prolong(L, c->mg[l], c->mg[l + 1], cur[l + 1], cur[l], skip);
-        for (int k = s - 1; k >= 0; --k) smooth(l, c->mg_om[k], l == 0 && k == 0);
+        for (int k = s - 1; k >= 0; --k) smooth(l, c->mg_om[k], l == 0 && k == 0, l == 0 && k == s - 1 && k > 0);
     }
     if (parts && np == 0) np = launch_dot2<float>(L, c->N, r, cur[0], parts);
     if (nparts) *nparts = np;
@@ -396,12 +405,12 @@ int pcg_mixed(mm_ctx *c, const double *b, double *x, double tol, double atol, in
     int i = 0, nB = 0, mode = PCG_RESTART;
     int *const done = &c->dsc->pcg_done;
     // z = M r; the V-cycle's result buffer becomes z32 (the other one its scratch)
-    auto precondition = [&]() {
-        float *res = mg_vcycle(c, c->r32, c->z32, c->mg_xb[0], done, true, c->partsB, &nB);
+    auto precondition = [&](int it) {
+        float *res = mg_vcycle(c, c->r32, c->z32, c->mg_xb[0], done, true, c->partsB, &nB, it);
         if (res != c->z32) std::swap(c->z32, c->mg_xb[0]);
     };
     if (mg) {   // for the initial true residual
-        precondition();
+        precondition(-1);
         mode = PCG_RESTART_PARTS;
     }
     while (!c->hsc->ir_done) {
@@ -425,7 +434,7 @@ int pcg_mixed(mm_ctx *c, const double *b, double *x, double tol, double atol, in
             for (int k = 0; k < batch; ++k, ++i) {
                 pcg_iteration<float>(c, i, mode, tol, atol, maxiter, c->z32, pb, c->q32, c->dx32, c->r32, invD,
                                      c->K32, c->P32, &nB);
-                if (mg) precondition();
+                if (mg) precondition(i);
                 mode = PCG_NORMAL;
             }
             if ((rc = sync_scalars(c))) return rc;
@@ -442,7 +451,9 @@ int pcg_mixed(mm_ctx *c, const double *b, double *x, double tol, double atol, in
         if (c->hsc->pcg_done) i = c->hsc->pcg_iter;
         if (c->timing) {
             const int used = i - seg0;  // event pairs are indexed by the global iteration number
-            if ((rc = collect(c->t_spmv, seg0, used)) || (rc = collect(c->t_upd, seg0, used))) return rc;
+            if ((rc = collect(c, c->t_spmv, seg0, used)) || (rc = collect(c, c->t_upd, seg0, used)) ||
+                (mg && (rc = collect(c, c->t_mg, seg0, used))))
+                return rc;
         }
         // reliable update: fold dx into x, replace r by the true residual
         launch_add_correction(L, n, x, c->dx32);
@@ -457,7 +468,7 @@ int pcg_mixed(mm_ctx *c, const double *b, double *x, double tol, double atol, in
             // replacements, so the true residual's r.z can exceed the recursive one by
             // orders of magnitude: continuing the direction (beta = gamma_true /
             // gamma_recursive) would wreck the conjugacy.
-            precondition();
+            precondition(-1);
             mode = PCG_RESTART_PARTS;
         }
     }
@@ -852,6 +863,7 @@ int mm_ctx_create(const mm_mesh_desc *d, const mm_options *opts, mm_ctx **out) {
                                  : c->k_nnz * (s_k + 4) + 4 * (c->N + 1) + 4 * (int64_t)s_k * c->N;
     // PCG update: reads x, r, p, q, invD; writes x, r, z
     c->t_upd.bytes = 8 * (int64_t)s_k * c->N;
+    c->t_mg.bytes = 3 * (int64_t)sizeof(float) * c->N;   // b, x read; the new x written
 
     if ((rc = c->alloc(&c->dsc, 1))) return bail(rc);
     {
@@ -1544,7 +1556,7 @@ int mm_warm_start_stats(mm_ctx *c, int64_t *calls, double *last_gamma) {
 int mm_timing_enable(mm_ctx *c, int32_t on) {
     if (!c) return fail(MM_E_ARG, "null context");
     c->timing = on != 0;
-    for (KernelTimer *t : {&c->t_spmv, &c->t_upd}) {
+    for (KernelTimer *t : {&c->t_spmv, &c->t_upd, &c->t_mg}) {
         if (c->timing && t->ev.empty()) {
             t->ev.resize(2 * 1024);   // sampled pairs: iterations up to 16 x 1024 per solve
             for (auto &e : t->ev) HIPC(hipEventCreate(&e));
@@ -1563,6 +1575,9 @@ int mm_timing_get(mm_ctx *c, mm_kernel_timing *o) {
     o->update_launches = c->t_upd.launches;
     o->update_ms = c->t_upd.ms;
     o->update_bytes = c->t_upd.bytes;
+    o->mg_smooth_launches = c->t_mg.launches;
+    o->mg_smooth_ms = c->t_mg.ms;
+    o->mg_smooth_bytes = c->t_mg.bytes;
     return 0;
 }
 

@@ -10,6 +10,11 @@ namespace mm {
 
 constexpr int kMgMaxLevels = 8;
 
+// Tile grid of a per-level kernel: x tiles, y tiles, z chunks (launched as x * y * z blocks)
+struct MgGrid {
+    int x, y, z;
+};
+
 // One level as the kernels see it, with the transfers to the next coarser level:
 // restriction lists per coarse index (3 (fine index, weight) pairs, weight 0 = unused)
 // and prolongation pairs per fine index (coarse c0, weight t of c0 + 1).

@@ -11,6 +11,7 @@
 // [2/30, 2] (2 bounds the spectrum of D^-1 A by Gershgorin), the post-smoother in the
 // reverse order, so that the V-cycle is a symmetric operator; the coarsest level is a
 // fixed Chebyshev-Richardson polynomial.  FP32 vectors, FP32 coefficients.
+#include <hip/hip_ext.h>
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
@@ -25,6 +26,20 @@ namespace {
 // Column tiles of the per-level kernels: kMgTX x kMgTY (x, y) columns per 256-thread block
 // (64 x 4 measured faster than 32 x 8 on the 154^3 grid)
 constexpr int kMgTX = 64, kMgTY = 256 / kMgTX;
+
+// Logical tile of this block in a 1-D launch of gx * gy * gz blocks.  Blocks are
+// dispatched round-robin over the 8 XCDs (block b on XCD b % 8), so each XCD is given a
+// contiguous range of logical tiles, z chunk fastest, then x, then y: the halo rows and
+// planes that neighbouring tiles share are then read through one L2.
+struct Tile {
+    int bx, by, bz;
+};
+__device__ __forceinline__ Tile mg_tile(const MgGrid &g) {
+    const int nb = g.x * g.y * g.z, b = blockIdx.x;
+    const int q = nb / 8, r = nb % 8, x = b % 8, l = b / 8;
+    const int t = x < r ? x * (q + 1) + l : r * (q + 1) + (x - r) * q + l;
+    return Tile{(t / g.z) % g.x, t / (g.z * g.x), t % g.z};
+}
 
 template <typename T>
 __device__ __forceinline__ T edge_sum_mg(const T *__restrict__ h1p, const T *__restrict__ h2p, int a, int b) {
@@ -41,15 +56,16 @@ __device__ __forceinline__ T edge_sum_mg(const T *__restrict__ h1p, const T *__r
 // block into parts[0 / kMaxPartials + block] (the next PCG step's r.z and r.r).
 enum { OP_FIRST = 0, OP_SMOOTH = 1, OP_RESID = 2 };
 template <int OP, bool DOT>
-__global__ __launch_bounds__(256) void k_mg_stencil(PoissonGrid<float> P, const float *__restrict__ b,
+__global__ __launch_bounds__(256) void k_mg_stencil(PoissonGrid<float> P, MgGrid G, const float *__restrict__ b,
                                                     const float *__restrict__ x, float *__restrict__ out, float w,
                                                     const int *skip, double *parts) {
     if (skip && *skip) return;   // block-uniform
     const int nx = P.nx, ny = P.ny, nz = P.nz;
-    const int i = blockIdx.x * kMgTX + (threadIdx.x % kMgTX);
-    const int j = blockIdx.y * kMgTY + (threadIdx.x / kMgTX);
-    const int chunk = (nz + gridDim.z - 1) / gridDim.z;
-    const int k0 = blockIdx.z * chunk, k1 = min(nz, k0 + chunk);
+    const Tile T = mg_tile(G);
+    const int i = T.bx * kMgTX + (threadIdx.x % kMgTX);
+    const int j = T.by * kMgTY + (threadIdx.x / kMgTX);
+    const int chunk = (nz + G.z - 1) / G.z;
+    const int k0 = T.bz * chunk, k1 = min(nz, k0 + chunk);
     const bool active = i < nx && j < ny && k0 < k1;
     if (!DOT && !active) return;
     double sbz = 0.0, sbb = 0.0;
@@ -151,9 +167,8 @@ __global__ __launch_bounds__(256) void k_mg_stencil(PoissonGrid<float> P, const 
                 a += sh[0][q];
                 c += sh[1][q];
             }
-            const int bid = blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z);
-            parts[bid] = a;
-            parts[kMaxPartials + bid] = c;
+            parts[blockIdx.x] = a;
+            parts[kMaxPartials + blockIdx.x] = c;
         }
     }
 }
@@ -164,7 +179,7 @@ __global__ __launch_bounds__(256) void k_mg_stencil(PoissonGrid<float> P, const 
 // step from zero (invw_c: the level's first Chebyshev weight over the diagonal, 0 on the
 // faces).  Tiles of 64 x 4 coarse (x, y) columns, each thread marching a z chunk; the
 // z and y pairs are uniform over a wave, the x pairs per lane.
-__global__ __launch_bounds__(256) void k_mg_restrict(int nxc, int nyc, int nzc, int nxf, int nyf,
+__global__ __launch_bounds__(256) void k_mg_restrict(MgGrid G, int nxc, int nyc, int nzc, int nxf, int nyf,
                                                      const int *__restrict__ rxi, const float *__restrict__ rxw,
                                                      const int *__restrict__ ryi, const float *__restrict__ ryw,
                                                      const int *__restrict__ rzi, const float *__restrict__ rzw,
@@ -172,10 +187,11 @@ __global__ __launch_bounds__(256) void k_mg_restrict(int nxc, int nyc, int nzc, 
                                                      const float *__restrict__ invwc, float *__restrict__ x1c,
                                                      const int *skip) {
     if (skip && *skip) return;
-    const int I = blockIdx.x * kMgTX + (threadIdx.x % kMgTX);
-    const int J = blockIdx.y * kMgTY + (threadIdx.x / kMgTX);
-    const int chunk = (nzc + gridDim.z - 1) / gridDim.z;
-    const int k0 = blockIdx.z * chunk, k1 = min(nzc, k0 + chunk);
+    const Tile T = mg_tile(G);
+    const int I = T.bx * kMgTX + (threadIdx.x % kMgTX);
+    const int J = T.by * kMgTY + (threadIdx.x / kMgTX);
+    const int chunk = (nzc + G.z - 1) / G.z;
+    const int k0 = T.bz * chunk, k1 = min(nzc, k0 + chunk);
     if (I >= nxc || J >= nyc) return;
     const bool edge = I == 0 || J == 0 || I == nxc - 1 || J == nyc - 1;
     const int64_t sf = (int64_t)nxf * nyf, sc = (int64_t)nxc * nyc;
@@ -216,17 +232,18 @@ __global__ __launch_bounds__(256) void k_mg_restrict(int nxc, int nyc, int nzc, 
 // 1 - t[i] and c0[i] + 1 with weight t[i] (t = 0 on coarse nodes; an interior fine node
 // has c0 + 1 inside the coarse axis).  Fine faces stay zero.  Tiles of 64 x 4 fine
 // columns marching z chunks.
-__global__ __launch_bounds__(256) void k_mg_prolong(int nxf, int nyf, int nzf, int nxc, int nyc,
+__global__ __launch_bounds__(256) void k_mg_prolong(MgGrid G, int nxf, int nyf, int nzf, int nxc, int nyc,
                                                     const int *__restrict__ pxc, const float *__restrict__ pxt,
                                                     const int *__restrict__ pyc, const float *__restrict__ pyt,
                                                     const int *__restrict__ pzc, const float *__restrict__ pzt,
                                                     const float *__restrict__ xc, float *__restrict__ xf,
                                                     const int *skip) {
     if (skip && *skip) return;
-    const int i = blockIdx.x * kMgTX + (threadIdx.x % kMgTX);
-    const int j = blockIdx.y * kMgTY + (threadIdx.x / kMgTX);
-    const int chunk = (nzf + gridDim.z - 1) / gridDim.z;
-    const int k0 = max(1, blockIdx.z * chunk), k1 = min(nzf - 1, blockIdx.z * chunk + chunk);
+    const Tile T = mg_tile(G);
+    const int i = T.bx * kMgTX + (threadIdx.x % kMgTX);
+    const int j = T.by * kMgTY + (threadIdx.x / kMgTX);
+    const int chunk = (nzf + G.z - 1) / G.z;
+    const int k0 = max(1, T.bz * chunk), k1 = min(nzf - 1, T.bz * chunk + chunk);
     if (i <= 0 || j <= 0 || i >= nxf - 1 || j >= nyf - 1) return;
     const int64_t sf = (int64_t)nxf * nyf, sc = (int64_t)nxc * nyc;
     const float tx = pxt[i], ty = pyt[j];
@@ -498,34 +515,41 @@ __global__ __launch_bounds__(kTailThreads) void k_mg_tail(MgTailArgs a, const in
 
 // At most kMaxPartials blocks (the fused dot writes one partial per block); the setup
 // checks that one z chunk of tiles fits.
-dim3 stencil_grid(const PoissonGrid<float> &P) {
+MgGrid stencil_grid(const PoissonGrid<float> &P) {
     const int gx = (P.nx + kMgTX - 1) / kMgTX, gy = (P.ny + kMgTY - 1) / kMgTY;
     const int64_t per_plane = (int64_t)gx * gy;
-    int gz = (int)std::min<int64_t>(P.nz, std::max<int64_t>(1, kMaxPartials / per_plane));
-    return dim3(gx, gy, gz);
+    const int gz = (int)std::min<int64_t>(P.nz, std::max<int64_t>(1, kMaxPartials / per_plane));
+    return MgGrid{gx, gy, gz};
 }
+inline dim3 blocks(const MgGrid &g) { return dim3(g.x * g.y * g.z); }
 
 
 }  // namespace
 
 void launch_mg_first(const Launch &L, const PoissonGrid<float> &P, const float *b, float *out, float w,
                      const int *skip) {
-    hipLaunchKernelGGL((k_mg_stencil<OP_FIRST, false>), stencil_grid(P), dim3(256), 0, L.s, P, b,
+    const MgGrid g = stencil_grid(P);
+    hipLaunchKernelGGL((k_mg_stencil<OP_FIRST, false>), blocks(g), dim3(256), 0, L.s, P, g, b,
                        (const float *)nullptr, out, w, skip, (double *)nullptr);
 }
 int launch_mg_smooth(const Launch &L, const PoissonGrid<float> &P, const float *b, const float *x, float *out,
                      float w, const int *skip, double *dot_parts) {
-    const dim3 g = stencil_grid(P);
+    const MgGrid g = stencil_grid(P);
     if (dot_parts)
-        hipLaunchKernelGGL((k_mg_stencil<OP_SMOOTH, true>), g, dim3(256), 0, L.s, P, b, x, out, w, skip, dot_parts);
+        hipLaunchKernelGGL((k_mg_stencil<OP_SMOOTH, true>), blocks(g), dim3(256), 0, L.s, P, g, b, x, out, w, skip,
+                           dot_parts);
+    else if (L.t0)   // timed: events recorded on the dispatch itself
+        hipExtLaunchKernelGGL((k_mg_stencil<OP_SMOOTH, false>), blocks(g), dim3(256), 0, L.s, L.t0, L.t1, 0, P, g,
+                              b, x, out, w, skip, (double *)nullptr);
     else
-        hipLaunchKernelGGL((k_mg_stencil<OP_SMOOTH, false>), g, dim3(256), 0, L.s, P, b, x, out, w, skip,
+        hipLaunchKernelGGL((k_mg_stencil<OP_SMOOTH, false>), blocks(g), dim3(256), 0, L.s, P, g, b, x, out, w, skip,
                            (double *)nullptr);
-    return (int)(g.x * g.y * g.z);
+    return g.x * g.y * g.z;
 }
 void launch_mg_resid(const Launch &L, const PoissonGrid<float> &P, const float *b, const float *x, float *out,
                      const int *skip) {
-    hipLaunchKernelGGL((k_mg_stencil<OP_RESID, false>), stencil_grid(P), dim3(256), 0, L.s, P, b, x, out, 0.f, skip,
+    const MgGrid g = stencil_grid(P);
+    hipLaunchKernelGGL((k_mg_stencil<OP_RESID, false>), blocks(g), dim3(256), 0, L.s, P, g, b, x, out, 0.f, skip,
                        (double *)nullptr);
 }
 int mg_tail_layout(MgTailArgs &a) {
@@ -560,12 +584,14 @@ bool mg_stencil_fits(const PoissonGrid<float> &P) {
 }
 void launch_mg_restrict(const Launch &L, const MgLevelDev &f, const MgLevelDev &c, const float *rf, float *bc,
                         float *x1c, const int *skip) {
-    hipLaunchKernelGGL(k_mg_restrict, stencil_grid(c.P), dim3(256), 0, L.s, c.P.nx, c.P.ny, c.P.nz, f.P.nx,
+    const MgGrid g = stencil_grid(c.P);
+    hipLaunchKernelGGL(k_mg_restrict, blocks(g), dim3(256), 0, L.s, g, c.P.nx, c.P.ny, c.P.nz, f.P.nx,
                        f.P.ny, f.rxi, f.rxw, f.ryi, f.ryw, f.rzi, f.rzw, rf, bc, c.invw, x1c, skip);
 }
 void launch_mg_prolong(const Launch &L, const MgLevelDev &f, const MgLevelDev &c, const float *xc, float *xf,
                        const int *skip) {
-    hipLaunchKernelGGL(k_mg_prolong, stencil_grid(f.P), dim3(256), 0, L.s, f.P.nx, f.P.ny, f.P.nz, c.P.nx,
+    const MgGrid g = stencil_grid(f.P);
+    hipLaunchKernelGGL(k_mg_prolong, blocks(g), dim3(256), 0, L.s, g, f.P.nx, f.P.ny, f.P.nz, c.P.nx,
                        c.P.ny, f.pxc, f.pxt, f.pyc, f.pyt, f.pzc, f.pzt, xc, xf, skip);
 }
 

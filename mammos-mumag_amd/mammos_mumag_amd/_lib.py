@@ -70,7 +70,9 @@ class Stats(ctypes.Structure):
 class KernelTiming(ctypes.Structure):
     _fields_ = [("spmv_launches", ctypes.c_int64), ("spmv_ms", ctypes.c_double),
                 ("spmv_bytes", ctypes.c_int64), ("update_launches", ctypes.c_int64),
-                ("update_ms", ctypes.c_double), ("update_bytes", ctypes.c_int64)]
+                ("update_ms", ctypes.c_double), ("update_bytes", ctypes.c_int64),
+                ("mg_smooth_launches", ctypes.c_int64), ("mg_smooth_ms", ctypes.c_double),
+                ("mg_smooth_bytes", ctypes.c_int64)]
 
 
 VP = ctypes.c_void_p
