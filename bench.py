@@ -30,13 +30,15 @@ for _p in (ROOT, ROOT / "mammos-mumag_amd"):
 
 METRIC = "hysteresis field steps/s @2M nodes: learned+FP32 vs plain FP64 on one MI355X"
 HBM_PEAK_GBS = 8000.0
-# HBM bytes per launch of the two PCG kernels on the voronoi64_250nm grid (structured
-# path), from rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE (median KiB per dispatch) scaled
-# by the same counters on the 512 MiB device copy of --traffic-probe (factors 0.5000 and
-# 1.0000); see profiles/pcg_kernels_{fp64,mixed}_voronoi125.md.
+# HBM bytes per launch of the two PCG kernels and of the level-0 multigrid smoothing step
+# on the voronoi64_250nm grid (structured path), from rocprofv3 --pmc FETCH_SIZE and
+# WRITE_SIZE (median KiB per dispatch) scaled by the same counters on the 512 MiB device
+# copy of --traffic-probe (factors 0.5000 and 1.0000); see
+# profiles/pcg_kernels_{fp64,mixed}_voronoi125.md and profiles/mg_vcycle_mixed_voronoi125.md.
 TRAFFIC = {
     ("fp64", "spmv"): 134047365, ("fp64", "update"): 233836081,
     ("mixed", "spmv"): 67436398, ("mixed", "update"): 116955348,
+    ("mixed", "mg_smooth"): 47887360,
 }
 
 

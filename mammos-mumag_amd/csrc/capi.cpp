@@ -1559,7 +1559,9 @@ int mm_timing_enable(mm_ctx *c, int32_t on) {
     for (KernelTimer *t : {&c->t_spmv, &c->t_upd, &c->t_mg}) {
         if (c->timing && t->ev.empty()) {
             t->ev.resize(2 * 1024);   // sampled pairs: iterations up to 16 x 1024 per solve
-            for (auto &e : t->ev) HIPC(hipEventCreate(&e));
+            // no system-scope fence at the events: they would write back the L2 around
+            // the timed dispatch and slow the kernel they measure
+            for (auto &e : t->ev) HIPC(hipEventCreateWithFlags(&e, hipEventDisableSystemFence));
         }
         t->launches = 0;
         t->ms = 0.0;

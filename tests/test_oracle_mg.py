@@ -1,6 +1,7 @@
 """The multigrid restatement (oracle/mg.py): its level operator is the Poisson K of the
 FP64 oracle on the interior, the V-cycle is symmetric and positive (a valid CG
-preconditioner) and cuts the iteration count of CG on K."""
+preconditioner) and cuts the iteration count of CG on K, and on one level it is the
+Chebyshev minimax polynomial of its interval (closed form)."""
 import numpy as np
 import pytest
 
@@ -67,3 +68,38 @@ def test_vcycle_symmetric_positive_and_effective(n, layers):
     xm, itm = pcg(V.apply)
     assert itm < itj / 2
     assert np.linalg.norm(xm - xj) <= 1e-8 * np.linalg.norm(xj)
+
+
+def test_one_level_is_the_chebyshev_polynomial(grid):
+    """On one level the V-cycle is the coarsest-level Chebyshev-Richardson polynomial:
+    its error propagation I - M A acts on each eigenvector of D^-1 A (eigenvalue lam) as
+    prod_k (1 - w_k lam), whose size on [2/300, 2] is bounded by 1 / T_6(b) with
+    b = (2 + 2/300) / (2 - 2/300) (the Chebyshev minimax property), and lam <= 2
+    (Gershgorin).  A wrong weight, a dropped step or a wrong diagonal breaks one of these."""
+    V = mg.VCycle(grid.xs, grid.ys, grid.zs)
+    assert len(V.levels) == 1
+    A, face = mg.operator(grid.xs, grid.ys, grid.zs)
+    inner = np.flatnonzero(~face)
+    Ai = A[inner][:, inner].toarray()
+    d = np.diag(Ai)
+    S = Ai / np.sqrt(np.outer(d, d))            # D^-1/2 A D^-1/2, same spectrum as D^-1 A
+    lam, Q = np.linalg.eigh(S)
+    assert lam.max() <= 2.0 + 1e-12 and lam.min() > 0
+    lo = 2.0 / 300.0
+    b = (2.0 + lo) / (2.0 - lo)
+    t6 = np.cosh(6 * np.arccosh(b))
+    poly = lambda x: np.prod([1.0 - w * x for w in V.omc])
+    vals = np.array([poly(x) for x in lam])
+    inband = lam >= lo
+    assert np.max(np.abs(vals[inband])) <= 1.0 / t6 * (1 + 1e-9)
+    # and it is the minimax polynomial of [2/300, 2]: it equioscillates with amplitude
+    # 1 / T_6(b) at the seven Chebyshev extrema of the interval
+    xj = (2.0 + lo) / 2 + (2.0 - lo) / 2 * np.cos(np.arange(7) * np.pi / 6)
+    assert np.allclose([poly(x) for x in xj], (-1.0) ** np.arange(7) / t6, rtol=1e-9, atol=1e-15)
+    # the oracle's V-cycle applies exactly that polynomial: E v = p(lam) v for
+    # eigenvectors v of D^-1 A (v = D^-1/2 q)
+    for idx in (0, 1, len(lam) // 2, len(lam) - 1):
+        v = np.zeros(A.shape[0])
+        v[inner] = Q[:, idx] / np.sqrt(d)
+        Ev = v - V.apply(A @ v)
+        assert np.linalg.norm(Ev - vals[idx] * v) <= 1e-9 * np.linalg.norm(v)
