@@ -470,23 +470,27 @@ __device__ __forceinline__ int64_t gnode(const GridDev &G, int i, int j, int k) 
     return (int64_t)(G.m0x + i) + (int64_t)G.nx * ((G.m0y + j) + (int64_t)G.ny * (G.m0z + k));
 }
 
+// Material coefficients of one voxel as the node kernels use them
+template <typename TC>
+struct VoxCoef {
+    TC cex, cK, Js, u0, u1, u2;
+};
+
 // (C v)_node from the values nv[(dx+1) + 3 (dy+1) + 9 (dz+1)][a] of v at the node's 27
-// neighbours (clamped to the block: clamped entries only meet zero-material voxels);
-// with DIV also the node's divergence term sum Js (B v).  Unscaled: y = fS ex + fM an.
-template <typename TC, bool DIV>
-__device__ __forceinline__ void grid_c_node(const GridDev &G, const MatCoef *__restrict__ mats,
-                                            const TC (&nv)[27][3], int i, int j, int k, TC (&y)[3], TC &bdiv) {
+// neighbours (clamped to the block: clamped entries only meet zero-material voxels) and
+// vox(w), the coefficients of voxel (i-1+vx, j-1+vy, k-1+vz), w = vx + 2 vy + 4 vz; with
+// DIV also the node's divergence term sum Js (B v).  Unscaled: y = fS ex + fM an.
+template <typename TC, bool DIV, typename Vox>
+__device__ __forceinline__ void grid_c_core(const GridDev &G, const TC (&nv)[27][3], Vox vox, TC (&y)[3],
+                                            TC &bdiv) {
     TC ex[3] = {0, 0, 0}, an[3] = {0, 0, 0}, bd = 0;
 #pragma unroll
     for (int w = 0; w < 8; ++w) {
-        const int vx = w & 1, vy = (w >> 1) & 1, vz = (w >> 2) & 1;  // voxel (i-1+vx, j-1+vy, k-1+vz)
-        const int xi = i - 1 + vx, yj = j - 1 + vy, zk = k - 1 + vz;
-        const bool inside = xi >= 0 && yj >= 0 && zk >= 0 && xi < G.mn && yj < G.mn && zk < G.mn;
-        const int mat = inside ? (int)G.vmat[xi + (int64_t)G.mn * (yj + (int64_t)G.mn * zk)] : G.n_mats;
-        const MatCoef mc = mats[mat];
+        const int vx = w & 1, vy = (w >> 1) & 1, vz = (w >> 2) & 1;
+        const VoxCoef<TC> mc = vox(w);
         const int p = (1 - vx) + 2 * (1 - vy) + 4 * (1 - vz);  // corner of the node in this voxel
         TC sS[3] = {0, 0, 0}, sM = 0, sB = 0;
-        const TC u0 = (TC)mc.u[0], u1 = (TC)mc.u[1], u2 = (TC)mc.u[2];
+        const TC u0 = mc.u0, u1 = mc.u1, u2 = mc.u2;
 #pragma unroll
         for (int c = 0; c < 8; ++c) {
             const int cx = c & 1, cy = (c >> 1) & 1, cz = (c >> 2) & 1;
@@ -504,18 +508,35 @@ __device__ __forceinline__ void grid_c_node(const GridDev &G, const MatCoef *__r
                 if (B2 != 0) sB += (TC)B2 * nv[s][2];
             }
         }
-        const TC cex = (TC)mc.cex, cK = (TC)mc.cK;
 #pragma unroll
-        for (int a = 0; a < 3; ++a) ex[a] += cex * sS[a];
-        an[0] += cK * sM * u0;
-        an[1] += cK * sM * u1;
-        an[2] += cK * sM * u2;
-        if (DIV) bd += (TC)mc.Js * sB;
+        for (int a = 0; a < 3; ++a) ex[a] += mc.cex * sS[a];
+        an[0] += mc.cK * sM * u0;
+        an[1] += mc.cK * sM * u1;
+        an[2] += mc.cK * sM * u2;
+        if (DIV) bd += mc.Js * sB;
     }
     const TC fS = (TC)G.fS, fM = (TC)G.fM;
 #pragma unroll
     for (int a = 0; a < 3; ++a) y[a] = fS * ex[a] + fM * an[a];
     if (DIV) bdiv = (TC)G.fB * bd;
+}
+
+template <typename TC>
+__device__ __forceinline__ VoxCoef<TC> vox_coef(const MatCoef &mc) {
+    return VoxCoef<TC>{(TC)mc.cex, (TC)mc.cK, (TC)mc.Js, (TC)mc.u[0], (TC)mc.u[1], (TC)mc.u[2]};
+}
+
+// grid_c_core with the voxel materials read from the block's material ids
+template <typename TC, bool DIV>
+__device__ __forceinline__ void grid_c_node(const GridDev &G, const MatCoef *__restrict__ mats,
+                                            const TC (&nv)[27][3], int i, int j, int k, TC (&y)[3], TC &bdiv) {
+    auto vox = [&](int w) {
+        const int xi = i - 1 + (w & 1), yj = j - 1 + ((w >> 1) & 1), zk = k - 1 + ((w >> 2) & 1);
+        const bool inside = xi >= 0 && yj >= 0 && zk >= 0 && xi < G.mn && yj < G.mn && zk < G.mn;
+        const int mat = inside ? (int)G.vmat[xi + (int64_t)G.mn * (yj + (int64_t)G.mn * zk)] : G.n_mats;
+        return vox_coef<TC>(mats[mat]);
+    };
+    grid_c_core<TC, DIV>(G, nv, vox, y, bdiv);
 }
 
 // y = C v at magnet nodes; with DIV also b = Dx vx + Dy vy + Dz vz at magnet nodes.
@@ -896,55 +917,109 @@ __global__ __launch_bounds__(kBlock) void k_tcgc_init(GridDev G, const double *_
     block_partials<2>(s, 0u, partials);
 }
 
-// d' = FIRST ? d : -z + beta d at the node and its neighbours, dout = d', Hd = H d';
-// partial d'.H d'.  H v = m0 x ((C v) x m0) - (m0 . gF) v.
-template <typename T, typename TC, bool FIRST>
-__global__ __launch_bounds__(kBlock) void k_tcgc_dh(GridDev G, const MatCoef *__restrict__ mats,
+// d' = FIRST ? d : -z + beta d, dout = d', Hd = H d'; partial d'.H d'.
+// H v = m0 x ((C v) x m0) - (m0 . gF) v.  A block owns a kCTX x kCTY tile of block nodes
+// and marches a z chunk: d' is formed once per node into a three-plane LDS ring with a
+// one-node halo (clamped to the block, as the 27-point gather clamps), the voxels'
+// coefficients into a two-plane ring; the node then reads its 27 neighbours and 8
+// voxels from LDS.
+constexpr int kCTX = 32, kCTY = 8;
+struct TcgTiles {
+    int gx, gy, gz;
+};
+template <typename T, bool FIRST>
+__global__ __launch_bounds__(kBlock) void k_tcgc_dh(GridDev G, TcgTiles tl, const MatCoef *__restrict__ mats,
                                                     const T *__restrict__ z, const T *__restrict__ din,
                                                     T *__restrict__ dout, const T *__restrict__ m0c,
                                                     const T *__restrict__ mgc, T *__restrict__ Hd,
                                                     const DevScalars *dsc, double *partials) {
-    if (dsc->tcg_done) return;
-    const int n1 = G.mn + 1;
-    const int64_t nb = (int64_t)n1 * n1 * n1;
+    static_assert(kCTX * kCTY == kBlock, "one node per thread and plane");
+    __shared__ T nd[3][kCTY + 2][kCTX + 2][3];
+    __shared__ T vc[2][kCTY + 1][kCTX + 1][5];
+    if (dsc->tcg_done) return;   // block-uniform
+    const int mn = G.mn, n1 = mn + 1;
+    const int b = blockIdx.x;
+    const int cz = b % tl.gz, bx = (b / tl.gz) % tl.gx, by = b / (tl.gz * tl.gx);
+    const int tx = threadIdx.x % kCTX, ty = threadIdx.x / kCTX;
+    const int i0 = bx * kCTX, j0 = by * kCTY, i = i0 + tx, j = j0 + ty;
+    const int chunk = (n1 + tl.gz - 1) / tl.gz;
+    const int k0 = cz * chunk, k1 = min(n1, k0 + chunk);
     const T beta = (T)dsc->tcg_beta;
-    double den = 0.0;
-    for (int64_t t = blockIdx.x * (int64_t)kBlock + threadIdx.x; t < nb; t += (int64_t)gridDim.x * kBlock) {
-        int i, j, k;
-        block_ijk(n1, t, i, j, k);
-        TC nv[27][3];
+    auto ring3 = [](int p) { return ((p % 3) + 3) % 3; };
+    auto load_nodes = [&](int kp) {   // d' of plane kp (clamped) with its halo
+        const int kk = min(max(kp, 0), mn), slot = ring3(kp);
+        for (int e = threadIdx.x; e < (kCTY + 2) * (kCTX + 2); e += kBlock) {
+            const int ex = e % (kCTX + 2), ey = e / (kCTX + 2);
+            const int ii = min(max(i0 - 1 + ex, 0), mn), jj = min(max(j0 - 1 + ey, 0), mn);
+            const int64_t tn = ii + (int64_t)n1 * (jj + (int64_t)n1 * kk);
 #pragma unroll
-        for (int dz = -1; dz <= 1; ++dz)
-#pragma unroll
-            for (int dy = -1; dy <= 1; ++dy)
-#pragma unroll
-                for (int dx = -1; dx <= 1; ++dx) {
-                    const int ii = min(max(i + dx, 0), G.mn), jj = min(max(j + dy, 0), G.mn),
-                              kk = min(max(k + dz, 0), G.mn);
-                    const int64_t tn = ii + (int64_t)n1 * (jj + (int64_t)n1 * kk);
-                    const int s = (dx + 1) + 3 * (dy + 1) + 9 * (dz + 1);
-#pragma unroll
-                    for (int a = 0; a < 3; ++a)
-                        nv[s][a] = (TC)(FIRST ? din[3 * tn + a] : -z[3 * tn + a] + beta * din[3 * tn + a]);
-                }
-        TC cv[3], unused = 0;
-        grid_c_node<TC, false>(G, mats, nv, i, j, k, cv, unused);
-        double mm[3], cd[3], dd[3], c[3], pr[3];
-#pragma unroll
-        for (int a = 0; a < 3; ++a) {
-            mm[a] = (double)m0c[3 * t + a];
-            cd[a] = (double)cv[a];
-            dd[a] = (double)nv[13][a];
-            dout[3 * t + a] = (T)nv[13][a];
+            for (int a = 0; a < 3; ++a)
+                nd[slot][ey][ex][a] = FIRST ? din[3 * tn + a] : -z[3 * tn + a] + beta * din[3 * tn + a];
         }
-        const double mg = (double)mgc[t];
-        cross3(cd, mm, c);
-        cross3(mm, c, pr);
+    };
+    auto load_voxels = [&](int kz) {   // voxel plane kz (zero material outside the block)
+        const int slot = kz & 1;
+        for (int e = threadIdx.x; e < (kCTY + 1) * (kCTX + 1); e += kBlock) {
+            const int ex = e % (kCTX + 1), ey = e / (kCTX + 1);
+            const int xi = i0 - 1 + ex, yj = j0 - 1 + ey;
+            const bool inside = xi >= 0 && yj >= 0 && kz >= 0 && xi < mn && yj < mn && kz < mn;
+            const int mat = inside ? (int)G.vmat[xi + (int64_t)mn * (yj + (int64_t)mn * kz)] : G.n_mats;
+            const VoxCoef<T> c = vox_coef<T>(mats[mat]);
+            vc[slot][ey][ex][0] = c.cex;
+            vc[slot][ey][ex][1] = c.cK;
+            vc[slot][ey][ex][2] = c.u0;
+            vc[slot][ey][ex][3] = c.u1;
+            vc[slot][ey][ex][4] = c.u2;
+        }
+    };
+    double den = 0.0;
+    if (k0 < k1) {   // block-uniform
+        load_nodes(k0 - 1);
+        load_nodes(k0);
+        load_voxels(k0 - 1);
+        for (int k = k0; k < k1; ++k) {
+            load_nodes(k + 1);
+            load_voxels(k);
+            __syncthreads();
+            if (i < n1 && j < n1) {
+                T nv[27][3];
 #pragma unroll
-        for (int a = 0; a < 3; ++a) {
-            const T h = (T)(pr[a] - mg * dd[a]);
-            Hd[3 * t + a] = h;
-            den += dd[a] * (double)h;
+                for (int dz = -1; dz <= 1; ++dz)
+#pragma unroll
+                    for (int dy = -1; dy <= 1; ++dy)
+#pragma unroll
+                        for (int dx = -1; dx <= 1; ++dx) {
+                            const int sl = ring3(k + dz), s = (dx + 1) + 3 * (dy + 1) + 9 * (dz + 1);
+#pragma unroll
+                            for (int a = 0; a < 3; ++a) nv[s][a] = nd[sl][ty + 1 + dy][tx + 1 + dx][a];
+                        }
+                auto vox = [&](int w) {
+                    const int vx = w & 1, vy = (w >> 1) & 1, sl = (k - 1 + ((w >> 2) & 1)) & 1;
+                    const T *q = vc[sl][ty + vy][tx + vx];
+                    return VoxCoef<T>{q[0], q[1], (T)0, q[2], q[3], q[4]};
+                };
+                T cv[3], unused = 0;
+                grid_c_core<T, false>(G, nv, vox, cv, unused);
+                const int64_t t = i + (int64_t)n1 * (j + (int64_t)n1 * k);
+                double mm[3], cd[3], dd[3], c[3], pr[3];
+#pragma unroll
+                for (int a = 0; a < 3; ++a) {
+                    mm[a] = (double)m0c[3 * t + a];
+                    cd[a] = (double)cv[a];
+                    dd[a] = (double)nv[13][a];
+                    dout[3 * t + a] = nv[13][a];
+                }
+                const double mg = (double)mgc[t];
+                cross3(cd, mm, c);
+                cross3(mm, c, pr);
+#pragma unroll
+                for (int a = 0; a < 3; ++a) {
+                    const T h = (T)(pr[a] - mg * dd[a]);
+                    Hd[3 * t + a] = h;
+                    den += dd[a] * (double)h;
+                }
+            }
+            __syncthreads();
         }
     }
     double v[1] = {den};
@@ -1005,7 +1080,11 @@ void tcgc_run(const Launch &L, const GridDev &G, int64_t n, const TcgBlock<T> &B
               const double *gF, const double *g, const double *D, int max_iter, double *zout) {
     const int64_t n1 = G.mn + 1, nb = n1 * n1 * n1, n3 = 3 * nb;
     const int gi = grid_for(nb, 2), gu = grid_for(n3, 4);
-    const int gd = grid_for(nb, 1);
+    TcgTiles tl;
+    tl.gx = (int)((n1 + kCTX - 1) / kCTX);
+    tl.gy = (int)((n1 + kCTY - 1) / kCTY);
+    tl.gz = (int)std::max<int64_t>(1, std::min<int64_t>(n1, std::min(1024, kMaxPartials) / (tl.gx * tl.gy)));
+    const int gd = tl.gx * tl.gy * tl.gz;
     hipLaunchKernelGGL(k_tcgc_init<T>, dim3(gi), dim3(kBlock), 0, L.s, G, g, gF, m0, D, B.m0, B.mg, B.D, B.p, B.r,
                        B.z, B.d[0], L.red.partials);
     FinArgs ai{};
@@ -1017,10 +1096,10 @@ void tcgc_run(const Launch &L, const GridDev &G, int64_t n, const TcgBlock<T> &B
         const T *din = B.d[it & 1];
         T *dout = B.d[(it + 1) & 1];
         if (it == 0)
-            hipLaunchKernelGGL((k_tcgc_dh<T, T, true>), dim3(gd), dim3(kBlock), 0, L.s, G, L.mats, B.z, din, dout,
+            hipLaunchKernelGGL((k_tcgc_dh<T, true>), dim3(gd), dim3(kBlock), 0, L.s, G, tl, L.mats, B.z, din, dout,
                                B.m0, B.mg, B.Hd, L.dsc, L.red.partials);
         else
-            hipLaunchKernelGGL((k_tcgc_dh<T, T, false>), dim3(gd), dim3(kBlock), 0, L.s, G, L.mats, B.z, din, dout,
+            hipLaunchKernelGGL((k_tcgc_dh<T, false>), dim3(gd), dim3(kBlock), 0, L.s, G, tl, L.mats, B.z, din, dout,
                                B.m0, B.mg, B.Hd, L.dsc, L.red.partials);
         launch_fin(L, FinArgs::store(&L.dsc->tcg_den, 1, 0u, gd, &L.dsc->tcg_done));
         hipLaunchKernelGGL(k_tcgc_upd<T>, dim3(gu), dim3(kBlock), 0, L.s, n3, B.p, B.r, B.z, dout, B.Hd, B.D, L.dsc,
