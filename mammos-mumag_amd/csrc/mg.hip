@@ -23,7 +23,7 @@ namespace mm {
 
 namespace {
 
-// Column tiles of the per-level kernels: kMgTX x kMgTY (x, y) columns per 256-thread block
+// Column tiles of the transfer kernels: kMgTX x kMgTY (x, y) columns per 256-thread block
 // (64 x 4 measured faster than 32 x 8 on the 154^3 grid)
 constexpr int kMgTX = 64, kMgTY = 256 / kMgTX;
 
@@ -50,9 +50,9 @@ __device__ __forceinline__ T edge_sum_mg(const T *__restrict__ h1p, const T *__r
 // OP_FIRST:  out = w b / diag                (x = 0)
 // OP_SMOOTH: out = x + w (b - A x) / diag
 // OP_RESID:  out = b - A x
-// Zero on the faces.  Tiles of 64 x 4 (x, y) nodes, each thread marching a z chunk with a
-// sliding window for the z neighbours; the loads of plane k + 1 are issued before the
-// arithmetic of plane k.  DOT (OP_SMOOTH): partials of b.out and b.b per
+// Zero on the faces.  Strips of 256 consecutive nodes of a plane, each thread marching a
+// z chunk with a sliding window for the z neighbours; the loads of plane k + 1 are issued
+// before the arithmetic of plane k.  DOT (OP_SMOOTH): partials of b.out and b.b per
 // block into parts[0 / kMaxPartials + block] (the next PCG step's r.z and r.r).
 enum { OP_FIRST = 0, OP_SMOOTH = 1, OP_RESID = 2 };
 template <int OP, bool DOT>
@@ -62,8 +62,9 @@ __global__ __launch_bounds__(256) void k_mg_stencil(PoissonGrid<float> P, MgGrid
     if (skip && *skip) return;   // block-uniform
     const int nx = P.nx, ny = P.ny, nz = P.nz;
     const Tile T = mg_tile(G);
-    const int i = T.bx * kMgTX + (threadIdx.x % kMgTX);
-    const int j = T.by * kMgTY + (threadIdx.x / kMgTX);
+    // a strip of 256 consecutive nodes of a plane (G.x strips per plane, G.y = 1)
+    const int e = T.bx * 256 + threadIdx.x;
+    const int j = e / nx, i = e - j * nx;
     const int chunk = (nz + G.z - 1) / G.z;
     const int k0 = T.bz * chunk, k1 = min(nz, k0 + chunk);
     const bool active = i < nx && j < ny && k0 < k1;
@@ -515,8 +516,16 @@ __global__ __launch_bounds__(kTailThreads) void k_mg_tail(MgTailArgs a, const in
 
 // At most kMaxPartials blocks (the fused dot writes one partial per block); the setup
 // checks that one z chunk of tiles fits.
-MgGrid stencil_grid(const PoissonGrid<float> &P) {
+MgGrid tile_grid(const PoissonGrid<float> &P) {
     const int gx = (P.nx + kMgTX - 1) / kMgTX, gy = (P.ny + kMgTY - 1) / kMgTY;
+    const int64_t per_plane = (int64_t)gx * gy;
+    const int gz = (int)std::min<int64_t>(P.nz, std::max<int64_t>(1, kMaxPartials / per_plane));
+    return MgGrid{gx, gy, gz};
+}
+// The stencil kernels march strips of 256 consecutive nodes of a plane (long contiguous
+// runs per block and plane: 3% faster than 64 x 4 tiles on the 154^3 grid).
+MgGrid stencil_grid(const PoissonGrid<float> &P) {
+    const int gx = (P.nx * P.ny + 255) / 256, gy = 1;
     const int64_t per_plane = (int64_t)gx * gy;
     const int gz = (int)std::min<int64_t>(P.nz, std::max<int64_t>(1, kMaxPartials / per_plane));
     return MgGrid{gx, gy, gz};
@@ -580,17 +589,18 @@ void launch_mg_tail(const Launch &L, const MgTailArgs &a, const int *skip) {
     }
 }
 bool mg_stencil_fits(const PoissonGrid<float> &P) {
-    return (int64_t)((P.nx + kMgTX - 1) / kMgTX) * ((P.ny + kMgTY - 1) / kMgTY) <= kMaxPartials;
+    const MgGrid g = stencil_grid(P);
+    return (int64_t)g.x * g.y <= kMaxPartials;
 }
 void launch_mg_restrict(const Launch &L, const MgLevelDev &f, const MgLevelDev &c, const float *rf, float *bc,
                         float *x1c, const int *skip) {
-    const MgGrid g = stencil_grid(c.P);
+    const MgGrid g = tile_grid(c.P);
     hipLaunchKernelGGL(k_mg_restrict, blocks(g), dim3(256), 0, L.s, g, c.P.nx, c.P.ny, c.P.nz, f.P.nx,
                        f.P.ny, f.rxi, f.rxw, f.ryi, f.ryw, f.rzi, f.rzw, rf, bc, c.invw, x1c, skip);
 }
 void launch_mg_prolong(const Launch &L, const MgLevelDev &f, const MgLevelDev &c, const float *xc, float *xf,
                        const int *skip) {
-    const MgGrid g = stencil_grid(f.P);
+    const MgGrid g = tile_grid(f.P);
     hipLaunchKernelGGL(k_mg_prolong, blocks(g), dim3(256), 0, L.s, g, f.P.nx, f.P.ny, f.P.nz, c.P.nx,
                        c.P.ny, f.pxc, f.pxt, f.pyc, f.pyt, f.pzc, f.pzt, xc, xf, skip);
 }
