@@ -23,7 +23,7 @@ namespace mm {
 
 namespace {
 
-// Column tiles of the transfer kernels: kMgTX x kMgTY (x, y) columns per 256-thread block
+// Column tiles of the restriction kernel: kMgTX x kMgTY (x, y) columns per 256-thread block
 // (64 x 4 measured faster than 32 x 8 on the 154^3 grid)
 constexpr int kMgTX = 64, kMgTY = 256 / kMgTX;
 
@@ -231,8 +231,8 @@ __global__ __launch_bounds__(256) void k_mg_restrict(MgGrid G, int nxc, int nyc,
 
 // x_f += P x_c on the interior: per fine index i of an axis, coarse c0[i] with weight
 // 1 - t[i] and c0[i] + 1 with weight t[i] (t = 0 on coarse nodes; an interior fine node
-// has c0 + 1 inside the coarse axis).  Fine faces stay zero.  Tiles of 64 x 4 fine
-// columns marching z chunks.
+// has c0 + 1 inside the coarse axis).  Fine faces stay zero.  Strips of 256 consecutive
+// fine plane nodes marching z chunks.
 __global__ __launch_bounds__(256) void k_mg_prolong(MgGrid G, int nxf, int nyf, int nzf, int nxc, int nyc,
                                                     const int *__restrict__ pxc, const float *__restrict__ pxt,
                                                     const int *__restrict__ pyc, const float *__restrict__ pyt,
@@ -241,8 +241,8 @@ __global__ __launch_bounds__(256) void k_mg_prolong(MgGrid G, int nxf, int nyf, 
                                                     const int *skip) {
     if (skip && *skip) return;
     const Tile T = mg_tile(G);
-    const int i = T.bx * kMgTX + (threadIdx.x % kMgTX);
-    const int j = T.by * kMgTY + (threadIdx.x / kMgTX);
+    const int e = T.bx * 256 + threadIdx.x;   // strips of 256 consecutive plane nodes
+    const int j = e / nxf, i = e - j * nxf;
     const int chunk = (nzf + G.z - 1) / G.z;
     const int k0 = max(1, T.bz * chunk), k1 = min(nzf - 1, T.bz * chunk + chunk);
     if (i <= 0 || j <= 0 || i >= nxf - 1 || j >= nyf - 1) return;
@@ -600,7 +600,7 @@ void launch_mg_restrict(const Launch &L, const MgLevelDev &f, const MgLevelDev &
 }
 void launch_mg_prolong(const Launch &L, const MgLevelDev &f, const MgLevelDev &c, const float *xc, float *xf,
                        const int *skip) {
-    const MgGrid g = tile_grid(f.P);
+    const MgGrid g = stencil_grid(f.P);
     hipLaunchKernelGGL(k_mg_prolong, blocks(g), dim3(256), 0, L.s, g, f.P.nx, f.P.ny, f.P.nz, c.P.nx,
                        c.P.ny, f.pxc, f.pxt, f.pyc, f.pyt, f.pzc, f.pzt, xc, xf, skip);
 }
