@@ -540,37 +540,101 @@ __device__ __forceinline__ void grid_c_node(const GridDev &G, const MatCoef *__r
 }
 
 // y = C v at magnet nodes; with DIV also b = Dx vx + Dy vy + Dz vz at magnet nodes.
+// A block owns a kCTX x kCTY tile of block nodes and marches a z chunk: v (in TC) is
+// staged once per node into a three-plane LDS ring with a one-node halo (clamped to the
+// block, as the 27-point gather clamps), the voxels' coefficients into a two-plane ring;
+// each node then reads its 27 neighbours and 8 voxels from LDS.
+constexpr int kCTX = 32, kCTY = 8;
+struct TcgTiles {
+    int gx, gy, gz;
+};
+TcgTiles block_tiles(const GridDev &G) {
+    const int n1 = G.mn + 1;
+    TcgTiles tl;
+    tl.gx = (n1 + kCTX - 1) / kCTX;
+    tl.gy = (n1 + kCTY - 1) / kCTY;
+    tl.gz = std::max(1, std::min(n1, std::min(1024, kMaxPartials) / (tl.gx * tl.gy)));
+    return tl;
+}
 template <typename TC, bool DIV>
-__global__ __launch_bounds__(kBlock) void k_grid_exani(GridDev G, const MatCoef *__restrict__ mats,
+__global__ __launch_bounds__(kBlock) void k_grid_exani(GridDev G, TcgTiles tl, const MatCoef *__restrict__ mats,
                                                        const double *__restrict__ v,
                                                        double *__restrict__ y, double *__restrict__ b,
                                                        const int *skip) {
-    if (skip && *skip) return;
-    const int n1 = G.mn + 1;
-    const int64_t nmag = (int64_t)n1 * n1 * n1;
-    const int64_t t = blockIdx.x * (int64_t)kBlock + threadIdx.x;
-    if (t >= nmag) return;
-    const int i = (int)(t % n1), j = (int)((t / n1) % n1), k = (int)(t / ((int64_t)n1 * n1));
-    TC nv[27][3];
+    static_assert(kCTX * kCTY == kBlock, "one node per thread and plane");
+    constexpr int NV = DIV ? 6 : 5;
+    __shared__ TC nd[3][kCTY + 2][kCTX + 2][3];
+    __shared__ TC vc[2][kCTY + 1][kCTX + 1][NV];
+    if (skip && *skip) return;   // block-uniform
+    const int mn = G.mn, n1 = mn + 1;
+    const int bl = blockIdx.x;
+    const int cz = bl % tl.gz, bx = (bl / tl.gz) % tl.gx, by = bl / (tl.gz * tl.gx);
+    const int tx = threadIdx.x % kCTX, ty = threadIdx.x / kCTX;
+    const int i0 = bx * kCTX, j0 = by * kCTY, i = i0 + tx, j = j0 + ty;
+    const int chunk = (n1 + tl.gz - 1) / tl.gz;
+    const int k0 = cz * chunk, k1 = min(n1, k0 + chunk);
+    auto ring3 = [](int p) { return ((p % 3) + 3) % 3; };
+    auto load_nodes = [&](int kp) {
+        const int kk = min(max(kp, 0), mn), slot = ring3(kp);
+        for (int e = threadIdx.x; e < (kCTY + 2) * (kCTX + 2); e += kBlock) {
+            const int ex = e % (kCTX + 2), ey = e / (kCTX + 2);
+            const int ii = min(max(i0 - 1 + ex, 0), mn), jj = min(max(j0 - 1 + ey, 0), mn);
+            const int64_t g = gnode(G, ii, jj, kk);
 #pragma unroll
-    for (int dz = -1; dz <= 1; ++dz)
+            for (int a = 0; a < 3; ++a) nd[slot][ey][ex][a] = (TC)v[3 * g + a];
+        }
+    };
+    auto load_voxels = [&](int kz) {
+        const int slot = kz & 1;
+        for (int e = threadIdx.x; e < (kCTY + 1) * (kCTX + 1); e += kBlock) {
+            const int ex = e % (kCTX + 1), ey = e / (kCTX + 1);
+            const int xi = i0 - 1 + ex, yj = j0 - 1 + ey;
+            const bool inside = xi >= 0 && yj >= 0 && kz >= 0 && xi < mn && yj < mn && kz < mn;
+            const int mat = inside ? (int)G.vmat[xi + (int64_t)mn * (yj + (int64_t)mn * kz)] : G.n_mats;
+            const VoxCoef<TC> c = vox_coef<TC>(mats[mat]);
+            TC *q = vc[slot][ey][ex];
+            q[0] = c.cex;
+            q[1] = c.cK;
+            q[2] = c.u0;
+            q[3] = c.u1;
+            q[4] = c.u2;
+            if (DIV) q[NV - 1] = c.Js;
+        }
+    };
+    if (k0 >= k1) return;   // block-uniform, before any barrier
+    load_nodes(k0 - 1);
+    load_nodes(k0);
+    load_voxels(k0 - 1);
+    for (int k = k0; k < k1; ++k) {
+        load_nodes(k + 1);
+        load_voxels(k);
+        __syncthreads();
+        if (i < n1 && j < n1) {
+            TC nv[27][3];
 #pragma unroll
-        for (int dy = -1; dy <= 1; ++dy)
+            for (int dz = -1; dz <= 1; ++dz)
 #pragma unroll
-            for (int dx = -1; dx <= 1; ++dx) {
-                const int ii = min(max(i + dx, 0), G.mn), jj = min(max(j + dy, 0), G.mn),
-                          kk = min(max(k + dz, 0), G.mn);
-                const int64_t g = gnode(G, ii, jj, kk);
-                const int s = (dx + 1) + 3 * (dy + 1) + 9 * (dz + 1);
+                for (int dy = -1; dy <= 1; ++dy)
 #pragma unroll
-                for (int a = 0; a < 3; ++a) nv[s][a] = (TC)v[3 * g + a];
-            }
-    TC yv[3], bd = 0;
-    grid_c_node<TC, DIV>(G, mats, nv, i, j, k, yv, bd);
-    const int64_t g = gnode(G, i, j, k);
+                    for (int dx = -1; dx <= 1; ++dx) {
+                        const int sl = ring3(k + dz), s = (dx + 1) + 3 * (dy + 1) + 9 * (dz + 1);
 #pragma unroll
-    for (int a = 0; a < 3; ++a) y[3 * g + a] = (double)yv[a];
-    if (DIV) b[g] = (double)bd;
+                        for (int a = 0; a < 3; ++a) nv[s][a] = nd[sl][ty + 1 + dy][tx + 1 + dx][a];
+                    }
+            auto vox = [&](int w) {
+                const int vx = w & 1, vy = (w >> 1) & 1, sl = (k - 1 + ((w >> 2) & 1)) & 1;
+                const TC *q = vc[sl][ty + vy][tx + vx];
+                return VoxCoef<TC>{q[0], q[1], DIV ? q[NV - 1] : (TC)0, q[2], q[3], q[4]};
+            };
+            TC yv[3], bd = 0;
+            grid_c_core<TC, DIV>(G, nv, vox, yv, bd);
+            const int64_t g = gnode(G, i, j, k);
+#pragma unroll
+            for (int a = 0; a < 3; ++a) y[3 * g + a] = (double)yv[a];
+            if (DIV) b[g] = (double)bd;
+        }
+        __syncthreads();
+    }
 }
 
 // g = interleave(Gx u, Gy u, Gz u) at magnet nodes.
@@ -630,17 +694,18 @@ static inline int grid_mag_blocks(const GridDev &G) {
 
 void launch_grid_exani(const Launch &L, const GridDev &G, bool fp32, const double *v, double *y,
                        double *b, const int *skip_flag) {
-    const dim3 g(grid_mag_blocks(G)), blk(kBlock);
+    const TcgTiles tl = block_tiles(G);
+    const dim3 g(tl.gx * tl.gy * tl.gz), blk(kBlock);
     if (fp32) {
         if (b)
-            hipLaunchKernelGGL((k_grid_exani<float, true>), g, blk, 0, L.s, G, L.mats, v, y, b, skip_flag);
+            hipLaunchKernelGGL((k_grid_exani<float, true>), g, blk, 0, L.s, G, tl, L.mats, v, y, b, skip_flag);
         else
-            hipLaunchKernelGGL((k_grid_exani<float, false>), g, blk, 0, L.s, G, L.mats, v, y, b, skip_flag);
+            hipLaunchKernelGGL((k_grid_exani<float, false>), g, blk, 0, L.s, G, tl, L.mats, v, y, b, skip_flag);
     } else {
         if (b)
-            hipLaunchKernelGGL((k_grid_exani<double, true>), g, blk, 0, L.s, G, L.mats, v, y, b, skip_flag);
+            hipLaunchKernelGGL((k_grid_exani<double, true>), g, blk, 0, L.s, G, tl, L.mats, v, y, b, skip_flag);
         else
-            hipLaunchKernelGGL((k_grid_exani<double, false>), g, blk, 0, L.s, G, L.mats, v, y, b, skip_flag);
+            hipLaunchKernelGGL((k_grid_exani<double, false>), g, blk, 0, L.s, G, tl, L.mats, v, y, b, skip_flag);
     }
 }
 void launch_grid_grad(const Launch &L, const GridDev &G, bool fp32, const double *u, double *g) {
@@ -923,10 +988,6 @@ __global__ __launch_bounds__(kBlock) void k_tcgc_init(GridDev G, const double *_
 // one-node halo (clamped to the block, as the 27-point gather clamps), the voxels'
 // coefficients into a two-plane ring; the node then reads its 27 neighbours and 8
 // voxels from LDS.
-constexpr int kCTX = 32, kCTY = 8;
-struct TcgTiles {
-    int gx, gy, gz;
-};
 template <typename T, bool FIRST>
 __global__ __launch_bounds__(kBlock) void k_tcgc_dh(GridDev G, TcgTiles tl, const MatCoef *__restrict__ mats,
                                                     const T *__restrict__ z, const T *__restrict__ din,
@@ -1080,10 +1141,7 @@ void tcgc_run(const Launch &L, const GridDev &G, int64_t n, const TcgBlock<T> &B
               const double *gF, const double *g, const double *D, int max_iter, double *zout) {
     const int64_t n1 = G.mn + 1, nb = n1 * n1 * n1, n3 = 3 * nb;
     const int gi = grid_for(nb, 2), gu = grid_for(n3, 4);
-    TcgTiles tl;
-    tl.gx = (int)((n1 + kCTX - 1) / kCTX);
-    tl.gy = (int)((n1 + kCTY - 1) / kCTY);
-    tl.gz = (int)std::max<int64_t>(1, std::min<int64_t>(n1, std::min(1024, kMaxPartials) / (tl.gx * tl.gy)));
+    const TcgTiles tl = block_tiles(G);
     const int gd = tl.gx * tl.gy * tl.gz;
     hipLaunchKernelGGL(k_tcgc_init<T>, dim3(gi), dim3(kBlock), 0, L.s, G, g, gF, m0, D, B.m0, B.mg, B.D, B.p, B.r,
                        B.z, B.d[0], L.red.partials);
