@@ -553,8 +553,8 @@ int total_eg(mm_ctx *c, const double *m, double hext, const double *hdir, double
     }
     const double zc0 = -hext * hdir[0] / c->V, zc1 = -hext * hdir[1] / c->V,
                  zc2 = -hext * hdir[2] / c->V;
-    launch_finish_eg(L, c->N, m, c->meas, zc0, zc1, zc2, c->gC, c->gd, gfull, gproj, &c->dsc->energy,
-                     hmag_on > 0);
+    launch_finish_eg(L, c->N, c->structured ? &c->G : nullptr, m, c->meas, zc0, zc1, zc2, c->gC, c->gd, gfull,
+                     gproj, &c->dsc->energy, hmag_on > 0);
     int rc2 = sync_scalars(c);
     if (rc2) return rc2;
     *E = c->hsc->energy;

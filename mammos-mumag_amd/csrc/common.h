@@ -184,8 +184,9 @@ void launch_grid_exani(const Launch &L, const GridDev &G, bool fp32, const doubl
 void launch_grid_grad(const Launch &L, const GridDev &G, bool fp32, const double *u, double *g);
 
 // Node kernels.
-void launch_finish_eg(const Launch &L, int64_t n, const double *m, const double *meas, double zc0,
-                      double zc1, double zc2, const double *gC, const double *gd, double *g_full,
+// G: the structured grid (nodes off its magnet block are zero on input), or null
+void launch_finish_eg(const Launch &L, int64_t n, const GridDev *G, const double *m, const double *meas,
+                      double zc0, double zc1, double zc2, const double *gC, const double *gd, double *g_full,
                       double *g_proj, double *E_out, int with_hmag);
 void launch_external(const Launch &L, int64_t n, const double *m, const double *meas, double zc0,
                      double zc1, double zc2, double *g, double *E_out);

@@ -725,7 +725,9 @@ __device__ __forceinline__ void cross3(const double *a, const double *b, double 
 }
 
 // g = gz + gC (+ gd); E = m.gz + m.gC/2 (+ m.gd/2); g_proj = m x (g x m)
-__global__ __launch_bounds__(kBlock) void k_finish_eg(int64_t n, const double *__restrict__ m,
+// With a structured grid (G.nx > 0) the nodes outside the magnet block, where m, meas, gC
+// and gd are zero, get zeros without being read.
+__global__ __launch_bounds__(kBlock) void k_finish_eg(int64_t n, GridDev G, const double *__restrict__ m,
                                                       const double *__restrict__ meas, double zc0,
                                                       double zc1, double zc2,
                                                       const double *__restrict__ gC,
@@ -736,6 +738,18 @@ __global__ __launch_bounds__(kBlock) void k_finish_eg(int64_t n, const double *_
     double e = 0.0;
     const double zc[3] = {zc0, zc1, zc2};
     for (int64_t i = blockIdx.x * (int64_t)kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock) {
+        if (G.nx > 0) {
+            const int64_t r = i / G.nx, k = r / G.ny;
+            const int gi = (int)(i - r * G.nx) - G.m0x, gj = (int)(r - k * G.ny) - G.m0y, gk = (int)k - G.m0z;
+            if (gi < 0 || gj < 0 || gk < 0 || gi > G.mn || gj > G.mn || gk > G.mn) {
+#pragma unroll
+                for (int a = 0; a < 3; ++a) {
+                    g_full[3 * i + a] = 0.0;
+                    g_proj[3 * i + a] = 0.0;
+                }
+                continue;
+            }
+        }
         const double mi = meas[i];
         double mm[3], g[3];
 #pragma unroll
@@ -758,11 +772,12 @@ __global__ __launch_bounds__(kBlock) void k_finish_eg(int64_t n, const double *_
     block_partials<1>(v, 0u, partials);
 }
 
-void launch_finish_eg(const Launch &L, int64_t n, const double *m, const double *meas, double zc0,
-                      double zc1, double zc2, const double *gC, const double *gd, double *g_full,
+void launch_finish_eg(const Launch &L, int64_t n, const GridDev *G, const double *m, const double *meas,
+                      double zc0, double zc1, double zc2, const double *gC, const double *gd, double *g_full,
                       double *g_proj, double *E_out, int with_hmag) {
     const int nb = grid_for(n, 2);
-    hipLaunchKernelGGL(k_finish_eg, dim3(nb), dim3(kBlock), 0, L.s, n, m, meas, zc0, zc1, zc2, gC, gd,
+    const GridDev g = G ? *G : GridDev{};
+    hipLaunchKernelGGL(k_finish_eg, dim3(nb), dim3(kBlock), 0, L.s, n, g, m, meas, zc0, zc1, zc2, gC, gd,
                        g_full, g_proj, with_hmag, L.red.partials);
     launch_fin(L, FinArgs::store(E_out, 1, 0u, nb, nullptr));
 }
