@@ -67,6 +67,7 @@ struct mm_ctx {
     ElemDev E;
     GridDev G;
     TcgBlock<float> tb32;    // truncated CG on the magnet block (structured grids)
+    bool ncg_block = false;  // the NCG's reductions visit the magnet block only (this call)
     TcgBlock<double> tb64;
     PoissonGrid<double> P;
     PoissonGrid<float> P32;
@@ -594,6 +595,16 @@ int reduce(mm_ctx *c, RedOp op, int64_t n, const double *a, const double *b, con
     for (int k = 0; k < nout; ++k) out[k] = c->hsc->red[k];
     return 0;
 }
+// reduce over the NCG's interleaved 3-vectors; on the magnet block only when the field
+// step's m is zero off it (c->ncg_block: then so are the gradient, direction and iterates)
+int reduce3(mm_ctx *c, RedOp op, const double *a, const double *b, const double *cc, const double *d,
+            double *out, int nout) {
+    launch_reduce(c->L(), op, 3 * c->N, a, b, cc, d, 0, 0, 0, c->dsc->red, c->ncg_block ? &c->G : nullptr, 3);
+    int rc = sync_scalars(c);
+    if (rc) return rc;
+    for (int k = 0; k < nout; ++k) out[k] = c->hsc->red[k];
+    return 0;
+}
 
 std::vector<MatCoef> material_table(const mm_mesh_desc *d, double V) {
     const double mu0 = 4.0e-7 * M_PI, size = d->length_scale > 0 ? d->length_scale : 1e-9;
@@ -1092,7 +1103,9 @@ int mm_minimize(mm_ctx *c, double *m_inout, double hext, const double hdir[3],
     int rc = 0;
     double *X = c->X, *X0 = c->X0, *X1 = c->X1, *G = c->Gp, *G1 = c->G1, *D = c->Dd, *Z1 = c->Z1;
     HIPC(hipMemcpyAsync(X, m_inout, bytes3, hipMemcpyDeviceToDevice, c->s));
+    if (c->structured) launch_outside_max(L, n, c->G, X, &c->dsc->red[0]);
     if ((rc = sync_scalars(c))) return rc;
+    c->ncg_block = c->structured && c->hsc->red[0] == 0.0;
     const long long tcg0 = c->hsc->tcg_total;
 
     auto feval = [&](const double *x, double *E, double *g) -> int {
@@ -1120,14 +1133,14 @@ int mm_minimize(mm_ctx *c, double *m_inout, double hext, const double hdir[3],
     const double sqrt_eps = std::sqrt(DBL_EPSILON);
     for (;;) {
         double s3[3];
-        if ((rc = reduce(c, RED_STOP3, n3, X0, X, G, nullptr, s3, 3))) return rc;
+        if ((rc = reduce3(c, RED_STOP3, X0, X, G, nullptr, s3, 3))) return rc;
         const bool a = (f0 - f) > tol * (1 + std::fabs(f));
         const bool b = (s3[0] > sqrt_tol * (1 + s3[1])) || k == 0;
         const bool cc = s3[2] > cbrt_tol * (1 + std::fabs(f));
         if (!((a || b || cc) && k < iter_max)) break;
         // restart if d is not a sufficient descent direction (minimizers.py:171-173)
         double dg3[3];
-        if ((rc = reduce(c, RED_DG3, n3, D, G, nullptr, nullptr, dg3, 3))) return rc;
+        if ((rc = reduce3(c, RED_DG3, D, G, nullptr, nullptr, dg3, 3))) return rc;
         if (dg3[0] > -0.001 * std::sqrt(dg3[1]) * std::sqrt(dg3[2])) launch_axpby(L, n3, -1.0, G, 0.0, nullptr, D);
         // ---- line search (minimizers.py:92-137)
         double f1 = 0.0;
@@ -1135,14 +1148,14 @@ int mm_minimize(mm_ctx *c, double *m_inout, double hext, const double hdir[3],
             const double rho = 0.5, cA = 1e-4, delta = 0.1, eps = 1e-6;
             const int max_ls = 20;
             double h3[3];
-            if ((rc = reduce(c, RED_H3, n3, X, D, nullptr, nullptr, h3, 3))) return rc;
+            if ((rc = reduce3(c, RED_H3, X, D, nullptr, nullptr, h3, 3))) return rc;
             const double t1 = 2.0 * sqrt_eps * (1.0 + std::sqrt(h3[0])) / std::sqrt(h3[1]);
             const double t2 = sqrt_eps / h3[2];
             const double h = t1 < t2 ? t1 : t2;
             launch_update_m(L, n, X, h, D, X1);
             if ((rc = feval(X1, &f1, G1))) return rc;
             double gd2[2];
-            if ((rc = reduce(c, RED_GD2, n3, G, G1, D, nullptr, gd2, 2))) return rc;
+            if ((rc = reduce3(c, RED_GD2, G, G1, D, nullptr, gd2, 2))) return rc;
             const double gd = gd2[0];
             double alpha;
             {  // alpha_init (minimizers.py:34-57), c = 0.1, o = 0.99
@@ -1155,7 +1168,7 @@ int mm_minimize(mm_ctx *c, double *m_inout, double hext, const double hdir[3],
             launch_update_m(L, n, X, alpha, D, X1);
             if ((rc = feval(X1, &f1, G1))) return rc;
             double gd1;
-            if ((rc = reduce(c, RED_DOT, n3, G1, D, nullptr, nullptr, &gd1, 1))) return rc;
+            if ((rc = reduce3(c, RED_DOT, G1, D, nullptr, nullptr, &gd1, 1))) return rc;
             int i = 0;
             for (;;) {
                 const double eps_k = eps * std::fabs(f);
@@ -1165,7 +1178,7 @@ int mm_minimize(mm_ctx *c, double *m_inout, double hext, const double hdir[3],
                 alpha *= rho;
                 launch_update_m(L, n, X, alpha, D, X1);
                 if ((rc = feval(X1, &f1, G1))) return rc;
-                if ((rc = reduce(c, RED_DOT, n3, G1, D, nullptr, nullptr, &gd1, 1))) return rc;
+                if ((rc = reduce3(c, RED_DOT, G1, D, nullptr, nullptr, &gd1, 1))) return rc;
                 ++i;
                 loc.backtracks++;
             }
@@ -1173,10 +1186,10 @@ int mm_minimize(mm_ctx *c, double *m_inout, double hext, const double hdir[3],
         // ---- HS+ direction (minimizers.py:175-184)
         if ((rc = minv(G1, Z1))) return rc;
         double yzd[2];
-        if ((rc = reduce(c, RED_YZD, n3, G, G1, Z1, D, yzd, 2))) return rc;
+        if ((rc = reduce3(c, RED_YZD, G, G1, Z1, D, yzd, 2))) return rc;
         const double beta = std::max(yzd[0] / yzd[1], 0.0);
         launch_axpby(L, n3, -1.0, Z1, beta, D, D);
-        if ((rc = reduce(c, RED_DG3, n3, D, G, nullptr, nullptr, dg3, 3))) return rc;
+        if ((rc = reduce3(c, RED_DG3, D, G, nullptr, nullptr, dg3, 3))) return rc;
         if (dg3[0] > -0.001 * std::sqrt(dg3[1]) * std::sqrt(dg3[2])) launch_axpby(L, n3, -1.0, Z1, 0.0, nullptr, D);
         // shift state: x0 <- x, x <- x1, f0 <- f, f <- f1, g <- g1
         std::swap(X0, X);

@@ -227,7 +227,12 @@ enum RedOp {
     RED_MH = 7,           // out0 = sum_i (h . m_i) meas_i  (a = m, b = meas)
     RED_SS = 8,           // out0 = a.a
 };
+// out = max |m| over the nodes off the magnet block of a structured grid
+void launch_outside_max(const Launch &L, int64_t n, const GridDev &G, const double *m, double *out);
+// G (optional): a structured grid whose vectors are zero off the magnet block, visited
+// on the block only (comp = 3 for interleaved 3-vectors of n = 3 N, 1 for node vectors)
 void launch_reduce(const Launch &L, RedOp op, int64_t n, const double *a, const double *b,
-                   const double *c, const double *d, double h0, double h1, double h2, double *out);
+                   const double *c, const double *d, double h0, double h1, double h2, double *out,
+                   const GridDev *G = nullptr, int comp = 1);
 
 }  // namespace mm

@@ -1227,8 +1227,11 @@ void launch_axpby(const Launch &L, int64_t n, double a, const double *x, double 
 
 // ------------------------------------------------------------------ generic reductions
 
+// With G.nx > 0 (structured grid; the vectors are zero off the magnet block) only the
+// block's entries are visited: element e of the block range maps to the grid element
+// comp * node(e / comp) + e % comp (comp = 3 for interleaved 3-vectors, 1 for nodes).
 template <int OP>
-__global__ __launch_bounds__(kBlock) void k_reduce(int64_t n, const double *__restrict__ a,
+__global__ __launch_bounds__(kBlock) void k_reduce(int64_t n, GridDev G, int comp, const double *__restrict__ a,
                                                    const double *__restrict__ b,
                                                    const double *__restrict__ c,
                                                    const double *__restrict__ d, double h0, double h1,
@@ -1240,7 +1243,16 @@ __global__ __launch_bounds__(kBlock) void k_reduce(int64_t n, const double *__re
     double s[NV];
 #pragma unroll
     for (int k = 0; k < NV; ++k) s[k] = 0.0;
-    for (int64_t i = blockIdx.x * (int64_t)kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock) {
+    const int n1 = G.mn + 1;
+    const int64_t ne = G.nx > 0 ? (int64_t)n1 * n1 * n1 * comp : n;
+    for (int64_t e = blockIdx.x * (int64_t)kBlock + threadIdx.x; e < ne; e += (int64_t)gridDim.x * kBlock) {
+        int64_t i = e;
+        if (G.nx > 0) {
+            const int64_t t = e / comp;
+            const int r = (int)(e - t * comp);
+            const int64_t tj = t / n1;
+            i = comp * gnode(G, (int)(t - tj * n1), (int)(tj % n1), (int)(tj / n1)) + r;
+        }
         if constexpr (OP == RED_DOT) {
             s[0] += a[i] * b[i];
         } else if constexpr (OP == RED_NORMINF) {
@@ -1273,18 +1285,40 @@ __global__ __launch_bounds__(kBlock) void k_reduce(int64_t n, const double *__re
     block_partials<NV>(s, MASK, partials);
 }
 
+// max |m| over the nodes off the magnet block of a structured grid (m interleaved [3 N])
+__global__ __launch_bounds__(kBlock) void k_outside_max(int64_t n, GridDev G, const double *__restrict__ m,
+                                                        double *partials) {
+    double s[1] = {0.0};
+    for (int64_t i = blockIdx.x * (int64_t)kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock) {
+        const int64_t r = i / G.nx, k = r / G.ny;
+        const int gi = (int)(i - r * G.nx) - G.m0x, gj = (int)(r - k * G.ny) - G.m0y, gk = (int)k - G.m0z;
+        if (gi < 0 || gj < 0 || gk < 0 || gi > G.mn || gj > G.mn || gk > G.mn)
+#pragma unroll
+            for (int a = 0; a < 3; ++a) s[0] = fmax(s[0], fabs(m[3 * i + a]));
+    }
+    block_partials<1>(s, 1u, partials);
+}
+void launch_outside_max(const Launch &L, int64_t n, const GridDev &G, const double *m, double *out) {
+    const int nb = grid_for(n, 4);
+    hipLaunchKernelGGL(k_outside_max, dim3(nb), dim3(kBlock), 0, L.s, n, G, m, L.red.partials);
+    launch_fin(L, FinArgs::store(out, 1, 1u, nb, nullptr));
+}
+
 void launch_reduce(const Launch &L, RedOp op, int64_t n, const double *a, const double *b,
-                   const double *c, const double *d, double h0, double h1, double h2, double *out) {
-    const int nb = grid_for(n, 8);
+                   const double *c, const double *d, double h0, double h1, double h2, double *out,
+                   const GridDev *G, int comp) {
+    const GridDev gd = G ? *G : GridDev{};
+    const int64_t n1 = gd.mn + 1;
+    const int nb = grid_for(G ? n1 * n1 * n1 * comp : n, 8);
     const dim3 g(nb), blk(kBlock);
     int nv = 1;
     unsigned mask = 0;
     switch (op) {
-#define MM_RED_CASE(OPC, NV, MASK)                                                                     \
-    case OPC:                                                                                          \
-        hipLaunchKernelGGL(k_reduce<OPC>, g, blk, 0, L.s, n, a, b, c, d, h0, h1, h2, L.red.partials); \
-        nv = NV;                                                                                       \
-        mask = MASK;                                                                                   \
+#define MM_RED_CASE(OPC, NV, MASK)                                                                        \
+    case OPC:                                                                                             \
+        hipLaunchKernelGGL(k_reduce<OPC>, g, blk, 0, L.s, n, gd, comp, a, b, c, d, h0, h1, h2, L.red.partials); \
+        nv = NV;                                                                                          \
+        mask = MASK;                                                                                      \
         break;
         MM_RED_CASE(RED_DOT, 1, 0u)
         MM_RED_CASE(RED_NORMINF, 1, 1u)
