@@ -34,11 +34,11 @@ HBM_PEAK_GBS = 8000.0
 # on the voronoi64_250nm grid (structured path), from rocprofv3 --pmc FETCH_SIZE and
 # WRITE_SIZE (median KiB per dispatch) scaled by the same counters on the 512 MiB device
 # copy of --traffic-probe (factors 0.5000 and 1.0000); see
-# profiles/pcg_kernels_{fp64,mixed}_voronoi125.md and profiles/mg_vcycle_mixed_voronoi125.md.
+# profiles/pcg_kernels_{fp64,mixed_jacobi}_voronoi125.md and profiles/mg_vcycle_mixed_voronoi125.md.
 TRAFFIC = {
-    ("fp64", "spmv"): 134047365, ("fp64", "update"): 233836081,
-    ("mixed", "spmv"): 67436398, ("mixed", "update"): 116955348,
-    ("mixed", "mg_smooth"): 47887360,
+    ("fp64", "spmv"): 132921483, ("fp64", "update"): 233838276,
+    ("mixed", "spmv"): 67524786, ("mixed", "update"): 116970655,
+    ("mixed", "mg_smooth"): 46100553,
 }
 
 
