@@ -62,29 +62,39 @@ static inline int grid_for(int64_t n, int per_thread = 1) {
     return (int)b;
 }
 
-// One block: totals of the partials, then the scalar logic selected by op.
-__global__ __launch_bounds__(kBlock) void k_fin(const double *__restrict__ partials, FinArgs a,
-                                                DevScalars *dsc) {
+// One block: totals of the partials, then the scalar logic selected by op.  All values'
+// partials are loaded in one pass (each value summed in the same order as one value at a
+// time: per thread over j = t, t + 256, ..., then the wave, then the waves in order), one
+// barrier.
+constexpr int kFinThreads = kBlock;
+__global__ __launch_bounds__(kFinThreads) void k_fin(const double *__restrict__ partials, FinArgs a,
+                                                     DevScalars *dsc) {
     if (a.skip && *a.skip) return;
-    __shared__ double sh[kBlock / kWave];
+    __shared__ double sh[4][kFinThreads / kWave];
     const int lane = threadIdx.x & (kWave - 1), w = threadIdx.x / kWave;
+    double acc[4] = {0.0, 0.0, 0.0, 0.0};
+    for (int j = threadIdx.x; j < a.nparts; j += kFinThreads) {
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            if (i >= a.nv) break;
+            const double p = partials[i * kMaxPartials + j];
+            acc[i] = ((a.maxmask >> i) & 1u) ? fmax(acc[i], p) : acc[i] + p;
+        }
+    }
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        acc[i] = ((a.maxmask >> i) & 1u) ? wave_max(acc[i]) : wave_sum(acc[i]);
+        if (lane == 0) sh[i][w] = acc[i];
+    }
+    __syncthreads();
+    if (threadIdx.x != 0) return;
     double tot[4] = {0.0, 0.0, 0.0, 0.0};
     for (int i = 0; i < a.nv; ++i) {
         const bool mx = (a.maxmask >> i) & 1u;
-        double s = 0.0;
-        for (int j = threadIdx.x; j < a.nparts; j += kBlock) {
-            const double p = partials[i * kMaxPartials + j];
-            s = mx ? fmax(s, p) : s + p;
-        }
-        s = mx ? wave_max(s) : wave_sum(s);
-        if (lane == 0) sh[w] = s;
-        __syncthreads();
-        double t = sh[0];
-        for (int k = 1; k < kBlock / kWave; ++k) t = mx ? fmax(t, sh[k]) : t + sh[k];
+        double t = sh[i][0];
+        for (int k = 1; k < kFinThreads / kWave; ++k) t = mx ? fmax(t, sh[i][k]) : t + sh[i][k];
         tot[i] = t;
-        __syncthreads();
     }
-    if (threadIdx.x != 0) return;
     switch (a.op) {
         case FIN_STORE:
             for (int i = 0; i < a.nv; ++i) a.out[i] = tot[i];
@@ -133,7 +143,7 @@ __global__ __launch_bounds__(kBlock) void k_fin(const double *__restrict__ parti
 }
 
 void launch_fin(const Launch &L, const FinArgs &a) {
-    hipLaunchKernelGGL(k_fin, dim3(1), dim3(kBlock), 0, L.s, L.red.partials, a, L.dsc);
+    hipLaunchKernelGGL(k_fin, dim3(1), dim3(kFinThreads), 0, L.s, L.red.partials, a, L.dsc);
 }
 
 // ------------------------------------------------------------------ SELL SpMV
