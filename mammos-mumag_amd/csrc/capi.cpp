@@ -96,11 +96,11 @@ struct mm_ctx {
     int mg_nl = 0;                      // levels (0: Jacobi)
     int mg_batch = 8;                   // most inner iterations between host checks with multigrid
     double mg_rate2 = 0.05;             // running r.r reduction per multigrid-PCG iteration
-    int mg_deg = 3, mg_cdeg = 6;        // smoothing degree, coarsest-level polynomial degree
+    int mg_deg = 3, mg_cdeg = 12;       // smoothing degree, coarsest-level polynomial degree
     MgLevelDev mg[kMgMaxLevels];
     float *mg_b[kMgMaxLevels] = {}, *mg_xa[kMgMaxLevels] = {}, *mg_xb[kMgMaxLevels] = {},
           *mg_r[kMgMaxLevels] = {};
-    float mg_om[16] = {}, mg_omc[16] = {};
+    float mg_om[kMgMaxDeg] = {}, mg_omc[kMgMaxDeg] = {};
     int mg_tail = 0;                    // levels >= mg_tail run in the one-workgroup tail kernel
     MgTailArgs tail;
 
@@ -1299,14 +1299,35 @@ int mm_multigrid(mm_ctx *c, int32_t max_levels) {
             *ip[a] = di;
         }
     }
-    // Chebyshev-Richardson steps: reciprocal Chebyshev nodes of [lo, 2]
-    auto omegas = [](float *om, int deg, double lo) {
+    // Chebyshev-Richardson steps: reciprocals of the Chebyshev nodes of [lo, 2], from the
+    // largest node down; with leja set, in greedy Leja order instead (the largest node,
+    // then each time the node with the largest product of distances to those taken; ties,
+    // which the symmetric nodes make, go to the lower index).  Same polynomial; in the
+    // descending order the last steps of the coarsest polynomial (w up to ~100) multiply
+    // the FP32 rounding of the earlier ones by ~1e3, in Leja order it stays at ~4e-7.
+    auto omegas = [](float *om, int deg, double lo, bool leja) {
         const double hi = 2.0;
-        for (int k = 0; k < deg; ++k)
-            om[k] = (float)(1.0 / ((hi + lo) / 2 + (hi - lo) / 2 * std::cos(M_PI * (2 * k + 1) / (2.0 * deg))));
+        double node[kMgMaxDeg];
+        int order[kMgMaxDeg];
+        bool used[kMgMaxDeg] = {};
+        for (int k = 0; k < deg; ++k) {
+            node[k] = (hi + lo) / 2 + (hi - lo) / 2 * std::cos(M_PI * (2 * k + 1) / (2.0 * deg));
+            order[k] = k;
+        }
+        for (int t = 1; leja && t < deg; ++t) {
+            used[order[t - 1]] = true;
+            double best = -1;
+            for (int j = 0; j < deg; ++j) {
+                if (used[j]) continue;
+                double p = 1;
+                for (int i = 0; i < t; ++i) p *= std::fabs(node[j] - node[order[i]]);
+                if (p > best * (1 + 1e-9)) best = p, order[t] = j;
+            }
+        }
+        for (int k = 0; k < deg; ++k) om[k] = (float)(1.0 / node[order[k]]);
     };
-    omegas(c->mg_om, c->mg_deg, 2.0 / 30.0);
-    omegas(c->mg_omc, c->mg_cdeg, 2.0 / 300.0);
+    omegas(c->mg_om, c->mg_deg, 2.0 / 15.0, false);
+    omegas(c->mg_omc, c->mg_cdeg, 2.0 / 100.0, true);
     // invw of every level: its first step's weight over the stencil's own diagonal (the
     // first-step kernel applied to ones), 0 on the faces
     {
@@ -1330,7 +1351,7 @@ int mm_multigrid(mm_ctx *c, int32_t max_levels) {
         a.nt = nl - t;
         a.s = c->mg_deg;
         a.cd = c->mg_cdeg;
-        for (int k = 0; k < 8; ++k) {
+        for (int k = 0; k < kMgMaxDeg; ++k) {
             a.om[k] = c->mg_om[k];
             a.omc[k] = c->mg_omc[k];
         }

@@ -34,6 +34,7 @@ struct MgLevelDev {
 // tr_off[q] of its transfer lists (mg_tail_layout fills the offsets and lds_bytes).
 constexpr int kMgTailMax = 4;
 constexpr int kMgTailLdsBytes = 64 * 1024;
+constexpr int kMgMaxDeg = 16;      // smoothing / coarsest-level polynomial degrees at most
 struct MgTailArgs {
     int nt = 0, s = 0, cd = 0;
     int n[kMgTailMax] = {}, off[kMgTailMax] = {}, coef_off[kMgTailMax] = {}, tr_off[kMgTailMax] = {};
@@ -41,7 +42,7 @@ struct MgTailArgs {
     MgLevelDev lev[kMgTailMax];
     const float *b0 = nullptr;
     float *x0 = nullptr;
-    float om[8] = {}, omc[8] = {};
+    float om[kMgMaxDeg] = {}, omc[kMgMaxDeg] = {};
 };
 int mg_tail_layout(MgTailArgs &a);
 void launch_mg_tail(const Launch &L, const MgTailArgs &a, const int *skip);

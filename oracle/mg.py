@@ -7,8 +7,9 @@ intervals merge when both are at most 3 hmin, hmin the level's smallest spacing 
 three axes; down to at most 16 nodes per axis),
 the 7-point operator of each level's rectilinear grid with zero on the faces, linear
 interpolation and its transpose, Chebyshev-Richardson smoothing x += w_k (b - A x) / diag
-with w_k the reciprocal Chebyshev nodes of [2/30, 2] (reverse order after the coarse
-correction), the coarsest level a degree-6 polynomial on [2/300, 2].
+with w_k the reciprocal Chebyshev nodes of [2/15, 2] (reverse order after the coarse
+correction), the coarsest level a degree-12 polynomial on [2/100, 2] with its steps in
+Leja order.
 
 Pinned by: its own operator against the oracle's assembled K (interior rows), its
 symmetry, and V-cycle properties (exact on the coarsest level's polynomial, linearity);
@@ -93,12 +94,33 @@ def interp(xf, keep):
 
 
 def omegas(deg, lo, hi=2.0):
+    """Chebyshev-Richardson weights: reciprocals of the Chebyshev nodes of [lo, hi],
+    from the largest node down."""
     k = np.arange(deg)
     return 1.0 / ((hi + lo) / 2 + (hi - lo) / 2 * np.cos(np.pi * (2 * k + 1) / (2 * deg)))
 
 
+def leja(om):
+    """The weights in greedy Leja order of their nodes 1/w: the largest node first, then
+    each time the node with the largest product of distances to those already taken
+    (ties, which symmetric nodes make, to the lower index).  Same polynomial; in FP32 the
+    partial products stay bounded."""
+    nodes = 1.0 / np.asarray(om, float)
+    order = [int(np.argmax(nodes))]
+    while len(order) < len(nodes):
+        best, pick = -1.0, -1
+        for j in range(len(nodes)):
+            if j in order:
+                continue
+            p = np.prod(np.abs(nodes[j] - nodes[order]))
+            if p > best * (1 + 1e-9):
+                best, pick = p, j
+        order.append(pick)
+    return np.asarray(om)[order]
+
+
 class VCycle:
-    def __init__(self, xs, ys, zs, max_levels=8, deg=3, cdeg=6):
+    def __init__(self, xs, ys, zs, max_levels=8, deg=3, cdeg=12):
         self.levels, self.axes = [], []
         axes = [np.asarray(xs, float), np.asarray(ys, float), np.asarray(zs, float)]
         while True:
@@ -115,8 +137,8 @@ class VCycle:
             lev["P"] = sp.kron(sp.kron(interp(axes[2], keeps[2]), interp(axes[1], keeps[1])),
                                interp(axes[0], keeps[0])).tocsr()
             axes = [a[k] for a, k in zip(axes, keeps)]
-        self.om = omegas(deg, 2.0 / 30.0)
-        self.omc = omegas(cdeg, 2.0 / 300.0)
+        self.om = omegas(deg, 2.0 / 15.0)
+        self.omc = leja(omegas(cdeg, 2.0 / 100.0))
 
     def _smooth(self, lev, x, b, oms):
         for w in oms:
