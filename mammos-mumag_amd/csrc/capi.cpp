@@ -390,15 +390,20 @@ float *mg_vcycle(mm_ctx *c, const float *r, float *z, float *alt, const int *ski
 // b - K x; with Jacobi the search direction continues as p = z + beta p with gamma from
 // the true residual, with multigrid the recursion restarts from p = M r.  Stops when the
 // true residual satisfies r.r <= max(tol^2 b.b, atol^2), the FP64 path's bound.
+// With multigrid the segment's target is the inner iteration's own stop bound, tested
+// after each KB before the V-cycle (k_pcg_check), and the V-cycle of a batch's last
+// iteration is launched after the host has read the flags, only when the segment goes on:
+// no V-cycle runs for a residual that is about to be replaced or has converged.
 int pcg_mixed(mm_ctx *c, const double *b, double *x, double tol, double atol, int maxiter, int *iters) {
     Launch L = c->L();
     const int64_t n = c->N;
     const double delta2 = c->ru_delta * c->ru_delta;
     op_K(c, x, c->q, nullptr, nullptr);
     const bool mg = c->mg_nl > 0;
+    const double seg2 = mg ? delta2 : 0.0;
     // with multigrid, every kernel that forms z from r writes the V-cycle's first step
     const float *invD = mg ? c->mg[0].invw : c->invDK32;
-    launch_ru_residual(L, n, b, c->q, c->r32, c->z32, c->p32, invD, tol, atol, maxiter, true);
+    launch_ru_residual(L, n, b, c->q, c->r32, c->z32, c->p32, invD, tol, atol, maxiter, true, seg2);
     launch_fill_zero32(L, n, c->dx32, nullptr);
     int rc = sync_scalars(c);
     if (rc) return rc;
@@ -410,13 +415,14 @@ int pcg_mixed(mm_ctx *c, const double *b, double *x, double tol, double atol, in
         float *res = mg_vcycle(c, c->r32, c->z32, c->mg_xb[0], done, true, c->partsB, &nB, it);
         if (res != c->z32) std::swap(c->z32, c->mg_xb[0]);
     };
-    if (mg) {   // for the initial true residual
+    if (mg && !c->hsc->ir_done) {   // for the initial true residual
         precondition(-1);
         mode = PCG_RESTART_PARTS;
     }
     while (!c->hsc->ir_done) {
         // FP32 iterations until the recursive residual has dropped by ru_delta since the
-        // last true residual (checked between batches) or claims convergence
+        // last true residual (with multigrid tested on the device after every KB, else
+        // between batches) or claims convergence
         const double rmax2 = c->hsc->ir_rmax2;
         const int seg0 = i;
         // multigrid: each batch is sized to reach the segment's target at the measured
@@ -435,7 +441,10 @@ int pcg_mixed(mm_ctx *c, const double *b, double *x, double tol, double atol, in
             for (int k = 0; k < batch; ++k, ++i) {
                 pcg_iteration<float>(c, i, mode, tol, atol, maxiter, c->z32, pb, c->q32, c->dx32, c->r32, invD,
                                      c->K32, c->P32, &nB);
-                if (mg) precondition(i);
+                if (mg) {
+                    launch_pcg_check(L, c->partsB, nB, i, maxiter);
+                    if (k + 1 < batch) precondition(i);
+                }
                 mode = PCG_NORMAL;
             }
             if ((rc = sync_scalars(c))) return rc;
@@ -446,22 +455,24 @@ int pcg_mixed(mm_ctx *c, const double *b, double *x, double tol, double atol, in
                 j_ref = i - 1;
             }
             if (c->hsc->pcg_done || c->hsc->pcg_rr < delta2 * rmax2) break;
+            if (mg) precondition(i - 1);   // the batch's last iteration goes on
         }
         // the stopped iteration index; the next segment continues from it with p kept
         // (the kernels queued after it in the batch did nothing)
         if (c->hsc->pcg_done) i = c->hsc->pcg_iter;
         if (c->timing) {
             const int used = i - seg0;  // event pairs are indexed by the global iteration number
+            // (the segment's last iteration ran no V-cycle)
             if ((rc = collect(c, c->t_spmv, seg0, used)) || (rc = collect(c, c->t_upd, seg0, used)) ||
-                (mg && (rc = collect(c, c->t_mg, seg0, used))))
+                (mg && (rc = collect(c, c->t_mg, seg0, used - 1))))
                 return rc;
         }
         // reliable update: fold dx into x, replace r by the true residual
         launch_add_correction(L, n, x, c->dx32);
         op_K(c, x, c->q, nullptr, nullptr);
-        launch_ru_residual(L, n, b, c->q, c->r32, c->z32, c->p32, invD, tol, atol, maxiter, false);
+        launch_ru_residual(L, n, b, c->q, c->r32, c->z32, c->p32, invD, tol, atol, maxiter, false, seg2);
         if ((rc = sync_scalars(c))) return rc;
-        if (!c->hsc->ir_done && i >= maxiter) break;
+        if (c->hsc->ir_done || i >= maxiter) break;
         mode = PCG_KEEP;
         if (mg) {
             // z = M r for the replaced residual and a restart (p = z).  The multigrid

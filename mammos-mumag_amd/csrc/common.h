@@ -100,6 +100,7 @@ struct FinArgs {
     double *out = nullptr;
     double tol = 0, atol = 0;
     int maxiter = 0;
+    double delta2 = 0;   // FIN_RU_*: > 0 sets the inner stop bound to max(outer, delta2 r.r)
     static FinArgs store(double *out, int nv, unsigned mask, int nparts, const int *skip) {
         FinArgs a;
         a.op = FIN_STORE;
@@ -165,12 +166,18 @@ int launch_ka_sell(const Launch &L, const SellDev<T> &A, const PcgIterArgs &a, c
 template <typename T>
 int launch_kb(const Launch &L, int64_t n, const PcgIterArgs &a, T *x, T *r, T *z, const T *p,
               const T *q, const T *invD);
+// Stop test after KB(i) from its r.r partials (parts slot 1, nparts of them), before a
+// preconditioner that runs after KB: stops after i + 1 iterations when r.r <= the stop
+// bound or i + 1 >= maxiter.
+void launch_pcg_check(const Launch &L, const double *parts, int nparts, int iter, int maxiter);
 // partials (r.z, r.r) of the preconditioned residual, for the next KA; skipped when done.
 template <typename T>
 int launch_dot2(const Launch &L, int64_t n, const T *r, const T *z, double *parts_out);
+// delta2 > 0: the inner iteration stops by itself once its recursive r.r has dropped to
+// delta2 times this true r.r (or to the outer bound)
 void launch_ru_residual(const Launch &L, int64_t n, const double *b, const double *Ax, float *r32,
                         float *z32, float *p32, const float *invD32, double tol, double atol,
-                        int maxiter, bool init);
+                        int maxiter, bool init, double delta2 = 0);
 void launch_add_correction(const Launch &L, int64_t n, double *x, float *dx);
 
 // Element-by-element operators (general meshes), accumulated into zeroed outputs.

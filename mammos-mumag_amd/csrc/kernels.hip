@@ -133,6 +133,7 @@ __global__ __launch_bounds__(kFinThreads) void k_fin(const double *__restrict__ 
             dsc->ir_rr = tot[0];
             dsc->ir_rmax2 = tot[0];
             dsc->ir_done = tot[0] <= dsc->ir_bound;
+            if (a.delta2 > 0) dsc->pcg_bound = fmax(dsc->ir_bound, a.delta2 * tot[0]);
             dsc->ir_outer += 1;
             dsc->ru_gamma = tot[1];
             dsc->pcg_rr = tot[0];
@@ -325,7 +326,7 @@ __global__ __launch_bounds__(kBlock) void k_ru_residual(int64_t n, const double 
 }
 void launch_ru_residual(const Launch &L, int64_t n, const double *b, const double *Ax, float *r32,
                         float *z32, float *p32, const float *invD32, double tol, double atol,
-                        int maxiter, bool init) {
+                        int maxiter, bool init, double delta2) {
     const int nb = grid_for(n, 4);
     const int *skip = init ? nullptr : &L.dsc->ir_done;
     if (init)
@@ -341,6 +342,7 @@ void launch_ru_residual(const Launch &L, int64_t n, const double *b, const doubl
     a.tol = tol;
     a.atol = atol;
     a.maxiter = maxiter;
+    a.delta2 = delta2;
     a.skip = skip;
     launch_fin(L, a);
 }

@@ -410,6 +410,24 @@ __global__ __launch_bounds__(kFB) void k_dot2(int64_t n, const T *__restrict__ r
     fblock_partials<2, kFB>(s, sh, parts_out);
 }
 
+// Stop test between KB(i) and a preconditioner that runs after it (the multigrid
+// V-cycle): r.r from KB's partials (slot 1); when it is at or below the stop bound, or
+// iteration i + 1 would reach maxiter (KA(i + 1)'s own test), the iteration stops after
+// i + 1 iterations, so that the preconditioner and every kernel queued after it are
+// skipped.  One block.
+constexpr int kCheckThreads = 256;
+__global__ __launch_bounds__(kCheckThreads) void k_pcg_check(const double *__restrict__ parts, int nparts,
+                                                            int iter, int maxiter, DevScalars *dsc) {
+    __shared__ double sh[kCheckThreads / kWave];
+    if (dsc->pcg_done) return;
+    const double rr = block_sum_parts<kCheckThreads>(parts + kMaxPartials, nparts, sh);
+    if (threadIdx.x == 0 && !(rr > dsc->pcg_bound && iter + 1 < maxiter)) {
+        dsc->pcg_rr = rr;
+        dsc->pcg_iter = iter + 1;
+        dsc->pcg_done = 1;
+    }
+}
+
 // Init: r = b - A x0, z = r * invD; partials (r.z, r.r, b.b) for KA(0) in FIRST mode.
 // Clears the stop flag left by the previous solve (KA(0) reads it on entry).
 template <typename T>
@@ -507,6 +525,10 @@ int launch_kb(const Launch &L, int64_t n, const PcgIterArgs &a, T *x, T *r, T *z
     else
         launch_timed(L, k_kb<T, false>, dim3(nb), dim3(kFB), n, a, x, r, z, p, q, invD, L.dsc);
     return nb;
+}
+
+void launch_pcg_check(const Launch &L, const double *parts, int nparts, int iter, int maxiter) {
+    hipLaunchKernelGGL(k_pcg_check, dim3(1), dim3(kCheckThreads), 0, L.s, parts, nparts, iter, maxiter, L.dsc);
 }
 
 template <typename T>
