@@ -435,7 +435,7 @@ int pcg_mixed(mm_ctx *c, const double *b, double *x, double tol, double atol, in
             int batch = c->pcg_batch;
             if (mg) {
                 const double need = std::log(target2 / rr_ref) / std::log(c->mg_rate2);
-                batch = (int)std::ceil(std::max(need, 0.0)) + (i == seg0 ? 1 : 0);
+                batch = (int)std::ceil(std::max(need, 0.0));
                 batch = std::min(std::max(batch, 1), c->mg_batch);
             }
             for (int k = 0; k < batch; ++k, ++i) {
@@ -448,11 +448,14 @@ int pcg_mixed(mm_ctx *c, const double *b, double *x, double tol, double atol, in
                 mode = PCG_NORMAL;
             }
             if ((rc = sync_scalars(c))) return rc;
-            if (mg && !c->hsc->pcg_done && i - 1 > j_ref && c->hsc->pcg_rr > 0.0 && rr_ref > 0.0) {
-                const double r2 = std::pow(c->hsc->pcg_rr / rr_ref, 1.0 / (i - 1 - j_ref));
+            // pcg_rr is the r.r entering iteration jr: the last KA's, or at a stop the
+            // stopped index's
+            const int jr = c->hsc->pcg_done ? c->hsc->pcg_iter : i - 1;
+            if (mg && jr > j_ref && c->hsc->pcg_rr > 0.0 && rr_ref > 0.0) {
+                const double r2 = std::pow(c->hsc->pcg_rr / rr_ref, 1.0 / (jr - j_ref));
                 c->mg_rate2 = std::min(std::max(0.5 * c->mg_rate2 + 0.5 * r2, 1e-6), 0.9);
                 rr_ref = c->hsc->pcg_rr;
-                j_ref = i - 1;
+                j_ref = jr;
             }
             if (c->hsc->pcg_done || c->hsc->pcg_rr < delta2 * rmax2) break;
             if (mg) precondition(i - 1);   // the batch's last iteration goes on
