@@ -23,6 +23,12 @@ namespace mm {
 
 namespace {
 
+// The skip flag of launches that have none (never written): the flag is then one global
+// load without a branch (a global, not a flat load, whose wait would also hold the
+// kernel-argument loads), so that its test waits only once the kernel's first loads are
+// issued.
+__device__ int kNoSkip = 0;
+
 // Column tiles of the restriction kernel: kMgTX x kMgTY (x, y) columns per 256-thread block
 // (64 x 4 measured faster than 32 x 8 on the 154^3 grid)
 constexpr int kMgTX = 64, kMgTY = 256 / kMgTX;
@@ -54,12 +60,14 @@ __device__ __forceinline__ T edge_sum_mg(const T *__restrict__ h1p, const T *__r
 // z chunk with a sliding window for the z neighbours; the loads of plane k + 1 are issued
 // before the arithmetic of plane k.  DOT (OP_SMOOTH): partials of b.out and b.b per
 // block into parts[0 / kMaxPartials + block] (the next PCG step's r.z and r.r).
+// The skip flag is tested after the first loads are issued (the coarse levels' kernels
+// are latency-bound: the flag's load then overlaps theirs).
 enum { OP_FIRST = 0, OP_SMOOTH = 1, OP_RESID = 2 };
 template <int OP, bool DOT>
 __global__ __launch_bounds__(256) void k_mg_stencil(PoissonGrid<float> P, MgGrid G, const float *__restrict__ b,
                                                     const float *__restrict__ x, float *__restrict__ out, float w,
                                                     const int *skip, double *parts) {
-    if (skip && *skip) return;   // block-uniform
+    const int sk = *(skip ? skip : &kNoSkip);   // block-uniform
     const int nx = P.nx, ny = P.ny, nz = P.nz;
     const Tile T = mg_tile(G);
     // a strip of 256 consecutive nodes of a plane (G.x strips per plane, G.y = 1)
@@ -70,45 +78,65 @@ __global__ __launch_bounds__(256) void k_mg_stencil(PoissonGrid<float> P, MgGrid
     const bool active = i < nx && j < ny && k0 < k1;
     if (!DOT && !active) return;
     double sbz = 0.0, sbb = 0.0;
+    const int64_t sy = nx, sz = (int64_t)nx * ny;
+    const int64_t base = i + sy * j;
+    const bool edge_ij = i == 0 || j == 0 || i == nx - 1 || j == ny - 1;
+    auto inner = [&](int k) { return !(edge_ij || k == 0 || k == nz - 1); };
+    // spacings of the column; loads of plane k (and x of plane k + 1), issued one plane
+    // ahead of their use
+    float hxl = 0.f, hxu = 0.f, hyl = 0.f, hyu = 0.f, ixm = 0.f, ixq = 0.f, iym = 0.f, iyq = 0.f;
+    float bk = 0.f, xw = 0.f, xe = 0.f, xs = 0.f, xn = 0.f, xm = 0.f, xc = 0.f, xp = 0.f;
+    auto load_plane = [&](int k, float &bv, float &w_, float &e_, float &s_, float &n_) {
+        const int64_t idx = base + sz * k;
+        const bool in = inner(k);
+        bv = (in || DOT) ? b[idx] : 0.f;
+        if (OP != OP_FIRST && in) {
+            w_ = x[idx - 1];
+            e_ = x[idx + 1];
+            s_ = x[idx - sy];
+            n_ = x[idx + sy];
+        }
+    };
     if (active) {
-        const int64_t sy = nx, sz = (int64_t)nx * ny;
-        const int64_t base = i + sy * j;
-        const bool edge_ij = i == 0 || j == 0 || i == nx - 1 || j == ny - 1;
+        // straight-line loads (clamped to valid indices, the unused values zeroed after),
+        // so that they are all in flight together: the spacing arrays hold n + 1 entries
+        // (pads at 0 and n), so i + 1 and j + 1 are valid; the column's factors are unused
+        // on an edge column
+        hxl = P.hxp[i];
+        hxu = P.hxp[i + 1];
+        hyl = P.hyp[j];
+        hyu = P.hyp[j + 1];
+        ixm = P.ixp[i];
+        ixq = P.ixp[i + 1];
+        iym = P.iyp[j];
+        iyq = P.iyp[j + 1];
+        const int64_t idx = base + sz * k0;
+        const bool in = inner(k0);
+        bk = b[idx];
+        if (!(in || DOT)) bk = 0.f;
+        if (OP != OP_FIRST) {
+            const int64_t dx1 = in ? 1 : 0, dy1 = in ? sy : 0;   // in-bounds neighbours
+            xw = x[idx - dx1];
+            xe = x[idx + dx1];
+            xs = x[idx - dy1];
+            xn = x[idx + dy1];
+            xm = x[k0 > 0 ? idx - sz : idx];
+            xc = x[idx];
+            xp = x[k0 + 1 < nz ? idx + sz : idx];
+            if (!in) xw = xe = xs = xn = 0.f;
+            if (k0 == 0) xm = 0.f;
+            if (k0 + 1 >= nz) xp = 0.f;
+        }
+    }
+    // the loads above complete before the flag's test (rather than being moved after it)
+    asm volatile("" : "+v"(hxl), "+v"(hxu), "+v"(hyl), "+v"(hyu), "+v"(ixm), "+v"(ixq), "+v"(iym), "+v"(iyq));
+    asm volatile("" : "+v"(xm), "+v"(xc), "+v"(xp), "+v"(bk), "+v"(xw), "+v"(xe), "+v"(xs), "+v"(xn));
+    if (sk) return;
+    if (active) {
         // per-column factors: edge sums regrouped per axis, e.g.
         // edge_sum(hz, hx; k, i) = hzl (2 hxl + hxu) + hzu (hxl + 2 hxu)
-        float hyl = 0.f, hyu = 0.f, cx = 0.f, dx = 0.f, szz = 0.f, ixm = 0.f, ixq = 0.f, iym = 0.f, iyq = 0.f;
-        if (!edge_ij) {
-            const float hxl = P.hxp[i], hxu = P.hxp[i + 1];
-            hyl = P.hyp[j];
-            hyu = P.hyp[j + 1];
-            cx = 2.f * hxl + hxu;
-            dx = hxl + 2.f * hxu;
-            szz = hxl * (2.f * hyl + hyu) + hxu * (hyl + 2.f * hyu);
-            ixm = P.ixp[i];
-            ixq = P.ixp[i + 1];
-            iym = P.iyp[j];
-            iyq = P.iyp[j + 1];
-        }
-        auto inner = [&](int k) { return !(edge_ij || k == 0 || k == nz - 1); };
-        // loads of plane k (and x of plane k + 1), issued one plane ahead of their use
-        float bk = 0.f, xw = 0.f, xe = 0.f, xs = 0.f, xn = 0.f, xm = 0.f, xc = 0.f, xp = 0.f;
-        auto load_plane = [&](int k, float &bv, float &w_, float &e_, float &s_, float &n_) {
-            const int64_t idx = base + sz * k;
-            const bool in = inner(k);
-            bv = (in || DOT) ? b[idx] : 0.f;
-            if (OP != OP_FIRST && in) {
-                w_ = x[idx - 1];
-                e_ = x[idx + 1];
-                s_ = x[idx - sy];
-                n_ = x[idx + sy];
-            }
-        };
-        if (OP != OP_FIRST) {
-            xm = k0 > 0 ? x[base + sz * (k0 - 1)] : 0.f;
-            xc = x[base + sz * k0];
-            xp = k0 + 1 < nz ? x[base + sz * (k0 + 1)] : 0.f;
-        }
-        load_plane(k0, bk, xw, xe, xs, xn);
+        const float cx = 2.f * hxl + hxu, dx = hxl + 2.f * hxu;
+        const float szz = hxl * (2.f * hyl + hyu) + hxu * (hyl + 2.f * hyu);
         for (int k = k0; k < k1; ++k) {
             const int64_t idx = base + sz * k;
             float bn = 0.f, wn = 0.f, en = 0.f, sn = 0.f, nn = 0.f, xpp = 0.f;
@@ -187,7 +215,7 @@ __global__ __launch_bounds__(256) void k_mg_restrict(MgGrid G, int nxc, int nyc,
                                                      const float *__restrict__ rf, float *__restrict__ bc,
                                                      const float *__restrict__ invwc, float *__restrict__ x1c,
                                                      const int *skip) {
-    if (skip && *skip) return;
+    const int sk = *(skip ? skip : &kNoSkip);   // tested once the transfer lists' loads are issued
     const Tile T = mg_tile(G);
     const int I = T.bx * kMgTX + (threadIdx.x % kMgTX);
     const int J = T.by * kMgTY + (threadIdx.x / kMgTX);
@@ -205,6 +233,10 @@ __global__ __launch_bounds__(256) void k_mg_restrict(MgGrid G, int nxc, int nyc,
         yo[q] = nxf * ryi[3 * J + q];
         yw[q] = ryw[3 * J + q];
     }
+    // the list loads complete before the flag's test (rather than being moved after it)
+    asm volatile("" : "+v"(xi[0]), "+v"(xi[1]), "+v"(xi[2]), "+v"(xw[0]), "+v"(xw[1]), "+v"(xw[2]));
+    asm volatile("" : "+v"(yo[0]), "+v"(yo[1]), "+v"(yo[2]), "+v"(yw[0]), "+v"(yw[1]), "+v"(yw[2]));
+    if (sk) return;
     for (int K = k0; K < k1; ++K) {
         const int64_t idx = I + (int64_t)nxc * J + sc * K;
         float s = 0.f;
@@ -239,7 +271,7 @@ __global__ __launch_bounds__(256) void k_mg_prolong(MgGrid G, int nxf, int nyf, 
                                                     const int *__restrict__ pzc, const float *__restrict__ pzt,
                                                     const float *__restrict__ xc, float *__restrict__ xf,
                                                     const int *skip) {
-    if (skip && *skip) return;
+    const int sk = *(skip ? skip : &kNoSkip);   // tested once the transfer lists' loads are issued
     const Tile T = mg_tile(G);
     const int e = T.bx * 256 + threadIdx.x;   // strips of 256 consecutive plane nodes
     const int j = e / nxf, i = e - j * nxf;
@@ -247,8 +279,11 @@ __global__ __launch_bounds__(256) void k_mg_prolong(MgGrid G, int nxf, int nyf, 
     const int k0 = max(1, T.bz * chunk), k1 = min(nzf - 1, T.bz * chunk + chunk);
     if (i <= 0 || j <= 0 || i >= nxf - 1 || j >= nyf - 1) return;
     const int64_t sf = (int64_t)nxf * nyf, sc = (int64_t)nxc * nyc;
-    const float tx = pxt[i], ty = pyt[j];
-    const int64_t oxy = pxc[i] + (int64_t)nxc * pyc[j];
+    float tx = pxt[i], ty = pyt[j];
+    int cx = pxc[i], cy = pyc[j];
+    asm volatile("" : "+v"(tx), "+v"(ty), "+v"(cx), "+v"(cy));   // loaded before the flag's test
+    if (sk) return;
+    const int64_t oxy = cx + (int64_t)nxc * cy;
     for (int k = k0; k < k1; ++k) {
         const float tz = pzt[k];
         const float *c0 = xc + oxy + sc * pzc[k];

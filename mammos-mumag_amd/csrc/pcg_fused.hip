@@ -377,8 +377,9 @@ __global__ __launch_bounds__(kFB) void k_kb(int64_t n, PcgIterArgs a, T *__restr
                                             const T *__restrict__ p, const T *__restrict__ q,
                                             const T *__restrict__ invD, DevScalars *dsc) {
     __shared__ double sh[kFB / kWave];
-    if (dsc->pcg_done) return;
+    const int stop = dsc->pcg_done;   // tested after the partials' loads (block-uniform)
     const double pq = block_sum_parts<kFB>(a.parts_in, a.nparts_in, sh);
+    if (stop) return;
     const T alpha = (T)(dsc->pcg_gam[a.iter & 1] / pq);
     double s[2] = {0.0, 0.0};
     for (int64_t i = blockIdx.x * (int64_t)kFB + threadIdx.x; i < n; i += (int64_t)gridDim.x * kFB) {
@@ -419,8 +420,9 @@ constexpr int kCheckThreads = 256;
 __global__ __launch_bounds__(kCheckThreads) void k_pcg_check(const double *__restrict__ parts, int nparts,
                                                             int iter, int maxiter, DevScalars *dsc) {
     __shared__ double sh[kCheckThreads / kWave];
-    if (dsc->pcg_done) return;
+    const int stop = dsc->pcg_done;   // tested after the partials' loads
     const double rr = block_sum_parts<kCheckThreads>(parts + kMaxPartials, nparts, sh);
+    if (stop) return;
     if (threadIdx.x == 0 && !(rr > dsc->pcg_bound && iter + 1 < maxiter)) {
         dsc->pcg_rr = rr;
         dsc->pcg_iter = iter + 1;
