@@ -398,12 +398,21 @@ int pcg_mixed(mm_ctx *c, const double *b, double *x, double tol, double atol, in
     Launch L = c->L();
     const int64_t n = c->N;
     const double delta2 = c->ru_delta * c->ru_delta;
-    op_K(c, x, c->q, nullptr, nullptr);
     const bool mg = c->mg_nl > 0;
     const double seg2 = mg ? delta2 : 0.0;
     // with multigrid, every kernel that forms z from r writes the V-cycle's first step
     const float *invD = mg ? c->mg[0].invw : c->invDK32;
-    launch_ru_residual(L, n, b, c->q, c->r32, c->z32, c->p32, invD, tol, atol, maxiter, true, seg2);
+    // r = b - K x in FP64 (one pass on the structured grid), its FP32 copy and z
+    const bool fused = c->structured && grid_ru_residual_fits(c->P);
+    auto true_residual = [&](bool init) {
+        if (fused) {
+            launch_grid_ru_residual(L, c->P, x, b, c->r32, c->z32, c->p32, invD, tol, atol, maxiter, init, seg2);
+        } else {
+            op_K(c, x, c->q, nullptr, nullptr);
+            launch_ru_residual(L, n, b, c->q, c->r32, c->z32, c->p32, invD, tol, atol, maxiter, init, seg2);
+        }
+    };
+    true_residual(true);
     launch_fill_zero32(L, n, c->dx32, nullptr);
     int rc = sync_scalars(c);
     if (rc) return rc;
@@ -472,8 +481,7 @@ int pcg_mixed(mm_ctx *c, const double *b, double *x, double tol, double atol, in
         }
         // reliable update: fold dx into x, replace r by the true residual
         launch_add_correction(L, n, x, c->dx32);
-        op_K(c, x, c->q, nullptr, nullptr);
-        launch_ru_residual(L, n, b, c->q, c->r32, c->z32, c->p32, invD, tol, atol, maxiter, false, seg2);
+        true_residual(false);
         if ((rc = sync_scalars(c))) return rc;
         if (c->hsc->ir_done || i >= maxiter) break;
         mode = PCG_KEEP;

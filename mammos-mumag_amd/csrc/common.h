@@ -178,6 +178,12 @@ int launch_dot2(const Launch &L, int64_t n, const T *r, const T *z, double *part
 void launch_ru_residual(const Launch &L, int64_t n, const double *b, const double *Ax, float *r32,
                         float *z32, float *p32, const float *invD32, double tol, double atol,
                         int maxiter, bool init, double delta2 = 0);
+// The same from x on the structured grid, K x formed in the pass (as launch_grid_poisson
+// forms it); grid_ru_residual_fits: its blocks fit the partial slots
+bool grid_ru_residual_fits(const PoissonGrid<double> &P);
+void launch_grid_ru_residual(const Launch &L, const PoissonGrid<double> &P, const double *x, const double *b,
+                             float *r32, float *z32, float *p32, const float *invD32, double tol, double atol,
+                             int maxiter, bool init, double delta2 = 0);
 void launch_add_correction(const Launch &L, int64_t n, double *x, float *dx);
 
 // Element-by-element operators (general meshes), accumulated into zeroed outputs.

@@ -324,6 +324,21 @@ __global__ __launch_bounds__(kBlock) void k_ru_residual(int64_t n, const double 
     }
     block_partials<3>(s, 0u, partials);
 }
+// The finalize of a true residual from nb blocks' partials (r.r FP64, r32.z32, b.b).
+static void launch_ru_fin(const Launch &L, int nb, double tol, double atol, int maxiter, bool init,
+                          double delta2) {
+    const int *skip = init ? nullptr : &L.dsc->ir_done;
+    FinArgs a{};
+    a.op = init ? FIN_RU_INIT : FIN_RU_REPLACE;
+    a.nv = 3;
+    a.nparts = nb;
+    a.tol = tol;
+    a.atol = atol;
+    a.maxiter = maxiter;
+    a.delta2 = delta2;
+    a.skip = skip;
+    launch_fin(L, a);
+}
 void launch_ru_residual(const Launch &L, int64_t n, const double *b, const double *Ax, float *r32,
                         float *z32, float *p32, const float *invD32, double tol, double atol,
                         int maxiter, bool init, double delta2) {
@@ -335,16 +350,86 @@ void launch_ru_residual(const Launch &L, int64_t n, const double *b, const doubl
     else
         hipLaunchKernelGGL(k_ru_residual<false>, dim3(nb), dim3(kBlock), 0, L.s, n, b, Ax, r32, z32, p32,
                            invD32, skip, L.red.partials);
-    FinArgs a{};
-    a.op = init ? FIN_RU_INIT : FIN_RU_REPLACE;
-    a.nv = 3;
-    a.nparts = nb;
-    a.tol = tol;
-    a.atol = atol;
-    a.maxiter = maxiter;
-    a.delta2 = delta2;
-    a.skip = skip;
-    launch_fin(L, a);
+    launch_ru_fin(L, nb, tol, atol, maxiter, init, delta2);
+}
+
+// The true residual on the structured grid in one pass: K x formed as k_grid_poisson forms
+// it (same tiles, same arithmetic), r = b - K x, r32, z32 = r32 invD32 (INIT: p32 = z32)
+// and the partials (r.r FP64, r32.z32, b.b) of k_ru_residual; skipped when *skip.
+template <bool INIT>
+__global__ __launch_bounds__(kBlock) void k_grid_ru_residual(PoissonGrid<double> P, const double *__restrict__ x,
+                                                             const double *__restrict__ b,
+                                                             float *__restrict__ r32, float *__restrict__ z32,
+                                                             float *__restrict__ p32,
+                                                             const float *__restrict__ invD32, const int *skip,
+                                                             double *partials) {
+    if (skip && *skip) return;
+    const int nx = P.nx, ny = P.ny, nz = P.nz;
+    const int i = blockIdx.x * 64 + (threadIdx.x & 63);
+    const int j = blockIdx.y * 4 + (threadIdx.x >> 6);
+    const int chunk = (nz + gridDim.z - 1) / gridDim.z;
+    const int k0 = blockIdx.z * chunk, k1 = min(nz, k0 + chunk);
+    const int64_t sy = nx, sz = (int64_t)nx * ny;
+    double s[3] = {0.0, 0.0, 0.0};
+    if (i < nx && j < ny && k0 < k1) {
+        const bool edge_ij = i == 0 || j == 0 || i == nx - 1 || j == ny - 1;
+        const double sz_ij = edge_sum(P.hxp, P.hyp, i, j);
+        const double mxm = (i - 1 > 0) ? 1.0 : 0.0, mxp = (i + 1 < nx - 1) ? 1.0 : 0.0;
+        const double mym = (j - 1 > 0) ? 1.0 : 0.0, myp = (j + 1 < ny - 1) ? 1.0 : 0.0;
+        const double ixm = P.ixp[i], ixp = P.ixp[i + 1], iym = P.iyp[j], iyp = P.iyp[j + 1];
+        const int64_t base = i + sy * j;
+        double xm = k0 > 0 ? x[base + sz * (k0 - 1)] : 0.0;
+        double xc = x[base + sz * k0];
+        for (int k = k0; k < k1; ++k) {
+            const int64_t idx = base + sz * k;
+            const double xp = k + 1 < nz ? x[idx + sz] : 0.0;
+            double out;
+            if (edge_ij || k == 0 || k == nz - 1) {
+                out = xc;  // Dirichlet row
+            } else {
+                const double xw = x[idx - 1], xe = x[idx + 1], xs = x[idx - sy], xn = x[idx + sy];
+                const double sx = edge_sum(P.hyp, P.hzp, j, k);
+                const double syy = edge_sum(P.hzp, P.hxp, k, i);
+                const double cxm = -sx * ixm, cxp = -sx * ixp;
+                const double cym = -syy * iym, cyp = -syy * iyp;
+                const double czm = -sz_ij * P.izp[k], czp = -sz_ij * P.izp[k + 1];
+                const double diag = -(cxm + cxp + cym + cyp + czm + czp);
+                const double mzm = (k - 1 > 0) ? 1.0 : 0.0, mzp = (k + 1 < nz - 1) ? 1.0 : 0.0;
+                out = diag * xc + mxm * cxm * xw + mxp * cxp * xe + mym * cym * xs + myp * cyp * xn +
+                      mzm * czm * xm + mzp * czp * xp;
+            }
+            const double bi = b[idx];
+            const double ri = bi - out;
+            const float rf = (float)ri;
+            const float zf = rf * invD32[idx];
+            r32[idx] = rf;
+            z32[idx] = zf;
+            if (INIT) p32[idx] = zf;
+            s[0] += ri * ri;
+            s[1] += (double)rf * (double)zf;
+            s[2] += bi * bi;
+            xm = xc;
+            xc = xp;
+        }
+    }
+    block_partials<3>(s, 0u, partials);
+}
+
+bool grid_ru_residual_fits(const PoissonGrid<double> &P) {
+    return nblocks<double>(poisson_grid(P)) <= kMaxPartials;
+}
+void launch_grid_ru_residual(const Launch &L, const PoissonGrid<double> &P, const double *x, const double *b,
+                             float *r32, float *z32, float *p32, const float *invD32, double tol, double atol,
+                             int maxiter, bool init, double delta2) {
+    const dim3 g = poisson_grid(P);
+    const int *skip = init ? nullptr : &L.dsc->ir_done;
+    if (init)
+        hipLaunchKernelGGL(k_grid_ru_residual<true>, g, dim3(kBlock), 0, L.s, P, x, b, r32, z32, p32, invD32,
+                           skip, L.red.partials);
+    else
+        hipLaunchKernelGGL(k_grid_ru_residual<false>, g, dim3(kBlock), 0, L.s, P, x, b, r32, z32, p32, invD32,
+                           skip, L.red.partials);
+    launch_ru_fin(L, nblocks<double>(g), tol, atol, maxiter, init, delta2);
 }
 
 // x += (double) dx; dx = 0.  Skipped once the true residual has converged.
