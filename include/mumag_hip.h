@@ -233,8 +233,9 @@ int mm_warm_start_stats(mm_ctx *ctx, int64_t *calls_host, double *last_gamma_hos
 /* When the network is applied: MM_WS_EVERY (default) before every potential solve;
  * MM_WS_MEASURED per solve in the form, warm-started or plain, with the lower mean wall
  * time (warm start plus PCG, timed on the host) over the solves that used it; the first
- * 3 solves of each form alternate and every 16th solve takes the other form.  The
- * answer is the plain method's either way. */
+ * 3 solves of each form alternate, then the other form is re-measured on solve 15 and at
+ * an interval that doubles (16 .. 256 solves) while it stays slower, back to 16 when it
+ * wins.  The answer is the plain method's either way. */
 #define MM_WS_EVERY 0
 #define MM_WS_MEASURED 1
 int mm_warm_start_policy(mm_ctx *ctx, int32_t policy);

@@ -116,6 +116,8 @@ struct mm_ctx {
     // MM_WS_MEASURED: host-timed solves with and without the warm start (running means)
     int ws_policy = 0;
     int64_t ws_solves = 0, ws_n[2] = {0, 0};
+    int64_t ws_every = 16, ws_next = 15;   // re-measurement interval and the next one
+    bool ws_probe = false;                 // the current solve re-measures the other form
     double ws_mean[2] = {0.0, 0.0};
 
     Launch L() const {
@@ -534,19 +536,30 @@ void learned_warm_start(mm_ctx *c, const double *m, const double *b, double *u) 
 // MM_WS_EVERY: every solve is warm-started.  MM_WS_MEASURED: each solve takes the form
 // (warm-started or plain) with the lower mean wall time of the solves that used it, the
 // warm start plus the PCG, measured on the host; the first 3 solves of each form
-// alternate, and every 16th solve takes the other form so that both means stay current.
+// alternate; then the other form is re-measured on solve 15 and after that at an
+// interval that doubles (16, 32, ..., 256 solves) while it stays the slower one and
+// returns to 16 when it wins.
+constexpr int64_t kWsEvery = 16, kWsEveryMax = 256;
 bool ws_choose(mm_ctx *c) {
+    c->ws_probe = false;
     if (c->ws_policy != MM_WS_MEASURED) return true;
     const int64_t k = c->ws_solves;
     if (c->ws_n[0] < 3 || c->ws_n[1] < 3) return (k & 1) == 0;
     const bool best = c->ws_mean[1] < c->ws_mean[0];
-    return (k % 16 == 15) ? !best : best;
+    c->ws_probe = k >= c->ws_next;
+    return c->ws_probe ? !best : best;
 }
 void ws_record(mm_ctx *c, bool ws, double seconds) {
     const int a = ws ? 1 : 0;
     c->ws_n[a]++;
     const double w = c->ws_n[a] < 8 ? 1.0 / c->ws_n[a] : 0.125;   // mean, then a running mean
     c->ws_mean[a] += w * (seconds - c->ws_mean[a]);
+    if (c->ws_probe) {
+        const bool still_slower = ws ? c->ws_mean[1] >= c->ws_mean[0] : c->ws_mean[0] > c->ws_mean[1];
+        c->ws_every = still_slower ? std::min(2 * c->ws_every, kWsEveryMax) : kWsEvery;
+        c->ws_next = c->ws_solves + c->ws_every;
+        c->ws_probe = false;
+    }
     c->ws_solves++;
 }
 
@@ -563,9 +576,11 @@ int potential_solve(mm_ctx *c, const double *m, double *u, double tol, int *it) 
     return rc;
 }
 
-// E, g_proj, g_full for magnetisation m (energies.py:67-99).
+// E, g_proj, g_full for magnetisation m (energies.py:67-99).  off_zero: on a structured
+// grid, m is zero off the magnet block and gproj, gfull already hold zero there.
 int total_eg(mm_ctx *c, const double *m, double hext, const double *hdir, double tol_u,
-             int hmag_on, double *u, double *E, double *gproj, double *gfull, int *pcg_it) {
+             int hmag_on, double *u, double *E, double *gproj, double *gfull, int *pcg_it,
+             bool off_zero = false) {
     Launch L = c->L();
     op_C(c, m, c->gC, hmag_on > 0 ? c->b : nullptr, nullptr);
     int it = 0, rc = 0;
@@ -577,7 +592,7 @@ int total_eg(mm_ctx *c, const double *m, double hext, const double *hdir, double
     const double zc0 = -hext * hdir[0] / c->V, zc1 = -hext * hdir[1] / c->V,
                  zc2 = -hext * hdir[2] / c->V;
     launch_finish_eg(L, c->N, c->structured ? &c->G : nullptr, m, c->meas, zc0, zc1, zc2, c->gC, c->gd, gfull,
-                     gproj, &c->dsc->energy, hmag_on > 0);
+                     gproj, &c->dsc->energy, hmag_on > 0, !(off_zero && c->structured));
     int rc2 = sync_scalars(c);
     if (rc2) return rc2;
     *E = c->hsc->energy;
@@ -586,15 +601,17 @@ int total_eg(mm_ctx *c, const double *m, double hext, const double *hdir, double
 }
 
 // M_inv (loop.py:40-55): truncated CG on H with grad_fk = -g.  No host sync.
+// zout_off_zero: on a structured grid, g is zero off the magnet block and zout already
+// holds zero there.
 int precond(mm_ctx *c, const double *m_ref, const double *gF, const double *g, int max_iter,
-            double *zout) {
+            double *zout, bool zout_off_zero = false) {
     Launch L = c->L();
     const int64_t n = c->N, n3 = 3 * c->N;
     if (c->structured) {   // on the magnet block only (FP32 vectors in the mixed form)
         if (c->precision == MM_PREC_MIXED)
-            tcgc_run<float>(L, c->G, n, c->tb32, m_ref, gF, g, c->Dex, max_iter, zout);
+            tcgc_run<float>(L, c->G, n, c->tb32, m_ref, gF, g, c->Dex, max_iter, zout, !zout_off_zero);
         else
-            tcgc_run<double>(L, c->G, n, c->tb64, m_ref, gF, g, c->Dex, max_iter, zout);
+            tcgc_run<double>(L, c->G, n, c->tb64, m_ref, gF, g, c->Dex, max_iter, zout, !zout_off_zero);
         return 0;
     }
     launch_tcg_init(L, n3, g, c->Dex, c->tp, c->tr, c->tz, c->td);
@@ -1129,10 +1146,24 @@ int mm_minimize(mm_ctx *c, double *m_inout, double hext, const double hdir[3],
     if ((rc = sync_scalars(c))) return rc;
     c->ncg_block = c->structured && c->hsc->red[0] == 0.0;
     const long long tcg0 = c->hsc->tcg_total;
+    // When m is zero off the magnet block (ncg_block), so are every iterate, gradient and
+    // direction of this call: each buffer's entries off the block are written (zero) the
+    // first time this call writes it, and after that only its block entries.
+    const double *zeroed[8] = {};
+    int nzeroed = 0;
+    auto off_zero = [&](const double *p) {
+        if (!c->ncg_block) return false;
+        for (int q = 0; q < nzeroed; ++q)
+            if (zeroed[q] == p) return true;
+        if (nzeroed < 8) zeroed[nzeroed++] = p;   // written in full by the launch that asked
+        return false;
+    };
+    auto blk = [&](const double *out) { return off_zero(out) ? &c->G : nullptr; };
 
     auto feval = [&](const double *x, double *E, double *g) -> int {
         int it = 0;
-        int r = total_eg(c, x, hext, hdir, P->tol_u, P->hmag_on, u, E, g, gF, &it);
+        const bool oz_g = off_zero(g), oz_f = off_zero(gF);
+        int r = total_eg(c, x, hext, hdir, P->tol_u, P->hmag_on, u, E, g, gF, &it, oz_g && oz_f);
         loc.fcalls++;
         loc.pcg_iters += it;
         loc.pcg_calls += P->hmag_on > 0;
@@ -1141,16 +1172,18 @@ int mm_minimize(mm_ctx *c, double *m_inout, double hext, const double hdir[3],
     };
     auto minv = [&](const double *g, double *z) -> int {
         loc.c_applies += P->precond_iter;
-        return precond(c, m_ref, gF, g, P->precond_iter, z);
+        return precond(c, m_ref, gF, g, P->precond_iter, z, off_zero(z));
     };
 
     const double FMAX = DBL_MAX;
     double f0 = FMAX, f = 0.0;
     if ((rc = feval(X, &f, G))) return rc;
     if ((rc = minv(G, Z1))) return rc;
-    launch_axpby(L, n3, -1.0, Z1, 0.0, nullptr, D);  // d = -M_inv(g)
+    launch_axpby(L, n3, -1.0, Z1, 0.0, nullptr, D, blk(D));  // d = -M_inv(g)
     int64_t k = 1;
     HIPC(hipMemcpyAsync(X0, X, bytes3, hipMemcpyDeviceToDevice, c->s));
+    off_zero(X);    // copies of m in full
+    off_zero(X0);
     const double sqrt_tol = std::sqrt(tol), cbrt_tol = std::cbrt(tol);
     const double sqrt_eps = std::sqrt(DBL_EPSILON);
     for (;;) {
@@ -1163,7 +1196,8 @@ int mm_minimize(mm_ctx *c, double *m_inout, double hext, const double hdir[3],
         // restart if d is not a sufficient descent direction (minimizers.py:171-173)
         double dg3[3];
         if ((rc = reduce3(c, RED_DG3, D, G, nullptr, nullptr, dg3, 3))) return rc;
-        if (dg3[0] > -0.001 * std::sqrt(dg3[1]) * std::sqrt(dg3[2])) launch_axpby(L, n3, -1.0, G, 0.0, nullptr, D);
+        if (dg3[0] > -0.001 * std::sqrt(dg3[1]) * std::sqrt(dg3[2]))
+            launch_axpby(L, n3, -1.0, G, 0.0, nullptr, D, blk(D));
         // ---- line search (minimizers.py:92-137)
         double f1 = 0.0;
         {
@@ -1174,7 +1208,7 @@ int mm_minimize(mm_ctx *c, double *m_inout, double hext, const double hdir[3],
             const double t1 = 2.0 * sqrt_eps * (1.0 + std::sqrt(h3[0])) / std::sqrt(h3[1]);
             const double t2 = sqrt_eps / h3[2];
             const double h = t1 < t2 ? t1 : t2;
-            launch_update_m(L, n, X, h, D, X1);
+            launch_update_m(L, n, X, h, D, X1, blk(X1));
             if ((rc = feval(X1, &f1, G1))) return rc;
             double gd2[2];
             if ((rc = reduce3(c, RED_GD2, G, G1, D, nullptr, gd2, 2))) return rc;
@@ -1187,7 +1221,7 @@ int mm_minimize(mm_ctx *c, double *m_inout, double hext, const double hdir[3],
                 if (term2 < 0) term2 = 1.0e30;
                 alpha = term1 < 0.99 * term2 ? term1 : term2;
             }
-            launch_update_m(L, n, X, alpha, D, X1);
+            launch_update_m(L, n, X, alpha, D, X1, blk(X1));
             if ((rc = feval(X1, &f1, G1))) return rc;
             double gd1;
             if ((rc = reduce3(c, RED_DOT, G1, D, nullptr, nullptr, &gd1, 1))) return rc;
@@ -1198,7 +1232,7 @@ int mm_minimize(mm_ctx *c, double *m_inout, double hext, const double hdir[3],
                 const bool done = wolfe || (f1 <= f + cA * alpha * gd) || (i > max_ls);
                 if (done) break;
                 alpha *= rho;
-                launch_update_m(L, n, X, alpha, D, X1);
+                launch_update_m(L, n, X, alpha, D, X1, blk(X1));
                 if ((rc = feval(X1, &f1, G1))) return rc;
                 if ((rc = reduce3(c, RED_DOT, G1, D, nullptr, nullptr, &gd1, 1))) return rc;
                 ++i;
@@ -1210,9 +1244,10 @@ int mm_minimize(mm_ctx *c, double *m_inout, double hext, const double hdir[3],
         double yzd[2];
         if ((rc = reduce3(c, RED_YZD, G, G1, Z1, D, yzd, 2))) return rc;
         const double beta = std::max(yzd[0] / yzd[1], 0.0);
-        launch_axpby(L, n3, -1.0, Z1, beta, D, D);
+        launch_axpby(L, n3, -1.0, Z1, beta, D, D, blk(D));
         if ((rc = reduce3(c, RED_DG3, D, G, nullptr, nullptr, dg3, 3))) return rc;
-        if (dg3[0] > -0.001 * std::sqrt(dg3[1]) * std::sqrt(dg3[2])) launch_axpby(L, n3, -1.0, Z1, 0.0, nullptr, D);
+        if (dg3[0] > -0.001 * std::sqrt(dg3[1]) * std::sqrt(dg3[2]))
+            launch_axpby(L, n3, -1.0, Z1, 0.0, nullptr, D, blk(D));
         // shift state: x0 <- x, x <- x1, f0 <- f, f <- f1, g <- g1
         std::swap(X0, X);
         std::swap(X, X1);
@@ -1582,6 +1617,9 @@ int mm_warm_start_policy(mm_ctx *c, int32_t policy) {
     if (policy != MM_WS_EVERY && policy != MM_WS_MEASURED) return fail(MM_E_ARG, "unknown warm-start policy");
     c->ws_policy = policy;
     c->ws_solves = c->ws_n[0] = c->ws_n[1] = 0;
+    c->ws_every = kWsEvery;
+    c->ws_next = kWsEvery - 1;
+    c->ws_probe = false;
     c->ws_mean[0] = c->ws_mean[1] = 0.0;
     return 0;
 }

@@ -197,10 +197,11 @@ void launch_grid_exani(const Launch &L, const GridDev &G, bool fp32, const doubl
 void launch_grid_grad(const Launch &L, const GridDev &G, bool fp32, const double *u, double *g);
 
 // Node kernels.
-// G: the structured grid (nodes off its magnet block are zero on input), or null
+// G: the structured grid (nodes off its magnet block are zero on input), or null;
+// write_off = false: g_full and g_proj already hold zero off the block and are left there
 void launch_finish_eg(const Launch &L, int64_t n, const GridDev *G, const double *m, const double *meas,
                       double zc0, double zc1, double zc2, const double *gC, const double *gd, double *g_full,
-                      double *g_proj, double *E_out, int with_hmag);
+                      double *g_proj, double *E_out, int with_hmag, bool write_off = true);
 void launch_external(const Launch &L, int64_t n, const double *m, const double *meas, double zc0,
                      double zc1, double zc2, double *g, double *E_out);
 void launch_hvec(const Launch &L, int64_t n, const double *m0, const double *gF, const double *Cd,
@@ -218,13 +219,19 @@ struct TcgBlock {
     T *m0 = nullptr, *mg = nullptr, *D = nullptr, *p = nullptr, *r = nullptr, *z = nullptr, *Hd = nullptr;
     T *d[2] = {nullptr, nullptr};
 };
+// write_off = false: zout already holds zero off the block and is written on it only
 template <typename T>
 void tcgc_run(const Launch &L, const GridDev &G, int64_t n, const TcgBlock<T> &B, const double *m0,
-              const double *gF, const double *g, const double *D, int max_iter, double *zout);
+              const double *gF, const double *g, const double *D, int max_iter, double *zout,
+              bool write_off = true);
+// G (optional): a structured grid whose x and d are zero off the magnet block and whose
+// out already holds zero there: the block's nodes only
 void launch_update_m(const Launch &L, int64_t n, const double *x, double alpha, const double *d,
-                     double *out);
+                     double *out, const GridDev *G = nullptr);
+// G3 (optional): n = 3 N interleaved 3-vectors, x and y zero off the magnet block of G3 and
+// out already zero there: the block's nodes only
 void launch_axpby(const Launch &L, int64_t n, double a, const double *x, double b, const double *y,
-                  double *out);
+                  double *out, const GridDev *G3 = nullptr);
 void launch_fill_zero(const Launch &L, int64_t n, double *x, const int *skip_flag);
 void launch_fill_zero32(const Launch &L, int64_t n, float *x, const int *skip_flag);
 

@@ -566,6 +566,12 @@ void launch_ebe_grad(const Launch &L, const ElemDev &E, const double *u, double 
 __device__ __forceinline__ int64_t gnode(const GridDev &G, int i, int j, int k) {
     return (int64_t)(G.m0x + i) + (int64_t)G.nx * ((G.m0y + j) + (int64_t)G.ny * (G.m0z + k));
 }
+// Grid node of magnet-block node t = i + n1 (j + n1 k), n1 = mn + 1 (n1^3 < 2^32)
+__device__ __forceinline__ int64_t block_gnode(const GridDev &G, uint32_t t) {
+    const uint32_t n1 = (uint32_t)G.mn + 1u;
+    const uint32_t tj = t / n1, k = tj / n1;
+    return gnode(G, (int)(t - tj * n1), (int)(tj - k * n1), (int)k);
+}
 
 // Material coefficients of one voxel as the node kernels use them
 template <typename TC>
@@ -831,7 +837,7 @@ __global__ __launch_bounds__(kBlock) void k_finish_eg(int64_t n, GridDev G, cons
                                                       const double *__restrict__ gd,
                                                       double *__restrict__ g_full,
                                                       double *__restrict__ g_proj, int with_hmag,
-                                                      double *partials) {
+                                                      int write_off, double *partials) {
     double e = 0.0;
     const double zc[3] = {zc0, zc1, zc2};
     for (int64_t i = blockIdx.x * (int64_t)kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock) {
@@ -839,10 +845,12 @@ __global__ __launch_bounds__(kBlock) void k_finish_eg(int64_t n, GridDev G, cons
             const int64_t r = i / G.nx, k = r / G.ny;
             const int gi = (int)(i - r * G.nx) - G.m0x, gj = (int)(r - k * G.ny) - G.m0y, gk = (int)k - G.m0z;
             if (gi < 0 || gj < 0 || gk < 0 || gi > G.mn || gj > G.mn || gk > G.mn) {
+                if (write_off) {
 #pragma unroll
-                for (int a = 0; a < 3; ++a) {
-                    g_full[3 * i + a] = 0.0;
-                    g_proj[3 * i + a] = 0.0;
+                    for (int a = 0; a < 3; ++a) {
+                        g_full[3 * i + a] = 0.0;
+                        g_proj[3 * i + a] = 0.0;
+                    }
                 }
                 continue;
             }
@@ -871,11 +879,11 @@ __global__ __launch_bounds__(kBlock) void k_finish_eg(int64_t n, GridDev G, cons
 
 void launch_finish_eg(const Launch &L, int64_t n, const GridDev *G, const double *m, const double *meas,
                       double zc0, double zc1, double zc2, const double *gC, const double *gd, double *g_full,
-                      double *g_proj, double *E_out, int with_hmag) {
+                      double *g_proj, double *E_out, int with_hmag, bool write_off) {
     const int nb = grid_for(n, 2);
     const GridDev g = G ? *G : GridDev{};
     hipLaunchKernelGGL(k_finish_eg, dim3(nb), dim3(kBlock), 0, L.s, n, g, m, meas, zc0, zc1, zc2, gC, gd,
-                       g_full, g_proj, with_hmag, L.red.partials);
+                       g_full, g_proj, with_hmag, (int)write_off, L.red.partials);
     launch_fin(L, FinArgs::store(E_out, 1, 0u, nb, nullptr));
 }
 
@@ -1231,6 +1239,20 @@ __global__ __launch_bounds__(kBlock) void k_tcgc_pmax(int64_t n3, const T *__res
     block_partials<1>(s, 1u, partials);
 }
 
+// The same on the block's nodes only (zout keeps its entries off the block)
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_tcgc_final_block(GridDev G, const T *__restrict__ p,
+                                                             const double *__restrict__ g,
+                                                             double *__restrict__ zout, const DevScalars *dsc) {
+    const bool zero = dsc->tcg_pmax == 0.0;
+    const uint32_t n1 = (uint32_t)G.mn + 1u, nb = n1 * n1 * n1;
+    for (uint32_t t = blockIdx.x * kBlock + threadIdx.x; t < nb; t += gridDim.x * kBlock) {
+        const int64_t gn = block_gnode(G, t);
+#pragma unroll
+        for (int a = 0; a < 3; ++a) zout[3 * gn + a] = zero ? g[3 * gn + a] : (double)p[3 * (int64_t)t + a];
+    }
+}
+
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_tcgc_final(GridDev G, int64_t n, const T *__restrict__ p,
                                                        const double *__restrict__ g, double *__restrict__ zout,
@@ -1250,7 +1272,7 @@ __global__ __launch_bounds__(kBlock) void k_tcgc_final(GridDev G, int64_t n, con
 
 template <typename T>
 void tcgc_run(const Launch &L, const GridDev &G, int64_t n, const TcgBlock<T> &B, const double *m0,
-              const double *gF, const double *g, const double *D, int max_iter, double *zout) {
+              const double *gF, const double *g, const double *D, int max_iter, double *zout, bool write_off) {
     const int64_t n1 = G.mn + 1, nb = n1 * n1 * n1, n3 = 3 * nb;
     const int gi = grid_for(nb, 2), gu = grid_for(n3, 4);
     const TcgTiles tl = block_tiles(G);
@@ -1283,12 +1305,16 @@ void tcgc_run(const Launch &L, const GridDev &G, int64_t n, const TcgBlock<T> &B
     }
     hipLaunchKernelGGL(k_tcgc_pmax<T>, dim3(gu), dim3(kBlock), 0, L.s, n3, B.p, L.red.partials);
     launch_fin(L, FinArgs::store(&L.dsc->tcg_pmax, 1, 1u, gu, nullptr));
-    hipLaunchKernelGGL(k_tcgc_final<T>, dim3(grid_for(n, 2)), dim3(kBlock), 0, L.s, G, n, B.p, g, zout, L.dsc);
+    if (write_off)
+        hipLaunchKernelGGL(k_tcgc_final<T>, dim3(grid_for(n, 2)), dim3(kBlock), 0, L.s, G, n, B.p, g, zout, L.dsc);
+    else
+        hipLaunchKernelGGL(k_tcgc_final_block<T>, dim3(grid_for(nb, 2)), dim3(kBlock), 0, L.s, G, B.p, g, zout,
+                           L.dsc);
 }
 template void tcgc_run<float>(const Launch &, const GridDev &, int64_t, const TcgBlock<float> &, const double *,
-                              const double *, const double *, const double *, int, double *);
+                              const double *, const double *, const double *, int, double *, bool);
 template void tcgc_run<double>(const Launch &, const GridDev &, int64_t, const TcgBlock<double> &, const double *,
-                               const double *, const double *, const double *, int, double *);
+                               const double *, const double *, const double *, int, double *, bool);
 
 // ------------------------------------------------------------------ vector kernels
 
@@ -1306,8 +1332,30 @@ __global__ __launch_bounds__(kBlock) void k_update_m(int64_t n, const double *__
         for (int a = 0; a < 3; ++a) out[3 * i + a] = v[a] / nrm;
     }
 }
+// The same on the magnet block's nodes only (out keeps its entries off the block)
+__global__ __launch_bounds__(kBlock) void k_update_m_block(GridDev G, const double *__restrict__ x,
+                                                           double alpha, const double *__restrict__ d,
+                                                           double *__restrict__ out) {
+    const uint32_t n1 = (uint32_t)G.mn + 1u, nb = n1 * n1 * n1;
+    for (uint32_t t = blockIdx.x * kBlock + threadIdx.x; t < nb; t += gridDim.x * kBlock) {
+        const int64_t i = block_gnode(G, t);
+        double v[3];
+#pragma unroll
+        for (int a = 0; a < 3; ++a) v[a] = x[3 * i + a] + alpha * d[3 * i + a];
+        double nrm = sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
+        if (nrm == 0.0) nrm = 1.0;
+#pragma unroll
+        for (int a = 0; a < 3; ++a) out[3 * i + a] = v[a] / nrm;
+    }
+}
 void launch_update_m(const Launch &L, int64_t n, const double *x, double alpha, const double *d,
-                     double *out) {
+                     double *out, const GridDev *G) {
+    if (G) {
+        const int64_t n1 = G->mn + 1;
+        hipLaunchKernelGGL(k_update_m_block, dim3(grid_for(n1 * n1 * n1, 2)), dim3(kBlock), 0, L.s, *G, x, alpha,
+                           d, out);
+        return;
+    }
     hipLaunchKernelGGL(k_update_m, dim3(grid_for(n, 2)), dim3(kBlock), 0, L.s, n, x, alpha, d, out);
 }
 
@@ -1317,8 +1365,24 @@ __global__ __launch_bounds__(kBlock) void k_axpby(int64_t n, double a, const dou
     for (int64_t i = blockIdx.x * (int64_t)kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock)
         out[i] = (y ? a * x[i] + b * y[i] : a * x[i]);
 }
+// The same over interleaved 3-vectors on the magnet block's nodes only
+__global__ __launch_bounds__(kBlock) void k_axpby3_block(GridDev G, double a, const double *__restrict__ x,
+                                                         double b, const double *y, double *out) {
+    const uint32_t n1 = (uint32_t)G.mn + 1u, nb = n1 * n1 * n1;
+    for (uint32_t t = blockIdx.x * kBlock + threadIdx.x; t < nb; t += gridDim.x * kBlock) {
+        const int64_t i = 3 * block_gnode(G, t);
+#pragma unroll
+        for (int c = 0; c < 3; ++c) out[i + c] = (y ? a * x[i + c] + b * y[i + c] : a * x[i + c]);
+    }
+}
 void launch_axpby(const Launch &L, int64_t n, double a, const double *x, double b, const double *y,
-                  double *out) {
+                  double *out, const GridDev *G3) {
+    if (G3) {
+        const int64_t n1 = G3->mn + 1;
+        hipLaunchKernelGGL(k_axpby3_block, dim3(grid_for(n1 * n1 * n1, 2)), dim3(kBlock), 0, L.s, *G3, a, x, b, y,
+                           out);
+        return;
+    }
     hipLaunchKernelGGL(k_axpby, dim3(grid_for(n, 4)), dim3(kBlock), 0, L.s, n, a, x, b, y, out);
 }
 
