@@ -86,6 +86,9 @@ __global__ __launch_bounds__(256) void k_mg_stencil(PoissonGrid<float> P, MgGrid
     // ahead of their use
     float hxl = 0.f, hxu = 0.f, hyl = 0.f, hyu = 0.f, ixm = 0.f, ixq = 0.f, iym = 0.f, iyq = 0.f;
     float bk = 0.f, xw = 0.f, xe = 0.f, xs = 0.f, xn = 0.f, xm = 0.f, xc = 0.f, xp = 0.f;
+    // z spacings of plane k (hz[k], hz[k + 1], iz[k], iz[k + 1]; n + 1 entries each, so
+    // k + 1 <= nz is valid), also one plane ahead
+    float hzl = 0.f, hzu = 0.f, izl = 0.f, izu = 0.f;
     auto load_plane = [&](int k, float &bv, float &w_, float &e_, float &s_, float &n_) {
         const int64_t idx = base + sz * k;
         const bool in = inner(k);
@@ -110,6 +113,10 @@ __global__ __launch_bounds__(256) void k_mg_stencil(PoissonGrid<float> P, MgGrid
         ixq = P.ixp[i + 1];
         iym = P.iyp[j];
         iyq = P.iyp[j + 1];
+        hzl = P.hzp[k0];
+        hzu = P.hzp[k0 + 1];
+        izl = P.izp[k0];
+        izu = P.izp[k0 + 1];
         const int64_t idx = base + sz * k0;
         const bool in = inner(k0);
         bk = b[idx];
@@ -131,6 +138,7 @@ __global__ __launch_bounds__(256) void k_mg_stencil(PoissonGrid<float> P, MgGrid
     // the loads above complete before the flag's test (rather than being moved after it)
     asm volatile("" : "+v"(hxl), "+v"(hxu), "+v"(hyl), "+v"(hyu), "+v"(ixm), "+v"(ixq), "+v"(iym), "+v"(iyq));
     asm volatile("" : "+v"(xm), "+v"(xc), "+v"(xp), "+v"(bk), "+v"(xw), "+v"(xe), "+v"(xs), "+v"(xn));
+    asm volatile("" : "+v"(hzl), "+v"(hzu), "+v"(izl), "+v"(izu));
     if (sk) return;
     if (active) {
         // per-column factors: edge sums regrouped per axis, e.g.
@@ -139,14 +147,16 @@ __global__ __launch_bounds__(256) void k_mg_stencil(PoissonGrid<float> P, MgGrid
         const float szz = hxl * (2.f * hyl + hyu) + hxu * (hyl + 2.f * hyu);
         for (int k = k0; k < k1; ++k) {
             const int64_t idx = base + sz * k;
-            float bn = 0.f, wn = 0.f, en = 0.f, sn = 0.f, nn = 0.f, xpp = 0.f;
+            float bn = 0.f, wn = 0.f, en = 0.f, sn = 0.f, nn = 0.f, xpp = 0.f, hzn = 0.f, izn = 0.f;
             if (k + 1 < k1) {
                 load_plane(k + 1, bn, wn, en, sn, nn);
                 if (OP != OP_FIRST && k + 2 < nz) xpp = x[idx + 2 * sz];
+                hzn = P.hzp[k + 2];   // k + 2 <= k1 <= nz
+                izn = P.izp[k + 2];
             }
             float o = 0.f;
             if (inner(k)) {
-                const float hzl = P.hzp[k], hzu = P.hzp[k + 1], izm = P.izp[k], izq = P.izp[k + 1];
+                const float izm = izl, izq = izu;
                 const float sx = hyl * (2.f * hzl + hzu) + hyu * (hzl + 2.f * hzu);
                 const float syy = hzl * cx + hzu * dx;
                 const float diag = sx * (ixm + ixq) + syy * (iym + iyq) + szz * (izm + izq);
@@ -174,6 +184,10 @@ __global__ __launch_bounds__(256) void k_mg_stencil(PoissonGrid<float> P, MgGrid
             xe = en;
             xs = sn;
             xn = nn;
+            hzl = hzu;
+            hzu = hzn;
+            izl = izu;
+            izu = izn;
         }
     }
     if (DOT) {
@@ -233,19 +247,39 @@ __global__ __launch_bounds__(256) void k_mg_restrict(MgGrid G, int nxc, int nyc,
         yo[q] = nxf * ryi[3 * J + q];
         yw[q] = ryw[3 * J + q];
     }
+    // the z pairs of plane K (loaded one plane ahead)
+    int zi[3] = {0, 0, 0};
+    float zw[3] = {0.f, 0.f, 0.f};
+    if (k0 < k1) {
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            zi[c] = rzi[3 * k0 + c];
+            zw[c] = rzw[3 * k0 + c];
+        }
+    }
     // the list loads complete before the flag's test (rather than being moved after it)
     asm volatile("" : "+v"(xi[0]), "+v"(xi[1]), "+v"(xi[2]), "+v"(xw[0]), "+v"(xw[1]), "+v"(xw[2]));
     asm volatile("" : "+v"(yo[0]), "+v"(yo[1]), "+v"(yo[2]), "+v"(yw[0]), "+v"(yw[1]), "+v"(yw[2]));
+    asm volatile("" : "+v"(zi[0]), "+v"(zi[1]), "+v"(zi[2]), "+v"(zw[0]), "+v"(zw[1]), "+v"(zw[2]));
     if (sk) return;
     for (int K = k0; K < k1; ++K) {
         const int64_t idx = I + (int64_t)nxc * J + sc * K;
+        int zin[3] = {0, 0, 0};
+        float zwn[3] = {0.f, 0.f, 0.f};
+        if (K + 1 < k1) {
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                zin[c] = rzi[3 * (K + 1) + c];
+                zwn[c] = rzw[3 * (K + 1) + c];
+            }
+        }
         float s = 0.f;
         if (!(edge || K == 0 || K == nzc - 1)) {
 #pragma unroll
             for (int c = 0; c < 3; ++c) {
-                const float wz = rzw[3 * K + c];
+                const float wz = zw[c];
                 if (wz == 0.f) continue;
-                const float *pz = rf + sf * rzi[3 * K + c];
+                const float *pz = rf + sf * zi[c];
                 float sz = 0.f;
 #pragma unroll
                 for (int bq = 0; bq < 3; ++bq) {
@@ -258,6 +292,11 @@ __global__ __launch_bounds__(256) void k_mg_restrict(MgGrid G, int nxc, int nyc,
         }
         bc[idx] = s;
         x1c[idx] = s * invwc[idx];
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            zi[c] = zin[c];
+            zw[c] = zwn[c];
+        }
     }
 }
 
@@ -281,12 +320,23 @@ __global__ __launch_bounds__(256) void k_mg_prolong(MgGrid G, int nxf, int nyf, 
     const int64_t sf = (int64_t)nxf * nyf, sc = (int64_t)nxc * nyc;
     float tx = pxt[i], ty = pyt[j];
     int cx = pxc[i], cy = pyc[j];
-    asm volatile("" : "+v"(tx), "+v"(ty), "+v"(cx), "+v"(cy));   // loaded before the flag's test
+    float tzk = 0.f;   // the z pair of plane k, loaded one plane ahead
+    int czk = 0;
+    if (k0 < k1) {
+        tzk = pzt[k0];
+        czk = pzc[k0];
+    }
+    // loaded before the flag's test
+    asm volatile("" : "+v"(tx), "+v"(ty), "+v"(cx), "+v"(cy), "+v"(tzk), "+v"(czk));
     if (sk) return;
     const int64_t oxy = cx + (int64_t)nxc * cy;
     for (int k = k0; k < k1; ++k) {
-        const float tz = pzt[k];
-        const float *c0 = xc + oxy + sc * pzc[k];
+        const float tz = tzk;
+        const float *c0 = xc + oxy + sc * czk;
+        if (k + 1 < k1) {
+            tzk = pzt[k + 1];
+            czk = pzc[k + 1];
+        }
         const float *c1 = c0 + sc;
         const float a0 = (1.f - ty) * ((1.f - tx) * c0[0] + tx * c0[1]) + ty * ((1.f - tx) * c0[nxc] + tx * c0[nxc + 1]);
         float s = (1.f - tz) * a0;
