@@ -677,40 +677,87 @@ __global__ __launch_bounds__(kBlock) void k_grid_exani(GridDev G, TcgTiles tl, c
     const int chunk = (n1 + tl.gz - 1) / tl.gz;
     const int k0 = cz * chunk, k1 = min(n1, k0 + chunk);
     auto ring3 = [](int p) { return ((p % 3) + 3) % 3; };
-    auto load_nodes = [&](int kp) {
-        const int kk = min(max(kp, 0), mn), slot = ring3(kp);
-        for (int e = threadIdx.x; e < (kCTY + 2) * (kCTX + 2); e += kBlock) {
+    // A thread's ring entries: nodes e = t, t + kBlock < (kCTY + 2)(kCTX + 2), voxels
+    // e = t, t + kBlock < (kCTY + 1)(kCTX + 1).  Plane k + 2's node values and plane
+    // k + 1's voxel materials are loaded into registers before plane k is computed and
+    // written to the rings at the next step.
+    constexpr int kNodeE = (kCTY + 2) * (kCTX + 2), kVoxE = (kCTY + 1) * (kCTX + 1);
+    static_assert(kNodeE <= 2 * kBlock && kVoxE <= 2 * kBlock, "two ring entries per thread");
+    struct NodeRegs {
+        TC v[2][3];
+    };
+    struct VoxRegs {
+        int mat[2];
+    };
+    auto fetch_nodes = [&](int kp) {
+        NodeRegs r{};
+        const int kk = min(max(kp, 0), mn);
+#pragma unroll
+        for (int q = 0; q < 2; ++q) {
+            const int e = threadIdx.x + q * kBlock;
+            if (e >= kNodeE) break;
             const int ex = e % (kCTX + 2), ey = e / (kCTX + 2);
             const int ii = min(max(i0 - 1 + ex, 0), mn), jj = min(max(j0 - 1 + ey, 0), mn);
             const int64_t g = gnode(G, ii, jj, kk);
 #pragma unroll
-            for (int a = 0; a < 3; ++a) nd[slot][ey][ex][a] = (TC)v[3 * g + a];
+            for (int a = 0; a < 3; ++a) r.v[q][a] = (TC)v[3 * g + a];
+        }
+        return r;
+    };
+    auto store_nodes = [&](const NodeRegs &r, int kp) {
+        const int slot = ring3(kp);
+#pragma unroll
+        for (int q = 0; q < 2; ++q) {
+            const int e = threadIdx.x + q * kBlock;
+            if (e >= kNodeE) break;
+            const int ex = e % (kCTX + 2), ey = e / (kCTX + 2);
+#pragma unroll
+            for (int a = 0; a < 3; ++a) nd[slot][ey][ex][a] = r.v[q][a];
         }
     };
-    auto load_voxels = [&](int kz) {
-        const int slot = kz & 1;
-        for (int e = threadIdx.x; e < (kCTY + 1) * (kCTX + 1); e += kBlock) {
+    auto fetch_voxels = [&](int kz) {
+        VoxRegs r{};
+#pragma unroll
+        for (int q = 0; q < 2; ++q) {
+            const int e = threadIdx.x + q * kBlock;
+            if (e >= kVoxE) break;
             const int ex = e % (kCTX + 1), ey = e / (kCTX + 1);
             const int xi = i0 - 1 + ex, yj = j0 - 1 + ey;
             const bool inside = xi >= 0 && yj >= 0 && kz >= 0 && xi < mn && yj < mn && kz < mn;
-            const int mat = inside ? (int)G.vmat[xi + (int64_t)mn * (yj + (int64_t)mn * kz)] : G.n_mats;
-            const VoxCoef<TC> c = vox_coef<TC>(mats[mat]);
-            TC *q = vc[slot][ey][ex];
-            q[0] = c.cex;
-            q[1] = c.cK;
-            q[2] = c.u0;
-            q[3] = c.u1;
-            q[4] = c.u2;
-            if (DIV) q[NV - 1] = c.Js;
+            r.mat[q] = inside ? (int)G.vmat[xi + (int64_t)mn * (yj + (int64_t)mn * kz)] : G.n_mats;
+        }
+        return r;
+    };
+    auto store_voxels = [&](const VoxRegs &r, int kz) {
+        const int slot = kz & 1;
+#pragma unroll
+        for (int q = 0; q < 2; ++q) {
+            const int e = threadIdx.x + q * kBlock;
+            if (e >= kVoxE) break;
+            const int ex = e % (kCTX + 1), ey = e / (kCTX + 1);
+            const VoxCoef<TC> c = vox_coef<TC>(mats[r.mat[q]]);
+            TC *s = vc[slot][ey][ex];
+            s[0] = c.cex;
+            s[1] = c.cK;
+            s[2] = c.u0;
+            s[3] = c.u1;
+            s[4] = c.u2;
+            if (DIV) s[NV - 1] = c.Js;
         }
     };
     if (k0 >= k1) return;   // block-uniform, before any barrier
-    load_nodes(k0 - 1);
-    load_nodes(k0);
-    load_voxels(k0 - 1);
+    store_nodes(fetch_nodes(k0 - 1), k0 - 1);
+    store_nodes(fetch_nodes(k0), k0);
+    store_voxels(fetch_voxels(k0 - 1), k0 - 1);
+    NodeRegs rn = fetch_nodes(k0 + 1);
+    VoxRegs rv = fetch_voxels(k0);
     for (int k = k0; k < k1; ++k) {
-        load_nodes(k + 1);
-        load_voxels(k);
+        store_nodes(rn, k + 1);
+        store_voxels(rv, k);
+        if (k + 1 < k1) {
+            rn = fetch_nodes(k + 2);
+            rv = fetch_voxels(k + 1);
+        }
         __syncthreads();
         if (i < n1 && j < n1) {
             TC nv[27][3];
@@ -1127,25 +1174,68 @@ __global__ __launch_bounds__(kBlock) void k_tcgc_dh(GridDev G, TcgTiles tl, cons
     const int k0 = cz * chunk, k1 = min(n1, k0 + chunk);
     const T beta = (T)dsc->tcg_beta;
     auto ring3 = [](int p) { return ((p % 3) + 3) % 3; };
-    auto load_nodes = [&](int kp) {   // d' of plane kp (clamped) with its halo
-        const int kk = min(max(kp, 0), mn), slot = ring3(kp);
-        for (int e = threadIdx.x; e < (kCTY + 2) * (kCTX + 2); e += kBlock) {
+    // A thread's ring entries: nodes e = t, t + kBlock < (kCTY + 2)(kCTX + 2), voxels
+    // e = t, t + kBlock < (kCTY + 1)(kCTX + 1).  Plane k + 2's raw node values (d, and z
+    // unless FIRST) and plane k + 1's voxel materials are loaded into registers before
+    // plane k is computed; d' = -z + beta d is formed when they are written to the ring.
+    constexpr int kNodeE = (kCTY + 2) * (kCTX + 2), kVoxE = (kCTY + 1) * (kCTX + 1);
+    static_assert(kNodeE <= 2 * kBlock && kVoxE <= 2 * kBlock, "two ring entries per thread");
+    struct NodeRegs {
+        T d[2][3], z[2][3];
+    };
+    struct VoxRegs {
+        int mat[2];
+    };
+    auto fetch_nodes = [&](int kp) {   // plane kp (clamped) with its halo
+        NodeRegs r{};
+        const int kk = min(max(kp, 0), mn);
+#pragma unroll
+        for (int q = 0; q < 2; ++q) {
+            const int e = threadIdx.x + q * kBlock;
+            if (e >= kNodeE) break;
             const int ex = e % (kCTX + 2), ey = e / (kCTX + 2);
             const int ii = min(max(i0 - 1 + ex, 0), mn), jj = min(max(j0 - 1 + ey, 0), mn);
             const int64_t tn = ii + (int64_t)n1 * (jj + (int64_t)n1 * kk);
 #pragma unroll
-            for (int a = 0; a < 3; ++a)
-                nd[slot][ey][ex][a] = FIRST ? din[3 * tn + a] : -z[3 * tn + a] + beta * din[3 * tn + a];
+            for (int a = 0; a < 3; ++a) {
+                r.d[q][a] = din[3 * tn + a];
+                if (!FIRST) r.z[q][a] = z[3 * tn + a];
+            }
+        }
+        return r;
+    };
+    auto store_nodes = [&](const NodeRegs &r, int kp) {
+        const int slot = ring3(kp);
+#pragma unroll
+        for (int q = 0; q < 2; ++q) {
+            const int e = threadIdx.x + q * kBlock;
+            if (e >= kNodeE) break;
+            const int ex = e % (kCTX + 2), ey = e / (kCTX + 2);
+#pragma unroll
+            for (int a = 0; a < 3; ++a) nd[slot][ey][ex][a] = FIRST ? r.d[q][a] : -r.z[q][a] + beta * r.d[q][a];
         }
     };
-    auto load_voxels = [&](int kz) {   // voxel plane kz (zero material outside the block)
-        const int slot = kz & 1;
-        for (int e = threadIdx.x; e < (kCTY + 1) * (kCTX + 1); e += kBlock) {
+    auto fetch_voxels = [&](int kz) {   // voxel plane kz (zero material outside the block)
+        VoxRegs r{};
+#pragma unroll
+        for (int q = 0; q < 2; ++q) {
+            const int e = threadIdx.x + q * kBlock;
+            if (e >= kVoxE) break;
             const int ex = e % (kCTX + 1), ey = e / (kCTX + 1);
             const int xi = i0 - 1 + ex, yj = j0 - 1 + ey;
             const bool inside = xi >= 0 && yj >= 0 && kz >= 0 && xi < mn && yj < mn && kz < mn;
-            const int mat = inside ? (int)G.vmat[xi + (int64_t)mn * (yj + (int64_t)mn * kz)] : G.n_mats;
-            const VoxCoef<T> c = vox_coef<T>(mats[mat]);
+            r.mat[q] = inside ? (int)G.vmat[xi + (int64_t)mn * (yj + (int64_t)mn * kz)] : G.n_mats;
+        }
+        return r;
+    };
+    auto store_voxels = [&](const VoxRegs &r, int kz) {
+        const int slot = kz & 1;
+#pragma unroll
+        for (int q = 0; q < 2; ++q) {
+            const int e = threadIdx.x + q * kBlock;
+            if (e >= kVoxE) break;
+            const int ex = e % (kCTX + 1), ey = e / (kCTX + 1);
+            const VoxCoef<T> c = vox_coef<T>(mats[r.mat[q]]);
             vc[slot][ey][ex][0] = c.cex;
             vc[slot][ey][ex][1] = c.cK;
             vc[slot][ey][ex][2] = c.u0;
@@ -1155,12 +1245,18 @@ __global__ __launch_bounds__(kBlock) void k_tcgc_dh(GridDev G, TcgTiles tl, cons
     };
     double den = 0.0;
     if (k0 < k1) {   // block-uniform
-        load_nodes(k0 - 1);
-        load_nodes(k0);
-        load_voxels(k0 - 1);
+        store_nodes(fetch_nodes(k0 - 1), k0 - 1);
+        store_nodes(fetch_nodes(k0), k0);
+        store_voxels(fetch_voxels(k0 - 1), k0 - 1);
+        NodeRegs rn = fetch_nodes(k0 + 1);
+        VoxRegs rv = fetch_voxels(k0);
         for (int k = k0; k < k1; ++k) {
-            load_nodes(k + 1);
-            load_voxels(k);
+            store_nodes(rn, k + 1);
+            store_voxels(rv, k);
+            if (k + 1 < k1) {
+                rn = fetch_nodes(k + 2);
+                rv = fetch_voxels(k + 1);
+            }
             __syncthreads();
             if (i < n1 && j < n1) {
                 T nv[27][3];
