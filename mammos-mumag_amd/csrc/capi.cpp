@@ -1319,8 +1319,10 @@ int mm_multigrid(mm_ctx *c, int32_t max_levels) {
         for (int a = 0; a < 3; ++a) {   // host-side bounds of every index the kernels use
             if (t[a].coarse.size() < 3) return fail(MM_E_STATE, "multigrid: degenerate coarse axis");
             const int nf = (int)ax[a].size(), nc = (int)t[a].coarse.size();
+            // the prolongation reads coarse nodes c0 and c0 + 1 of every interior fine node
             for (int i = 0; i < nf; ++i)
-                if (t[a].pc[i] < 0 || t[a].pc[i] >= nc || (t[a].pt[i] != 0.f && t[a].pc[i] + 1 >= nc))
+                if (t[a].pc[i] < 0 || t[a].pc[i] >= nc ||
+                    ((t[a].pt[i] != 0.f || (i > 0 && i < nf - 1)) && t[a].pc[i] + 1 >= nc))
                     return fail(MM_E_STATE, "multigrid: prolongation index out of range");
             for (int q = 0; q < 3 * nc; ++q)
                 if (t[a].ri[q] < 0 || t[a].ri[q] >= nf) return fail(MM_E_STATE, "multigrid: restriction index out of range");

@@ -354,8 +354,11 @@ void launch_ru_residual(const Launch &L, int64_t n, const double *b, const doubl
 }
 
 // The true residual on the structured grid in one pass: K x formed as k_grid_poisson forms
-// it (same tiles, same arithmetic), r = b - K x, r32, z32 = r32 invD32 (INIT: p32 = z32)
-// and the partials (r.r FP64, r32.z32, b.b) of k_ru_residual; skipped when *skip.
+// it (same tiles, same expressions, the edge sums written out), r = b - K x, r32,
+// z32 = r32 invD32 (INIT: p32 = z32) and the partials (r.r FP64, r32.z32, b.b) of
+// k_ru_residual; skipped when *skip.  Each plane's loads are issued one plane ahead
+// (in-plane neighbours clamped to the node itself on Dirichlet rows, where they are
+// unused).
 template <bool INIT>
 __global__ __launch_bounds__(kBlock) void k_grid_ru_residual(PoissonGrid<double> P, const double *__restrict__ x,
                                                              const double *__restrict__ b,
@@ -377,31 +380,60 @@ __global__ __launch_bounds__(kBlock) void k_grid_ru_residual(PoissonGrid<double>
         const double mxm = (i - 1 > 0) ? 1.0 : 0.0, mxp = (i + 1 < nx - 1) ? 1.0 : 0.0;
         const double mym = (j - 1 > 0) ? 1.0 : 0.0, myp = (j + 1 < ny - 1) ? 1.0 : 0.0;
         const double ixm = P.ixp[i], ixp = P.ixp[i + 1], iym = P.iyp[j], iyp = P.iyp[j + 1];
+        const double hxl = P.hxp[i], hxu = P.hxp[i + 1], hyl = P.hyp[j], hyu = P.hyp[j + 1];
         const int64_t base = i + sy * j;
+        // plane k: x at k + 1, its in-plane neighbours, b, invD, hz and iz at k and k + 1
+        // (the spacing arrays hold n + 1 entries)
+        struct Plane {
+            double xp, xw, xe, xs, xn, bi, hzl, hzu, izl, izu;
+            float iv;
+        };
+        auto load = [&](int k) {
+            Plane p;
+            const int64_t idx = base + sz * k;
+            const bool in = !(edge_ij || k == 0 || k == nz - 1);
+            const int64_t ox = in ? 1 : 0, oy = in ? sy : 0;
+            p.xp = x[k + 1 < nz ? idx + sz : idx];
+            if (k + 1 >= nz) p.xp = 0.0;
+            p.xw = x[idx - ox];
+            p.xe = x[idx + ox];
+            p.xs = x[idx - oy];
+            p.xn = x[idx + oy];
+            p.bi = b[idx];
+            p.iv = invD32[idx];
+            p.hzl = P.hzp[k];
+            p.hzu = P.hzp[k + 1];
+            p.izl = P.izp[k];
+            p.izu = P.izp[k + 1];
+            return p;
+        };
         double xm = k0 > 0 ? x[base + sz * (k0 - 1)] : 0.0;
         double xc = x[base + sz * k0];
+        Plane cur = load(k0);
         for (int k = k0; k < k1; ++k) {
             const int64_t idx = base + sz * k;
-            const double xp = k + 1 < nz ? x[idx + sz] : 0.0;
+            Plane nxt = cur;
+            if (k + 1 < k1) nxt = load(k + 1);
+            const double xp = cur.xp;
             double out;
             if (edge_ij || k == 0 || k == nz - 1) {
                 out = xc;  // Dirichlet row
             } else {
-                const double xw = x[idx - 1], xe = x[idx + 1], xs = x[idx - sy], xn = x[idx + sy];
-                const double sx = edge_sum(P.hyp, P.hzp, j, k);
-                const double syy = edge_sum(P.hzp, P.hxp, k, i);
+                // edge_sum(hy, hz; j, k) and edge_sum(hz, hx; k, i)
+                const double sx = 2.0 * hyl * cur.hzl + hyl * cur.hzu + hyu * cur.hzl + 2.0 * hyu * cur.hzu;
+                const double syy = 2.0 * cur.hzl * hxl + cur.hzl * hxu + cur.hzu * hxl + 2.0 * cur.hzu * hxu;
                 const double cxm = -sx * ixm, cxp = -sx * ixp;
                 const double cym = -syy * iym, cyp = -syy * iyp;
-                const double czm = -sz_ij * P.izp[k], czp = -sz_ij * P.izp[k + 1];
+                const double czm = -sz_ij * cur.izl, czp = -sz_ij * cur.izu;
                 const double diag = -(cxm + cxp + cym + cyp + czm + czp);
                 const double mzm = (k - 1 > 0) ? 1.0 : 0.0, mzp = (k + 1 < nz - 1) ? 1.0 : 0.0;
-                out = diag * xc + mxm * cxm * xw + mxp * cxp * xe + mym * cym * xs + myp * cyp * xn +
-                      mzm * czm * xm + mzp * czp * xp;
+                out = diag * xc + mxm * cxm * cur.xw + mxp * cxp * cur.xe + mym * cym * cur.xs +
+                      myp * cyp * cur.xn + mzm * czm * xm + mzp * czp * xp;
             }
-            const double bi = b[idx];
+            const double bi = cur.bi;
             const double ri = bi - out;
             const float rf = (float)ri;
-            const float zf = rf * invD32[idx];
+            const float zf = rf * cur.iv;
             r32[idx] = rf;
             z32[idx] = zf;
             if (INIT) p32[idx] = zf;
@@ -410,6 +442,7 @@ __global__ __launch_bounds__(kBlock) void k_grid_ru_residual(PoissonGrid<double>
             s[2] += bi * bi;
             xm = xc;
             xc = xp;
+            cur = nxt;
         }
     }
     block_partials<3>(s, 0u, partials);

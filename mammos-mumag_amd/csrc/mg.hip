@@ -275,19 +275,24 @@ __global__ __launch_bounds__(256) void k_mg_restrict(MgGrid G, int nxc, int nyc,
         }
         float s = 0.f;
         if (!(edge || K == 0 || K == nzc - 1)) {
+            // all 27 loads at once (no branch between them: the kernel is latency-bound);
+            // unused pairs (index 0, weight 0) add exactly zero
+            float f[3][3][3];
 #pragma unroll
-            for (int c = 0; c < 3; ++c) {
-                const float wz = zw[c];
-                if (wz == 0.f) continue;
-                const float *pz = rf + sf * zi[c];
-                float sz = 0.f;
+            for (int c = 0; c < 3; ++c)
 #pragma unroll
                 for (int bq = 0; bq < 3; ++bq) {
-                    if (yw[bq] == 0.f) continue;
-                    const float *py = pz + yo[bq];
-                    sz += yw[bq] * (xw[0] * py[xi[0]] + xw[1] * py[xi[1]] + xw[2] * py[xi[2]]);
+                    const float *py = rf + sf * zi[c] + yo[bq];
+#pragma unroll
+                    for (int q = 0; q < 3; ++q) f[c][bq][q] = py[xi[q]];
                 }
-                s += wz * sz;
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                float sz = 0.f;
+#pragma unroll
+                for (int bq = 0; bq < 3; ++bq)
+                    sz += yw[bq] * (xw[0] * f[c][bq][0] + xw[1] * f[c][bq][1] + xw[2] * f[c][bq][2]);
+                s += zw[c] * sz;
             }
         }
         bc[idx] = s;
@@ -337,14 +342,13 @@ __global__ __launch_bounds__(256) void k_mg_prolong(MgGrid G, int nxf, int nyf, 
             tzk = pzt[k + 1];
             czk = pzc[k + 1];
         }
-        const float *c1 = c0 + sc;
+        const float *c1 = c0 + sc;   // inside the coarse axis for an interior fine node
+        // both coarse planes' loads at once; with tz = 0 the second adds exactly zero
         const float a0 = (1.f - ty) * ((1.f - tx) * c0[0] + tx * c0[1]) + ty * ((1.f - tx) * c0[nxc] + tx * c0[nxc + 1]);
+        const float a1 =
+            (1.f - ty) * ((1.f - tx) * c1[0] + tx * c1[1]) + ty * ((1.f - tx) * c1[nxc] + tx * c1[nxc + 1]);
         float s = (1.f - tz) * a0;
-        if (tz != 0.f) {   // uniform over the block
-            const float a1 =
-                (1.f - ty) * ((1.f - tx) * c1[0] + tx * c1[1]) + ty * ((1.f - tx) * c1[nxc] + tx * c1[nxc + 1]);
-            s += tz * a1;
-        }
+        s += tz * a1;
         const int64_t idx = i + (int64_t)nxf * j + sf * k;
         xf[idx] += s;
     }
