@@ -820,59 +820,126 @@ __global__ __launch_bounds__(kBlock) void k_grid_exani(GridDev G, TcgTiles tl, c
     }
 }
 
-// g = interleave(Gx u, Gy u, Gz u) at magnet nodes.
+// g = interleave(Gx u, Gy u, Gz u) at magnet nodes.  Tiles of kCTX x kCTY nodes marching
+// a z chunk, as k_grid_exani: u in a three-plane LDS ring with a one-node halo (clamped to
+// the block), the voxels' Js in a two-plane ring, the next planes' loads in registers
+// ahead of the compute; the 27 neighbours and 8 voxels then come from LDS.
 template <typename TC>
-__global__ __launch_bounds__(kBlock) void k_grid_grad(GridDev G, const MatCoef *__restrict__ mats,
+__global__ __launch_bounds__(kBlock) void k_grid_grad(GridDev G, TcgTiles tl, const MatCoef *__restrict__ mats,
                                                       const double *__restrict__ u,
                                                       double *__restrict__ g) {
-    const int n1 = G.mn + 1;
-    const int64_t nmag = (int64_t)n1 * n1 * n1;
-    const int64_t t = blockIdx.x * (int64_t)kBlock + threadIdx.x;
-    if (t >= nmag) return;
-    const int i = (int)(t % n1), j = (int)((t / n1) % n1), k = (int)(t / ((int64_t)n1 * n1));
-    TC nu[27];
+    static_assert(kCTX * kCTY == kBlock, "one node per thread and plane");
+    __shared__ TC nd[3][kCTY + 2][kCTX + 2];
+    __shared__ TC vj[2][kCTY + 1][kCTX + 1];
+    const int mn = G.mn, n1 = mn + 1;
+    const int bl = blockIdx.x;
+    const int cz = bl % tl.gz, bx = (bl / tl.gz) % tl.gx, by = bl / (tl.gz * tl.gx);
+    const int tx = threadIdx.x % kCTX, ty = threadIdx.x / kCTX;
+    const int i0 = bx * kCTX, j0 = by * kCTY, i = i0 + tx, j = j0 + ty;
+    const int chunk = (n1 + tl.gz - 1) / tl.gz;
+    const int k0 = cz * chunk, k1 = min(n1, k0 + chunk);
+    auto ring3 = [](int p) { return ((p % 3) + 3) % 3; };
+    constexpr int kNodeE = (kCTY + 2) * (kCTX + 2), kVoxE = (kCTY + 1) * (kCTX + 1);
+    static_assert(kNodeE <= 2 * kBlock && kVoxE <= 2 * kBlock, "two ring entries per thread");
+    struct NodeRegs {
+        TC v[2];
+    };
+    struct VoxRegs {
+        int mat[2];
+    };
+    auto fetch_nodes = [&](int kp) {
+        NodeRegs r{};
+        const int kk = min(max(kp, 0), mn);
 #pragma unroll
-    for (int dz = -1; dz <= 1; ++dz)
-#pragma unroll
-        for (int dy = -1; dy <= 1; ++dy)
-#pragma unroll
-            for (int dx = -1; dx <= 1; ++dx) {
-                const int ii = min(max(i + dx, 0), G.mn), jj = min(max(j + dy, 0), G.mn),
-                          kk = min(max(k + dz, 0), G.mn);
-                nu[(dx + 1) + 3 * (dy + 1) + 9 * (dz + 1)] = (TC)u[gnode(G, ii, jj, kk)];
-            }
-    TC acc[3] = {0, 0, 0};
-#pragma unroll
-    for (int w = 0; w < 8; ++w) {
-        const int vx = w & 1, vy = (w >> 1) & 1, vz = (w >> 2) & 1;
-        const int xi = i - 1 + vx, yj = j - 1 + vy, zk = k - 1 + vz;
-        const bool inside = xi >= 0 && yj >= 0 && zk >= 0 && xi < G.mn && yj < G.mn && zk < G.mn;
-        const int mat = inside ? (int)G.vmat[xi + (int64_t)G.mn * (yj + (int64_t)G.mn * zk)] : G.n_mats;
-        const TC js = (TC)mats[mat].Js;
-        const int p = (1 - vx) + 2 * (1 - vy) + 4 * (1 - vz);
-        TC s[3] = {0, 0, 0};
-#pragma unroll
-        for (int c = 0; c < 8; ++c) {
-            const int cx = c & 1, cy = (c >> 1) & 1, cz = (c >> 2) & 1;
-            const int sidx = (cx - (1 - vx) + 1) + 3 * (cy - (1 - vy) + 1) + 9 * (cz - (1 - vz) + 1);
-#pragma unroll
-            for (int a = 0; a < 3; ++a) {
-                const int B = kKB[c][p][a];
-                if (B != 0) s[a] += (TC)B * nu[sidx];
-            }
+        for (int q = 0; q < 2; ++q) {
+            const int e = threadIdx.x + q * kBlock;
+            if (e >= kNodeE) break;
+            const int ex = e % (kCTX + 2), ey = e / (kCTX + 2);
+            const int ii = min(max(i0 - 1 + ex, 0), mn), jj = min(max(j0 - 1 + ey, 0), mn);
+            r.v[q] = (TC)u[gnode(G, ii, jj, kk)];
         }
+        return r;
+    };
+    auto store_nodes = [&](const NodeRegs &r, int kp) {
+        const int slot = ring3(kp);
 #pragma unroll
-        for (int a = 0; a < 3; ++a) acc[a] += js * s[a];
-    }
-    const int64_t gn = gnode(G, i, j, k);
+        for (int q = 0; q < 2; ++q) {
+            const int e = threadIdx.x + q * kBlock;
+            if (e >= kNodeE) break;
+            nd[slot][e / (kCTX + 2)][e % (kCTX + 2)] = r.v[q];
+        }
+    };
+    auto fetch_voxels = [&](int kz) {
+        VoxRegs r{};
+#pragma unroll
+        for (int q = 0; q < 2; ++q) {
+            const int e = threadIdx.x + q * kBlock;
+            if (e >= kVoxE) break;
+            const int xi = i0 - 1 + e % (kCTX + 1), yj = j0 - 1 + e / (kCTX + 1);
+            const bool inside = xi >= 0 && yj >= 0 && kz >= 0 && xi < mn && yj < mn && kz < mn;
+            r.mat[q] = inside ? (int)G.vmat[xi + (int64_t)mn * (yj + (int64_t)mn * kz)] : G.n_mats;
+        }
+        return r;
+    };
+    auto store_voxels = [&](const VoxRegs &r, int kz) {
+        const int slot = kz & 1;
+#pragma unroll
+        for (int q = 0; q < 2; ++q) {
+            const int e = threadIdx.x + q * kBlock;
+            if (e >= kVoxE) break;
+            vj[slot][e / (kCTX + 1)][e % (kCTX + 1)] = (TC)mats[r.mat[q]].Js;
+        }
+    };
+    if (k0 >= k1) return;   // block-uniform, before any barrier
+    store_nodes(fetch_nodes(k0 - 1), k0 - 1);
+    store_nodes(fetch_nodes(k0), k0);
+    store_voxels(fetch_voxels(k0 - 1), k0 - 1);
+    NodeRegs rn = fetch_nodes(k0 + 1);
+    VoxRegs rv = fetch_voxels(k0);
     const TC f = (TC)G.fG;
+    for (int k = k0; k < k1; ++k) {
+        store_nodes(rn, k + 1);
+        store_voxels(rv, k);
+        if (k + 1 < k1) {
+            rn = fetch_nodes(k + 2);
+            rv = fetch_voxels(k + 1);
+        }
+        __syncthreads();
+        if (i < n1 && j < n1) {
+            TC nu[27];
 #pragma unroll
-    for (int a = 0; a < 3; ++a) g[3 * gn + a] = (double)(f * acc[a]);
-}
-
-static inline int grid_mag_blocks(const GridDev &G) {
-    const int64_t n1 = G.mn + 1;
-    return (int)((n1 * n1 * n1 + kBlock - 1) / kBlock);
+            for (int dz = -1; dz <= 1; ++dz)
+#pragma unroll
+                for (int dy = -1; dy <= 1; ++dy)
+#pragma unroll
+                    for (int dx = -1; dx <= 1; ++dx)
+                        nu[(dx + 1) + 3 * (dy + 1) + 9 * (dz + 1)] = nd[ring3(k + dz)][ty + 1 + dy][tx + 1 + dx];
+            TC acc[3] = {0, 0, 0};
+#pragma unroll
+            for (int w = 0; w < 8; ++w) {
+                const int vx = w & 1, vy = (w >> 1) & 1, vz = (w >> 2) & 1;
+                const TC js = vj[(k - 1 + vz) & 1][ty + vy][tx + vx];
+                const int p = (1 - vx) + 2 * (1 - vy) + 4 * (1 - vz);
+                TC s[3] = {0, 0, 0};
+#pragma unroll
+                for (int c = 0; c < 8; ++c) {
+                    const int cx = c & 1, cy = (c >> 1) & 1, cz2 = (c >> 2) & 1;
+                    const int sidx = (cx - (1 - vx) + 1) + 3 * (cy - (1 - vy) + 1) + 9 * (cz2 - (1 - vz) + 1);
+#pragma unroll
+                    for (int a = 0; a < 3; ++a) {
+                        const int B = kKB[c][p][a];
+                        if (B != 0) s[a] += (TC)B * nu[sidx];
+                    }
+                }
+#pragma unroll
+                for (int a = 0; a < 3; ++a) acc[a] += js * s[a];
+            }
+            const int64_t gn = gnode(G, i, j, k);
+#pragma unroll
+            for (int a = 0; a < 3; ++a) g[3 * gn + a] = (double)(f * acc[a]);
+        }
+        __syncthreads();
+    }
 }
 
 void launch_grid_exani(const Launch &L, const GridDev &G, bool fp32, const double *v, double *y,
@@ -892,11 +959,12 @@ void launch_grid_exani(const Launch &L, const GridDev &G, bool fp32, const doubl
     }
 }
 void launch_grid_grad(const Launch &L, const GridDev &G, bool fp32, const double *u, double *g) {
-    const dim3 gr(grid_mag_blocks(G)), blk(kBlock);
+    const TcgTiles tl = block_tiles(G);
+    const dim3 gr(tl.gx * tl.gy * tl.gz), blk(kBlock);
     if (fp32)
-        hipLaunchKernelGGL(k_grid_grad<float>, gr, blk, 0, L.s, G, L.mats, u, g);
+        hipLaunchKernelGGL(k_grid_grad<float>, gr, blk, 0, L.s, G, tl, L.mats, u, g);
     else
-        hipLaunchKernelGGL(k_grid_grad<double>, gr, blk, 0, L.s, G, L.mats, u, g);
+        hipLaunchKernelGGL(k_grid_grad<double>, gr, blk, 0, L.s, G, tl, L.mats, u, g);
 }
 
 // ------------------------------------------------------------------ node kernels
