@@ -62,6 +62,21 @@ static inline int grid_for(int64_t n, int per_thread = 1) {
     return (int)b;
 }
 
+// The total of n partials in k_fin's order (per thread over j = t, t + 256, ..., then the
+// wave, then the waves in order), in every thread of a kFinThreads-thread block; sh holds
+// kFinThreads / kWave doubles.  Contains a barrier.
+__device__ double fin_order_sum(const double *__restrict__ parts, int n, double *sh) {
+    const int lane = threadIdx.x & (kWave - 1), w = threadIdx.x / kWave;
+    double acc = 0.0;
+    for (int j = threadIdx.x; j < n; j += kBlock) acc = acc + parts[j];
+    acc = wave_sum(acc);
+    if (lane == 0) sh[w] = acc;
+    __syncthreads();
+    double t = sh[0];
+    for (int k = 1; k < kBlock / kWave; ++k) t = t + sh[k];
+    return t;
+}
+
 // One block: totals of the partials, then the scalar logic selected by op.  All values'
 // partials are loaded in one pass (each value summed in the same order as one value at a
 // time: per thread over j = t, t + 256, ..., then the wave, then the waves in order), one
@@ -1404,13 +1419,20 @@ __global__ __launch_bounds__(kBlock) void k_tcgc_dh(GridDev G, TcgTiles tl, cons
     block_partials<1>(v, 0u, partials);
 }
 
+// den = d'.H d' from k_tcgc_dh's nden partials, summed in every block in k_fin's order
+// (so alpha is what a finalize launch would give); block 0 publishes it.
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_tcgc_upd(int64_t n3, T *__restrict__ p, T *__restrict__ r,
                                                      T *__restrict__ z, const T *__restrict__ d,
                                                      const T *__restrict__ Hd, const T *__restrict__ Dc,
-                                                     const DevScalars *dsc, double *partials) {
-    if (dsc->tcg_done) return;
-    const double den = dsc->tcg_den;
+                                                     DevScalars *dsc, const double *__restrict__ dparts,
+                                                     int nden, double *partials) {
+    static_assert(kFinThreads == kBlock, "the finalize order needs the finalize's block width");
+    __shared__ double sh[kBlock / kWave];
+    const int done = dsc->tcg_done;   // block-uniform; tested after the barrier
+    const double den = fin_order_sum(dparts, nden, sh);
+    if (done) return;
+    if (blockIdx.x == 0 && threadIdx.x == 0) dsc->tcg_den = den;
     const bool neg = den <= 0.0;
     const double alpha = neg ? 0.0 : dsc->tcg_rz / den;
     double s[2] = {0.0, 0.0};
@@ -1484,15 +1506,16 @@ void tcgc_run(const Launch &L, const GridDev &G, int64_t n, const TcgBlock<T> &B
     for (int it = 0; it < max_iter; ++it) {
         const T *din = B.d[it & 1];
         T *dout = B.d[(it + 1) & 1];
+        // d'.H d' partials in slot 3, read by the update kernel while it writes slots 0, 1
+        double *const dparts = L.red.partials + 3 * kMaxPartials;
         if (it == 0)
             hipLaunchKernelGGL((k_tcgc_dh<T, true>), dim3(gd), dim3(kBlock), 0, L.s, G, tl, L.mats, B.z, din, dout,
-                               B.m0, B.mg, B.Hd, L.dsc, L.red.partials);
+                               B.m0, B.mg, B.Hd, L.dsc, dparts);
         else
             hipLaunchKernelGGL((k_tcgc_dh<T, false>), dim3(gd), dim3(kBlock), 0, L.s, G, tl, L.mats, B.z, din, dout,
-                               B.m0, B.mg, B.Hd, L.dsc, L.red.partials);
-        launch_fin(L, FinArgs::store(&L.dsc->tcg_den, 1, 0u, gd, &L.dsc->tcg_done));
+                               B.m0, B.mg, B.Hd, L.dsc, dparts);
         hipLaunchKernelGGL(k_tcgc_upd<T>, dim3(gu), dim3(kBlock), 0, L.s, n3, B.p, B.r, B.z, dout, B.Hd, B.D, L.dsc,
-                           L.red.partials);
+                           dparts, gd, L.red.partials);
         FinArgs a{};
         a.op = FIN_TCG_UPDATE;
         a.nv = 2;
