@@ -1004,9 +1004,10 @@ __global__ __launch_bounds__(kBlock) void k_finish_eg(int64_t n, GridDev G, cons
     double e = 0.0;
     const double zc[3] = {zc0, zc1, zc2};
     for (int64_t i = blockIdx.x * (int64_t)kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock) {
-        if (G.nx > 0) {
-            const int64_t r = i / G.nx, k = r / G.ny;
-            const int gi = (int)(i - r * G.nx) - G.m0x, gj = (int)(r - k * G.ny) - G.m0y, gk = (int)k - G.m0z;
+        if (G.nx > 0) {   // 32-bit index arithmetic (a structured grid has fewer than 2^32 nodes)
+            const uint32_t iu = (uint32_t)i, r = iu / (uint32_t)G.nx, k = r / (uint32_t)G.ny;
+            const int gi = (int)(iu - r * (uint32_t)G.nx) - G.m0x, gj = (int)(r - k * (uint32_t)G.ny) - G.m0y,
+                      gk = (int)k - G.m0z;
             if (gi < 0 || gj < 0 || gk < 0 || gi > G.mn || gj > G.mn || gk > G.mn) {
                 if (write_off) {
 #pragma unroll
@@ -1223,9 +1224,10 @@ void launch_tcg_final(const Launch &L, int64_t n3, const double *p, const double
 //   k_tcgc_final  zout = (max|p| == 0) ? g : p scattered to the grid (0 off the block)
 
 __device__ __forceinline__ void block_ijk(int n1, int64_t t, int &i, int &j, int &k) {
-    i = (int)(t % n1);
-    j = (int)((t / n1) % n1);
-    k = (int)(t / ((int64_t)n1 * n1));
+    const uint32_t tu = (uint32_t)t, n1u = (uint32_t)n1, tj = tu / n1u, kk = tj / n1u;   // n1^3 < 2^32
+    i = (int)(tu - tj * n1u);
+    j = (int)(tj - kk * n1u);
+    k = (int)kk;
 }
 
 template <typename T>
@@ -1477,10 +1479,11 @@ __global__ __launch_bounds__(kBlock) void k_tcgc_final(GridDev G, int64_t n, con
                                                        const double *__restrict__ g, double *__restrict__ zout,
                                                        const DevScalars *dsc) {
     const bool zero = dsc->tcg_pmax == 0.0;
-    const int64_t sxy = (int64_t)G.nx * G.ny;
     const int n1 = G.mn + 1;
     for (int64_t gn = blockIdx.x * (int64_t)kBlock + threadIdx.x; gn < n; gn += (int64_t)gridDim.x * kBlock) {
-        const int i = (int)(gn % G.nx) - G.m0x, j = (int)((gn / G.nx) % G.ny) - G.m0y, k = (int)(gn / sxy) - G.m0z;
+        const uint32_t gu = (uint32_t)gn, r = gu / (uint32_t)G.nx, kz = r / (uint32_t)G.ny;   // N < 2^32
+        const int i = (int)(gu - r * (uint32_t)G.nx) - G.m0x, j = (int)(r - kz * (uint32_t)G.ny) - G.m0y,
+                  k = (int)kz - G.m0z;
         const bool in = i >= 0 && j >= 0 && k >= 0 && i < n1 && j < n1 && k < n1;
         const int64_t t = in ? i + (int64_t)n1 * (j + (int64_t)n1 * k) : 0;
 #pragma unroll
@@ -1628,11 +1631,12 @@ __global__ __launch_bounds__(kBlock) void k_reduce(int64_t n, GridDev G, int com
     const int64_t ne = G.nx > 0 ? (int64_t)n1 * n1 * n1 * comp : n;
     for (int64_t e = blockIdx.x * (int64_t)kBlock + threadIdx.x; e < ne; e += (int64_t)gridDim.x * kBlock) {
         int64_t i = e;
-        if (G.nx > 0) {
-            const int64_t t = e / comp;
-            const int r = (int)(e - t * comp);
-            const int64_t tj = t / n1;
-            i = comp * gnode(G, (int)(t - tj * n1), (int)(tj % n1), (int)(tj / n1)) + r;
+        if (G.nx > 0) {   // 32-bit index arithmetic (the block holds fewer than 2^32 entries)
+            const uint32_t eu = (uint32_t)e, n1u = (uint32_t)n1;
+            const uint32_t t = comp == 3 ? eu / 3u : eu;
+            const uint32_t r = eu - t * (uint32_t)comp;
+            const uint32_t tj = t / n1u, k = tj / n1u;
+            i = comp * gnode(G, (int)(t - tj * n1u), (int)(tj - k * n1u), (int)k) + r;
         }
         if constexpr (OP == RED_DOT) {
             s[0] += a[i] * b[i];
@@ -1671,8 +1675,9 @@ __global__ __launch_bounds__(kBlock) void k_outside_max(int64_t n, GridDev G, co
                                                         double *partials) {
     double s[1] = {0.0};
     for (int64_t i = blockIdx.x * (int64_t)kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock) {
-        const int64_t r = i / G.nx, k = r / G.ny;
-        const int gi = (int)(i - r * G.nx) - G.m0x, gj = (int)(r - k * G.ny) - G.m0y, gk = (int)k - G.m0z;
+        const uint32_t iu = (uint32_t)i, r = iu / (uint32_t)G.nx, k = r / (uint32_t)G.ny;
+        const int gi = (int)(iu - r * (uint32_t)G.nx) - G.m0x, gj = (int)(r - k * (uint32_t)G.ny) - G.m0y,
+                  gk = (int)k - G.m0z;
         if (gi < 0 || gj < 0 || gk < 0 || gi > G.mn || gj > G.mn || gk > G.mn)
 #pragma unroll
             for (int a = 0; a < 3; ++a) s[0] = fmax(s[0], fabs(m[3 * i + a]));
