@@ -754,6 +754,8 @@ int mm_ctx_create(const mm_mesh_desc *d, const mm_options *opts, mm_ctx **out) {
         if (!(S.V > 0.0)) return fail(MM_E_MESH, "no magnetic element (all Js <= 0)");
         if (!ka_grid_fits(d->grid_n[0], d->grid_n[1], d->grid_n[2]))
             return fail(MM_E_MESH, "structured: grid too large for the PCG tiles; use use_structured = 0");
+        if (!tcg_tiles_fit(S.mn))
+            return fail(MM_E_MESH, "structured: magnet block too large for the truncated-CG tiles; use use_structured = 0");
     } else {
         std::string err = assemble(*d, o.drop_tol, H, true);
         if (!err.empty()) return fail(MM_E_MESH, err);

@@ -219,6 +219,8 @@ struct TcgBlock {
     T *m0 = nullptr, *mg = nullptr, *D = nullptr, *p = nullptr, *r = nullptr, *z = nullptr, *Hd = nullptr;
     T *d[2] = {nullptr, nullptr};
 };
+// The magnet block's node tiles fit the partial slots (one partial per tile)
+bool tcg_tiles_fit(int mn);
 // write_off = false: zout already holds zero off the block and is written on it only
 template <typename T>
 void tcgc_run(const Launch &L, const GridDev &G, int64_t n, const TcgBlock<T> &B, const double *m0,

@@ -707,6 +707,12 @@ TcgTiles block_tiles(const GridDev &G) {
     tl.gz = std::max(1, std::min(n1, std::min(1024, kMaxPartials) / (tl.gx * tl.gy)));
     return tl;
 }
+bool tcg_tiles_fit(int mn) {   // the truncated CG's d'.H d' writes one partial per tile
+    GridDev G;
+    G.mn = mn;
+    const TcgTiles tl = block_tiles(G);
+    return (int64_t)tl.gx * tl.gy * tl.gz <= kMaxPartials;
+}
 template <typename TC, bool DIV>
 __global__ __launch_bounds__(kBlock) void k_grid_exani(GridDev G, TcgTiles tl, const MatCoef *__restrict__ mats,
                                                        const double *__restrict__ v,
