@@ -386,20 +386,30 @@ __host__ __device__ inline int tail_transfer_words(int nxf, int nyf, int nzf, in
     return 6 * (nxc + nyc + nzc) + 2 * (nxf + nyf + nzf);
 }
 
+// A thread's first node t = threadIdx.x on an nx x ny x nz level and the (i, j, k) step of
+// t += kTailThreads: computed once per level (they need divisions), then every pass over
+// the level walks its nodes from them without divisions (TailIter).
+struct TailPos {
+    int i, j, k, si, sj, sk;
+};
+__device__ TailPos tail_pos(int nx, int ny) {
+    TailPos p;
+    const int r = threadIdx.x / nx;
+    p.i = threadIdx.x - r * nx;
+    p.j = r % ny;
+    p.k = r / ny;
+    const int s = kTailThreads / nx;
+    p.si = kTailThreads - s * nx;
+    p.sj = s % ny;
+    p.sk = s / ny;
+    return p;
+}
 // Node coordinates of t = threadIdx.x, threadIdx.x + kTailThreads, ... on an nx x ny x nz
 // level, advanced without divisions.
 struct TailIter {
     int i, j, k, si, sj, sk, nx, ny;
-    __device__ TailIter(int nx_, int ny_) : nx(nx_), ny(ny_) {
-        const int r = threadIdx.x / nx;
-        i = threadIdx.x - r * nx;
-        j = r % ny;
-        k = r / ny;
-        const int s = kTailThreads / nx;
-        si = kTailThreads - s * nx;
-        sj = s % ny;
-        sk = s / ny;
-    }
+    __device__ TailIter(const TailPos &p, int nx_, int ny_)
+        : i(p.i), j(p.j), k(p.k), si(p.si), sj(p.sj), sk(p.sk), nx(nx_), ny(ny_) {}
     __device__ void next() {
         i += si;
         j += sj;
@@ -473,10 +483,10 @@ __device__ void tail_fill_transfer(const MgLevelDev &f, const PoissonGrid<float>
 }
 
 // out = op(b, x) on level P; b at LDS offset ob, or the global bg when ob < 0
-__device__ __forceinline__ void tail_stencil(const TailLevel &P, int ob, const float *bg, int ox, int oo, float w,
-                                             int op) {
+__device__ __forceinline__ void tail_stencil(const TailLevel &P, const TailPos &pos, int ob, const float *bg, int ox,
+                                             int oo, float w, int op) {
     const int nx = P.nx, ny = P.ny, nz = P.nz, nxy = nx * ny, n = nxy * nz;
-    TailIter it(nx, ny);
+    TailIter it(pos, nx, ny);
     for (int t = threadIdx.x; t < n; t += kTailThreads, it.next()) {
         const int i = it.i, j = it.j, k = it.k;
         float o = 0.f;
@@ -502,11 +512,11 @@ __device__ __forceinline__ void tail_stencil(const TailLevel &P, int ob, const f
 }
 
 // bc = R rf, zero on the coarse faces (fine level F with its lists T, coarse level C)
-__device__ __forceinline__ void tail_restrict(const TailLevel &F, const TailTransfer &T, const TailLevel &C, int orf,
-                                              int obc) {
+__device__ __forceinline__ void tail_restrict(const TailLevel &F, const TailTransfer &T, const TailLevel &C,
+                                              const TailPos &cpos, int orf, int obc) {
     const int nxc = C.nx, nyc = C.ny, nzc = C.nz, nxyc = nxc * nyc, nc = nxyc * nzc;
     const int nxf = F.nx, nxyf = F.nx * F.ny;
-    TailIter it(nxc, nyc);
+    TailIter it(cpos, nxc, nyc);
     for (int t = threadIdx.x; t < nc; t += kTailThreads, it.next()) {
         const int I = it.i, J = it.j, K = it.k;
         float s = 0.f;
@@ -533,11 +543,11 @@ __device__ __forceinline__ void tail_restrict(const TailLevel &F, const TailTran
 }
 
 // xf += P xc on the interior of the fine level F
-__device__ __forceinline__ void tail_prolong(const TailLevel &F, const TailTransfer &T, const TailLevel &C, int oxc,
-                                             int oxf) {
+__device__ __forceinline__ void tail_prolong(const TailLevel &F, const TailPos &fpos, const TailTransfer &T,
+                                             const TailLevel &C, int oxc, int oxf) {
     const int nx = F.nx, ny = F.ny, nz = F.nz, nxy = nx * ny, n = nxy * nz;
     const int nxc = C.nx, nxyc = C.nx * C.ny;
-    TailIter it(nx, ny);
+    TailIter it(fpos, nx, ny);
     for (int t = threadIdx.x; t < n; t += kTailThreads, it.next()) {
         const int i = it.i, j = it.j, k = it.k;
         if (i == 0 || j == 0 || k == 0 || i == nx - 1 || j == ny - 1 || k == nz - 1) continue;
@@ -559,9 +569,11 @@ __global__ __launch_bounds__(kTailThreads) void k_mg_tail(MgTailArgs a, const in
     int cur[NT], oth[NT], ob[NT];
     TailLevel TL[NT];
     TailTransfer TT[NT];
+    TailPos pos[NT];
 #pragma unroll
     for (int q = 0; q < NT; ++q) {
         const PoissonGrid<float> &P = a.lev[q].P;
+        pos[q] = tail_pos(P.nx, P.ny);
         cur[q] = a.off[q];
         oth[q] = a.off[q] + a.n[q];
         ob[q] = q == 0 ? -1 : a.off[q] + 2 * a.n[q];
@@ -576,27 +588,27 @@ __global__ __launch_bounds__(kTailThreads) void k_mg_tail(MgTailArgs a, const in
     __syncthreads();
 #pragma unroll
     for (int q = 0; q + 1 < NT; ++q) {
-        tail_stencil(TL[q], ob[q], a.b0, 0, cur[q], a.om[0], OP_FIRST);
+        tail_stencil(TL[q], pos[q], ob[q], a.b0, 0, cur[q], a.om[0], OP_FIRST);
         for (int k = 1; k < a.s; ++k) {
-            tail_stencil(TL[q], ob[q], a.b0, cur[q], oth[q], a.om[k], OP_SMOOTH);
+            tail_stencil(TL[q], pos[q], ob[q], a.b0, cur[q], oth[q], a.om[k], OP_SMOOTH);
             const int t = cur[q]; cur[q] = oth[q]; oth[q] = t;
         }
-        tail_stencil(TL[q], ob[q], a.b0, cur[q], oth[q], 0.f, OP_RESID);
-        tail_restrict(TL[q], TT[q], TL[q + 1], oth[q], ob[q + 1]);
+        tail_stencil(TL[q], pos[q], ob[q], a.b0, cur[q], oth[q], 0.f, OP_RESID);
+        tail_restrict(TL[q], TT[q], TL[q + 1], pos[q + 1], oth[q], ob[q + 1]);
     }
     {
         constexpr int q = NT - 1;
-        tail_stencil(TL[q], ob[q], a.b0, 0, cur[q], a.omc[0], OP_FIRST);
+        tail_stencil(TL[q], pos[q], ob[q], a.b0, 0, cur[q], a.omc[0], OP_FIRST);
         for (int k = 1; k < a.cd; ++k) {
-            tail_stencil(TL[q], ob[q], a.b0, cur[q], oth[q], a.omc[k], OP_SMOOTH);
+            tail_stencil(TL[q], pos[q], ob[q], a.b0, cur[q], oth[q], a.omc[k], OP_SMOOTH);
             const int t = cur[q]; cur[q] = oth[q]; oth[q] = t;
         }
     }
 #pragma unroll
     for (int q = NT - 2; q >= 0; --q) {
-        tail_prolong(TL[q], TT[q], TL[q + 1], cur[q + 1], cur[q]);
+        tail_prolong(TL[q], pos[q], TT[q], TL[q + 1], cur[q + 1], cur[q]);
         for (int k = a.s - 1; k >= 0; --k) {
-            tail_stencil(TL[q], ob[q], a.b0, cur[q], oth[q], a.om[k], OP_SMOOTH);
+            tail_stencil(TL[q], pos[q], ob[q], a.b0, cur[q], oth[q], a.om[k], OP_SMOOTH);
             const int t = cur[q]; cur[q] = oth[q]; oth[q] = t;
         }
     }
