@@ -205,3 +205,41 @@ def test_mixed_multigrid_loop():
         assert np.max(np.abs(r1[:, 1] - r0[:, 1])) < 1e-6, form
         assert np.max(np.abs(r1[:, 0] / r0[:, 0] - 1)) < 1e-6, form
     assert out["mixed+mg"][1]["pcg_iters"] < out["mixed"][1]["pcg_iters"] / 10
+
+
+def test_mixed_multigrid_pcg_limits():
+    """mm_pcg in the mixed form with the multigrid preconditioner (its segment target and
+    iteration cap are tested on the device between KB and the V-cycle): from zero the true
+    residual meets the FP64 path's bound r.r <= tol^2 b.b, with far fewer iterations than
+    plain FP64; a cap of 3 iterations returns the warning code after exactly 3; a converged
+    start does nothing."""
+    import torch
+    from mammos_mumag_amd import Model
+    from oracle import mg
+    g = generate.build(edge=39.0, n=39, layers=13, grains=2, cone=15.0, seed=3)
+    A, face = mg.operator(g.xs, g.ys, g.zs)
+    rng = np.random.default_rng(5)
+    b = rng.normal(size=A.shape[0])
+    b[face] = 0.0
+    ref = Model.from_structured(g, precision="fp64")
+    xr = ref.empty1()
+    itr, rc = ref.pcg(ref.to_device(b), xr, 1e-10)
+    assert rc == 0
+    ref.close()
+    model = Model.from_structured(g, precision="mixed")
+    model.multigrid()
+    x = model.empty1()
+    it, rc = model.pcg(model.to_device(b), x, 1e-10)
+    xs = x.cpu().numpy()
+    res = np.linalg.norm(b - A @ xs) / np.linalg.norm(b)
+    print(f"[gpu] mixed+mg pcg: {it} iterations (fp64 Jacobi {itr}), true residual {res:.2e}", flush=True)
+    assert rc == 0 and it < itr / 10
+    assert res <= 1e-10 * (1 + 1e-6)
+    x = model.empty1()
+    it, rc = model.pcg(model.to_device(b), x, 1e-14, maxiter=3)
+    assert it == 3 and rc == 1
+    x = model.to_device(xs)
+    x0 = x.clone()
+    it, rc = model.pcg(model.to_device(b), x, 1e-6)
+    assert it == 0 and rc == 0 and torch.equal(x, x0)
+    model.close()
