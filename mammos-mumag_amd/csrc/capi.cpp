@@ -360,13 +360,20 @@ float *mg_vcycle(mm_ctx *c, const float *r, float *z, float *alt, const int *ski
         if (last0 && fuse) np = nb;
         std::swap(cur[l], oth[l]);
     };
+    // two steps in one launch on the coarse levels (l >= 1: launch-bound kernels)
+    auto smooth2 = [&](int l, float w1, float w2) {
+        launch_mg_smooth2(L, c->mg[l].P, B[l], cur[l], oth[l], w1, w2, skip);
+        std::swap(cur[l], oth[l]);
+    };
     // levels below top run one kernel per step; from top on, the tail kernel (or, with no
     // tail, the coarsest level's polynomial)
     const bool tail = c->mg_tail < nl;
     const int top = tail ? c->mg_tail : nl - 1;
     // pre-smoothing (the first step of every level is already in cur[l]), residual, restriction
     for (int l = 0; l < top; ++l) {
-        for (int k = 1; k < s; ++k) smooth(l, c->mg_om[k], false);
+        int k = 1;
+        for (; l > 0 && k + 1 < s; k += 2) smooth2(l, c->mg_om[k], c->mg_om[k + 1]);
+        for (; k < s; ++k) smooth(l, c->mg_om[k], false);
         launch_mg_resid(L, c->mg[l].P, B[l], cur[l], c->mg_r[l], skip);
         launch_mg_restrict(L, c->mg[l], c->mg[l + 1], c->mg_r[l], c->mg_b[l + 1], cur[l + 1], skip);
     }
@@ -378,7 +385,9 @@ float *mg_vcycle(mm_ctx *c, const float *r, float *z, float *alt, const int *ski
     }
     for (int l = top - 1; l >= 0; --l) {
         launch_mg_prolong(L, c->mg[l], c->mg[l + 1], cur[l + 1], cur[l], skip);
-        for (int k = s - 1; k >= 0; --k) smooth(l, c->mg_om[k], l == 0 && k == 0, l == 0 && k == s - 1 && k > 0);
+        int k = s - 1;
+        for (; l > 0 && k >= 1; k -= 2) smooth2(l, c->mg_om[k], c->mg_om[k - 1]);
+        for (; k >= 0; --k) smooth(l, c->mg_om[k], l == 0 && k == 0, l == 0 && k == s - 1 && k > 0);
     }
     if (parts && np == 0) np = launch_dot2<float>(L, c->N, r, cur[0], parts);
     if (nparts) *nparts = np;

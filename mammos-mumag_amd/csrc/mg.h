@@ -54,6 +54,9 @@ int launch_mg_smooth(const Launch &L, const PoissonGrid<float> &P, const float *
                      float w, const int *skip, double *dot_parts = nullptr);
 void launch_mg_resid(const Launch &L, const PoissonGrid<float> &P, const float *b, const float *x, float *out,
                      const int *skip);
+// out = two smoothing steps (weights w1 then w2) from x, in one pass (for small levels)
+void launch_mg_smooth2(const Launch &L, const PoissonGrid<float> &P, const float *b, const float *x, float *out,
+                       float w1, float w2, const int *skip);
 // b_c = R r_f and the coarse level's first smoothing step x1c = c.invw b_c
 void launch_mg_restrict(const Launch &L, const MgLevelDev &f, const MgLevelDev &c, const float *rf, float *bc,
                         float *x1c, const int *skip);

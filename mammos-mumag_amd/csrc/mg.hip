@@ -216,6 +216,68 @@ __global__ __launch_bounds__(256) void k_mg_stencil(PoissonGrid<float> P, MgGrid
     }
 }
 
+// Two smoothing steps in one pass, for the coarse levels whose kernels are launch-bound:
+// out = S_w2(S_w1(x)), S_w(x) = x + w (b - A x) / diag on the interior, 0 on the faces,
+// with k_mg_stencil's expressions.  One node per thread: it forms S_w1(x) at its node and
+// at its six neighbours (the interior ones; faces hold zero) from the 25 x values within
+// two axis steps and the spacings of the three columns / rows / planes around it (read
+// through the caches), then S_w2 at its node.
+__device__ __forceinline__ float mg_smooth_value(float hxl, float hxu, float hyl, float hyu, float hzl, float hzu,
+                                                 float ixm, float ixq, float iym, float iyq, float izm, float izq,
+                                                 float w, float bk, float xc, float xw, float xe, float xs,
+                                                 float xn, float xm, float xp) {
+    const float cx = 2.f * hxl + hxu, dx = hxl + 2.f * hxu;
+    const float szz = hxl * (2.f * hyl + hyu) + hxu * (hyl + 2.f * hyu);
+    const float sx = hyl * (2.f * hzl + hzu) + hyu * (hzl + 2.f * hzu);
+    const float syy = hzl * cx + hzu * dx;
+    const float diag = sx * (ixm + ixq) + syy * (iym + iyq) + szz * (izm + izq);
+    const float ax = diag * xc - sx * (ixm * xw + ixq * xe) - syy * (iym * xs + iyq * xn) - szz * (izm * xm + izq * xp);
+    return xc + w * (bk - ax) / diag;
+}
+__global__ __launch_bounds__(256) void k_mg_smooth2(PoissonGrid<float> P, const float *__restrict__ b,
+                                                    const float *__restrict__ x, float *__restrict__ out, float w1,
+                                                    float w2, const int *skip) {
+    const int nx = P.nx, ny = P.ny, nz = P.nz;
+    const int64_t n = (int64_t)nx * ny * nz;
+    const int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (t >= n) return;
+    if (skip && *skip) return;
+    const uint32_t tu = (uint32_t)t, r = tu / (uint32_t)nx, kk = r / (uint32_t)ny;
+    const int i = (int)(tu - r * (uint32_t)nx), j = (int)(r - kk * (uint32_t)ny), k = (int)kk;
+    if (i == 0 || j == 0 || k == 0 || i == nx - 1 || j == ny - 1 || k == nz - 1) {
+        out[t] = 0.f;
+        return;
+    }
+    const int64_t sy = nx, sz = (int64_t)nx * ny;
+    // spacings of index - 1 .. index + 2 per axis (n + 1 entries each; 1 <= i <= nx - 2)
+    float hx[4], ix[4], hy[4], iy[4], hz[4], iz[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        hx[q] = P.hxp[i - 1 + q];
+        ix[q] = P.ixp[i - 1 + q];
+        hy[q] = P.hyp[j - 1 + q];
+        iy[q] = P.iyp[j - 1 + q];
+        hz[q] = P.hzp[k - 1 + q];
+        iz[q] = P.izp[k - 1 + q];
+    }
+    auto X = [&](int di, int dj, int dk) { return x[t + di + sy * dj + sz * dk]; };
+    auto B = [&](int di, int dj, int dk) { return b[t + di + sy * dj + sz * dk]; };
+    // step 1 at (i + di, j + dj, k + dk), one offset non-zero at most; 0 on the faces
+    auto s1 = [&](int di, int dj, int dk) -> float {
+        const int ii = i + di, jj = j + dj, kz = k + dk;
+        if (ii == 0 || jj == 0 || kz == 0 || ii == nx - 1 || jj == ny - 1 || kz == nz - 1) return 0.f;
+        const int a = 1 + di, c = 1 + dj, e = 1 + dk;
+        return mg_smooth_value(hx[a], hx[a + 1], hy[c], hy[c + 1], hz[e], hz[e + 1], ix[a], ix[a + 1], iy[c],
+                               iy[c + 1], iz[e], iz[e + 1], w1, B(di, dj, dk), X(di, dj, dk), X(di - 1, dj, dk),
+                               X(di + 1, dj, dk), X(di, dj - 1, dk), X(di, dj + 1, dk), X(di, dj, dk - 1),
+                               X(di, dj, dk + 1));
+    };
+    const float yc = s1(0, 0, 0), yw = s1(-1, 0, 0), ye = s1(1, 0, 0), ys = s1(0, -1, 0), yn = s1(0, 1, 0),
+                ym = s1(0, 0, -1), yp = s1(0, 0, 1);
+    out[t] = mg_smooth_value(hx[1], hx[2], hy[1], hy[2], hz[1], hz[2], ix[1], ix[2], iy[1], iy[2], iz[1], iz[2], w2,
+                             B(0, 0, 0), yc, yw, ye, ys, yn, ym, yp);
+}
+
 // b_c = R r_f (transpose of the per-axis linear interpolation); zero on the coarse faces.
 // Per coarse index I of an axis, up to 3 (fine index, weight) pairs at [3 I + q] (unused
 // pairs: index 0, weight 0).  Also x1c = invw_c b_c, the coarse level's first smoothing
@@ -655,6 +717,12 @@ int launch_mg_smooth(const Launch &L, const PoissonGrid<float> &P, const float *
         hipLaunchKernelGGL((k_mg_stencil<OP_SMOOTH, false>), blocks(g), dim3(256), 0, L.s, P, g, b, x, out, w, skip,
                            (double *)nullptr);
     return g.x * g.y * g.z;
+}
+void launch_mg_smooth2(const Launch &L, const PoissonGrid<float> &P, const float *b, const float *x, float *out,
+                       float w1, float w2, const int *skip) {
+    const int64_t n = (int64_t)P.nx * P.ny * P.nz;
+    hipLaunchKernelGGL(k_mg_smooth2, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, L.s, P, b, x, out, w1, w2,
+                       skip);
 }
 void launch_mg_resid(const Launch &L, const PoissonGrid<float> &P, const float *b, const float *x, float *out,
                      const int *skip) {
