@@ -36,9 +36,9 @@ HBM_PEAK_GBS = 8000.0
 # copy of --traffic-probe (factors 0.5000 and 1.0000); see
 # profiles/pcg_kernels_{fp64,mixed_jacobi}_voronoi125.md and profiles/mg_vcycle_mixed_voronoi125.md.
 TRAFFIC = {
-    ("fp64", "spmv"): 132807621, ("fp64", "update"): 233822987,
-    ("mixed", "spmv"): 67669447, ("mixed", "update"): 116963435,
-    ("mixed", "mg_smooth"): 45881849,
+    ("fp64", "spmv"): 132805198, ("fp64", "update"): 233822847,
+    ("mixed", "spmv"): 67655278, ("mixed", "update"): 116963418,
+    ("mixed", "mg_smooth"): 45772351,
 }
 
 
