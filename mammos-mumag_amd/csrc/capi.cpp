@@ -1440,6 +1440,13 @@ int mm_multigrid(mm_ctx *c, int32_t max_levels) {
         c->mg_tail = t;
         c->tail = a;
     }
+    if (c->mg_tail < nl) {   // the tail's stencil factors and transfer lists, formed once
+        float *img = nullptr;
+        if ((rc = c->alloc(&img, (size_t)c->tail.img_n))) return rc;
+        launch_mg_tail_image(c->L(), c->tail, img);
+        HIPC(hipStreamSynchronize(c->s));
+        c->tail.img = img;
+    }
     // a multigrid-preconditioned FP32 recursion stays accurate over a longer stretch:
     // replace the residual every 1e-3 of reduction unless the caller chose otherwise
     if (!c->ru_delta_set) c->ru_delta = 1e-3;

@@ -31,7 +31,9 @@ struct MgLevelDev {
 // right-hand side of the first tail level (global), x0 receives its result (all of it,
 // zero on the faces); n[q] is tail level q's node count, off[q] the LDS offset (floats)
 // of its vectors (x, y, and b for q > 0), coef_off[q] of its stencil factors and
-// tr_off[q] of its transfer lists (mg_tail_layout fills the offsets and lds_bytes).
+// tr_off[q] of its transfer lists (mg_tail_layout fills the offsets, img_n and
+// lds_bytes).  The tables (img_n floats from coef_off[0]) are formed once by
+// launch_mg_tail_image into img (img_n floats, 16-byte aligned), which the tail copies.
 constexpr int kMgTailMax = 4;
 constexpr int kMgTailLdsBytes = 64 * 1024;
 constexpr int kMgMaxDeg = 16;      // smoothing / coarsest-level polynomial degrees at most
@@ -40,11 +42,14 @@ struct MgTailArgs {
     int n[kMgTailMax] = {}, off[kMgTailMax] = {}, coef_off[kMgTailMax] = {}, tr_off[kMgTailMax] = {};
     int lds_bytes = 0;
     MgLevelDev lev[kMgTailMax];
+    int img_n = 0;
+    const float *img = nullptr;
     const float *b0 = nullptr;
     float *x0 = nullptr;
     float om[kMgMaxDeg] = {}, omc[kMgMaxDeg] = {};
 };
 int mg_tail_layout(MgTailArgs &a);
+void launch_mg_tail_image(const Launch &L, const MgTailArgs &a, float *img);
 void launch_mg_tail(const Launch &L, const MgTailArgs &a, const int *skip);
 
 void launch_mg_first(const Launch &L, const PoissonGrid<float> &P, const float *b, float *out, float w,

@@ -426,8 +426,9 @@ __global__ __launch_bounds__(256) void k_mg_prolong(MgGrid G, int nxf, int nyf, 
 constexpr int kTailThreads = 1024;
 
 // The tail's dynamic LDS; every tail array is an offset into it (so that the accesses
-// are LDS instructions; integers are stored bit-cast).
-extern __shared__ float tail_lds[];
+// are LDS instructions; integers are stored bit-cast), 16-byte aligned for the float4
+// copy of the tables.
+extern __shared__ __align__(16) float tail_lds[];
 
 // A level's stencil factors (offsets): SX[j + ny k] = edge_sum(hy, hz; j, k),
 // SY[k + nz i] = edge_sum(hz, hx; k, i), SZ[i + nx j] = edge_sum(hx, hy; i, j),
@@ -624,10 +625,39 @@ __device__ __forceinline__ void tail_prolong(const TailLevel &F, const TailPos &
     __syncthreads();
 }
 
+// The tail's stencil factors and transfer lists (its LDS words from coef_off[0] on) for NT
+// levels, formed once into img: each tail launch then copies them in one pass instead of
+// gathering them from the level arrays (one memory round trip per table).
+template <int NT>
+__global__ __launch_bounds__(kTailThreads) void k_mg_tail_image(MgTailArgs a, float *img) {
+#pragma unroll
+    for (int q = 0; q < NT; ++q) {
+        const PoissonGrid<float> &P = a.lev[q].P;
+        tail_fill_level(P, tail_level(a.coef_off[q], P.nx, P.ny, P.nz));
+        if (q + 1 < NT) {
+            const PoissonGrid<float> &C = a.lev[q + 1].P;
+            tail_fill_transfer(a.lev[q], C, tail_transfer(a.tr_off[q], P.nx, P.ny, P.nz, C.nx, C.ny, C.nz));
+        }
+    }
+    __syncthreads();
+    for (int t = threadIdx.x; t < a.img_n; t += kTailThreads) img[t] = tail_lds[a.coef_off[0] + t];
+}
+
 // NT tail levels (compile-time, so that the per-level state stays in registers)
 template <int NT>
 __global__ __launch_bounds__(kTailThreads) void k_mg_tail(MgTailArgs a, const int *skip) {
-    if (skip && *skip) return;
+    // the image's first float4 per thread is loaded before the skip test (index clamped
+    // into the image, so the load is unconditional); coef_off[0] and img_n are multiples
+    // of 4 (mg_tail_layout)
+    const int n4 = a.img_n >> 2;
+    const float4 *src = reinterpret_cast<const float4 *>(a.img);
+    float4 *dst = reinterpret_cast<float4 *>(tail_lds + a.coef_off[0]);
+    float4 v = src[min((int)threadIdx.x, n4 - 1)];
+    const int sk = *(skip ? skip : &kNoSkip);
+    asm volatile("" : "+v"(v.x), "+v"(v.y), "+v"(v.z), "+v"(v.w));
+    if (sk) return;
+    if ((int)threadIdx.x < n4) dst[threadIdx.x] = v;
+    for (int t = threadIdx.x + kTailThreads; t < n4; t += kTailThreads) dst[t] = src[t];
     int cur[NT], oth[NT], ob[NT];
     TailLevel TL[NT];
     TailTransfer TT[NT];
@@ -640,11 +670,9 @@ __global__ __launch_bounds__(kTailThreads) void k_mg_tail(MgTailArgs a, const in
         oth[q] = a.off[q] + a.n[q];
         ob[q] = q == 0 ? -1 : a.off[q] + 2 * a.n[q];
         TL[q] = tail_level(a.coef_off[q], P.nx, P.ny, P.nz);
-        tail_fill_level(P, TL[q]);
         if (q + 1 < NT) {
             const PoissonGrid<float> &C = a.lev[q + 1].P;
             TT[q] = tail_transfer(a.tr_off[q], P.nx, P.ny, P.nz, C.nx, C.ny, C.nz);
-            tail_fill_transfer(a.lev[q], C, TT[q]);
         }
     }
     __syncthreads();
@@ -736,6 +764,7 @@ int mg_tail_layout(MgTailArgs &a) {
         a.off[q] = f;
         f += (q == 0 ? 2 : 3) * a.n[q];
     }
+    f = (f + 3) & ~3;   // the tables (the image) start on a float4
     for (int q = 0; q < a.nt; ++q) {
         const PoissonGrid<float> &P = a.lev[q].P;
         a.coef_off[q] = f;
@@ -746,8 +775,18 @@ int mg_tail_layout(MgTailArgs &a) {
             f += tail_transfer_words(P.nx, P.ny, P.nz, C.nx, C.ny, C.nz);
         }
     }
+    f = (f + 3) & ~3;
+    a.img_n = f - a.coef_off[0];
     a.lds_bytes = f * (int)sizeof(float);
     return a.lds_bytes;
+}
+void launch_mg_tail_image(const Launch &L, const MgTailArgs &a, float *img) {
+    switch (a.nt) {
+        case 1: hipLaunchKernelGGL(k_mg_tail_image<1>, dim3(1), dim3(kTailThreads), a.lds_bytes, L.s, a, img); break;
+        case 2: hipLaunchKernelGGL(k_mg_tail_image<2>, dim3(1), dim3(kTailThreads), a.lds_bytes, L.s, a, img); break;
+        case 3: hipLaunchKernelGGL(k_mg_tail_image<3>, dim3(1), dim3(kTailThreads), a.lds_bytes, L.s, a, img); break;
+        default: hipLaunchKernelGGL(k_mg_tail_image<4>, dim3(1), dim3(kTailThreads), a.lds_bytes, L.s, a, img); break;
+    }
 }
 void launch_mg_tail(const Launch &L, const MgTailArgs &a, const int *skip) {
     switch (a.nt) {
