@@ -1436,7 +1436,7 @@ int mm_multigrid(mm_ctx *c, int32_t max_levels) {
     };
     for (int t = nl - 1; t >= 1 && nl - t <= kMgTailMax; --t) {
         const MgTailArgs a = tail_args(t);
-        if (a.lds_bytes > kMgTailLdsBytes) break;
+        if (!mg_tail_fits(a)) break;
         c->mg_tail = t;
         c->tail = a;
     }

@@ -49,6 +49,8 @@ struct MgTailArgs {
     float om[kMgMaxDeg] = {}, omc[kMgMaxDeg] = {};
 };
 int mg_tail_layout(MgTailArgs &a);
+// the laid-out tail fits the kernel: its LDS, and at most one z column per thread per level
+bool mg_tail_fits(const MgTailArgs &a);
 void launch_mg_tail_image(const Launch &L, const MgTailArgs &a, float *img);
 void launch_mg_tail(const Launch &L, const MgTailArgs &a, const int *skip);
 

@@ -545,17 +545,37 @@ __device__ void tail_fill_transfer(const MgLevelDev &f, const PoissonGrid<float>
     tail_fill_pairs(f.P.nz, f.pzc, f.pzt, T.pzc, T.pzt);
 }
 
+// The stencil passes walk z columns: thread u takes column c = u mod (nx ny) and the
+// u div (nx ny)-th of the kTailThreads / (nx ny) equal k ranges (empty past nz), so each
+// node is one thread's once; the column's x and y factors and its face test are loaded
+// once per pass.  Needs nx ny <= kTailThreads (mg_tail_fits).
+struct TailCol {
+    int i, j, k0, k1;
+};
+__device__ TailCol tail_col(int nx, int ny, int nz) {
+    const int ncol = nx * ny, ch = kTailThreads / ncol, kc = (nz + ch - 1) / ch;
+    const int c = threadIdx.x % ncol, chunk = threadIdx.x / ncol;
+    TailCol p;
+    p.i = c % nx;
+    p.j = c / nx;
+    p.k0 = min(nz, chunk * kc);   // chunk >= ch gives k0 = nz: idle
+    p.k1 = min(nz, p.k0 + kc);
+    return p;
+}
+
 // out = op(b, x) on level P; b at LDS offset ob, or the global bg when ob < 0
-__device__ __forceinline__ void tail_stencil(const TailLevel &P, const TailPos &pos, int ob, const float *bg, int ox,
+__device__ __forceinline__ void tail_stencil(const TailLevel &P, const TailCol &col, int ob, const float *bg, int ox,
                                              int oo, float w, int op) {
-    const int nx = P.nx, ny = P.ny, nz = P.nz, nxy = nx * ny, n = nxy * nz;
-    TailIter it(pos, nx, ny);
-    for (int t = threadIdx.x; t < n; t += kTailThreads, it.next()) {
-        const int i = it.i, j = it.j, k = it.k;
+    const int nx = P.nx, ny = P.ny, nz = P.nz, nxy = nx * ny;
+    const int i = col.i, j = col.j;
+    const bool in_ij = i > 0 && j > 0 && i < nx - 1 && j < ny - 1;
+    const float szz = lf(P.SZ + i + nx * j);
+    const float ixm = lf(P.IX + i), ixq = lf(P.IX + i + 1), iym = lf(P.IY + j), iyq = lf(P.IY + j + 1);
+    int t = i + nx * j + nxy * col.k0;
+    for (int k = col.k0; k < col.k1; ++k, t += nxy) {
         float o = 0.f;
-        if (!(i == 0 || j == 0 || k == 0 || i == nx - 1 || j == ny - 1 || k == nz - 1)) {
-            const float sx = lf(P.SX + j + ny * k), syy = lf(P.SY + k + nz * i), szz = lf(P.SZ + i + nx * j);
-            const float ixm = lf(P.IX + i), ixq = lf(P.IX + i + 1), iym = lf(P.IY + j), iyq = lf(P.IY + j + 1);
+        if (in_ij && k > 0 && k < nz - 1) {
+            const float sx = lf(P.SX + j + ny * k), syy = lf(P.SY + k + nz * i);
             const float izm = lf(P.IZ + k), izq = lf(P.IZ + k + 1);
             const float diag = sx * (ixm + ixq) + syy * (iym + iyq) + szz * (izm + izq);
             const float bi = ob < 0 ? bg[t] : lf(ob + t);
@@ -662,10 +682,12 @@ __global__ __launch_bounds__(kTailThreads) void k_mg_tail(MgTailArgs a, const in
     TailLevel TL[NT];
     TailTransfer TT[NT];
     TailPos pos[NT];
+    TailCol col[NT];
 #pragma unroll
     for (int q = 0; q < NT; ++q) {
         const PoissonGrid<float> &P = a.lev[q].P;
         pos[q] = tail_pos(P.nx, P.ny);
+        col[q] = tail_col(P.nx, P.ny, P.nz);
         cur[q] = a.off[q];
         oth[q] = a.off[q] + a.n[q];
         ob[q] = q == 0 ? -1 : a.off[q] + 2 * a.n[q];
@@ -678,19 +700,19 @@ __global__ __launch_bounds__(kTailThreads) void k_mg_tail(MgTailArgs a, const in
     __syncthreads();
 #pragma unroll
     for (int q = 0; q + 1 < NT; ++q) {
-        tail_stencil(TL[q], pos[q], ob[q], a.b0, 0, cur[q], a.om[0], OP_FIRST);
+        tail_stencil(TL[q], col[q], ob[q], a.b0, 0, cur[q], a.om[0], OP_FIRST);
         for (int k = 1; k < a.s; ++k) {
-            tail_stencil(TL[q], pos[q], ob[q], a.b0, cur[q], oth[q], a.om[k], OP_SMOOTH);
+            tail_stencil(TL[q], col[q], ob[q], a.b0, cur[q], oth[q], a.om[k], OP_SMOOTH);
             const int t = cur[q]; cur[q] = oth[q]; oth[q] = t;
         }
-        tail_stencil(TL[q], pos[q], ob[q], a.b0, cur[q], oth[q], 0.f, OP_RESID);
+        tail_stencil(TL[q], col[q], ob[q], a.b0, cur[q], oth[q], 0.f, OP_RESID);
         tail_restrict(TL[q], TT[q], TL[q + 1], pos[q + 1], oth[q], ob[q + 1]);
     }
     {
         constexpr int q = NT - 1;
-        tail_stencil(TL[q], pos[q], ob[q], a.b0, 0, cur[q], a.omc[0], OP_FIRST);
+        tail_stencil(TL[q], col[q], ob[q], a.b0, 0, cur[q], a.omc[0], OP_FIRST);
         for (int k = 1; k < a.cd; ++k) {
-            tail_stencil(TL[q], pos[q], ob[q], a.b0, cur[q], oth[q], a.omc[k], OP_SMOOTH);
+            tail_stencil(TL[q], col[q], ob[q], a.b0, cur[q], oth[q], a.omc[k], OP_SMOOTH);
             const int t = cur[q]; cur[q] = oth[q]; oth[q] = t;
         }
     }
@@ -698,7 +720,7 @@ __global__ __launch_bounds__(kTailThreads) void k_mg_tail(MgTailArgs a, const in
     for (int q = NT - 2; q >= 0; --q) {
         tail_prolong(TL[q], pos[q], TT[q], TL[q + 1], cur[q + 1], cur[q]);
         for (int k = a.s - 1; k >= 0; --k) {
-            tail_stencil(TL[q], pos[q], ob[q], a.b0, cur[q], oth[q], a.om[k], OP_SMOOTH);
+            tail_stencil(TL[q], col[q], ob[q], a.b0, cur[q], oth[q], a.om[k], OP_SMOOTH);
             const int t = cur[q]; cur[q] = oth[q]; oth[q] = t;
         }
     }
@@ -779,6 +801,11 @@ int mg_tail_layout(MgTailArgs &a) {
     a.img_n = f - a.coef_off[0];
     a.lds_bytes = f * (int)sizeof(float);
     return a.lds_bytes;
+}
+bool mg_tail_fits(const MgTailArgs &a) {
+    for (int q = 0; q < a.nt; ++q)
+        if (a.lev[q].P.nx * a.lev[q].P.ny > kTailThreads) return false;
+    return a.lds_bytes <= kMgTailLdsBytes;
 }
 void launch_mg_tail_image(const Launch &L, const MgTailArgs &a, float *img) {
     switch (a.nt) {
