@@ -284,6 +284,14 @@ __global__ __launch_bounds__(256) void k_mg_smooth2(PoissonGrid<float> P, const 
 // step from zero (invw_c: the level's first Chebyshev weight over the diagonal, 0 on the
 // faces).  Tiles of 64 x 4 coarse (x, y) columns, each thread marching a z chunk; the
 // z and y pairs are uniform over a wave, the x pairs per lane.
+// Each wave restricts 64 consecutive coarse nodes (I) of one coarse row (J, K): it first
+// combines the 3 x 3 fine rows (z pairs x y pairs) with their weights over the fine span
+// the 64 nodes use (at most 131 nodes from the first coarse node's first fine index: a
+// coarse interval covers at most two fine ones), each lane loading 2 (+1) consecutive fine
+// nodes per row, so every load is contiguous; the x pairs are then read from that
+// combined row in LDS.  Loads are clamped into the fine row and LDS reads into the span,
+// so every access stays in bounds; unused pairs (weight 0) add exactly zero.
+constexpr int kRSpan = 132;
 __global__ __launch_bounds__(256) void k_mg_restrict(MgGrid G, int nxc, int nyc, int nzc, int nxf, int nyf,
                                                      const int *__restrict__ rxi, const float *__restrict__ rxw,
                                                      const int *__restrict__ ryi, const float *__restrict__ ryw,
@@ -291,23 +299,26 @@ __global__ __launch_bounds__(256) void k_mg_restrict(MgGrid G, int nxc, int nyc,
                                                      const float *__restrict__ rf, float *__restrict__ bc,
                                                      const float *__restrict__ invwc, float *__restrict__ x1c,
                                                      const int *skip) {
+    __shared__ float row[kMgTY][kRSpan + 4];
     const int sk = *(skip ? skip : &kNoSkip);   // tested once the transfer lists' loads are issued
     const Tile T = mg_tile(G);
-    const int I = T.bx * kMgTX + (threadIdx.x % kMgTX);
-    const int J = T.by * kMgTY + (threadIdx.x / kMgTX);
+    const int lane = threadIdx.x % kMgTX, w = threadIdx.x / kMgTX;
+    const int I = T.bx * kMgTX + lane, J = T.by * kMgTY + w;
+    const int Ic = min(I, nxc - 1), Jc = min(J, nyc - 1);   // list indices (clamped)
+    const bool act = I < nxc && J < nyc;
+    const bool edge = I == 0 || J == 0 || I >= nxc - 1 || J >= nyc - 1;
     const int chunk = (nzc + G.z - 1) / G.z;
     const int k0 = T.bz * chunk, k1 = min(nzc, k0 + chunk);
-    if (I >= nxc || J >= nyc) return;
-    const bool edge = I == 0 || J == 0 || I == nxc - 1 || J == nyc - 1;
     const int64_t sf = (int64_t)nxf * nyf, sc = (int64_t)nxc * nyc;
+    int base = rxi[3 * (T.bx * kMgTX)];   // the wave's first coarse node's first fine index
     int xi[3], yo[3];
     float xw[3], yw[3];
 #pragma unroll
     for (int q = 0; q < 3; ++q) {
-        xi[q] = rxi[3 * I + q];
-        xw[q] = rxw[3 * I + q];
-        yo[q] = nxf * ryi[3 * J + q];
-        yw[q] = ryw[3 * J + q];
+        xi[q] = rxi[3 * Ic + q];
+        xw[q] = rxw[3 * Ic + q];
+        yo[q] = nxf * ryi[3 * Jc + q];
+        yw[q] = ryw[3 * Jc + q];
     }
     // the z pairs of plane K (loaded one plane ahead)
     int zi[3] = {0, 0, 0};
@@ -320,12 +331,20 @@ __global__ __launch_bounds__(256) void k_mg_restrict(MgGrid G, int nxc, int nyc,
         }
     }
     // the list loads complete before the flag's test (rather than being moved after it)
-    asm volatile("" : "+v"(xi[0]), "+v"(xi[1]), "+v"(xi[2]), "+v"(xw[0]), "+v"(xw[1]), "+v"(xw[2]));
+    asm volatile("" : "+v"(xi[0]), "+v"(xi[1]), "+v"(xi[2]), "+v"(xw[0]), "+v"(xw[1]), "+v"(xw[2]), "+v"(base));
     asm volatile("" : "+v"(yo[0]), "+v"(yo[1]), "+v"(yo[2]), "+v"(yw[0]), "+v"(yw[1]), "+v"(yw[2]));
     asm volatile("" : "+v"(zi[0]), "+v"(zi[1]), "+v"(zi[2]), "+v"(zw[0]), "+v"(zw[1]), "+v"(zw[2]));
-    if (sk) return;
+    if (sk) return;   // the flag is the same for every block: no barrier is left waiting
+    int xo[3];
+#pragma unroll
+    for (int q = 0; q < 3; ++q) xo[q] = min(max(xi[q] - base, 0), kRSpan - 1);
+    // this lane's fine nodes of the span: base + lane, base + 64 + lane, and for lanes
+    // below kRSpan - 128 base + 128 + lane (clamped into the row)
+    const int f0 = min(base + lane, nxf - 1), f1 = min(base + kMgTX + lane, nxf - 1);
+    const int f2 = min(base + 2 * kMgTX + min(lane, kRSpan - 2 * kMgTX - 1), nxf - 1);
     for (int K = k0; K < k1; ++K) {
-        const int64_t idx = I + (int64_t)nxc * J + sc * K;
+        const int64_t idx = Ic + (int64_t)nxc * Jc + sc * K;
+        const float iwc = invwc[idx];
         int zin[3] = {0, 0, 0};
         float zwn[3] = {0.f, 0.f, 0.f};
         if (K + 1 < k1) {
@@ -335,30 +354,36 @@ __global__ __launch_bounds__(256) void k_mg_restrict(MgGrid G, int nxc, int nyc,
                 zwn[c] = rzw[3 * (K + 1) + c];
             }
         }
-        float s = 0.f;
-        if (!(edge || K == 0 || K == nzc - 1)) {
-            // all 27 loads at once (no branch between them: the kernel is latency-bound);
-            // unused pairs (index 0, weight 0) add exactly zero
-            float f[3][3][3];
+        float v[3][3][3];
 #pragma unroll
-            for (int c = 0; c < 3; ++c)
+        for (int c = 0; c < 3; ++c)
 #pragma unroll
-                for (int bq = 0; bq < 3; ++bq) {
-                    const float *py = rf + sf * zi[c] + yo[bq];
-#pragma unroll
-                    for (int q = 0; q < 3; ++q) f[c][bq][q] = py[xi[q]];
-                }
-#pragma unroll
-            for (int c = 0; c < 3; ++c) {
-                float sz = 0.f;
-#pragma unroll
-                for (int bq = 0; bq < 3; ++bq)
-                    sz += yw[bq] * (xw[0] * f[c][bq][0] + xw[1] * f[c][bq][1] + xw[2] * f[c][bq][2]);
-                s += zw[c] * sz;
+            for (int bq = 0; bq < 3; ++bq) {
+                const float *py = rf + sf * zi[c] + yo[bq];
+                v[c][bq][0] = py[f0];
+                v[c][bq][1] = py[f1];
+                v[c][bq][2] = py[f2];
             }
+        float g[3];
+#pragma unroll
+        for (int m = 0; m < 3; ++m) {
+            float sz = 0.f;
+#pragma unroll
+            for (int c = 0; c < 3; ++c) sz += zw[c] * (yw[0] * v[c][0][m] + yw[1] * v[c][1][m] + yw[2] * v[c][2][m]);
+            g[m] = sz;
         }
-        bc[idx] = s;
-        x1c[idx] = s * invwc[idx];
+        row[w][lane] = g[0];
+        row[w][kMgTX + lane] = g[1];
+        if (lane < kRSpan - 2 * kMgTX) row[w][2 * kMgTX + lane] = g[2];
+        __syncthreads();
+        float s = 0.f;
+        if (!(edge || K == 0 || K == nzc - 1))
+            s = xw[0] * row[w][xo[0]] + xw[1] * row[w][xo[1]] + xw[2] * row[w][xo[2]];
+        __syncthreads();   // the rows are rewritten for the next plane
+        if (act) {
+            bc[idx] = s;
+            x1c[idx] = s * iwc;
+        }
 #pragma unroll
         for (int c = 0; c < 3; ++c) {
             zi[c] = zin[c];
