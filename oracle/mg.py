@@ -7,9 +7,9 @@ intervals merge when both are at most 3 hmin, hmin the level's smallest spacing 
 three axes; down to at most 16 nodes per axis),
 the 7-point operator of each level's rectilinear grid with zero on the faces, linear
 interpolation and its transpose, Chebyshev-Richardson smoothing x += w_k (b - A x) / diag
-with w_k the reciprocal Chebyshev nodes of [2/15, 2] (reverse order after the coarse
-correction), the coarsest level a degree-12 polynomial on [2/100, 2] with its steps in
-Leja order.
+with w_k the reciprocal Chebyshev nodes of [2/8, 2] (degree 4) on the finest level and of
+[2/15, 2] (degree 3) on the others (reverse order after the coarse correction), the
+coarsest level a degree-12 polynomial on [2/100, 2] with its steps in Leja order.
 
 Pinned by: its own operator against the oracle's assembled K (interior rows), its
 symmetry, and V-cycle properties (exact on the coarsest level's polynomial, linearity);
@@ -120,7 +120,7 @@ def leja(om):
 
 
 class VCycle:
-    def __init__(self, xs, ys, zs, max_levels=8, deg=3, cdeg=12):
+    def __init__(self, xs, ys, zs, max_levels=8, deg=3, cdeg=12, deg0=4):
         self.levels, self.axes = [], []
         axes = [np.asarray(xs, float), np.asarray(ys, float), np.asarray(zs, float)]
         while True:
@@ -138,6 +138,7 @@ class VCycle:
                                interp(axes[0], keeps[0])).tocsr()
             axes = [a[k] for a, k in zip(axes, keeps)]
         self.om = omegas(deg, 2.0 / 15.0)
+        self.om0 = omegas(deg0, 2.0 / 8.0)   # the finest level's smoothing
         self.omc = leja(omegas(cdeg, 2.0 / 100.0))
 
     def _smooth(self, lev, x, b, oms):
@@ -151,9 +152,10 @@ class VCycle:
         r = np.where(lev["face"], 0.0, r)
         if l == len(self.levels) - 1:
             return self._smooth(lev, np.zeros_like(r), r, self.omc)
-        x = self._smooth(lev, np.zeros_like(r), r, self.om)
+        om = self.om0 if l == 0 else self.om
+        x = self._smooth(lev, np.zeros_like(r), r, om)
         rc = lev["P"].T @ (r - lev["A"] @ x)
         rc[self.levels[l + 1]["face"]] = 0.0
         x = x + lev["P"] @ self.apply(rc, l + 1)
         x[lev["face"]] = 0.0
-        return self._smooth(lev, x, r, self.om[::-1])
+        return self._smooth(lev, x, r, om[::-1])
