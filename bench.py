@@ -36,9 +36,9 @@ HBM_PEAK_GBS = 8000.0
 # copy of --traffic-probe (factors 0.5000 and 1.0000); see
 # profiles/pcg_kernels_{fp64,mixed_jacobi}_voronoi125.md and profiles/mg_vcycle_mixed_voronoi125.md.
 TRAFFIC = {
-    ("fp64", "spmv"): 132805198, ("fp64", "update"): 233822847,
-    ("mixed", "spmv"): 67655278, ("mixed", "update"): 116963418,
-    ("mixed", "mg_smooth"): 45772351,
+    ("fp64", "spmv"): 132804214, ("fp64", "update"): 233823056,
+    ("mixed", "spmv"): 67652230, ("mixed", "update"): 116963400,
+    ("mixed", "mg_smooth"): 45808470,
 }
 
 
@@ -186,8 +186,8 @@ def roofline(res, precision, args):
     # (timer, kernel, launches of that kernel per PCG iteration)
     for key, name, per_it in (("spmv", f"{kname}, {prec}", 1),
                               ("update", f"k_kb (x, r, z update, r.z and r.r partials), {prec}", 1),
-                              ("mg_smooth", "k_mg_stencil<SMOOTH> (level-0 Chebyshev smoothing step of the "
-                                            "multigrid V-cycle, 154^3), FP32 inner solve", 4)):
+                              ("mg_smooth", "k_mg_stencil<SMOOTH> (Chebyshev smoothing step of the multigrid "
+                                            "V-cycle's finest level), FP32 inner solve", 6)):
         if tm[f"{key}_launches"] > 0:
             avg_s = tm[f"{key}_ms"] / tm[f"{key}_launches"] / 1e3
             kernels.append(dict(key=key, kernel=name, total_ms=tm[f"{key}_ms"], avg_s=avg_s, per_it=per_it,
