@@ -1344,6 +1344,8 @@ int mm_multigrid(mm_ctx *c, int32_t max_levels) {
             for (int q = 0; q < 3 * nc; ++q)
                 if (t[a].ri[q] < 0 || t[a].ri[q] >= nf) return fail(MM_E_STATE, "multigrid: restriction index out of range");
         }
+        if (!mg_restrict_span_fits(t[0].ri.data(), t[0].rw.data(), (int)t[0].coarse.size()))
+            return fail(MM_E_STATE, "multigrid: an x restriction span exceeds the restriction kernel's staged row");
         MgLevelDev &f = c->mg[l];
         int *pci[3], *rii[3];
         float *ptf[3], *rwf[3];

@@ -68,6 +68,9 @@ void launch_mg_smooth2(const Launch &L, const PoissonGrid<float> &P, const float
 void launch_mg_restrict(const Launch &L, const MgLevelDev &f, const MgLevelDev &c, const float *rf, float *bc,
                         float *x1c, const int *skip);
 bool mg_stencil_fits(const PoissonGrid<float> &P);
+// the x restriction lists (ri, rw: 3 pairs per coarse index, nc coarse indices) keep every
+// used fine index of 64 consecutive coarse nodes inside the restriction kernel's staged row
+bool mg_restrict_span_fits(const int *ri, const float *rw, int nc);
 void launch_mg_prolong(const Launch &L, const MgLevelDev &f, const MgLevelDev &c, const float *xc, float *xf,
                        const int *skip);
 

@@ -858,6 +858,15 @@ void launch_mg_restrict(const Launch &L, const MgLevelDev &f, const MgLevelDev &
     hipLaunchKernelGGL(k_mg_restrict, blocks(g), dim3(256), 0, L.s, g, c.P.nx, c.P.ny, c.P.nz, f.P.nx,
                        f.P.ny, f.rxi, f.rxw, f.ryi, f.ryw, f.rzi, f.rzw, rf, bc, c.invw, x1c, skip);
 }
+bool mg_restrict_span_fits(const int *ri, const float *rw, int nc) {
+    for (int i0 = 0; i0 < nc; i0 += kMgTX) {
+        const int base = ri[3 * i0];
+        for (int I = i0; I < std::min(nc, i0 + kMgTX); ++I)
+            for (int q = 0; q < 3; ++q)
+                if (rw[3 * I + q] != 0.f && (ri[3 * I + q] < base || ri[3 * I + q] - base >= kRSpan)) return false;
+    }
+    return true;
+}
 void launch_mg_prolong(const Launch &L, const MgLevelDev &f, const MgLevelDev &c, const float *xc, float *xf,
                        const int *skip) {
     const MgGrid g = stencil_grid(f.P);
