@@ -133,3 +133,31 @@ def test_coarsest_steps_are_stable_in_fp32(grid):
     assert err(run(np.sort(V.omc), np.float32)) > 1e-4
     # the order does not change the polynomial
     assert err(run(np.sort(V.omc), np.float64)) < 1e-10
+
+
+def test_coarse_intervals_span_at_most_two_fine():
+    """The restriction kernel stages, for 64 consecutive coarse nodes, the fine nodes their
+    pairs use in one row of at most 132 (2 * 63 + 3 from the first coarse node's first
+    fine index): it relies on every coarse interval covering at most two fine ones, which
+    the coarsening rule gives (two adjacent intervals merge, never more).  Checked on the
+    level axes of the 125^3-voxel benchmark grid and of a graded 39-cell cube."""
+    from mammos_mumag_amd import generate
+    grids = (generate.build_config("voronoi64_250nm", seed=1),
+             generate.build(edge=39.0, n=39, layers=13, grains=2, cone=15.0, seed=3))
+    for g in grids:
+        axes = [np.asarray(g.xs, float), np.asarray(g.ys, float), np.asarray(g.zs, float)]
+        levels = 1
+        while max(len(a) for a in axes) > mg.COARSEST:
+            hmin = min(np.diff(a).min() for a in axes)
+            keeps = [mg.coarsen(a, hmin) for a in axes]
+            if all(len(k) == len(a) for k, a in zip(keeps, axes)):
+                break
+            for k, a in zip(keeps, axes):
+                assert k[0] == 0 and k[-1] == len(a) - 1
+                assert np.diff(k).min() >= 1 and np.diff(k).max() <= 2
+                # 64 consecutive coarse nodes and their outer neighbours' dropped nodes
+                span = max(k[min(i + 63, len(k) - 1)] - k[i] for i in range(len(k)))
+                assert span + 3 <= 132
+            axes = [a[k] for a, k in zip(axes, keeps)]
+            levels += 1
+        assert levels >= 3
