@@ -107,7 +107,9 @@ typedef struct {
     int64_t pcg_calls;
     int64_t tcg_iters;        /* truncated-CG iterations inside M_inv */
     int64_t c_applies;        /* exchange/anisotropy operator applications */
-    int64_t backtracks;       /* line-search halvings */
+    int64_t backtracks;       /* line-search halvings (minimizers.py:121-127) */
+    int64_t restarts_g;       /* direction reset to -g before the line search (minimizers.py:171-173) */
+    int64_t restarts_z;       /* direction reset to -z after the HS update (minimizers.py:182-184) */
 } mm_stats;
 
 const char *mm_last_error(void);

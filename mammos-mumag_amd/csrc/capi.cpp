@@ -1213,8 +1213,10 @@ int mm_minimize(mm_ctx *c, double *m_inout, double hext, const double hdir[3],
         // restart if d is not a sufficient descent direction (minimizers.py:171-173)
         double dg3[3];
         if ((rc = reduce3(c, RED_DG3, D, G, nullptr, nullptr, dg3, 3))) return rc;
-        if (dg3[0] > -0.001 * std::sqrt(dg3[1]) * std::sqrt(dg3[2]))
+        if (dg3[0] > -0.001 * std::sqrt(dg3[1]) * std::sqrt(dg3[2])) {
             launch_axpby(L, n3, -1.0, G, 0.0, nullptr, D, blk(D));
+            loc.restarts_g++;
+        }
         // ---- line search (minimizers.py:92-137)
         double f1 = 0.0;
         {
@@ -1263,8 +1265,10 @@ int mm_minimize(mm_ctx *c, double *m_inout, double hext, const double hdir[3],
         const double beta = std::max(yzd[0] / yzd[1], 0.0);
         launch_axpby(L, n3, -1.0, Z1, beta, D, D, blk(D));
         if ((rc = reduce3(c, RED_DG3, D, G, nullptr, nullptr, dg3, 3))) return rc;
-        if (dg3[0] > -0.001 * std::sqrt(dg3[1]) * std::sqrt(dg3[2]))
+        if (dg3[0] > -0.001 * std::sqrt(dg3[1]) * std::sqrt(dg3[2])) {
             launch_axpby(L, n3, -1.0, Z1, 0.0, nullptr, D, blk(D));
+            loc.restarts_z++;
+        }
         // shift state: x0 <- x, x <- x1, f0 <- f, f <- f1, g <- g1
         std::swap(X0, X);
         std::swap(X, X1);
@@ -1286,6 +1290,8 @@ int mm_minimize(mm_ctx *c, double *m_inout, double hext, const double hdir[3],
         st->tcg_iters += loc.tcg_iters;
         st->c_applies += loc.c_applies;
         st->backtracks += loc.backtracks;
+        st->restarts_g += loc.restarts_g;
+        st->restarts_z += loc.restarts_z;
     }
     // keep the rotated buffers for the next call
     c->X = X;

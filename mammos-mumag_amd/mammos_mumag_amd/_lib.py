@@ -61,7 +61,8 @@ class Stats(ctypes.Structure):
     _fields_ = [("ncg_iters", ctypes.c_int64), ("fcalls", ctypes.c_int64),
                 ("pcg_iters", ctypes.c_int64), ("pcg_calls", ctypes.c_int64),
                 ("tcg_iters", ctypes.c_int64), ("c_applies", ctypes.c_int64),
-                ("backtracks", ctypes.c_int64)]
+                ("backtracks", ctypes.c_int64), ("restarts_g", ctypes.c_int64),
+                ("restarts_z", ctypes.c_int64)]
 
     def as_dict(self):
         return {k: int(getattr(self, k)) for k, _ in self._fields_}

@@ -49,6 +49,50 @@ def boundary_setup(coords, tets, elem_tags, tags):
     return dict(airbox=False, dirichlet=bmask, shell=shell, R=R, Rinf=Rinf)
 
 
+
+def initial_state(coords, magnet, v, state_id):
+    """getM (scripts/magnetization.py:54-71): the state times the magnet mask
+    (wherePositive(meas), mapping.py:196), then escript's normalize (zero vectors stay 0).
+
+    state_id 0 uniform v; 1 flower (getFlower, :24-33); 2 vortex about the x axis
+    (getVortex, :13-21); 3 twisted (getTwisted, :36-51).  Points where a formula divides
+    0 by 0 (the vortex's or the twisted state's axis) take the limit 0 for that term.
+    """
+    x = np.asarray(coords, dtype=np.float64)
+    n = len(x)
+    if state_id == 4:
+        raise ValueError("state 'random' uses escript's RandomData generator, which has no "
+                         "reproducible counterpart here")
+    m = np.zeros((n, 3))
+    if state_id == 0:
+        m[:] = v
+    with np.errstate(divide="ignore", invalid="ignore"):
+        if state_id == 2:  # vortex
+            r = np.sqrt(x[:, 1] ** 2 + x[:, 2] ** 2)
+            rc = 0.14 * r.max()
+            m[:, 0] = np.exp(-2 * (r / rc))
+            m[:, 1] = -(x[:, 2] / r) * np.sqrt(1 - np.exp(-4 * (r * r) / (rc * rc)))
+            m[:, 2] = (x[:, 1] / r) * np.sqrt(1 - np.exp(-4 * (r * r) / (rc * rc)))
+        elif state_id == 1:  # flower
+            s = x.max()
+            m[:, 0] = (x[:, 0] * x[:, 2]) / (10 * s)
+            m[:, 1] = (x[:, 1] * x[:, 2]) / (10 * s)
+            m[:, 2] = 1.0
+        elif state_id == 3:  # twisted
+            r = np.sqrt(x[:, 1] ** 2 + x[:, 0] ** 2)
+            s = x.max()
+            pz, nz = (x[:, 2] > 0).astype(float), (x[:, 2] < 0).astype(float)
+            m[:, 0] = (x[:, 0] * x[:, 2]) / (10 * s) + 4 * (
+                pz * (-x[:, 1] / r) * (x[:, 2] / s) + nz * (x[:, 1] / r) * (-x[:, 2] / s))
+            m[:, 1] = (x[:, 1] * x[:, 2]) / (10 * s) + 4 * (
+                pz * (x[:, 0] / r) * (x[:, 2] / s) + nz * (-x[:, 0] / r) * (-x[:, 2] / s))
+            m[:, 2] = 1.0
+    m = np.nan_to_num(m)
+    m[~np.asarray(magnet, dtype=bool)] = 0.0
+    ln = np.linalg.norm(m, axis=1)
+    z = (ln == 0).astype(np.float64)
+    return (m * ((1.0 - z) / (ln + z))[:, None]).ravel()
+
 def _torch():
     import torch
     if not torch.cuda.is_available():
@@ -195,40 +239,8 @@ class Model:
         return (m * ((1.0 - z) / (ln + z))[:, None]).ravel()
 
     def state_m(self, v, state_id):
-        """Initial states of scripts/magnetization.py:9-71 (flower, vortex, twisted, uniform)."""
-        x = self.coords
-        if state_id == 4:
-            raise ValueError("state 'random' uses escript's RandomData generator, which has no "
-                             "reproducible counterpart here")
-        if state_id == 0:
-            return self.uniform_m(v)
-        m = np.zeros((self.N, 3))
-        with np.errstate(divide="ignore", invalid="ignore"):
-            if state_id == 2:  # vortex
-                r = np.sqrt(x[:, 1] ** 2 + x[:, 2] ** 2)
-                rc = 0.14 * r.max()
-                m[:, 0] = np.exp(-2 * (r / rc))
-                m[:, 1] = -(x[:, 2] / r) * np.sqrt(1 - np.exp(-4 * (r * r) / (rc * rc)))
-                m[:, 2] = (x[:, 1] / r) * np.sqrt(1 - np.exp(-4 * (r * r) / (rc * rc)))
-            elif state_id == 1:  # flower
-                s = x.max()
-                m[:, 0] = (x[:, 0] * x[:, 2]) / (10 * s)
-                m[:, 1] = (x[:, 1] * x[:, 2]) / (10 * s)
-                m[:, 2] = 1.0
-            elif state_id == 3:  # twisted
-                r = np.sqrt(x[:, 1] ** 2 + x[:, 0] ** 2)
-                s = x.max()
-                pz, nz = (x[:, 2] > 0).astype(float), (x[:, 2] < 0).astype(float)
-                m[:, 0] = (x[:, 0] * x[:, 2]) / (10 * s) + 4 * (
-                    pz * (-x[:, 1] / r) * (x[:, 2] / s) + nz * (x[:, 1] / r) * (-x[:, 2] / s))
-                m[:, 1] = (x[:, 1] * x[:, 2]) / (10 * s) + 4 * (
-                    pz * (x[:, 0] / r) * (x[:, 2] / s) + nz * (-x[:, 0] / r) * (-x[:, 2] / s))
-                m[:, 2] = 1.0
-        m = np.nan_to_num(m)
-        m[~self.magnet_mask()] = 0.0
-        ln = np.linalg.norm(m, axis=1)
-        z = (ln == 0).astype(np.float64)
-        return (m * ((1.0 - z) / (ln + z))[:, None]).ravel()
+        """Initial states of scripts/magnetization.py:54-71 getM (uniform, flower, vortex, twisted)."""
+        return initial_state(self.coords, self.magnet_mask(), v, state_id)
 
     def vortex_xm(self):
         """xM of scripts/magnetization.py:73-82 with scale = V**(1/3)."""

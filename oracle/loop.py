@@ -54,7 +54,8 @@ class Minimizer:
         return z
 
     def minimize(self, m, hext):
-        f, x, k = hestenes_stiefel_ncg(m, self.func(hext), self.M_inv, self.p["tol_fun"], update_m)
+        f, x, k = hestenes_stiefel_ncg(m, self.func(hext), self.M_inv, self.p["tol_fun"], update_m,
+                                       stats=self.stats)
         self.stats["ncg_iters"] = self.stats.get("ncg_iters", 0) + k
         return f, x, k
 

@@ -45,7 +45,7 @@ def alpha_init(f0, f, g, d, g1, h, c=0.1, o=0.99):
     return term1 if term1 < o * term2 else term2
 
 
-def line_search(x, f0, f, g, d, func, update_x=update_m):
+def line_search(x, f0, f, g, d, func, update_x=update_m, stats=None):
     rho, c, delta, eps, max_iter = 0.5, 1e-4, 0.1, 1e-6, 20
     h = compute_h(x, d)
     x1 = update_x(x, h, d)
@@ -67,11 +67,17 @@ def line_search(x, f0, f, g, d, func, update_x=update_m):
         f1, g1 = func(x1)
         gd1 = float(np.dot(g1, d))
         i += 1
+    if stats is not None:
+        stats["backtracks"] = stats.get("backtracks", 0) + i
     return x1, f1, g1
 
 
-def hestenes_stiefel_ncg(x, func, M_inv, tol, update_x=update_m, iter_max=100000):
-    """Returns (f, x, k)."""
+def hestenes_stiefel_ncg(x, func, M_inv, tol, update_x=update_m, iter_max=100000, stats=None):
+    """Returns (f, x, k).  ``stats`` (optional dict) counts the line-search halvings
+    (``backtracks``) and the two descent restarts: to -g before the line search
+    (``restarts_g``, minimizers.py:171-173) and to -z1 after the HS update
+    (``restarts_z``, minimizers.py:182-184)."""
+    st = stats if stats is not None else {}
     f0 = FMAX
     f, g = func(x)
     d = -M_inv(g)
@@ -85,12 +91,14 @@ def hestenes_stiefel_ncg(x, func, M_inv, tol, update_x=update_m, iter_max=100000
             break
         if np.dot(d, g) > -0.001 * np.linalg.norm(d) * np.linalg.norm(g):
             d = -g
-        x1, f1, g1 = line_search(x, f0, f, g, d, func, update_x)
+            st["restarts_g"] = st.get("restarts_g", 0) + 1
+        x1, f1, g1 = line_search(x, f0, f, g, d, func, update_x, st)
         z1 = M_inv(g1)
         y = g1 - g
         beta = max(float(np.dot(y, z1)) / float(np.dot(y, d)), 0.0)
         d = -z1 + beta * d
         if np.dot(d, g) > -0.001 * np.linalg.norm(d) * np.linalg.norm(g):
             d = -z1
+            st["restarts_z"] = st.get("restarts_z", 0) + 1
         k, x0, f0, x, f, g = k + 1, x, f, x1, f1, g1
     return f, x, k
