@@ -156,6 +156,16 @@ int mm_update_m(mm_ctx *ctx, const double *x, double alpha, const double *d, dou
 int mm_dot(mm_ctx *ctx, const double *a, const double *b, int64_t n, double *out_host);
 int mm_norm_inf(mm_ctx *ctx, const double *a, int64_t n, double *out_host);
 
+/* mm_dot without a context, for callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc); it has no
+ * counterpart in the reference.  *out_host = a.b over n doubles on `device`, with the
+ * same kernels and fixed summation order as mm_dot (bit-identical result).  It allocates
+ * nothing: its partials and result live in a static device array of 16.3 KB per device,
+ * enough for every n >= 0 (one partial per block, at most 2048 blocks).  Runs on the
+ * device's null stream and returns after the result is on the host.  Calls that overlap
+ * on one device share that array and race. */
+int mm_dot_noalloc(int32_t device, const double *a, const double *b, int64_t n, double *out_host);
+
 /* M.h = sum_i (h . m_i) meas_i / V in Tesla (loop.py:25-38 compute_mh). */
 int mm_compute_mh(mm_ctx *ctx, const double *m, const double hdir[3], double *mh_host);
 

@@ -11,6 +11,7 @@
 #include <cfloat>
 #include <cmath>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -1097,6 +1098,62 @@ int mm_update_m(mm_ctx *c, const double *x, double alpha, const double *d, doubl
 int mm_dot(mm_ctx *c, const double *a, const double *b, int64_t n, double *out) {
     if (!c || !a || !b || !out || n < 0) return fail(MM_E_ARG, "bad argument");
     return reduce(c, RED_DOT, n, a, b, nullptr, nullptr, out, 1);
+}
+
+// ---------------------------------------------------------------- mm_dot_noalloc
+// mm_dot for callers that cannot allocate (no context): its reduction keeps the per-block
+// partials and the result in this file-scope device array instead of a context's
+// workspace.  k_reduce runs at most kMaxPartials blocks (one partial each), so every
+// n >= 0 fits; the array is 2080 doubles (16.3 KB) per device, held until the process
+// ends.  The array's device address is looked up once per device (hipGetSymbolAddress,
+// with that device current) and only that address is handed to kernels and copies.
+static __device__ __align__(256) double scratch[kMaxPartials + 32];
+
+namespace {
+constexpr int kNoAllocMaxDevices = 64;
+constexpr int kNoAllocResult = kMaxPartials;   // result slot after the partials
+std::mutex g_scratch_mu;
+double *g_scratch_ptr[kNoAllocMaxDevices] = {};
+
+// The device address of `scratch` on the current device (set-up on its first use).
+int scratch_ptr(int device, double **out) {
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[device]) {
+        void *p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p)
+            return fail(MM_E_HIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+        g_scratch_ptr[device] = static_cast<double *>(p);
+    }
+    *out = g_scratch_ptr[device];
+    return 0;
+}
+}  // namespace
+
+int mm_dot_noalloc(int32_t device, const double *a, const double *b, int64_t n, double *out_host) {
+    if (!out_host || n < 0 || (n > 0 && (!a || !b))) return fail(MM_E_ARG, "bad argument");
+    int count = 0;
+    HIPC(hipGetDeviceCount(&count));
+    if (device < 0 || device >= count || device >= kNoAllocMaxDevices)
+        return fail(MM_E_ARG, "device ordinal out of range");
+    int prev = 0;
+    HIPC(hipGetDevice(&prev));
+    if (prev != device) HIPC(hipSetDevice(device));
+    double *ws = nullptr;
+    int rc = scratch_ptr(device, &ws);
+    hipError_t e = hipSuccess;
+    if (!rc) {
+        Launch L{};
+        L.s = nullptr;   // the device's null stream: ordered after the caller's default-stream work
+        L.red.partials = ws;
+        launch_reduce(L, RED_DOT, n, a, b, nullptr, nullptr, 0, 0, 0, ws + kNoAllocResult);
+        e = hipGetLastError();
+        if (e == hipSuccess) e = hipMemcpy(out_host, ws + kNoAllocResult, sizeof(double), hipMemcpyDeviceToHost);
+    }
+    if (prev != device) hipSetDevice(prev);
+    if (rc) return rc;
+    if (e != hipSuccess) return fail(MM_E_HIP, std::string("mm_dot_noalloc: ") + hipGetErrorString(e));
+    return 0;
 }
 
 int mm_norm_inf(mm_ctx *c, const double *a, int64_t n, double *out) {

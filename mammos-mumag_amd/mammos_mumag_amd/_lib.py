@@ -96,6 +96,7 @@ SIGNATURES = {
     "mm_update_m": [VP, VP, ctypes.c_double, VP, VP],
     "mm_dot": [VP, VP, VP, ctypes.c_int64, c_double_p],
     "mm_norm_inf": [VP, VP, ctypes.c_int64, c_double_p],
+    "mm_dot_noalloc": [ctypes.c_int32, VP, VP, ctypes.c_int64, c_double_p],
     "mm_compute_mh": [VP, VP, D3, c_double_p],
     "mm_minimize": [VP, VP, ctypes.c_double, D3, ctypes.POINTER(MinParams), VP, VP, VP,
                     c_double_p, ctypes.POINTER(Stats)],
