@@ -115,15 +115,22 @@ std::string assemble(const mm_mesh_desc &d, double drop_tol, HostOps &o, bool wa
     for (int e = 0; e < NE; ++e)
         if (d.elem_mat[e] < 0 || d.elem_mat[e] >= d.n_mats) return "element material index out of range";
 
-    // Volume V (scripts/materials.py:36-37) in element order.
-    double V = 0.0;
+    // Volume V (scripts/materials.py:36-37), a compensated (Neumaier) sum in element
+    // order: a plain running sum of the 11.7 M equal volumes of a 125^3-voxel magnet is off
+    // by 1.7e-10 relative, which every C and G entry (scaled by 1 / V) would carry.
+    double V = 0.0, Vc = 0.0;
     std::vector<uint8_t> magnetic(NE);
     for (int e = 0; e < NE; ++e) {
         Geo g;
         if (!geometry(X, T + 4 * (int64_t)e, g)) return "degenerate element " + std::to_string(e);
         magnetic[e] = d.mat_Js[d.elem_mat[e]] > 0.0;
-        if (magnetic[e]) V += g.vol;
+        if (magnetic[e]) {
+            const double t = V + g.vol;
+            Vc += std::fabs(V) >= std::fabs(g.vol) ? (V - t) + g.vol : (g.vol - t) + V;
+            V = t;
+        }
     }
+    V += Vc;
     if (!(V > 0.0)) return "no magnetic element (all Js <= 0)";
     o.V = V;
 

@@ -71,16 +71,32 @@ def t3d_tensor(xq, R, Rinf):
     return k
 
 
+def stiffness_local(vol, G):
+    """Element stiffness vol grad phi_k . grad phi_l of the P1 Laplacian (k = I), [NE, 4, 4]."""
+    return vol[:, None, None] * np.einsum("eka,ela->ekl", G, G)
+
+
+def laplacian(coords, tets):
+    """Assembled P1 Laplacian with no boundary rows replaced (the air-box K before
+    scripts/matrix.py:61-70 imposes the Dirichlet rows and columns)."""
+    vol, G = element_geometry(coords, tets)
+    return _coo_scalar(tets, stiffness_local(vol, G), len(coords))
+
+
 class Model:
     """Mesh + materials + operators of one model directory entry ``name``."""
 
-    def __init__(self, coords, tets, elem_tags, krn):
+    def __init__(self, coords, tets, elem_tags, krn, V=None, poisson=True):
+        """``V`` (optional) replaces the magnet volume that scales C and G, and
+        ``poisson=False`` skips K: a model of a sub-block of a larger mesh then holds that
+        mesh's C, D and G rows for the nodes whose elements all lie in the sub-block."""
         self.coords, self.tets, self.elem_tags = coords, tets, elem_tags
         self.N = len(coords)
         self.tags = sorted(set(int(t) for t in np.unique(elem_tags)))
         self.krn = krn
+        self._V = V
         self._materials()
-        self._assemble()
+        self._assemble(poisson)
 
     @classmethod
     def from_files(cls, base):
@@ -105,12 +121,12 @@ class Model:
                 self.Js[sel] = mat["Js"]
                 self.A[sel] = MU0_SCRIPT * mat["A"] / (1.0e-9 * 1.0e-9)  # size = 1e-9 (scripts/materials.py:13)
 
-    def _assemble(self):
+    def _assemble(self, poisson=True):
         N, tets = self.N, self.tets
         vol, G = element_geometry(self.coords, tets)
         self.vol, self.G = vol, G
         mag = self.Js > 0
-        self.V = float(vol[mag].sum())
+        self.V = float(vol[mag].sum()) if self._V is None else float(self._V)
         V = self.V
         # meas (escript_tools.py:13-17)
         self.meas = np.bincount(tets.ravel(), weights=np.repeat(self.Js * vol / 4.0, 4), minlength=N)
@@ -145,7 +161,8 @@ class Model:
             # loc[e, k, l] = Js vol/4 d_a phi_k   (row k test, column l trial)
             self.Dxyz.append(_coo_scalar(tm, loc, N))
         self.Gxyz = [(Dm.T / V).tocsr() for Dm in self.Dxyz]
-        self._poisson(vol, G)
+        if poisson:
+            self._poisson(vol, G)
 
     def _poisson(self, vol, G):
         """scripts/matrix.py:49-70 and trafo.py:4-45."""
@@ -155,8 +172,7 @@ class Model:
         bmask = where_zero(r - Rinf)
         self.airbox = abs(int(bmask.sum()) - 8) < 1e-8
         N, tets = self.N, self.tets
-        GG = np.einsum("eka,ela->ekl", G, G)
-        Kloc = vol[:, None, None] * GG
+        Kloc = stiffness_local(vol, G)
         if self.airbox:
             sup = x.max(axis=0)
             dmask = np.zeros(N, dtype=bool)

@@ -1,0 +1,385 @@
+"""The benchmark workloads at their full sizes on the GPU (structured Kuhn-grid path).
+
+* cube40_single (16^3 voxels, 24,389 nodes): the whole +8 T -> -8 T loop (161 steps) in
+  FP64 against the oracle's loop (tests/golden/config0_loop.json): M.h to 1e-6 (the M(H)
+  bound) and E to 1e-10 relative or within the oracle's own rounding spread, the same
+  reversal step; the chosen form (FP32 inner solve with the multigrid preconditioner,
+  FP64 outer) against FP64: reversal within one field step, |M - M_fp64| / Ms <= 1e-3.
+* voronoi8_100nm (60^3 voxels, 571,787 nodes): every operator, one total_eg and one
+  M_inv against the oracle assembled on the whole mesh.
+* voronoi64_250nm (125^3 voxels, 3,652,264 nodes, the benchmark): operator rows against
+  the oracle on voxel sub-blocks at the grid's far corner, the magnet's corners and its
+  centre; the structured kernels against the general (SELL / element-loop) kernels on the
+  whole mesh; closed forms (anisotropy energy and M.h of a uniform state, D / G
+  adjointness, K symmetry and Dirichlet rows, unit length after retraction); three field
+  steps of the chosen form against FP64 (M.h and E to 1e-6).
+* the learned workload (the benchmark geometry with a microstructure the network never
+  saw): the shipped network's warm start leaves the answer unchanged.
+* voronoi512_500nm (250^3 voxels, 22.7 M nodes, 134.6 M tets): the sub-block rows and
+  closed forms on the FP64 and mixed contexts, one potential solve of each form from zero
+  against the other, one field step of the chosen form.
+"""
+import json
+
+import numpy as np
+import pytest
+
+from gridsub import dirichlet_mask, magnet_volume_and_moment, submesh, uniform_anisotropy_energy
+
+pytestmark = pytest.mark.gpu
+
+
+def rel(a, b):
+    a, b = np.asarray(a), np.asarray(b)
+    return float(np.max(np.abs(a - b)) / max(np.max(np.abs(b)), 1e-300))
+
+
+def _free(*models):
+    import torch
+    for m in models:
+        m.close()
+    torch.cuda.empty_cache()
+
+
+def _loop(model, params, mg=False):
+    from mammos_mumag_amd.loop import run_loop
+    if mg:
+        model.multigrid(8)
+    steps = []
+    rec, stats, _ = run_loop(model, params, model.uniform_m(params["m"]), save=False,
+                             on_step=lambda i, h, mh, E, st, dt: steps.append(dict(st)))
+    return np.array(rec), steps
+
+
+# ---------------------------------------------------------------- cube40_single
+
+def test_config0_full_loop(golden):
+    from mammos_mumag_amd import Model, generate
+    ref = json.loads((golden / "config0_loop.json").read_text())
+    sp = ref["spread"]
+    mesh = generate.build_config("cube40_single")
+    params = generate.loop_params("cube40_single")
+    print("[gpu] cube40_single FP64 loop", flush=True)
+    fp64 = Model.from_structured(mesh)
+    rec, steps = _loop(fp64, params)
+    _free(fp64)
+    oref = np.array(ref["records"])
+    dmh = float(np.max(np.abs(rec[:, 2] - oref[:, 2])))
+    dE = float(np.max(np.abs(rec[:, 3] / oref[:, 3] - 1)))
+    sw, sw_ref = int(np.argmax(rec[:, 2] < 0)), int(np.argmax(oref[:, 2] < 0))
+    print(f"[gpu] fp64: {len(rec)} steps, reversal at step {sw} (oracle {sw_ref}), |dM.h| {dmh:.2e}, "
+          f"dE/E {dE:.2e}, counters {steps[-1]}", flush=True)
+    assert rec.shape == oref.shape
+    assert np.array_equal(rec[:, 0], oref[:, 0])
+    assert np.max(np.abs(rec[:, 1] - oref[:, 1])) < 1e-12
+    assert sw == sw_ref
+    assert dmh <= max(1e-6, 3 * sp["mh"])
+    assert dE <= max(1e-10, 3 * sp["E_rel"])
+    last = steps[-1]
+    for k in ("backtracks", "restarts_g", "restarts_z"):
+        assert last[k] == ref["counters"][-1][k]
+    lo, hi = sp["ncg_iters"]
+    assert 0.95 * lo <= last["ncg_iters"] <= 1.05 * hi
+
+    print("[gpu] cube40_single chosen form (mixed + multigrid) loop", flush=True)
+    mixed = Model.from_structured(mesh, precision="mixed")
+    mrec, msteps = _loop(mixed, params, mg=True)
+    _free(mixed)
+    Js = 1.61
+    dM = float(np.max(np.abs(mrec[:, 2] - rec[:, 2])) / Js)
+    msw = int(np.argmax(mrec[:, 2] < 0))
+    print(f"[gpu] mixed: reversal at step {msw}, max |dM|/Ms {dM:.2e}, PCG iterations "
+          f"{msteps[-1]['pcg_iters']} vs {steps[-1]['pcg_iters']}", flush=True)
+    assert mrec.shape == rec.shape
+    assert abs(msw - sw) <= 1
+    assert dM <= 1e-3
+
+
+# ---------------------------------------------------------------- voronoi8_100nm
+
+def test_config1_against_oracle():
+    from mammos_mumag_amd import Model, generate
+    from oracle.assemble import Model as OracleModel
+    from oracle.energies import total_eg
+    from oracle.loop import Minimizer
+    mesh = generate.build_config("voronoi8_100nm")
+    print("[gpu] voronoi8_100nm: oracle assembly", flush=True)
+    O = OracleModel(mesh.coords(), mesh.tets(), mesh.elem_tags(), mesh.krn())
+    gpu = Model.from_structured(mesh)
+    assert gpu.N == O.N == 571_787
+    assert abs(gpu.V / O.V - 1) < 1e-13 and rel(gpu.meas, O.meas) < 1e-13
+    rng = np.random.default_rng(5)
+    v = rng.normal(size=3 * O.N)
+    x = rng.normal(size=O.N)
+    assert rel(gpu.apply_C(gpu.to_device(v)).cpu().numpy(), O.C @ v) < 1e-12
+    assert rel(gpu.apply_K(gpu.to_device(x)).cpu().numpy(), O.K @ x) < 1e-12
+    b = gpu.divergence(gpu.to_device(v)).cpu().numpy()
+    assert rel(b, O.Dxyz[0] @ v[0::3] + O.Dxyz[1] @ v[1::3] + O.Dxyz[2] @ v[2::3]) < 1e-12
+    g = gpu.gradient(gpu.to_device(x)).cpu().numpy()
+    assert rel(g, np.stack([O.Gxyz[a] @ x for a in range(3)], axis=1).ravel()) < 1e-12
+    print("[gpu] voronoi8_100nm: operators match; total_eg and M_inv", flush=True)
+    p = generate.loop_params("voronoi8_100nm")
+    m = O.uniform_m([0.1, 0.2, 0.97]) + 0.05 * rng.normal(size=3 * O.N) * np.repeat(O.meas > 0, 3)
+    mm = m.reshape(-1, 3)
+    ln = np.linalg.norm(mm, axis=1)
+    m = (mm / np.where(ln == 0, 1, ln)[:, None]).ravel()
+    E_ref, gp_ref, u_ref, gf_ref = total_eg(O, m, 1e-10, 2.0, p["h"], 1, np.zeros(O.N))
+    u = gpu.empty1()
+    E, gp, gf, it = gpu.total_eg(gpu.to_device(m), 2.0, p["h"], 1e-10, 1, u)
+    print(f"[gpu] E {E!r} oracle {E_ref!r}, PCG iterations {it}", flush=True)
+    assert abs(E / E_ref - 1) < 1e-10
+    assert rel(gf.cpu().numpy(), gf_ref) < 1e-9
+    assert rel(gp.cpu().numpy(), gp_ref) < 1e-9
+    assert rel(u.cpu().numpy(), u_ref) < 1e-9
+    m0 = O.uniform_m(p["m"])
+    mini = Minimizer(O, p, m0)
+    mini.gF = gf_ref
+    z_ref = mini.M_inv(gp_ref)
+    z, _ = gpu.M_inv(gpu.to_device(m0), gpu.to_device(gf_ref), gpu.to_device(gp_ref), p["precond_iter"])
+    assert rel(z.cpu().numpy(), z_ref) < 1e-8
+    _free(gpu)
+
+
+# ---------------------------------------------------------------- sub-block rows + closed forms
+
+def _blocks(mesh):
+    """Voxel sub-blocks (lo, hi) of 8^3: the grid's far corner, the magnet's first and last
+    corners (half in the air), the magnet's centre."""
+    nv = len(mesh.xs) - 1
+    (m0, mn), = {b for b in mesh.magnet_box}
+    c = m0 + mn // 2
+    corners = [(nv - 8,) * 3, (m0 - 4,) * 3, (m0 + mn - 4,) * 3, (c - 4, c - 4, c - 4), (m0 + mn - 4, m0 - 4, c)]
+    return [(lo, tuple(l + 8 for l in lo)) for lo in corners]
+
+
+def _check_subblocks(gpu, mesh, tol, rng):
+    import oracle.assemble as oa
+    N = gpu.N
+    v = rng.normal(size=3 * N)
+    x = rng.normal(size=N)
+    dmask = dirichlet_mask(mesh)
+    xd = np.where(dmask, 0.0, x)
+    Cv = gpu.apply_C(gpu.to_device(v)).cpu().numpy()
+    Dv = gpu.divergence(gpu.to_device(v)).cpu().numpy()
+    Gx = gpu.gradient(gpu.to_device(x)).cpu().numpy()
+    Kx = gpu.apply_K(gpu.to_device(x)).cpu().numpy()
+    assert np.array_equal(Kx[dmask], x[dmask])                      # identity Dirichlet rows
+    for lo, hi in _blocks(mesh):
+        sub, full, inner = submesh(mesh, lo, hi)
+        coords, tets, tags = sub.coords(), sub.tets(), sub.elem_tags()
+        rows = full[inner]
+        L = oa.laplacian(coords, tets)
+        ok = inner & ~dmask[full]
+        ref = (L @ xd[full])[ok]
+        assert rel(Kx[full[ok]], ref) < tol, (lo, "K")
+        if not any(sub.tags_props[t]["Js"] > 0 for t in np.unique(tags).tolist()):
+            assert np.all(Cv.reshape(-1, 3)[rows] == 0) and np.all(Dv[rows] == 0)
+            continue
+        O = oa.Model(coords, tets, tags, sub.krn(), V=gpu.V, poisson=False)
+        vs = v.reshape(-1, 3)[full].ravel()
+        c_ref = (O.C @ vs).reshape(-1, 3)[inner]
+        d_ref = (O.Dxyz[0] @ vs[0::3] + O.Dxyz[1] @ vs[1::3] + O.Dxyz[2] @ vs[2::3])[inner]
+        g_ref = np.stack([O.Gxyz[a] @ x[full] for a in range(3)], axis=1)[inner]
+        scale_c = max(np.max(np.abs(c_ref)), 1e-300)
+        assert np.max(np.abs(Cv.reshape(-1, 3)[rows] - c_ref)) < tol * scale_c, (lo, "C")
+        assert rel(Dv[rows], d_ref) < tol, (lo, "D")
+        assert rel(Gx.reshape(-1, 3)[rows], g_ref) < tol, (lo, "G")
+    return v, x
+
+
+def _check_closed_forms(gpu, mesh, tol, rng):
+    import torch
+    N = gpu.N
+    V, moment = magnet_volume_and_moment(mesh)
+    assert abs(gpu.V / V - 1) < 1e-12
+    hdir = np.array([0.01745, 0.0, 0.99984]) / np.linalg.norm([0.01745, 0.0, 0.99984])
+    for mvec in ([0.0, 0.0, 1.0], [0.6, 0.0, 0.8]):
+        m = gpu.to_device(gpu.uniform_m(mvec))
+        E, _ = gpu.exani_eg(m)
+        E_ref = uniform_anisotropy_energy(mesh, np.array(mvec), gpu.V)
+        assert abs(E / E_ref - 1) < tol, (mvec, E, E_ref)
+        assert abs(gpu.compute_mh(m, hdir) / (float(np.dot(hdir, mvec)) * moment / V) - 1) < 1e-12
+    # u . D m = V m . G u (G = D^T / V);  x . K y = y . K x
+    m = torch.randn(3 * N, dtype=torch.float64, device="cuda")
+    u = torch.randn(N, dtype=torch.float64, device="cuda")
+    y = torch.randn(N, dtype=torch.float64, device="cuda")
+    Dm = gpu.divergence(m)
+    Gu = gpu.gradient(u)
+    a, b = float(torch.dot(u, Dm)), gpu.V * float(torch.dot(m, Gu))
+    assert abs(a - b) < tol * float(torch.dot(u.abs(), Dm.abs())), (a, b)
+    Ku, Ky = gpu.apply_K(u), gpu.apply_K(y)
+    a, b = float(torch.dot(y, Ku)), float(torch.dot(u, Ky))
+    assert abs(a - b) < 1e-12 * float(torch.dot(y.abs(), Ku.abs())), (a, b)
+    # unit length after the retraction on the magnet, zero elsewhere
+    m = gpu.to_device(gpu.uniform_m([0.0, 0.0, 1.0]))
+    d = torch.randn(3 * N, dtype=torch.float64, device="cuda") * torch.tensor(
+        np.repeat(gpu.meas > 0, 3), dtype=torch.float64, device="cuda")
+    out = gpu.update_m(m, 0.7, d).reshape(-1, 3)
+    ln = torch.linalg.norm(out, dim=1).cpu().numpy()
+    mag = gpu.meas > 0
+    assert np.max(np.abs(ln[mag] - 1)) < 1e-15 and np.all(ln[~mag] == 0)
+
+
+def test_config2_rows_and_closed_forms():
+    from mammos_mumag_amd import Model, generate
+    mesh = generate.build_config("voronoi64_250nm")
+    rng = np.random.default_rng(21)
+    for precision, tol in (("fp64", 1e-12), ("mixed", 2e-6)):
+        print(f"[gpu] voronoi64_250nm {precision}: sub-block rows and closed forms", flush=True)
+        gpu = Model.from_structured(mesh, precision=precision)
+        assert gpu.N == 3_652_264
+        _check_subblocks(gpu, mesh, tol, rng)
+        _check_closed_forms(gpu, mesh, tol, rng)
+        _free(gpu)
+
+
+def test_config2_structured_against_general():
+    """The two kernel families on the whole benchmark mesh, FP64."""
+    from mammos_mumag_amd import Model, generate
+    mesh = generate.build_config("voronoi64_250nm")
+    p = generate.loop_params("voronoi64_250nm")
+    print("[gpu] voronoi64_250nm: structured and general contexts", flush=True)
+    s = Model.from_structured(mesh)
+    g = Model.from_structured(mesh, structured=False)
+    rng = np.random.default_rng(22)
+    v = s.to_device(rng.normal(size=3 * s.N))
+    x = s.to_device(rng.normal(size=s.N))
+    assert rel(s.apply_C(v).cpu().numpy(), g.apply_C(v).cpu().numpy()) < 1e-12
+    assert rel(s.apply_K(x).cpu().numpy(), g.apply_K(x).cpu().numpy()) < 1e-12
+    assert rel(s.divergence(v).cpu().numpy(), g.divergence(v).cpu().numpy()) < 1e-12
+    assert rel(s.gradient(x).cpu().numpy(), g.gradient(x).cpu().numpy()) < 1e-12
+    m = s.to_device(s.uniform_m(p["m"]))
+    res = []
+    for model in (s, g):
+        u = model.empty1()
+        E, gp, gf, it = model.total_eg(m, 7.9, p["h"], 1e-10, 1, u)
+        z, _ = model.M_inv(m, gf, gp, p["precond_iter"])
+        res.append((E, gp.cpu().numpy(), u.cpu().numpy(), z.cpu().numpy(), it))
+    (E1, gp1, u1, z1, it1), (E2, gp2, u2, z2, it2) = res
+    print(f"[gpu] E {E1!r} / {E2!r}, PCG iterations {it1} / {it2}", flush=True)
+    assert abs(E1 / E2 - 1) < 1e-10
+    assert rel(u1, u2) < 1e-8 and rel(gp1, gp2) < 1e-8
+    assert rel(z1, z2) < 1e-8
+    _free(s, g)
+
+
+def test_config2_chosen_form_steps():
+    """Three field steps from +8 T: the chosen form against plain FP64 on the benchmark mesh."""
+    from mammos_mumag_amd import Model, _lib, generate
+    mesh = generate.build_config("voronoi64_250nm")
+    p = generate.loop_params("voronoi64_250nm")
+    out = {}
+    for form in ("fp64", "mixed"):
+        print(f"[gpu] voronoi64_250nm {form}: 3 field steps", flush=True)
+        model = Model.from_structured(mesh, precision=form)
+        if form == "mixed":
+            model.multigrid(8)
+        m = model.to_device(model.uniform_m(p["m"]))
+        m_ref, u, gF, st = m.clone(), model.empty1(), model.empty3(), _lib.Stats()
+        rec = []
+        for k in range(3):
+            hext = p["hstart"] + k * p["hstep"]
+            E = model.minimize(m, hext, p["h"], p, m_ref, u, gF, st)
+            rec.append((E, model.compute_mh(m, p["h"])))
+        out[form] = (np.array(rec), st.as_dict(), u.cpu().numpy())
+        _free(model)
+    (r0, s0, u0), (r1, s1, u1) = out["fp64"], out["mixed"]
+    print(f"[gpu] |dM.h| {np.max(np.abs(r1[:, 1] - r0[:, 1])):.2e}, dE/E {np.max(np.abs(r1[:, 0] / r0[:, 0] - 1)):.2e}, "
+          f"u {rel(u1, u0):.2e}; PCG {s0['pcg_iters']} / {s1['pcg_iters']}", flush=True)
+    assert abs(s0["ncg_iters"] - s1["ncg_iters"]) <= 3    # the NCG stop tests see the FP32 rounding
+    assert np.max(np.abs(r1[:, 1] - r0[:, 1])) < 1e-6
+    assert np.max(np.abs(r1[:, 0] / r0[:, 0] - 1)) < 1e-6
+    assert rel(u1, u0) < 1e-5
+
+
+def test_config3_learned_warm_start_unseen_microstructure():
+    """The benchmark geometry with seed 3001 (the network was trained on seeds 2001-2003):
+    with the shipped network before every solve the field steps end where plain FP64 does."""
+    from mammos_mumag_amd import Model, _lib, generate
+    import pathlib
+
+    import mammos_mumag_amd
+    from mammos_mumag_amd.learned import Network
+    shipped = pathlib.Path(mammos_mumag_amd.__file__).parent / "data" / "warmstart_voronoi64_250nm.npz"
+    mesh = generate.build_config("voronoi64_250nm", seed=3001)
+    p = generate.loop_params("voronoi64_250nm")
+    out = {}
+    for learned in (False, True):
+        print(f"[gpu] learned workload (seed 3001), network {'on' if learned else 'off'}", flush=True)
+        model = Model.from_structured(mesh, precision="mixed")
+        model.multigrid(8)
+        net = None
+        if learned:
+            net = Network(model, max_batch=32, seed=0)
+            meta = net.load(shipped)
+            assert 3001 not in [int(v) for v in np.ravel(meta.get("seeds", []))]
+            model.warm_start(net)
+        m = model.to_device(model.uniform_m(p["m"]))
+        m_ref, u, gF, st = m.clone(), model.empty1(), model.empty3(), _lib.Stats()
+        rec = []
+        for k in range(3):
+            E = model.minimize(m, p["hstart"] + k * p["hstep"], p["h"], p, m_ref, u, gF, st)
+            rec.append((E, model.compute_mh(m, p["h"])))
+        ws = model.warm_start_stats() if learned else None
+        out[learned] = (np.array(rec), st.as_dict(), ws)
+        _free(model)
+    (r0, s0, _), (r1, s1, ws) = out[False], out[True]
+    print(f"[gpu] PCG iterations {s0['pcg_iters']} plain / {s1['pcg_iters']} warm-started "
+          f"({ws['calls']} warm-started solves)", flush=True)
+    assert ws["calls"] == s1["pcg_calls"] > 0
+    assert np.max(np.abs(r1[:, 1] - r0[:, 1])) < 1e-6
+    assert np.max(np.abs(r1[:, 0] / r0[:, 0] - 1)) < 1e-8
+
+
+# ---------------------------------------------------------------- voronoi512_500nm
+
+def test_config4_rows_closed_forms_and_step():
+    """Both solves stop at the same 1e-10 relative residual, so their potentials differ by up
+    to cond(K) 1e-10 (1e-5 here): the check is that each solves the FP64 system, with the
+    FP64 operators, to the accuracy of its own right-hand side (FP64: the solver tolerance;
+    the chosen form: its FP32 divergence, ~1e-7), and that the energies agree."""
+    import torch
+    from mammos_mumag_amd import Model, _lib, generate
+    mesh = generate.build_config("voronoi512_500nm")
+    p = generate.loop_params("voronoi512_500nm")
+    rng = np.random.default_rng(41)
+    models, res = {}, {}
+    for precision, tol in (("fp64", 1e-12), ("mixed", 2e-6)):
+        print(f"[gpu] voronoi512_500nm {precision}: context", flush=True)
+        gpu = models[precision] = Model.from_structured(mesh, precision=precision)
+        assert gpu.N == len(mesh.xs) ** 3 == 22_665_187
+        print(f"[gpu] voronoi512_500nm {precision}: sub-block rows and closed forms", flush=True)
+        _check_subblocks(gpu, mesh, tol, rng)
+        _check_closed_forms(gpu, mesh, tol, rng)
+        if precision == "mixed":
+            gpu.multigrid(8)
+        m = gpu.to_device(gpu.uniform_m(p["m"]))
+        u = gpu.empty1()
+        E, gp, gf, it = gpu.total_eg(m, p["hstart"], p["h"], 1e-10, 1, u)
+        print(f"[gpu] voronoi512_500nm {precision}: potential solve from zero, {it} PCG iterations, E {E!r}",
+              flush=True)
+        res[precision] = (E, u)
+        del gp, gf
+    f64 = models["fp64"]
+    m = f64.to_device(f64.uniform_m(p["m"]))
+    b = f64.divergence(m)
+    nb = float(torch.linalg.norm(b))
+    for precision, bound in (("fp64", 1e-9), ("mixed", 1e-6)):
+        r = float(torch.linalg.norm(b - f64.apply_K(res[precision][1]))) / nb
+        print(f"[gpu] {precision} potential: FP64 residual {r:.2e}", flush=True)
+        assert r < bound, precision
+    (E64, _), (E32, _) = res["fp64"], res["mixed"]
+    assert abs(E32 / E64 - 1) < 1e-6
+    del m, b, res
+    _free(f64)
+    gpu = models["mixed"]
+    m = gpu.to_device(gpu.uniform_m(p["m"]))
+    m_ref, u, gF, st = m.clone(), gpu.empty1(), gpu.empty3(), _lib.Stats()
+    E1 = gpu.minimize(m, p["hstart"], p["h"], p, m_ref, u, gF, st)
+    ln = torch.linalg.norm(m.reshape(-1, 3), dim=1).cpu().numpy()
+    mag = gpu.meas > 0
+    print(f"[gpu] voronoi512_500nm chosen form: first field step E {E1!r}, {st.as_dict()}", flush=True)
+    assert E1 < E32
+    assert np.max(np.abs(ln[mag] - 1)) < 1e-14 and np.all(ln[~mag] == 0)
+    del m, m_ref, u, gF
+    _free(gpu)
