@@ -85,9 +85,11 @@ def cpu_worker(cfg, out_path, seed):
     n_tcg = mini.stats["tcg_iters"]
     t_tcg = (time.perf_counter() - t0) / max(n_tcg, 1)
     sample_s = t_pcg * it + t_eval * reps + t_tcg * n_tcg
+    threads = {k: os.environ.get(k) for k in ("OMP_NUM_THREADS", "OPENBLAS_NUM_THREADS", "MKL_NUM_THREADS")}
     with open(out_path, "w") as f:
         json.dump(dict(t_pcg_iter=t_pcg, t_eval=t_eval, t_tcg_iter=t_tcg, setup_s=t_setup,
-                       sample_s=sample_s, n_pcg=it, n_tcg=n_tcg, cores=1), f)
+                       sample_s=sample_s, n_pcg=it, n_tcg=n_tcg, cores=1,
+                       host_cores=len(os.sched_getaffinity(0)), threads_env=threads), f)
 
 
 def start_cpu_leg(cfg, seed):
@@ -115,7 +117,7 @@ def gpu_run(args, precision, traffic_probe=False, learned=None, multigrid=0):
     log(f"[bench] {precision}: mesh {args.config}: {n_nodes} nodes, {n_elems} tets ({time.perf_counter() - t0:.1f} s)")
     t0 = time.perf_counter()
     model = Model.from_structured(g, structured=not args.general, precision=precision,
-                                  inner_tol=args.inner_tol)
+                                  inner_tol=args.inner_tol, eval_precision=args.eval_precision)
     levels = []
     if multigrid:
         model.multigrid(multigrid)
@@ -174,6 +176,176 @@ def gpu_run(args, precision, traffic_probe=False, learned=None, multigrid=0):
     del m, m_ref, u, gF, model
     torch.cuda.empty_cache()
     return res
+
+
+def loop_run(args, precision, multigrid=0, learned=None, perturb=0.0):
+    """The field sweep of --config from hstart until ``--loop-after`` steps past the reversal
+    (the first step with M.h < 0), as scripts/loop.py:89-124 runs it with mfinal -2; per
+    step the field, M.h, E, wall seconds and the counters of that step."""
+    import torch
+
+    from mammos_mumag_amd import Model, _lib
+    from mammos_mumag_amd.generate import build_config, loop_params
+
+    g = build_config(args.config, seed=args.seed)
+    model = Model.from_structured(g, precision=precision, eval_precision=args.eval_precision)
+    if multigrid:
+        model.multigrid(multigrid)
+    tag = precision + (" + multigrid" if multigrid else "") + (" + learned" if learned else "")
+    if precision == "mixed":
+        tag += f" (evaluation {args.eval_precision})"
+    if learned:
+        from mammos_mumag_amd.learned import Network
+        net = Network(model, max_batch=32, seed=0)
+        net.load(learned)
+        model.warm_start(net)
+        model.warm_start_policy(args.ws_policy)
+    p = loop_params(args.config)
+    m0 = model.uniform_m(p["m"])
+    m_ref = model.to_device(m0)
+    if perturb:
+        import numpy as np
+        m0 = m0 * (1 + perturb * np.random.default_rng(1).normal(size=m0.shape))
+        tag += f", initial m * (1 + {perturb:g} N(0,1))"
+    m = model.to_device(m0)
+    u, gF = model.empty1(), model.empty3()
+    st = _lib.Stats()
+    n_steps = int(abs(p["hfinal"] - p["hstart"]) / abs(p["hstep"])) + 1
+    hext, rec, prev, after = p["hstart"], [], st.as_dict(), None
+    t_loop = time.perf_counter()
+    for k in range(n_steps):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        E = model.minimize(m, hext, p["h"], p, m_ref, u, gF, st)
+        mh = model.compute_mh(m, p["h"])
+        dt = time.perf_counter() - t0
+        cur = st.as_dict()
+        d = {key: cur[key] - prev[key] for key in cur}
+        prev = cur
+        rec.append(dict(h=hext, mh=mh, E=E, s=dt, **d))
+        if k % 20 == 0 or mh < 0 or d["ncg_iters"] > 50:
+            log(f"[loop] {tag}: step {k} h={hext:+.2f} T M.h={mh:+.6f} T {dt:.3f} s "
+                f"ncg {d['ncg_iters']} pcg {d['pcg_iters']} backtracks {d['backtracks']}")
+        if mh < 0 and after is None:
+            after = k
+        if after is not None and k - after >= args.loop_after:
+            break
+        hext += p["hstep"]
+    total = time.perf_counter() - t_loop
+    V, moment = model.V, float((model.meas).sum())
+    model.close()
+    del m, m_ref, u, gF, model
+    torch.cuda.empty_cache()
+    return dict(tag=tag, records=rec, seconds=total, js_sat=moment / V)
+
+
+def loop_main(args, learned):
+    """--loop: the chosen form and plain FP64 through the reversal; one JSON line."""
+    import numpy as np
+    forms = [("mixed", args.multigrid, learned), ("fp64", 0, None)]
+    if args.loop_perturb:
+        forms.append(("fp64", 0, None, args.loop_perturb))
+    if args.loop_fp64_only:
+        forms = forms[1:]
+    runs = [loop_run(args, *f) for f in forms]
+    out = {"loop": args.config, "forms": []}
+    ref = runs[0 if args.loop_fp64_only else 1]
+    n = min(len(r["records"]) for r in runs)
+    sat = args.loop_window
+    for r in runs:
+        rec = r["records"]
+        mh = np.array([x["mh"] for x in rec])
+        sw = int(np.argmax(mh < 0)) if np.any(mh < 0) else None
+        t = np.array([x["s"] for x in rec])
+        f = dict(form=r["tag"], steps=len(rec), reversal_step=sw,
+                 reversal_field_T=None if sw is None else rec[sw]["h"],
+                 steps_per_s_whole=round(len(rec) / float(t.sum()), 4),
+                 steps_per_s_saturated=round(sat / float(t[:sat].sum()), 4),
+                 slowest_step_s=round(float(t.max()), 3),
+                 ncg_per_step=round(float(np.mean([x["ncg_iters"] for x in rec])), 2),
+                 pcg_per_step=round(float(np.mean([x["pcg_iters"] for x in rec])), 2),
+                 backtracks=int(sum(x["backtracks"] for x in rec)),
+                 restarts=int(sum(x["restarts_g"] + x["restarts_z"] for x in rec)))
+        if r is not ref:
+            a = np.array([x["mh"] for x in rec[:n]])
+            b = np.array([x["mh"] for x in ref["records"][:n]])
+            ea = np.array([x["E"] for x in rec[:n]])
+            eb = np.array([x["E"] for x in ref["records"][:n]])
+            f["max_abs_dM_over_Ms"] = float(np.max(np.abs(a - b)) / ref["js_sat"])
+            f["max_dE_over_maxE"] = float(np.max(np.abs(ea - eb)) / np.max(np.abs(eb)))   # E crosses zero
+            f["speedup_whole"] = round(float(sum(x["s"] for x in ref["records"][:n]) / sum(x["s"] for x in rec[:n])), 3)
+        out["forms"].append(f)
+    if args.loop_out:
+        with open(args.loop_out, "w") as fh:
+            json.dump(dict(out, records={r["tag"]: r["records"] for r in runs}), fh)
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+def avalanche_main(args, learned):
+    """--avalanche K: plain FP64 runs the sweep to step K - 1; from that state, step K (the
+    reversal avalanche) is minimised again by plain FP64 from the exact state and from states
+    perturbed by 1e-15 relative (--avalanche-seeds), and by the chosen form.  Shows how far
+    rounding alone moves the end state of an avalanche in the FP64 method itself."""
+    import numpy as np
+    import torch
+
+    from mammos_mumag_amd import Model, _lib
+    from mammos_mumag_amd.generate import build_config, loop_params
+
+    g = build_config(args.config, seed=args.seed)
+    p = loop_params(args.config)
+    f64 = Model.from_structured(g)
+    m = f64.to_device(f64.uniform_m(p["m"]))
+    m_ref, u, gF, st = m.clone(), f64.empty1(), f64.empty3(), _lib.Stats()
+    hext = p["hstart"]
+    t0 = time.perf_counter()
+    for k in range(args.avalanche):
+        f64.minimize(m, hext, p["h"], p, m_ref, u, gF, st)
+        if k % 20 == 0 or k >= args.avalanche - 15:
+            log(f"[avalanche] fp64 step {k} h={hext:+.2f} T M.h={f64.compute_mh(m, p['h']):+.6f} T "
+                f"({time.perf_counter() - t0:.0f} s)")
+        hext += p["hstep"]
+    mh0 = f64.compute_mh(m, p["h"])
+    saved = (m.clone(), u.clone(), gF.clone())
+    out = dict(avalanche=args.config, step=args.avalanche, h=hext, mh_before=mh0, runs=[])
+
+    def one(model, tag, mm, uu, gg):
+        s2 = _lib.Stats()
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        E = model.minimize(mm, hext, p["h"], p, m_ref, uu, gg, s2)
+        dt = time.perf_counter() - t1
+        r = dict(run=tag, mh=model.compute_mh(mm, p["h"]), E=E, seconds=round(dt, 3),
+                 ncg_iters=s2.ncg_iters, pcg_iters=s2.pcg_iters, backtracks=s2.backtracks)
+        log(f"[avalanche] {r}")
+        out["runs"].append(r)
+
+    mag = torch.tensor(np.repeat(f64.meas > 0, 3), dtype=torch.float64, device="cuda")
+    runs = [(None, 0.0)] + [(sd, float(e)) for e in args.avalanche_eps.split(",")
+                            for sd in range(1, args.avalanche_seeds + 1)]
+    for seed, eps in runs:
+        mm, uu, gg = (x.clone() for x in saved)
+        if seed is not None:
+            gen = torch.Generator(device="cuda").manual_seed(seed)
+            mm *= 1 + eps * torch.randn(mm.shape, dtype=torch.float64, device="cuda", generator=gen) * mag
+        one(f64, "fp64" if seed is None else f"fp64, m * (1 + {eps:g} N(0,1)) seed {seed}", mm, uu, gg)
+    f64.close()
+    torch.cuda.empty_cache()
+    mix = Model.from_structured(g, precision="mixed", eval_precision=args.eval_precision)
+    mix.multigrid(args.multigrid)
+    if learned:
+        from mammos_mumag_amd.learned import Network
+        net = Network(mix, max_batch=32, seed=0)
+        net.load(learned)
+        mix.warm_start(net)
+        mix.warm_start_policy(args.ws_policy)
+    mm, uu, gg = (x.clone() for x in saved)
+    one(mix, "chosen form (FP32 inner solve, multigrid" + (", learned warm start" if learned else "") + ")",
+        mm, uu, gg)
+    mix.close()
+    print(json.dumps(out), flush=True)
+    return 0
 
 
 def roofline(res, precision, args):
@@ -242,10 +414,27 @@ def main():
     ap.add_argument("--seed", type=int, default=1000)
     ap.add_argument("--general", action="store_true", help="unstructured kernels (SELL K, element loops)")
     ap.add_argument("--inner-tol", type=float, default=0.0)
+    ap.add_argument("--eval-precision", default="fp64", choices=["fp64", "mixed"],
+                    help="mixed form: arithmetic of the energy evaluation's C m, D m and G u "
+                         "(the inner PCG and truncated CG are FP32 either way)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--traffic-probe", action="store_true",
                     help="after the run, copy 512 MiB on the device (calibrates the FETCH_SIZE / WRITE_SIZE counters)")
     ap.add_argument("--cpu-worker", default=None, help=argparse.SUPPRESS)
+    ap.add_argument("--loop", action="store_true",
+                    help="instead of the timed window: the chosen form and plain FP64 through the reversal")
+    ap.add_argument("--loop-after", type=int, default=4, help="--loop: steps run after the reversal")
+    ap.add_argument("--loop-window", type=int, default=24, help="--loop: saturated steps timed separately")
+    ap.add_argument("--loop-out", default="", help="--loop: also write the per-step records (JSON) here")
+    ap.add_argument("--loop-perturb", type=float, default=0.0,
+                    help="--loop: also run plain FP64 from the initial m perturbed by this relative size")
+    ap.add_argument("--loop-fp64-only", action="store_true", help="--loop: plain FP64 runs only")
+    ap.add_argument("--avalanche", type=int, default=0,
+                    help="step index K: FP64 to step K - 1, then step K from that state by FP64 (exact and "
+                         "perturbed by 1e-15) and by the chosen form")
+    ap.add_argument("--avalanche-seeds", type=int, default=2)
+    ap.add_argument("--avalanche-eps", default="1e-15",
+                    help="comma-separated relative sizes of the perturbations of m (each with every seed)")
     args = ap.parse_args()
     if args.cpu_worker:
         cpu_worker(args.config, args.cpu_worker, args.seed)
@@ -255,7 +444,7 @@ def main():
               file=sys.stderr)
         return 2
     cpu = None
-    if not args.no_cpu_baseline:
+    if not args.no_cpu_baseline and not args.loop and not args.avalanche:
         cpu = start_cpu_leg(args.config, args.seed)
     learned = None
     if args.learned != "none":
@@ -265,11 +454,18 @@ def main():
             learned = str(cand)
         elif args.learned:
             raise SystemExit(f"bench.py: trained warm-start network {cand} not found")
+    if args.loop:
+        return loop_main(args, learned)
+    if args.avalanche:
+        return avalanche_main(args, learned)
     mgl = args.multigrid if args.precision == "mixed" and not args.general else 0
     res = gpu_run(args, args.precision, traffic_probe=args.traffic_probe, learned=learned, multigrid=mgl)
     ablate = None
     if learned and not args.no_compare:
         ablate = gpu_run(args, args.precision, multigrid=mgl)   # the same form without the network
+    jacobi = None
+    if args.precision == "mixed" and mgl and not args.no_compare:
+        jacobi = gpu_run(args, "mixed")   # FP32 inner solve alone: the Jacobi preconditioner
     ref = None
     if (args.precision != "fp64" or learned) and not args.no_compare:
         ref = gpu_run(args, "fp64")
@@ -288,7 +484,7 @@ def main():
             per_step = (st["pcg_iters"] * c["t_pcg_iter"] + st["fcalls"] * c["t_eval"]
                         + st["tcg_iters"] * c["t_tcg_iter"]) / steps
             cpu_res = dict(value=round(1.0 / per_step, 6), unit="field steps/s", cores=c["cores"],
-                           kind="port",
+                           kind="port", host_cores=c["host_cores"], threads_env=c["threads_env"],
                            sample=(f"FP64 numpy/scipy oracle on the same {args.config} mesh: unit costs "
                                    f"timed over {c['n_pcg']} PCG iterations, 3 PCG-free evaluations and "
                                    f"{c['n_tcg']} truncated-CG iterations ({c['sample_s']:.1f} s), "
@@ -323,6 +519,11 @@ def main():
                            pcg_iters_per_step=round(ablate["stats"]["pcg_iters"] / steps, 2),
                            gain=round(ablate["elapsed"] / res["elapsed"], 4)))),
                    "per_step": {k: round(v / steps, 2) for k, v in res["stats"].items()}},
+        "mixed_jacobi": (None if jacobi is None else dict(
+            value=round(steps / jacobi["elapsed"], 6),
+            pcg_iters_per_step=round(jacobi["stats"]["pcg_iters"] / steps, 2),
+            speedup_over_fp64=None if ref is None else round(ref["elapsed"] / jacobi["elapsed"], 4),
+            errors=None if ref is None else form_errors(jacobi, ref))),
         "vs_fp64": (None if ref is None else dict(
             fp64_value=round(steps / ref["elapsed"], 6), speedup=round(ref["elapsed"] / res["elapsed"], 4),
             fp64_per_step={k: round(v / steps, 2) for k, v in ref["stats"].items()},

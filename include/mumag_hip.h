@@ -33,7 +33,7 @@ extern "C" {
 #define MM_W_MAXITER 1     /* iteration cap reached */
 
 #define MM_PREC_FP64 0     /* every operator and vector in FP64 */
-#define MM_PREC_MIXED 1    /* FP32 operators/fields, FP64 dots, energies, outer PCG residual */
+#define MM_PREC_MIXED 1    /* FP32 inner PCG and truncated CG, FP64 dots, energies, outer PCG residual */
 
 typedef struct mm_ctx mm_ctx;
 
@@ -80,6 +80,13 @@ typedef struct {
     int32_t use_structured;   /* 1: use the Kuhn-grid stencil kernels (needs grid_n) */
     double inner_tol;         /* MM_PREC_MIXED: FP32 residual reduction between two FP64
                                  reliable updates (0 = default 0.1) */
+    int32_t eval_precision;   /* MM_PREC_MIXED: arithmetic of the energy evaluation's operator
+                                 applications (C m, D m, G u of total_eg, and mm_apply_C /
+                                 mm_divergence / mm_gradient): MM_PREC_FP64 (0, default), so
+                                 that energies, their line-search differences and the PCG
+                                 right-hand side carry no FP32 rounding, or MM_PREC_MIXED
+                                 (FP32).  The PCG inner iterations and the truncated CG run in
+                                 FP32 either way. */
 } mm_options;
 
 typedef struct {

@@ -57,6 +57,9 @@ struct mm_ctx {
     int pcg_batch = 16;
     double ru_delta = 0.1;   // mixed precision: residual reduction between reliable updates
     bool ru_delta_set = false;  // given by the caller (mm_options.inner_tol)
+    // MM_PREC_MIXED with eval_precision MM_PREC_MIXED: the energy evaluation's C m, D m and
+    // G u in FP32 (structured kernels; the element kernels are FP64 either way)
+    bool eval32 = false;
     int64_t k_nnz = 0, k_entries = 0;
     size_t bytes = 0;
     std::vector<void *> allocs;
@@ -229,7 +232,7 @@ void op_K(mm_ctx *c, const double *x, double *y, double *dot, const int *skip) {
 void op_C(mm_ctx *c, const double *v, double *y, double *b, const int *skip) {
     Launch L = c->L();
     if (c->structured) {
-        launch_grid_exani(L, c->G, c->precision == MM_PREC_MIXED, v, y, b, skip);
+        launch_grid_exani(L, c->G, c->eval32, v, y, b, skip);
     } else {
         launch_fill_zero(L, 3 * c->N, y, skip);
         if (b) launch_fill_zero(L, c->N, b, skip);
@@ -239,7 +242,7 @@ void op_C(mm_ctx *c, const double *v, double *y, double *b, const int *skip) {
 void op_grad(mm_ctx *c, const double *u, double *g) {
     Launch L = c->L();
     if (c->structured) {
-        launch_grid_grad(L, c->G, c->precision == MM_PREC_MIXED, u, g);
+        launch_grid_grad(L, c->G, c->eval32, u, g);
     } else {
         launch_fill_zero(L, 3 * c->N, g, nullptr);
         launch_ebe_grad(L, c->E, u, g, 1.0 / c->V);
@@ -755,6 +758,8 @@ int mm_ctx_create(const mm_mesh_desc *d, const mm_options *opts, mm_ctx **out) {
     if (opts) o = *opts;
     if (o.precision != MM_PREC_FP64 && o.precision != MM_PREC_MIXED)
         return fail(MM_E_ARG, "unknown precision");
+    if (o.eval_precision != MM_PREC_FP64 && o.eval_precision != MM_PREC_MIXED)
+        return fail(MM_E_ARG, "unknown evaluation precision");
     for (int64_t i = 0; i < 4 * (int64_t)d->n_elems; ++i)
         if (d->tets[i] < 0 || d->tets[i] >= d->n_nodes) return fail(MM_E_MESH, "element node index out of range");
     for (int64_t e = 0; e < d->n_elems; ++e)
@@ -781,6 +786,7 @@ int mm_ctx_create(const mm_mesh_desc *d, const mm_options *opts, mm_ctx **out) {
     mm_ctx *c = new mm_ctx();
     c->device = o.device;
     c->precision = o.precision;
+    c->eval32 = o.precision == MM_PREC_MIXED && o.eval_precision == MM_PREC_MIXED;
     c->structured = structured;
     c->N = d->n_nodes;
     c->NE = d->n_elems;

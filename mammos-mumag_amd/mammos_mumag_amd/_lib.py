@@ -40,7 +40,8 @@ class MeshDesc(ctypes.Structure):
 class Options(ctypes.Structure):
     _fields_ = [("precision", ctypes.c_int32), ("device", ctypes.c_int32),
                 ("drop_tol", ctypes.c_double), ("pcg_batch", ctypes.c_int32),
-                ("use_structured", ctypes.c_int32), ("inner_tol", ctypes.c_double)]
+                ("use_structured", ctypes.c_int32), ("inner_tol", ctypes.c_double),
+                ("eval_precision", ctypes.c_int32)]
 
 
 class Info(ctypes.Structure):

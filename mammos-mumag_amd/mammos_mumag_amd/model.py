@@ -115,9 +115,11 @@ class Model:
     """Operators of one mesh + material set on the GPU."""
 
     def __init__(self, coords, tets, elem_tags, krn, precision="fp64", device=0, drop_tol=1e-12,
-                 pcg_batch=0, length_scale=1.0e-9, grid=None, inner_tol=0.0):
+                 pcg_batch=0, length_scale=1.0e-9, grid=None, inner_tol=0.0, eval_precision="fp64"):
         """``grid`` (optional): dict(xs, ys, zs, mag0, magn) of a structured Kuhn-grid mesh
-        written by ``generate``; the library then validates it and uses its stencil kernels."""
+        written by ``generate``; the library then validates it and uses its stencil kernels.
+        ``eval_precision`` ("fp64" or "mixed"): with precision "mixed", the arithmetic of the
+        energy evaluation's operator applications (the inner solves are FP32 either way)."""
         self.coords = np.ascontiguousarray(coords, dtype=np.float64)
         self.tets = np.ascontiguousarray(tets, dtype=np.int32)
         self.elem_tags = np.ascontiguousarray(elem_tags, dtype=np.int32)
@@ -177,6 +179,7 @@ class Model:
         opts.pcg_batch = pcg_batch
         opts.use_structured = 1 if grid is not None else 0
         opts.inner_tol = inner_tol
+        opts.eval_precision = {"fp64": _lib.MM_PREC_FP64, "mixed": _lib.MM_PREC_MIXED}[eval_precision]
         self._ctx = ctypes.c_void_p()
         _lib.check(lib.mm_ctx_create(ctypes.byref(desc), ctypes.byref(opts), ctypes.byref(self._ctx)),
                    "mm_ctx_create")

@@ -224,9 +224,9 @@ def test_config2_rows_and_closed_forms():
     from mammos_mumag_amd import Model, generate
     mesh = generate.build_config("voronoi64_250nm")
     rng = np.random.default_rng(21)
-    for precision, tol in (("fp64", 1e-12), ("mixed", 2e-6)):
-        print(f"[gpu] voronoi64_250nm {precision}: sub-block rows and closed forms", flush=True)
-        gpu = Model.from_structured(mesh, precision=precision)
+    for precision, ev, tol in (("fp64", "fp64", 1e-12), ("mixed", "fp64", 1e-12), ("mixed", "mixed", 2e-6)):
+        print(f"[gpu] voronoi64_250nm {precision} (evaluation {ev}): sub-block rows and closed forms", flush=True)
+        gpu = Model.from_structured(mesh, precision=precision, eval_precision=ev)
         assert gpu.N == 3_652_264
         _check_subblocks(gpu, mesh, tol, rng)
         _check_closed_forms(gpu, mesh, tol, rng)
@@ -346,7 +346,8 @@ def test_config4_rows_closed_forms_and_step():
     models, res = {}, {}
     for precision, tol in (("fp64", 1e-12), ("mixed", 2e-6)):
         print(f"[gpu] voronoi512_500nm {precision}: context", flush=True)
-        gpu = models[precision] = Model.from_structured(mesh, precision=precision)
+        # the mixed context with FP32 evaluation: every FP32 kernel at this size
+        gpu = models[precision] = Model.from_structured(mesh, precision=precision, eval_precision=precision)
         assert gpu.N == len(mesh.xs) ** 3 == 22_665_187
         print(f"[gpu] voronoi512_500nm {precision}: sub-block rows and closed forms", flush=True)
         _check_subblocks(gpu, mesh, tol, rng)

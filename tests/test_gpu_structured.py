@@ -46,8 +46,8 @@ def gpu_g(mesh):
 @pytest.fixture(scope="module")
 def gpu_mixed(mesh):
     from mammos_mumag_amd import Model
-    print("[gpu] mixed-precision structured context", flush=True)
-    return Model.from_structured(mesh, precision="mixed")
+    print("[gpu] mixed-precision structured context (FP32 evaluation kernels)", flush=True)
+    return Model.from_structured(mesh, precision="mixed", eval_precision="mixed")
 
 
 def _diag(model):
