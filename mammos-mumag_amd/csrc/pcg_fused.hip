@@ -29,11 +29,21 @@ namespace {
 constexpr int kFB = 1024;  // threads per block of the streaming fused kernels (16 wavefronts)
 // Tile rows, target block count and waves per SIMD of the structured KA (compile-time
 // tunables; the defaults are the measured best on MI355X).
+// (per precision: FP32 loads move half the bytes per request, so the FP32 kernel wants
+// more, smaller tiles; swept on the 154^3 benchmark grid: FP32 64 x 4 tiles with 768
+// blocks 15.3 us against 15.9 us for 64 x 8 / 512; FP64 64 x 8 with 256 blocks 24.2 us
+// against 26.5 us)
 #ifndef MM_KA_TY
 #define MM_KA_TY 8
 #endif
 #ifndef MM_KA_BLOCKS
-#define MM_KA_BLOCKS 512
+#define MM_KA_BLOCKS 256
+#endif
+#ifndef MM_KA_TY32
+#define MM_KA_TY32 4
+#endif
+#ifndef MM_KA_BLOCKS32
+#define MM_KA_BLOCKS32 768
 #endif
 #ifndef MM_KA_PF
 #define MM_KA_PF 2
@@ -44,7 +54,15 @@ constexpr int kFB = 1024;  // threads per block of the streaming fused kernels (
 #ifndef MM_KB_BLOCKS
 #define MM_KB_BLOCKS 256
 #endif
-constexpr int kKaTY = MM_KA_TY;  // rows of a KA tile on the structured grid (64 x kKaTY nodes)
+// rows of a KA tile on the structured grid (64 x TY nodes) and the target block count
+template <typename T>
+struct KaCfg {
+    static constexpr int TY = MM_KA_TY, blocks = MM_KA_BLOCKS;
+};
+template <>
+struct KaCfg<float> {
+    static constexpr int TY = MM_KA_TY32, blocks = MM_KA_BLOCKS32;
+};
 
 __device__ __forceinline__ double fwave_sum(double v) {
 #pragma unroll
@@ -477,10 +495,10 @@ __global__ __launch_bounds__(kFB) void k_pcg0(int64_t n, const T *__restrict__ b
     fblock_partials<3, kFB>(s, sh, parts_out);
 }
 
-// Blocks of the structured KA: 64 x kKaTY tiles times gz z-chunks, gz chosen so that
-// the launch has at most kKaBlocks blocks, at least one chunk per tile column and at
+// Blocks of the structured KA: 64 x TY tiles times gz z-chunks, gz chosen so that
+// the launch has at most KaCfg<T>::blocks blocks, at least one chunk per tile column and at
 // most kMaxPartials blocks.
-constexpr int kKaBlocks = MM_KA_BLOCKS;
+
 struct KaDims {
     int gx, gy, gz;
 };
@@ -488,9 +506,9 @@ template <typename T>
 KaDims ka_grid_dims(const PoissonGrid<T> &P) {
     KaDims d;
     d.gx = (P.nx + 63) / 64;
-    d.gy = (P.ny + kKaTY - 1) / kKaTY;
+    d.gy = (P.ny + KaCfg<T>::TY - 1) / KaCfg<T>::TY;
     const int per_plane = d.gx * d.gy;
-    d.gz = std::max(1, std::min(P.nz, kKaBlocks / per_plane));
+    d.gz = std::max(1, std::min(P.nz, KaCfg<T>::blocks / per_plane));
     d.gz = std::max(d.gz, (P.nz + kKaChunk - 1) / kKaChunk);   // chunks of at most kKaChunk planes
     return d;
 }
@@ -511,10 +529,13 @@ void launch_timed(const Launch &L, K kernel, dim3 grid, dim3 block, Args... args
 }  // namespace
 
 bool ka_grid_fits(int nx, int ny, int nz) {
-    const int64_t per_plane = (int64_t)((nx + 63) / 64) * ((ny + kKaTY - 1) / kKaTY);
     const int64_t gz = std::max<int64_t>(1, (nz + kKaChunk - 1) / kKaChunk);
     const int64_t n = (int64_t)nx * ny * nz;
-    return per_plane * gz <= kMaxPartials && n * (int64_t)sizeof(double) < ((int64_t)1 << 32);
+    for (const int ty : {KaCfg<double>::TY, KaCfg<float>::TY}) {
+        const int64_t per_plane = (int64_t)((nx + 63) / 64) * ((ny + ty - 1) / ty);
+        if (per_plane * gz > kMaxPartials) return false;
+    }
+    return n * (int64_t)sizeof(double) < ((int64_t)1 << 32);
 }
 
 template <typename T>
@@ -531,10 +552,12 @@ int launch_ka_grid(const Launch &L, const PoissonGrid<T> &P, const PcgIterArgs &
     const KaDims d = ka_grid_dims(P);
     const int nb = d.gx * d.gy * d.gz;
     if (a.mode == PCG_FIRST || a.mode == PCG_RESTART || a.mode == PCG_RESTART_PARTS)
-        launch_timed(L, k_ka_grid<T, kKaTY, true>, dim3(nb), dim3(64 * kKaTY), P, a, d.gx, d.gz, z, pold, pnew, q,
+        launch_timed(L, k_ka_grid<T, KaCfg<T>::TY, true>, dim3(nb), dim3(64 * KaCfg<T>::TY), P, a, d.gx, d.gz, z,
+                     pold, pnew, q,
                      L.dsc);
     else
-        launch_timed(L, k_ka_grid<T, kKaTY, false>, dim3(nb), dim3(64 * kKaTY), P, a, d.gx, d.gz, z, pold, pnew, q,
+        launch_timed(L, k_ka_grid<T, KaCfg<T>::TY, false>, dim3(nb), dim3(64 * KaCfg<T>::TY), P, a, d.gx, d.gz, z,
+                     pold, pnew, q,
                      L.dsc);
     return nb;
 }

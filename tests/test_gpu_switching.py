@@ -108,7 +108,8 @@ def test_quickstart_counters_and_mixed_form(qs, branches, cube_arrays):
 
 def _check_case(name, rec, steps, ref):
     """Branch counters exactly; iteration counts and the state within what rounding alone
-    moves in the oracle (its ``spread`` over initial states perturbed by 1e-15)."""
+    moves in the oracle (its ``spread`` over initial states perturbed by 1e-15), or within
+    the loop's own bounds where that is larger."""
     sp = ref["spread"]
     oref = np.array(ref["records"])
     dmh = float(np.max(np.abs(rec[:, 2] - oref[:, 2])))
@@ -125,8 +126,10 @@ def _check_case(name, rec, steps, ref):
         if lo == hi:
             _check_counters(steps, ref["counters"])
         assert 0.95 * lo <= last[k] <= 1.05 * hi, f"{k}: {last[k]} outside [{lo}, {hi}] +- 5%"
-    assert dmh <= max(1e-9, 3 * sp["mh"])
-    assert dE <= max(1e-10, 3 * sp["E_rel"])
+    # floors: the M(H) bound 1e-6, and 1e-9 in E (ten times tol_fun, the NCG's energy-decrease
+    # test, below which a slowly converging step's end point is not fixed)
+    assert dmh <= max(1e-6, 3 * sp["mh"])
+    assert dE <= max(1e-9, 3 * sp["E_rel"])
 
 
 @pytest.mark.parametrize("name", ["vortex_hmag0", "vortex_hmag1", "tilt1"])
