@@ -51,7 +51,7 @@ def log(*a):
 
 
 # --------------------------------------------------------------------------- CPU leg
-def cpu_worker(cfg, out_path, seed):
+def cpu_worker(cfg, out_path, seed, full_step=False):
     """Unit costs of the FP64 oracle on this host, at the benchmark's own mesh."""
     import numpy as np
 
@@ -86,10 +86,22 @@ def cpu_worker(cfg, out_path, seed):
     t_tcg = (time.perf_counter() - t0) / max(n_tcg, 1)
     sample_s = t_pcg * it + t_eval * reps + t_tcg * n_tcg
     threads = {k: os.environ.get(k) for k in ("OMP_NUM_THREADS", "OPENBLAS_NUM_THREADS", "MKL_NUM_THREADS")}
+    res = dict(t_pcg_iter=t_pcg, t_eval=t_eval, t_tcg_iter=t_tcg, setup_s=t_setup,
+               sample_s=sample_s, n_pcg=it, n_tcg=n_tcg, cores=1,
+               host_cores=len(os.sched_getaffinity(0)), threads_env=threads)
+    if full_step:
+        # one complete field step of the oracle (loop.py:58-65 at hstart from m0 = +z, u = 0),
+        # against the unit-cost model applied to its own counts
+        mini = Minimizer(M, p, m)
+        t0 = time.perf_counter()
+        mini.minimize(m.copy(), p["hstart"])
+        full_s = time.perf_counter() - t0
+        sc = mini.stats
+        model_s = sc["pcg_iters"] * t_pcg + sc["calls"] * t_eval + sc["tcg_iters"] * t_tcg
+        res.update(full_step_s=full_s, full_step_counts={k: int(v) for k, v in sc.items()},
+                   unit_cost_model_s=model_s, model_over_measured=model_s / full_s)
     with open(out_path, "w") as f:
-        json.dump(dict(t_pcg_iter=t_pcg, t_eval=t_eval, t_tcg_iter=t_tcg, setup_s=t_setup,
-                       sample_s=sample_s, n_pcg=it, n_tcg=n_tcg, cores=1,
-                       host_cores=len(os.sched_getaffinity(0)), threads_env=threads), f)
+        json.dump(res, f)
 
 
 def start_cpu_leg(cfg, seed):
@@ -348,6 +360,88 @@ def avalanche_main(args, learned):
     return 0
 
 
+def precond_eval_main(args):
+    """--precond-eval: the learned preconditioner on the benchmark geometry with a
+    microstructure no network saw (--seed): potential solves b = D m of converged field
+    steps of plain FP64, from zero and from the previous step's potential, by the Jacobi
+    PCG (fused production kernels, and the flexible-CG driver without the network), the
+    flexible CG with the network every `every`-th iteration, and the chosen form's FP32
+    multigrid solve.  One JSON line."""
+    import numpy as np
+    import torch
+
+    from mammos_mumag_amd import Model, _lib
+    from mammos_mumag_amd.generate import build_config, loop_params
+    from mammos_mumag_amd.learned import Network
+
+    g = build_config(args.config, seed=args.seed)
+    p = loop_params(args.config)
+    f64 = Model.from_structured(g)
+    f64.lp_setup()
+    net = Network(f64, max_batch=32, seed=0)
+    meta = net.load(args.precond)
+    log(f"[precond] network {args.precond}: trained on seeds {list(meta.get('seeds', []))}, "
+        f"loss {float(meta.get('loss_first', 0)):.3e} -> {float(meta.get('loss_last', 0)):.3e}")
+    mix = Model.from_structured(g, precision="mixed")
+    mix.multigrid(args.multigrid)
+    m = f64.to_device(f64.uniform_m(p["m"]))
+    m_ref, u, gF, st = m.clone(), f64.empty1(), f64.empty3(), _lib.Stats()
+    fields = [p["hstart"] + k * p["hstep"] for k in range(args.precond_steps)]
+    rows = []
+    u_prev = None
+
+    def timed(fn):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        r = fn()
+        torch.cuda.synchronize()
+        return r, time.perf_counter() - t0
+
+    for k, hext in enumerate(fields):
+        f64.minimize(m, hext, p["h"], p, m_ref, u, gF, st)
+        b = f64.divergence(m)
+        starts = [("zero", f64.empty1())] + ([("previous", u_prev)] if u_prev is not None else [])
+        for sname, x0 in starts:
+            res = dict(step=k, h=hext, start=sname)
+            x = x0.clone()
+            (it, _), t = timed(lambda: f64.pcg(b, x, 1e-10))
+            res["jacobi_fused"] = dict(iters=it, ms=round(1e3 * t, 2))
+            ref = x.clone()
+            x = x0.clone()
+            (it, _, _), t = timed(lambda: f64.lp_pcg(b, x, 1e-10, net=None))
+            res["jacobi_fcg"] = dict(iters=it, ms=round(1e3 * t, 2),
+                                     du=float(torch.max(torch.abs(x - ref)) / torch.max(torch.abs(ref))))
+            for every in args.precond_every:
+                x = x0.clone()
+                (it, apps, rc), t = timed(lambda: f64.lp_pcg(b, x, 1e-10, net=net, every=every))
+                res[f"learned_every{every}"] = dict(iters=it, apps=apps, ms=round(1e3 * t, 2), rc=rc,
+                                                    du=float(torch.max(torch.abs(x - ref)) / torch.max(torch.abs(ref))))
+            xm = x0.clone()
+            (it, _), t = timed(lambda: mix.pcg(b, xm, 1e-10))
+            res["multigrid_fp32"] = dict(iters=it, ms=round(1e3 * t, 2),
+                                         du=float(torch.max(torch.abs(xm - ref)) / torch.max(torch.abs(ref))))
+            log(f"[precond] {res}")
+            rows.append(res)
+        u_prev = u.clone()
+    summary = {}
+    for key in rows[0]:
+        if key in ("step", "h", "start"):
+            continue
+        for sname in ("zero", "previous"):
+            sel = [r[key] for r in rows if r["start"] == sname and key in r]
+            if sel:
+                summary[f"{key}/{sname}"] = dict(iters=round(float(np.mean([v["iters"] for v in sel])), 1),
+                                                 ms=round(float(np.mean([v["ms"] for v in sel])), 2))
+    out = dict(precond_eval=args.config, seed=args.seed, network=os.path.relpath(args.precond, ROOT),
+               trained_on=[int(v) for v in np.ravel(meta.get("seeds", []))], steps=len(fields), summary=summary,
+               rows=rows)
+    print(json.dumps(out), flush=True)
+    net.close()
+    mix.close()
+    f64.close()
+    return 0
+
+
 def roofline(res, precision, args):
     """Dominant PCG kernel of the timed steps: algorithmic bytes / measured launch time."""
     tm = res["timing"]
@@ -421,6 +515,8 @@ def main():
     ap.add_argument("--traffic-probe", action="store_true",
                     help="after the run, copy 512 MiB on the device (calibrates the FETCH_SIZE / WRITE_SIZE counters)")
     ap.add_argument("--cpu-worker", default=None, help=argparse.SUPPRESS)
+    ap.add_argument("--cpu-full-step", action="store_true",
+                    help="only the CPU baseline, plus one complete oracle field step against its unit-cost model")
     ap.add_argument("--loop", action="store_true",
                     help="instead of the timed window: the chosen form and plain FP64 through the reversal")
     ap.add_argument("--loop-after", type=int, default=4, help="--loop: steps run after the reversal")
@@ -433,18 +529,36 @@ def main():
                     help="step index K: FP64 to step K - 1, then step K from that state by FP64 (exact and "
                          "perturbed by 1e-15) and by the chosen form")
     ap.add_argument("--avalanche-seeds", type=int, default=2)
+    ap.add_argument("--precond-eval", action="store_true",
+                    help="the learned preconditioner against Jacobi and multigrid on potential solves (use --seed "
+                         "for a microstructure outside the training set)")
+    ap.add_argument("--precond", default="", help="--precond-eval: the preconditioner network (.npz)")
+    ap.add_argument("--precond-steps", type=int, default=4)
+    ap.add_argument("--precond-every", type=int, nargs="+", default=[16, 64, 256, 1000000],
+                    help="--precond-eval: iterations between the network's Galerkin steps (a large value: "
+                         "at the start only)")
     ap.add_argument("--avalanche-eps", default="1e-15",
                     help="comma-separated relative sizes of the perturbations of m (each with every seed)")
     args = ap.parse_args()
     if args.cpu_worker:
-        cpu_worker(args.config, args.cpu_worker, args.seed)
+        cpu_worker(args.config, args.cpu_worker, args.seed, full_step=args.cpu_full_step)
         return 0
+    if args.cpu_full_step:   # the CPU baseline alone, with one complete oracle field step
+        fd, path = tempfile.mkstemp(suffix=".json", prefix="mumag_cpu_")
+        os.close(fd)
+        env = dict(os.environ, OMP_NUM_THREADS="1", OPENBLAS_NUM_THREADS="1", MKL_NUM_THREADS="1")
+        rc = subprocess.call([sys.executable, str(ROOT / "bench.py"), "--cpu-worker", path, "--cpu-full-step",
+                              "--config", args.config, "--seed", str(args.seed)], env=env)
+        with open(path) as f:
+            print(json.dumps(dict(cpu_full_step=args.config, **json.load(f))), flush=True)
+        os.unlink(path)
+        return rc
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (this build has no multi-GPU path); got --gpus {args.gpus}",
               file=sys.stderr)
         return 2
     cpu = None
-    if not args.no_cpu_baseline and not args.loop and not args.avalanche:
+    if not args.no_cpu_baseline and not args.loop and not args.avalanche and not args.precond_eval:
         cpu = start_cpu_leg(args.config, args.seed)
     learned = None
     if args.learned != "none":
@@ -458,6 +572,10 @@ def main():
         return loop_main(args, learned)
     if args.avalanche:
         return avalanche_main(args, learned)
+    if args.precond_eval:
+        if not args.precond:
+            args.precond = str(ROOT / "mammos-mumag_amd" / "mammos_mumag_amd" / "data" / f"precond_{args.config}.npz")
+        return precond_eval_main(args)
     mgl = args.multigrid if args.precision == "mixed" and not args.general else 0
     res = gpu_run(args, args.precision, traffic_probe=args.traffic_probe, learned=learned, multigrid=mgl)
     ablate = None

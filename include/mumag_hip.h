@@ -263,6 +263,33 @@ int mm_warm_start_policy(mm_ctx *ctx, int32_t policy);
 int mm_warm_start_report(mm_ctx *ctx, int64_t *solves, int64_t *warm_solves, double *mean_s_plain,
                          double *mean_s_warm);
 
+/* ---- Learned preconditioner (no counterpart in the reference, whose potential solve is
+ * Jacobi-preconditioned, solvers.py:222-231): a second network of the same architecture
+ * (mm_mlp_create) maps features of the residual r over a node's neighbourhood to the error
+ * e = K^-1 r at the node, and serves as the preconditioner of a flexible CG.  Structured
+ * (Kuhn grid) contexts only.
+ *
+ * Residual features (bf16, node-major [N][MM_MLP_IN]): w = r / diag(K) of the node and of
+ * its neighbours one and two nodes away along each axis (13 values, 0 outside the grid),
+ * over s = RMS(w); the six adjacent spacings over their mean; the face flag; the magnetic
+ * fraction of the 8 adjacent voxels; the coordinates over the half box edge; 0 padding.
+ * The network's output y gives z = scale s y (0 on the faces). */
+/* Allocates the learned preconditioner's workspace (a set-up call). */
+int mm_lp_setup(mm_ctx *ctx);
+/* Residual features of r (device f64[N]) into X (device u16[N * MM_MLP_IN]). */
+int mm_lp_features(mm_ctx *ctx, const double *r, uint16_t *X);
+/* FP64 Jacobi CG on K from x_inout with the stop test of mm_pcg (r.r <= tol^2 b.b, the
+ * recursive residual; jax CG semantics, solvers.py:222-231), in which the network (net,
+ * output scale `scale`) contributes at the start and after every `every`-th iteration
+ * (0: never, which is the Jacobi PCG): a Galerkin step along its predicted error d,
+ * x += gamma d with gamma = d.r / d.K d (the K-norm of the error can only decrease), the
+ * first step of a flexible CG whose preconditioner alternates between the network and
+ * Jacobi; the CG then restarts from the corrected iterate.  The answer meets the Jacobi
+ * PCG's tolerance; only the iteration count changes.  iters_host counts CG iterations,
+ * lp_apps_host the network's steps.  Returns MM_W_MAXITER at the cap. */
+int mm_lp_pcg(mm_ctx *ctx, mm_mlp *net, double scale, const double *b, double *x_inout, double tol,
+              int32_t maxiter, int32_t every, int32_t *iters_host, int32_t *lp_apps_host);
+
 /* HIP-event timing of the two fused PCG kernels since the last mm_timing_enable: the
  * K apply with the search-direction update (SELL SpMV or 7-point stencil, FP64 or the
  * FP32 inner solve) and the vector update, and with the multigrid preconditioner one

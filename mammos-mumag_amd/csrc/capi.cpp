@@ -60,6 +60,7 @@ struct mm_ctx {
     // MM_PREC_MIXED with eval_precision MM_PREC_MIXED: the energy evaluation's C m, D m and
     // G u in FP32 (structured kernels; the element kernels are FP64 either way)
     bool eval32 = false;
+    int mg_built = 0, mg_built_max = 0;   // levels built by mm_multigrid and the bound they were built for
     int64_t k_nnz = 0, k_entries = 0;
     size_t bytes = 0;
     std::vector<void *> allocs;
@@ -123,6 +124,12 @@ struct mm_ctx {
     int64_t ws_solves = 0, ws_n[2] = {0, 0};
     int64_t ws_every = 16, ws_next = 15;   // re-measurement interval and the next one
     bool ws_probe = false;                 // the current solve re-measures the other form
+    // learned preconditioner (mm_lp_setup): FP64 flexible-PCG vectors, scaled residual,
+    // features and network output
+    double *lp_r = nullptr, *lp_z = nullptr, *lp_p = nullptr, *lp_q = nullptr, *lp_rold = nullptr,
+           *lp_w = nullptr;
+    uint16_t *lp_X = nullptr;
+    float *lp_y = nullptr;
     double ws_mean[2] = {0.0, 0.0};
 
     Launch L() const {
@@ -1370,13 +1377,23 @@ int mm_minimize(mm_ctx *c, double *m_inout, double hext, const double hdir[3],
 
 int mm_multigrid(mm_ctx *c, int32_t max_levels) {
     if (!c) return fail(MM_E_ARG, "null context");
-    if (max_levels == 0) {
+    if (max_levels == 0) {   // back to Jacobi, with Jacobi's reliable-update interval
         c->mg_nl = 0;
+        if (!c->ru_delta_set) c->ru_delta = 0.1;
         return 0;
     }
     if (!c->structured || c->precision != MM_PREC_MIXED)
         return fail(MM_E_STATE, "the multigrid preconditioner needs a structured mixed-precision context");
     if (max_levels < 2 || max_levels > kMgMaxLevels) return fail(MM_E_ARG, "levels outside [2, 8]");
+    // the levels are built once per context: a later call re-enables them (same bound) or
+    // is refused, so that no set-up rewrites levels in use or allocates them again
+    if (c->mg_built > 0) {
+        if (max_levels != c->mg_built_max)
+            return fail(MM_E_STATE, "multigrid already set up with another level bound; use a new context");
+        c->mg_nl = c->mg_built;
+        if (!c->ru_delta_set) c->ru_delta = 1e-3;
+        return 0;
+    }
     static_assert(kMgMaxLevels == MM_MG_MAX_LEVELS, "level bound of the header");
     int rc = 0;
     std::vector<double> ax[3] = {c->axis[0], c->axis[1], c->axis[2]};
@@ -1494,7 +1511,6 @@ int mm_multigrid(mm_ctx *c, int32_t max_levels) {
         }
         HIPC(hipStreamSynchronize(c->s));
     }
-    c->mg_nl = nl;
     // the coarse tail: the most levels (at least one, not level 0) whose vectors fit the
     // tail kernel's LDS
     c->mg_tail = nl;
@@ -1532,6 +1548,9 @@ int mm_multigrid(mm_ctx *c, int32_t max_levels) {
     // a multigrid-preconditioned FP32 recursion stays accurate over a longer stretch:
     // replace the residual every 1e-3 of reduction unless the caller chose otherwise
     if (!c->ru_delta_set) c->ru_delta = 1e-3;
+    c->mg_nl = nl;   // published last: a set-up that failed before leaves the Jacobi form in use
+    c->mg_built = nl;
+    c->mg_built_max = max_levels;
     return 0;
 }
 
@@ -1712,6 +1731,119 @@ int mm_warm_start(mm_ctx *c, mm_mlp *w, double scale) {
     c->ws_have_prev = false;
     c->ws_calls = 0;
     return 0;
+}
+
+// ---------------------------------------------------------------- learned preconditioner
+
+int mm_lp_setup(mm_ctx *c) {
+    if (!c) return fail(MM_E_ARG, "null context");
+    if (!c->structured) return fail(MM_E_STATE, "the learned preconditioner needs a structured (Kuhn grid) context");
+    if (c->lp_r) return 0;
+    const size_t n = (size_t)c->N;
+    if (c->alloc(&c->lp_r, n) || c->alloc(&c->lp_z, n) || c->alloc(&c->lp_p, n) || c->alloc(&c->lp_q, n) ||
+        c->alloc(&c->lp_rold, n) || c->alloc(&c->lp_w, n) || c->alloc(&c->lp_X, n * kMlpIn) ||
+        c->alloc(&c->lp_y, n))
+        return MM_E_HIP;
+    return 0;
+}
+
+namespace {
+// z = net correction of r (in units of the RMS of r / diag K), or z = r / diag K
+void lp_precond(mm_ctx *c, mm_mlp *net, double scale, const double *r, double *z, bool learned) {
+    Launch L = c->L();
+    if (!learned) {
+        launch_lp_scaled(L, c->N, r, c->invDK, z);
+        return;
+    }
+    launch_lp_scaled(L, c->N, r, c->invDK, c->lp_w);
+    launch_reduce(L, RED_SS, c->N, c->lp_w, nullptr, nullptr, nullptr, 0, 0, 0, &c->dsc->red[4]);
+    launch_lp_features(L, c->F, c->P, c->lp_w, &c->dsc->red[4], c->lp_X);
+    launch_mlp_forward(L, net->dev(), c->N, c->lp_X, c->lp_y);
+    launch_lp_apply(L, c->F, c->lp_y, &c->dsc->red[4], scale, z);
+}
+}  // namespace
+
+int mm_lp_features(mm_ctx *c, const double *r, uint16_t *X) {
+    if (!c || !r || !X) return fail(MM_E_ARG, "null argument");
+    if (!c->lp_r) return fail(MM_E_STATE, "call mm_lp_setup first");
+    Launch L = c->L();
+    launch_lp_scaled(L, c->N, r, c->invDK, c->lp_w);
+    launch_reduce(L, RED_SS, c->N, c->lp_w, nullptr, nullptr, nullptr, 0, 0, 0, &c->dsc->red[4]);
+    launch_lp_features(L, c->F, c->P, c->lp_w, &c->dsc->red[4], X);
+    HIPC(hipStreamSynchronize(c->s));
+    return 0;
+}
+
+int mm_lp_pcg(mm_ctx *c, mm_mlp *net, double scale, const double *b, double *x, double tol, int32_t maxiter,
+              int32_t every, int32_t *iters_host, int32_t *lp_apps_host) {
+    if (!c || !b || !x) return fail(MM_E_ARG, "null argument");
+    if (maxiter < 0 || every < 0) return fail(MM_E_ARG, "maxiter and every must be >= 0");
+    if (!c->lp_r) return fail(MM_E_STATE, "call mm_lp_setup first");
+    if (every > 0 && (!net || net->ctx != c)) return fail(MM_E_ARG, "a network of this context is needed");
+    Launch L = c->L();
+    const int64_t n = c->N;
+    double *r = c->lp_r, *z = c->lp_z, *p = c->lp_p, *q = c->lp_q, *d = c->lp_rold;
+    int rc = 0, it = 0, apps = 0;
+    auto dot = [&](const double *a, const double *bb, double *out) {
+        return reduce(c, RED_DOT, n, a, bb, nullptr, nullptr, out, 1);
+    };
+    double bb = 0, rr = 0, rz = 0;
+    auto residual = [&]() -> int {   // r = b - K x, r.r
+        op_K(c, x, q, nullptr, nullptr);
+        launch_axpby(L, n, 1.0, b, -1.0, q, r, nullptr);
+        return dot(r, r, &rr);
+    };
+    // Galerkin step along the network's predicted error d = N(r): gamma = d.r / d.K d
+    // minimises the K-norm of the error along d, so it can only decrease
+    auto galerkin = [&]() -> int {
+        lp_precond(c, net, scale, r, d, true);
+        ++apps;
+        double dr = 0, dkd = 0;
+        op_K(c, d, q, nullptr, nullptr);
+        int e = dot(d, r, &dr);
+        if (!e) e = dot(d, q, &dkd);
+        if (e) return e;
+        if (dkd > 0.0 && std::isfinite(dr / dkd)) {
+            launch_axpby(L, n, dr / dkd, d, 1.0, x, x, nullptr);
+            return residual();
+        }
+        return 0;
+    };
+    if ((rc = dot(b, b, &bb)) || (rc = residual())) return rc;
+    const double bound = tol * tol * bb;
+    bool restart = true;
+    while (rr > bound && it < maxiter) {
+        if (restart) {   // (re)start the Jacobi CG from the current iterate
+            if (every > 0 && (rc = galerkin())) return rc;
+            if (!(rr > bound)) break;
+            lp_precond(c, net, scale, r, z, false);
+            if ((rc = dot(r, z, &rz))) return rc;
+            HIPC(hipMemcpyAsync(p, z, n * sizeof(double), hipMemcpyDeviceToDevice, c->s));
+            restart = false;
+        }
+        double pq = 0;
+        op_K(c, p, q, nullptr, nullptr);
+        if ((rc = dot(p, q, &pq))) return rc;
+        const double alpha = rz / pq;
+        launch_axpby(L, n, alpha, p, 1.0, x, x, nullptr);    // x += alpha p
+        launch_axpby(L, n, -alpha, q, 1.0, r, r, nullptr);   // r -= alpha q
+        ++it;
+        if ((rc = dot(r, r, &rr))) return rc;
+        if (!(rr > bound)) break;
+        if (every > 0 && it % every == 0) {
+            restart = true;
+            continue;
+        }
+        double rz_new = 0;
+        lp_precond(c, net, scale, r, z, false);
+        if ((rc = dot(r, z, &rz_new))) return rc;
+        launch_axpby(L, n, 1.0, z, rz_new / rz, p, p, nullptr);   // p = z + beta p
+        rz = rz_new;
+    }
+    HIPC(hipStreamSynchronize(c->s));
+    if (iters_host) *iters_host = it;
+    if (lp_apps_host) *lp_apps_host = apps;
+    return rr > bound ? MM_W_MAXITER : 0;
 }
 
 int mm_warm_start_policy(mm_ctx *c, int32_t policy) {

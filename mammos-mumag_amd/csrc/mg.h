@@ -68,9 +68,23 @@ void launch_mg_smooth2(const Launch &L, const PoissonGrid<float> &P, const float
 void launch_mg_restrict(const Launch &L, const MgLevelDev &f, const MgLevelDev &c, const float *rf, float *bc,
                         float *x1c, const int *skip);
 bool mg_stencil_fits(const PoissonGrid<float> &P);
-// the x restriction lists (ri, rw: 3 pairs per coarse index, nc coarse indices) keep every
-// used fine index of 64 consecutive coarse nodes inside the restriction kernel's staged row
-bool mg_restrict_span_fits(const int *ri, const float *rw, int nc);
+// The restriction kernel's wave covers kMgRestrictTX consecutive coarse x indices and
+// stages the fine nodes they use in one row of kMgRestrictSpan nodes from the first coarse
+// node's first fine index (2 * 63 + 3 fine indices are enough when every coarse interval
+// covers at most two fine ones).  The x restriction lists (ri, rw: 3 pairs per coarse
+// index, nc coarse indices) fit when every used fine index lies in that row.
+constexpr int kMgRestrictTX = 64, kMgRestrictSpan = 132;
+inline bool mg_restrict_span_fits(const int *ri, const float *rw, int nc) {
+    for (int i0 = 0; i0 < nc; i0 += kMgRestrictTX) {
+        const int base = ri[3 * i0];
+        const int i1 = i0 + kMgRestrictTX < nc ? i0 + kMgRestrictTX : nc;
+        for (int I = i0; I < i1; ++I)
+            for (int q = 0; q < 3; ++q)
+                if (rw[3 * I + q] != 0.f && (ri[3 * I + q] < base || ri[3 * I + q] - base >= kMgRestrictSpan))
+                    return false;
+    }
+    return true;
+}
 void launch_mg_prolong(const Launch &L, const MgLevelDev &f, const MgLevelDev &c, const float *xc, float *xf,
                        const int *skip);
 

@@ -31,7 +31,7 @@ __device__ int kNoSkip = 0;
 
 // Column tiles of the restriction kernel: kMgTX x kMgTY (x, y) columns per 256-thread block
 // (64 x 4 measured faster than 32 x 8 on the 154^3 grid)
-constexpr int kMgTX = 64, kMgTY = 256 / kMgTX;
+constexpr int kMgTX = kMgRestrictTX, kMgTY = 256 / kMgTX;
 
 // Logical tile of this block in a 1-D launch of gx * gy * gz blocks.  Blocks are
 // dispatched round-robin over the 8 XCDs (block b on XCD b % 8), so each XCD is given a
@@ -291,7 +291,7 @@ __global__ __launch_bounds__(256) void k_mg_smooth2(PoissonGrid<float> P, const 
 // nodes per row, so every load is contiguous; the x pairs are then read from that
 // combined row in LDS.  Loads are clamped into the fine row and LDS reads into the span,
 // so every access stays in bounds; unused pairs (weight 0) add exactly zero.
-constexpr int kRSpan = 132;
+constexpr int kRSpan = kMgRestrictSpan;
 __global__ __launch_bounds__(256) void k_mg_restrict(MgGrid G, int nxc, int nyc, int nzc, int nxf, int nyf,
                                                      const int *__restrict__ rxi, const float *__restrict__ rxw,
                                                      const int *__restrict__ ryi, const float *__restrict__ ryw,
@@ -857,15 +857,6 @@ void launch_mg_restrict(const Launch &L, const MgLevelDev &f, const MgLevelDev &
     const MgGrid g = tile_grid(c.P);
     hipLaunchKernelGGL(k_mg_restrict, blocks(g), dim3(256), 0, L.s, g, c.P.nx, c.P.ny, c.P.nz, f.P.nx,
                        f.P.ny, f.rxi, f.rxw, f.ryi, f.ryw, f.rzi, f.rzw, rf, bc, c.invw, x1c, skip);
-}
-bool mg_restrict_span_fits(const int *ri, const float *rw, int nc) {
-    for (int i0 = 0; i0 < nc; i0 += kMgTX) {
-        const int base = ri[3 * i0];
-        for (int I = i0; I < std::min(nc, i0 + kMgTX); ++I)
-            for (int q = 0; q < 3; ++q)
-                if (rw[3 * I + q] != 0.f && (ri[3 * I + q] < base || ri[3 * I + q] - base >= kRSpan)) return false;
-    }
-    return true;
 }
 void launch_mg_prolong(const Launch &L, const MgLevelDev &f, const MgLevelDev &c, const float *xc, float *xf,
                        const int *skip) {

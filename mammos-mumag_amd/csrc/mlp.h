@@ -61,6 +61,12 @@ void launch_mlp_publish(const Launch &L, const float *p, uint16_t *w1t, uint16_t
 void launch_mlp_features(const Launch &L, const FeatGrid &F, const double *m, const double *mbar, uint16_t *X);
 void launch_mlp_direction(const Launch &L, const FeatGrid &F, const float *pred, const float *prev, const double *u,
                           double s, double *d);
+// learned preconditioner: w = r / diag(K); residual features of w (ss = w.w on the device);
+// z = c s y with s = sqrt(ss / N) and c the network's output scale, zero on the faces
+void launch_lp_scaled(const Launch &L, int64_t n, const double *r, const double *invD, double *w);
+void launch_lp_features(const Launch &L, const FeatGrid &F, const PoissonGrid<double> &P, const double *w,
+                        const double *ss, uint16_t *X);
+void launch_lp_apply(const Launch &L, const FeatGrid &F, const float *y, const double *ss, double c, double *z);
 void launch_mlp_galerkin(const Launch &L, int64_t n, const double *num, const double *den, const double *d,
                          double *u, double *gamma_out);
 

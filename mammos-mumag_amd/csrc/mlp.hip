@@ -512,4 +512,105 @@ void launch_mlp_galerkin(const Launch &L, int64_t n, const double *num, const do
                        gamma_out);
 }
 
+// ------------------------------------------------------------------ learned preconditioner
+// Features of the residual network (a second instance of the same MLP): per node, the
+// Jacobi-scaled residual w = r / diag(K) of the node and of its neighbours one and two
+// nodes away along each axis (13 values, 0 outside the grid), divided by s = RMS(w); the
+// six adjacent spacings over their mean; the face (Dirichlet) flag; the magnetic fraction
+// of the 8 adjacent voxels; the coordinates over the half box edge.  Output y: the
+// predicted error e = K^-1 r at the node in units of c s, c the network's output scale
+// (z = c s y, 0 on the faces).
+__global__ void k_lp_scaled(int64_t n, const double *__restrict__ r, const double *__restrict__ invD,
+                            double *__restrict__ w) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        w[i] = r[i] * invD[i];
+}
+
+__global__ void k_lp_features(FeatGrid F, PoissonGrid<double> P, const MatCoef *__restrict__ mats,
+                              const double *__restrict__ w, const double *__restrict__ ss, uint16_t *__restrict__ X) {
+    const int64_t N = (int64_t)F.nx * F.ny * F.nz;
+    const double s2 = *ss / (double)N;
+    const double inv_s = s2 > 0.0 ? 1.0 / sqrt(s2) : 0.0;
+    const int64_t sy = F.nx, sz = (int64_t)F.nx * F.ny;
+    for (int64_t idx = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; idx < N; idx += (int64_t)gridDim.x * blockDim.x) {
+        const int i = (int)(idx % F.nx);
+        const int j = (int)((idx / F.nx) % F.ny);
+        const int k = (int)(idx / sz);
+        float f[kMlpIn];
+        auto at = [&](int di, int dj, int dk) -> float {
+            const int a = i + di, b = j + dj, c = k + dk;
+            if (a < 0 || b < 0 || c < 0 || a >= F.nx || b >= F.ny || c >= F.nz) return 0.f;
+            return (float)(w[idx + di + sy * dj + sz * dk] * inv_s);
+        };
+        f[0] = at(0, 0, 0);
+        f[1] = at(-1, 0, 0); f[2] = at(1, 0, 0);
+        f[3] = at(0, -1, 0); f[4] = at(0, 1, 0);
+        f[5] = at(0, 0, -1); f[6] = at(0, 0, 1);
+        f[7] = at(-2, 0, 0); f[8] = at(2, 0, 0);
+        f[9] = at(0, -2, 0); f[10] = at(0, 2, 0);
+        f[11] = at(0, 0, -2); f[12] = at(0, 0, 2);
+        const double h[6] = {P.hxp[i], P.hxp[i + 1], P.hyp[j], P.hyp[j + 1], P.hzp[k], P.hzp[k + 1]};
+        double hs = 0.0;
+        int hc = 0;
+#pragma unroll
+        for (int q = 0; q < 6; ++q)
+            if (h[q] > 0.0) {
+                hs += h[q];
+                ++hc;
+            }
+        const double ih = hc ? hc / hs : 0.0;
+#pragma unroll
+        for (int q = 0; q < 6; ++q) f[13 + q] = (float)(h[q] * ih);
+        const bool face = i == 0 || j == 0 || k == 0 || i == F.nx - 1 || j == F.ny - 1 || k == F.nz - 1;
+        f[19] = face ? 1.f : 0.f;
+        int nmag = 0;
+        for (int d = 0; d < 8; ++d) {
+            const int vi = i - 1 + (d & 1) - F.m0x, vj = j - 1 + ((d >> 1) & 1) - F.m0y,
+                      vk = k - 1 + ((d >> 2) & 1) - F.m0z;
+            if (vi < 0 || vj < 0 || vk < 0 || vi >= F.mn || vj >= F.mn || vk >= F.mn) continue;
+            if (mats[F.vmat[((int64_t)vk * F.mn + vj) * F.mn + vi]].Js > 0.0) ++nmag;
+        }
+        f[20] = nmag * 0.125f;
+        f[21] = (float)(F.x[i] * F.inv_half);
+        f[22] = (float)(F.y[j] * F.inv_half);
+        f[23] = (float)(F.z[k] * F.inv_half);
+#pragma unroll
+        for (int q = 24; q < kMlpIn; ++q) f[q] = 0.f;
+        uint16_t *o = X + idx * kMlpIn;
+#pragma unroll
+        for (int q = 0; q < kMlpIn; q += 4) {
+            bf16x4 v = {(short)f2bf(f[q]), (short)f2bf(f[q + 1]), (short)f2bf(f[q + 2]), (short)f2bf(f[q + 3])};
+            *(bf16x4 *)(o + q) = v;
+        }
+    }
+}
+
+// z = c s y on the interior, 0 on the faces (s = RMS of the scaled residual, from ss; c the
+// network's output scale).
+__global__ void k_lp_apply(int nx, int ny, int nz, const float *__restrict__ y, const double *__restrict__ ss,
+                           double c, double *__restrict__ z) {
+    const int64_t N = (int64_t)nx * ny * nz;
+    const double s2 = *ss / (double)N;
+    const double s = s2 > 0.0 ? c * sqrt(s2) : 0.0;
+    for (int64_t idx = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; idx < N; idx += (int64_t)gridDim.x * blockDim.x) {
+        const int i = (int)(idx % nx), j = (int)((idx / nx) % ny), k = (int)(idx / ((int64_t)nx * ny));
+        const bool face = i == 0 || j == 0 || k == 0 || i == nx - 1 || j == ny - 1 || k == nz - 1;
+        z[idx] = face ? 0.0 : s * (double)y[idx];
+    }
+}
+
+void launch_lp_scaled(const Launch &L, int64_t n, const double *r, const double *invD, double *w) {
+    hipLaunchKernelGGL(k_lp_scaled, dim3(grid_blocks(n, 256, 4096)), dim3(256), 0, L.s, n, r, invD, w);
+}
+void launch_lp_features(const Launch &L, const FeatGrid &F, const PoissonGrid<double> &P, const double *w,
+                        const double *ss, uint16_t *X) {
+    const int64_t N = (int64_t)F.nx * F.ny * F.nz;
+    hipLaunchKernelGGL(k_lp_features, dim3(grid_blocks(N, 256, 8192)), dim3(256), 0, L.s, F, P, L.mats, w, ss, X);
+}
+void launch_lp_apply(const Launch &L, const FeatGrid &F, const float *y, const double *ss, double c, double *z) {
+    const int64_t N = (int64_t)F.nx * F.ny * F.nz;
+    hipLaunchKernelGGL(k_lp_apply, dim3(grid_blocks(N, 256, 4096)), dim3(256), 0, L.s, F.nx, F.ny, F.nz, y, ss, c,
+                       z);
+}
+
 }  // namespace mm

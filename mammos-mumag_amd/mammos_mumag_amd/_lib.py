@@ -122,6 +122,10 @@ SIGNATURES = {
     "mm_warm_start_stats": [VP, ctypes.POINTER(ctypes.c_int64), c_double_p],
     "mm_warm_start_policy": [VP, ctypes.c_int32],
     "mm_warm_start_report": [VP, VP, VP, VP, VP],
+    "mm_lp_setup": [VP],
+    "mm_lp_features": [VP, VP, VP],
+    "mm_lp_pcg": [VP, VP, ctypes.c_double, VP, VP, ctypes.c_double, ctypes.c_int32, ctypes.c_int32, c_int32_p,
+                  c_int32_p],
 }
 
 MLP_IN = 32

@@ -10,6 +10,15 @@ generates along hysteresis loops of other seeded microstructures, and used throu
 the change of the prediction, then the Jacobi PCG finishes to the unchanged tolerance —
 the network changes the iteration count, never the answer.
 
+A second network of the same architecture is the learned preconditioner
+(``Model.lp_pcg``): it maps features of a residual r over a node's neighbourhood (the
+Jacobi-scaled residual of the node and of its neighbours one and two nodes away along each
+axis, the local spacings, the face flag, the magnetic fraction, the coordinates) to the
+error e = K^-1 r at the node, and preconditions a flexible CG whose stop test is the Jacobi
+PCG's.  It is trained on (r, e) pairs from the same plain loops: at each field step, the
+residuals after j Jacobi-PCG iterations from zero and their errors against the converged
+potential.
+
 Structured (Kuhn grid) contexts only.
 """
 import ctypes
@@ -140,11 +149,73 @@ def collect(model, params, steps, nodes_per_step, seed=0, hstep=None, log=None):
     return torch.cat(Xs), torch.cat(us)
 
 
-def train(net, X, u, steps=2000, batch=16384, lr=1e-3, lr_min=1e-5, seed=0, log=None, log_every=200):
-    """Adam on mean((net(X) - u / scale)^2) with a cosine learning-rate schedule;
-    ``net.scale`` is set to the standard deviation of u.  Returns the loss history."""
+def lp_features(model, r, X=None):
+    """bf16 residual-feature rows (as int16) of every node for the residual r (device f64[N])."""
     torch = _torch()
-    net.scale = float(u.double().std().item()) or 1.0
+    X = torch.empty((model.N, N_IN), dtype=torch.int16, device=r.device) if X is None else X
+    model._sync()
+    _lib.check(model._lib.mm_lp_features(model._ctx, _ptr(r, model.N), _ptr(X, model.N * N_IN, "int16")),
+               "mm_lp_features")
+    return X
+
+
+def collect_both(model, params, steps, hstep, nodes_ws, nodes_lp, lp_iters=(0, 8, 32, 128, 512), seed=0,
+                 log=None):
+    """Training samples of both networks along the plain FP64 loop of ``model`` (``steps``
+    field steps of ``hstep`` from hstart): per converged field step, (features of m, u) at
+    ``nodes_ws`` random nodes for the warm start, and for each j in ``lp_iters`` the
+    residual r_j = b - K x_j after j Jacobi-PCG iterations from zero (b = D m) with its
+    error e_j = u - x_j: (features of r_j, e_j / s_j) at ``nodes_lp`` random nodes, s_j the
+    RMS of r_j / diag(K).  Returns (X_ws, u, X_lp, t_lp)."""
+    torch = _torch()
+    from . import _lib as L
+    dev = f"cuda:{model.device}"
+    gen = torch.Generator(device=dev)
+    gen.manual_seed(int(seed))
+    model.lp_setup()
+    dex, dk = np.empty(3 * model.N), np.empty(model.N)
+    _lib.check(model._lib.mm_get_diag_host(model._ctx, dex.ctypes.data_as(_lib.c_double_p),
+                                           dk.ctypes.data_as(_lib.c_double_p)), "diag")
+    invd = torch.tensor(1.0 / dk, dtype=torch.float64, device=dev)
+    m = model.to_device(model.uniform_m(params["m"]))
+    m_ref, u, gF = m.clone(), model.empty1(), model.empty3()
+    X = torch.empty((model.N, N_IN), dtype=torch.int16, device=dev)
+    Xw, uw, Xl, tl = [], [], [], []
+    hext = params["hstart"]
+    st = L.Stats()
+    for k in range(steps):
+        model.minimize(m, hext, params["h"], params, m_ref, u, gF, st)
+        features(model, m, X)
+        idx = torch.randint(0, model.N, (nodes_ws,), generator=gen, device=dev)
+        Xw.append(X[idx].clone())
+        uw.append(u[idx].float())
+        b = model.divergence(m)
+        for j in lp_iters:
+            xj = model.empty1()
+            if j:
+                model.pcg(b, xj, 1e-30, maxiter=j)
+            r = b - model.apply_K(xj)
+            e = u - xj
+            sj = float(torch.sqrt(torch.mean((r * invd) ** 2)))
+            if not sj > 0:
+                continue
+            lp_features(model, r, X)
+            idx = torch.randint(0, model.N, (nodes_lp,), generator=gen, device=dev)
+            Xl.append(X[idx].clone())
+            tl.append((e[idx] / sj).float())
+        if log:
+            log(f"[collect] step {k}: h = {hext:.2f} T, M.h = {model.compute_mh(m, params['h']):.4f}")
+        hext += hstep
+    return torch.cat(Xw), torch.cat(uw), torch.cat(Xl), torch.cat(tl)
+
+
+def train(net, X, u, steps=2000, batch=16384, lr=1e-3, lr_min=1e-5, seed=0, log=None, log_every=200,
+          scale=None):
+    """Adam on mean((net(X) - u / scale)^2) with a cosine learning-rate schedule;
+    ``net.scale`` is set to ``scale`` or else the standard deviation of u.  Returns the
+    loss history."""
+    torch = _torch()
+    net.scale = float(scale) if scale is not None else (float(u.double().std().item()) or 1.0)
     t = (u / net.scale).float().contiguous()
     gen = torch.Generator(device=X.device)
     gen.manual_seed(int(seed))
@@ -197,6 +268,48 @@ def train_for_config(config, seeds, steps, hstep, nodes_per_step, train_steps, b
     return hist
 
 
+def train_both(config, seeds, steps, hstep, nodes_ws, nodes_lp, train_steps, batch, lr, out_ws, out_lp,
+               log=lambda msg: print(msg, flush=True)):
+    """Both networks for a generated workload from the plain FP64 loops of the
+    microstructures ``seeds``: the warm start on (m, u) snapshots, the preconditioner on
+    (r, e) snapshots (its targets are already in units of the residual scale)."""
+    torch = _torch()
+    from .generate import build_config, loop_params
+    from .model import Model
+    p = loop_params(config)
+    parts = []
+    t0 = time.perf_counter()
+    for sd in seeds:
+        model = Model.from_structured(build_config(config, seed=sd))
+        parts.append(collect_both(model, p, steps, hstep, nodes_ws, nodes_lp, seed=sd,
+                                  log=lambda msg: log(f"{msg} (seed {sd}, {time.perf_counter() - t0:.1f} s)")
+                                  if msg.startswith("[collect] step 0") else None))
+        model.close()
+        log(f"[learned] seed {sd}: {parts[-1][0].shape[0]} warm-start and {parts[-1][2].shape[0]} "
+            f"preconditioner samples ({time.perf_counter() - t0:.1f} s)")
+    Xw = torch.cat([q[0] for q in parts])
+    uw = torch.cat([q[1] for q in parts])
+    Xl = torch.cat([q[2] for q in parts])
+    tl = torch.cat([q[3] for q in parts])
+    del parts
+    model = Model.from_structured(build_config(config, seed=seeds[0]))
+    snaps = dict(config=config, seeds=np.asarray(seeds), steps=steps, hstep=hstep, train_steps=train_steps,
+                 batch=batch, lr=lr)
+    hists = []
+    for name, X, t, out, scale in (("warm start", Xw, uw, out_ws, None), ("preconditioner", Xl, tl, out_lp, None)):
+        net = Network(model, max_batch=batch, seed=0)
+        hist = train(net, X, t, steps=train_steps, batch=batch, lr=lr, log=log,
+                     log_every=max(train_steps // 10, 1), scale=scale)
+        net.save(out, samples=np.int64(X.shape[0]), snapshots=np.int64(len(seeds) * steps *
+                 (1 if name == "warm start" else 5)), loss_first=hist[0][1], loss_last=hist[-1][1], **snaps)
+        log(f"[learned] {name}: {X.shape[0]} samples, loss {hist[0][1]:.3e} -> {hist[-1][1]:.3e}, saved {out} "
+            f"({time.perf_counter() - t0:.1f} s)")
+        net.close()
+        hists.append(hist)
+    model.close()
+    return hists
+
+
 if __name__ == "__main__":
     import argparse
     import pathlib
@@ -210,7 +323,18 @@ if __name__ == "__main__":
     ap.add_argument("--batch", type=int, default=32768)
     ap.add_argument("--lr", type=float, default=1e-3)
     ap.add_argument("--out", default=None)
+    ap.add_argument("--both", action="store_true",
+                    help="train the warm start and the learned preconditioner together (data/warmstart_*.npz, "
+                         "data/precond_*.npz)")
+    ap.add_argument("--nodes-lp", type=int, default=4096, help="--both: sampled nodes per residual snapshot")
+    ap.add_argument("--out-dir", default=None, help="--both: directory of the two .npz files (default: the package's)")
     a = ap.parse_args()
+    if a.both:
+        d = pathlib.Path(a.out_dir) if a.out_dir else pathlib.Path(__file__).resolve().parent / "data"
+        d.mkdir(exist_ok=True)
+        train_both(a.config, a.seeds, a.steps, a.hstep, a.nodes, a.nodes_lp, a.train_steps, a.batch, a.lr,
+                   str(d / f"warmstart_{a.config}.npz"), str(d / f"precond_{a.config}.npz"))
+        raise SystemExit(0)
     out = a.out or str(pathlib.Path(__file__).resolve().parent / "data" / f"warmstart_{a.config}.npz")
     pathlib.Path(out).parent.mkdir(exist_ok=True)
     train_for_config(a.config, a.seeds, a.steps, a.hstep, a.nodes, a.train_steps, a.batch, a.lr, out)

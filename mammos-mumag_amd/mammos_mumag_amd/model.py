@@ -431,6 +431,23 @@ class Model:
                                                   ctypes.byref(tw)), "mm_warm_start_report")
         return dict(solves=n.value, warm_solves=nw.value, mean_s_plain=tp.value, mean_s_warm=tw.value)
 
+    def lp_setup(self):
+        """Workspace of the learned preconditioner (structured contexts)."""
+        _lib.check(self._lib.mm_lp_setup(self._ctx), "mm_lp_setup")
+
+    def lp_pcg(self, b, x, tol, maxiter=10000, net=None, every=1):
+        """Flexible CG with the learned preconditioner ``net`` (``learned.Network``) every
+        ``every``-th iteration, Jacobi otherwise (``every=0`` or ``net=None``: the Jacobi
+        PCG in the same driver); returns (iterations, network applications, rc)."""
+        self._sync()
+        it, apps = ctypes.c_int32(), ctypes.c_int32()
+        rc = _lib.check(self._lib.mm_lp_pcg(self._ctx, None if net is None else net._net,
+                                            1.0 if net is None else float(net.scale), _ptr(b, self.N),
+                                            _ptr(x, self.N), float(tol), int(maxiter),
+                                            int(every) if net is not None else 0, ctypes.byref(it),
+                                            ctypes.byref(apps)), "mm_lp_pcg")
+        return it.value, apps.value, rc
+
     def timing(self, on=True):
         _lib.check(self._lib.mm_timing_enable(self._ctx, int(on)), "mm_timing_enable")
 

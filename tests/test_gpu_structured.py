@@ -243,3 +243,41 @@ def test_mixed_multigrid_pcg_limits():
     it, rc = model.pcg(model.to_device(b), x, 1e-6)
     assert it == 0 and rc == 0 and torch.equal(x, x0)
     model.close()
+
+
+def test_multigrid_setup_toggle():
+    """mm_multigrid(ctx, 0) restores the Jacobi form with its own reliable-update interval
+    (the same iteration count as a context that never had the multigrid); a second set-up
+    with the same bound re-enables the built levels without allocating; another bound is
+    refused."""
+    from mammos_mumag_amd import Model, _lib
+    g = generate.build(edge=39.0, n=39, layers=13, grains=2, cone=15.0, seed=3)
+    rng = np.random.default_rng(8)
+    fresh = Model.from_structured(g, precision="mixed")
+    b = fresh.divergence(fresh.to_device(fresh.state_m([0, 0, 1], 1)))
+    b = b + 0.01 * fresh.to_device(rng.normal(size=fresh.N)) * fresh.to_device((fresh.meas > 0).astype(float))
+    x = fresh.empty1()
+    it_fresh, _ = fresh.pcg(b, x, 1e-10)
+    fresh.close()
+    model = Model.from_structured(g, precision="mixed")
+    model.multigrid(8)
+    levels = model.mg_levels()
+    it_mg, _ = model.pcg(b, model.empty1(), 1e-10)
+    model.multigrid(0)
+    assert model.mg_levels() == []
+    it_back, _ = model.pcg(b, model.empty1(), 1e-10)
+    print(f"[gpu] Jacobi {it_fresh} iterations fresh, {it_back} after multigrid and back; multigrid {it_mg}",
+          flush=True)
+    assert it_back == it_fresh
+    import ctypes
+    info = _lib.Info()
+    _lib.check(model._lib.mm_ctx_info(model._ctx, ctypes.byref(info)), "info")
+    before = info.device_bytes
+    model.multigrid(8)
+    assert model.mg_levels() == levels
+    assert model.pcg(b, model.empty1(), 1e-10)[0] == it_mg
+    _lib.check(model._lib.mm_ctx_info(model._ctx, ctypes.byref(info)), "info")
+    assert info.device_bytes == before
+    with pytest.raises(_lib.MumagError):
+        model.multigrid(6)
+    model.close()
