@@ -241,9 +241,7 @@ void op_C(mm_ctx *c, const double *v, double *y, double *b, const int *skip) {
     if (c->structured) {
         launch_grid_exani(L, c->G, c->eval32, v, y, b, skip);
     } else {
-        launch_fill_zero(L, 3 * c->N, y, skip);
-        if (b) launch_fill_zero(L, c->N, b, skip);
-        launch_ebe_exani(L, c->E, v, y, b, skip);
+        launch_ebe_exani(L, c->E, v, y, b, skip);   // writes every node of y (and b)
     }
 }
 void op_grad(mm_ctx *c, const double *u, double *g) {
@@ -251,8 +249,7 @@ void op_grad(mm_ctx *c, const double *u, double *g) {
     if (c->structured) {
         launch_grid_grad(L, c->G, c->eval32, u, g);
     } else {
-        launch_fill_zero(L, 3 * c->N, g, nullptr);
-        launch_ebe_grad(L, c->E, u, g, 1.0 / c->V);
+        launch_ebe_grad(L, c->E, u, g, 1.0 / c->V);   // writes every node of g
     }
 }
 
@@ -935,6 +932,23 @@ int mm_ctx_create(const mm_mesh_desc *d, const mm_options *opts, mm_ctx **out) {
         c->E.G = dG;
         c->E.vol = dv;
         c->E.mat = dm;
+        {   // node -> slots 4 q + k of its magnet elements, ascending (a fixed summation order)
+            std::vector<int> iptr(c->N + 1, 0), islot(4 * (size_t)c->NEm);
+            for (size_t s4 = 0; s4 < islot.size(); ++s4) iptr[H.mag_tets[s4] + 1]++;
+            for (int64_t i = 0; i < c->N; ++i) iptr[i + 1] += iptr[i];
+            std::vector<int> fill(iptr.begin(), iptr.end() - 1);
+            for (size_t s4 = 0; s4 < islot.size(); ++s4) islot[fill[H.mag_tets[s4]]++] = (int)s4;
+            int *dp, *dsl;
+            double *dcy, *dcb;
+            if ((rc = c->upload(&dp, iptr)) || (rc = c->upload(&dsl, islot)) ||
+                (rc = c->alloc(&dcy, 12 * (size_t)c->NEm)) || (rc = c->alloc(&dcb, 4 * (size_t)c->NEm)))
+                return bail(rc);
+            c->E.nn = c->N;
+            c->E.inc_ptr = dp;
+            c->E.inc_slot = dsl;
+            c->E.cy = dcy;
+            c->E.cb = dcb;
+        }
     }
     // algorithmic bytes of one KA launch of the PCG loop: z and p_old read once, p and
     // q = K p written once, plus the stored matrix for SELL (entries: value + column,

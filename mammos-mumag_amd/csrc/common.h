@@ -57,6 +57,12 @@ struct ElemDev {
     const double *G = nullptr;        // SoA [9][n]: gradients of phi_1..phi_3 (phi_0 = -sum)
     const double *vol = nullptr;      // [n]
     const int *mat = nullptr;         // [n] material index
+    // node-centric sums: nodes [nn]; per node its slots 4 e + k (element e, local node k)
+    // in ascending order, CSR (inc_ptr [nn + 1], inc_slot [4 n]); contribution buffers
+    // cy [12 n] (C v per slot; the gradient uses its first 3 n) and cb [4 n] (D v)
+    int64_t nn = 0;
+    const int *inc_ptr = nullptr, *inc_slot = nullptr;
+    double *cy = nullptr, *cb = nullptr;
 };
 
 // Structured Kuhn grid: the magnet block of mn^3 uniform voxels (edge h) starting at

@@ -500,7 +500,9 @@ void launch_add_correction(const Launch &L, int64_t n, double *x, float *dx) {
 //   anisotropy:  y_k   += (-2K/V) (vol/20) u (u.v_k + sum_l u.v_l)
 //   divergence:  b_k   += Js vol G_k . mean(v)
 //   gradient:    g_k   += Js vol/(4V) grad(u)        (same for the 4 nodes)
-// Accumulated with FP64 atomics into zeroed outputs.
+// Each element writes its contributions (per element and local node; the gradient's once
+// per element) and a node-centric pass sums them in a fixed order, so the result is the
+// same in every run (atomic accumulation would depend on the order of arrival).
 
 struct ElemLocal {
     double G[4][3];
@@ -518,8 +520,8 @@ __device__ __forceinline__ void load_G(const ElemDev &E, int e, ElemLocal &L) {
 
 __global__ __launch_bounds__(kBlock) void k_ebe_exani(ElemDev E, const MatCoef *__restrict__ mats,
                                                       const double *__restrict__ v,
-                                                      double *__restrict__ y,
-                                                      double *__restrict__ b, const int *skip) {
+                                                      double *__restrict__ cy,
+                                                      double *__restrict__ cb, const int *skip) {
     if (skip && *skip) return;
     const int e = blockIdx.x * kBlock + threadIdx.x;
     if (e >= E.n) return;
@@ -555,23 +557,23 @@ __global__ __launch_bounds__(kBlock) void k_ebe_exani(ElemDev E, const MatCoef *
         for (int a = 0; a < 3; ++a) {
             const double yk = cex * (L.G[k][0] * gv[a][0] + L.G[k][1] * gv[a][1] + L.G[k][2] * gv[a][2]) +
                               ck * mc.u[a] * (s[k] + S);
-            unsafeAtomicAdd(&y[3 * (int64_t)tn[k] + a], yk);
+            cy[(4 * (int64_t)e + k) * 3 + a] = yk;
         }
     }
-    if (b) {
+    if (cb) {
         const double js = mc.Js * vol;
         double mb[3];
 #pragma unroll
         for (int a = 0; a < 3; ++a) mb[a] = 0.25 * (vl[0][a] + vl[1][a] + vl[2][a] + vl[3][a]);
 #pragma unroll
         for (int k = 0; k < 4; ++k)
-            unsafeAtomicAdd(&b[tn[k]], js * (L.G[k][0] * mb[0] + L.G[k][1] * mb[1] + L.G[k][2] * mb[2]));
+            cb[4 * (int64_t)e + k] = js * (L.G[k][0] * mb[0] + L.G[k][1] * mb[1] + L.G[k][2] * mb[2]);
     }
 }
 
 __global__ __launch_bounds__(kBlock) void k_ebe_grad(ElemDev E, const MatCoef *__restrict__ mats,
                                                      const double *__restrict__ u,
-                                                     double *__restrict__ g, double invV) {
+                                                     double *__restrict__ cg3, double invV) {
     const int e = blockIdx.x * kBlock + threadIdx.x;
     if (e >= E.n) return;
     const int4 t = E.tets[e];
@@ -587,21 +589,69 @@ __global__ __launch_bounds__(kBlock) void k_ebe_grad(ElemDev E, const MatCoef *_
     for (int c = 0; c < 3; ++c)
         gu[c] = cg * (L.G[0][c] * ul[0] + L.G[1][c] * ul[1] + L.G[2][c] * ul[2] + L.G[3][c] * ul[3]);
 #pragma unroll
-    for (int k = 0; k < 4; ++k)
+    for (int a = 0; a < 3; ++a) cg3[3 * (int64_t)e + a] = gu[a];
+}
+
+// Node-centric sums of the element contributions, in a fixed order: out[C i + a] = sum over
+// the slots s of node i (4 e + k for its magnet elements e, ascending) of c[C (s / D) + a]
+// (D = 1: one contribution per element and local node; D = 4: one per element, the same
+// for its four nodes).  kGatherLanes lanes share a node: lane l takes the node's slots l,
+// l + kGatherLanes, ..., then a butterfly over the lanes adds the partial sums.  Every
+// node is written.
+constexpr int kGatherLanes = 8;
+
+template <int C, int D>
+__global__ __launch_bounds__(kBlock) void k_ebe_gather(int64_t n, const int *__restrict__ ptr,
+                                                       const int *__restrict__ slot, const double *__restrict__ c,
+                                                       double *__restrict__ out, const int *skip) {
+    if (skip && *skip) return;
+    const int lane = threadIdx.x & (kGatherLanes - 1);
+    const int64_t stride = (int64_t)gridDim.x * (kBlock / kGatherLanes);
+    // every lane of a group holds the same i: the butterfly stays inside active groups
+    for (int64_t i = blockIdx.x * (int64_t)(kBlock / kGatherLanes) + threadIdx.x / kGatherLanes; i < n;
+         i += stride) {
+        double acc[C];
 #pragma unroll
-        for (int a = 0; a < 3; ++a) unsafeAtomicAdd(&g[3 * (int64_t)tn[k] + a], gu[a]);
+        for (int a = 0; a < C; ++a) acc[a] = 0.0;
+        const int q1 = ptr[i + 1];
+        for (int q = ptr[i] + lane; q < q1; q += kGatherLanes) {
+            const int64_t s = slot[q] / D;
+#pragma unroll
+            for (int a = 0; a < C; ++a) acc[a] += c[C * s + a];
+        }
+#pragma unroll
+        for (int off = kGatherLanes / 2; off > 0; off >>= 1)
+#pragma unroll
+            for (int a = 0; a < C; ++a) acc[a] += __shfl_xor(acc[a], off, kGatherLanes);
+        if (lane == 0)
+#pragma unroll
+            for (int a = 0; a < C; ++a) out[C * i + a] = acc[a];
+    }
+}
+
+static inline int gather_grid(int64_t nodes) {
+    const int64_t b = (nodes * kGatherLanes + kBlock - 1) / kBlock;
+    return (int)std::max<int64_t>(1, std::min<int64_t>(b, 16384));
 }
 
 void launch_ebe_exani(const Launch &L, const ElemDev &E, const double *v, double *y, double *b,
                       const int *skip_flag) {
-    if (E.n == 0) return;
-    hipLaunchKernelGGL(k_ebe_exani, dim3((E.n + kBlock - 1) / kBlock), dim3(kBlock), 0, L.s, E,
-                       L.mats, v, y, b, skip_flag);
+    if (E.n > 0)
+        hipLaunchKernelGGL(k_ebe_exani, dim3((E.n + kBlock - 1) / kBlock), dim3(kBlock), 0, L.s, E,
+                           L.mats, v, E.cy, b ? E.cb : nullptr, skip_flag);
+    const dim3 g(gather_grid(E.nn));
+    hipLaunchKernelGGL((k_ebe_gather<3, 1>), g, dim3(kBlock), 0, L.s, E.nn, E.inc_ptr, E.inc_slot, E.cy, y,
+                       skip_flag);
+    if (b)
+        hipLaunchKernelGGL((k_ebe_gather<1, 1>), g, dim3(kBlock), 0, L.s, E.nn, E.inc_ptr, E.inc_slot, E.cb, b,
+                           skip_flag);
 }
 void launch_ebe_grad(const Launch &L, const ElemDev &E, const double *u, double *g, double invV) {
-    if (E.n == 0) return;
-    hipLaunchKernelGGL(k_ebe_grad, dim3((E.n + kBlock - 1) / kBlock), dim3(kBlock), 0, L.s, E,
-                       L.mats, u, g, invV);
+    if (E.n > 0)
+        hipLaunchKernelGGL(k_ebe_grad, dim3((E.n + kBlock - 1) / kBlock), dim3(kBlock), 0, L.s, E,
+                           L.mats, u, E.cy, invV);
+    hipLaunchKernelGGL((k_ebe_gather<3, 4>), dim3(gather_grid(E.nn)), dim3(kBlock), 0, L.s, E.nn, E.inc_ptr,
+                       E.inc_slot, E.cy, g, nullptr);
 }
 
 // ------------------------------------------------------------------ structured magnet kernels

@@ -196,9 +196,10 @@ class Model:
     # -------------------------------------------------------------- construction
     @classmethod
     def from_structured(cls, mesh, structured=True, **kw):
-        """From a ``generate.StructuredMesh``; ``structured=False`` forces the general path."""
+        """From a ``generate.StructuredMesh``; ``structured=False`` forces the general path
+        (a ``generate.SphereMesh`` always takes it: its nodes are not a rectilinear grid)."""
         grid = None
-        if structured:
+        if structured and not getattr(mesh, "spherical", False):
             grid = dict(xs=mesh.xs, ys=mesh.ys, zs=mesh.zs, mag0=[b[0] for b in mesh.magnet_box],
                         magn=mesh.magnet_box[0][1])
         return cls(mesh.coords(), mesh.tets(), mesh.elem_tags(), mesh.krn(), grid=grid, **kw)

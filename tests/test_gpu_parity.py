@@ -137,3 +137,25 @@ def test_loop_matches_reference_fixture(gpu, cube_params, golden):
     z = np.load(golden / "cube_0001_vtu.npz")
     assert sorted(saved) == [1]
     assert np.allclose(z["m"], saved[1][0].reshape(-1, 3))
+
+
+def test_general_path_is_reproducible(gpu, cube_params):
+    """Element contributions are summed per node in a fixed order (no atomics): the
+    operators and a whole loop give the same bits in every run."""
+    from mammos_mumag_amd.loop import run_loop
+    rng = np.random.default_rng(11)
+    v = gpu.to_device(rng.normal(size=3 * gpu.N))
+    x = gpu.to_device(rng.normal(size=gpu.N))
+    for op, arg in ((gpu.apply_C, v), (gpu.divergence, v), (gpu.gradient, x)):
+        a = op(arg).cpu().numpy()
+        for _ in range(3):
+            assert np.array_equal(op(arg).cpu().numpy(), a)
+    runs = []
+    for _ in range(2):
+        saved = {}
+        rec, stats, _ = run_loop(gpu, cube_params, gpu.uniform_m(cube_params["m"]),
+                                 save_callback=lambda c, m, u: saved.setdefault(c, m.cpu().numpy()))
+        runs.append((np.array(rec), stats, saved[1]))
+    assert np.array_equal(runs[0][0], runs[1][0])
+    assert runs[0][1] == runs[1][1]
+    assert np.array_equal(runs[0][2], runs[1][2])
