@@ -181,7 +181,8 @@ def gpu_run(args, precision, traffic_probe=False, learned=None, multigrid=0):
         log("[bench] traffic probe: copied 512 MiB")
     ws = dict(model.warm_start_stats(), **model.warm_start_report()) if learned else None
     res = dict(elapsed=elapsed, stats=st.as_dict(), timing=timing, info=model.info, warm_start=ws,
-               mg_levels=" -> ".join("x".join(map(str, d)) for d in levels),
+               mg_levels=" -> ".join("x".join(map(str, d)) for d in levels), levels=levels,
+               n_voxels=int(CONFIGS_N(args.config)) ** 3,
                n_nodes=n_nodes, n_elems=n_elems, n_mag_nodes=int(np.sum(model.meas > 0)),
                mh=mh, energies=energies, u=u.cpu().numpy(), hd=model.gradient(u).cpu().numpy())
     model.close()
@@ -475,6 +476,59 @@ def roofline(res, precision, args):
                 per_pcg_iteration=k["per_it"])
 
 
+def CONFIGS_N(cfg):
+    from mammos_mumag_amd.generate import CONFIGS
+    return CONFIGS[cfg]["n"]
+
+
+def whole_step(res, precision, steps):
+    """Algorithmic HBM bytes of the timed field steps, by kernel class, from the run's exact
+    counts (PCG and truncated-CG iterations, energy evaluations, V-cycles that ran, true
+    residual passes and corrections) and per-launch byte counts on the structured grid
+    (N nodes, Nb magnet-block nodes, Nv voxels with 2-byte material ids); divided by the
+    measured time per step.  Not counted: the NCG's reductions, retractions and line-search
+    vector updates, finalize launches, copies and the network's inference."""
+    if not res["info"]["structured"]:
+        return None
+    st, tm = res["stats"], res["timing"]
+    N, Nb, Nv = res["n_nodes"], res["n_mag_nodes"], res["n_voxels"]
+    s = 4 if precision == "mixed" else 8   # inner PCG / truncated-CG vectors
+    cls = {}
+    # KA: z, p_old read, p, q written; KB: x, r, p, q, diag (or the V-cycle weights) read,
+    # x, r, z written
+    cls["pcg_iterations"] = st["pcg_iters"] * 12 * s * N
+    if precision == "mixed":
+        lv = res["levels"]
+        if lv and tm["vcycles"] >= 0:
+            n0 = lv[0][0] * lv[0][1] * lv[0][2]
+            coarse = sum(a * b * c for a, b, c in lv[1:])
+            # level 0: 7 smoothing steps and the residual (b, x read, one vector written),
+            # r read by the restriction, x read and written by the prolongation; coarser
+            # levels: 3 smoothing passes (two of them two-step), residual, transfers
+            cls["v_cycles"] = tm["vcycles"] * (108 * n0 + 72 * coarse)
+        # true residual: x, b (FP64), diag (FP32) read, r, z (FP32) written; correction:
+        # x (FP64), dx (FP32) read and written
+        cls["true_residuals"] = tm["ru_passes"] * 28 * N + tm["corrections"] * 24 * N
+    else:
+        # per solve: K x0 (x read, y written) and the initial residual (b, Kx0, diag read;
+        # r, z written)
+        cls["pcg_starts"] = st["pcg_calls"] * (16 + 40) * N
+    # truncated CG (magnet block): d' = -z + beta d, C d', H d' (z, d, m0, m0.gF read; d', Hd
+    # written; voxel materials) and the update (r, Hd, diag, p, d read; p, r, z written)
+    cls["truncated_cg"] = st["tcg_iters"] * ((40 * s) * Nb + 2 * Nv)
+    # evaluation (FP64): C m with D m (m read, gC and b written), G u (u read, g written),
+    # finish (meas, m, gC, gd read; g_full, g_proj written)
+    cls["evaluations"] = st["fcalls"] * (216 * Nb + 4 * Nv)
+    total = sum(cls.values())
+    t_step = res["elapsed"] / steps
+    bps = total / steps
+    return dict(bytes_per_step=int(bps), GBs=round(bps / t_step / 1e9, 1), frac=round(bps / t_step / 1e9 / HBM_PEAK_GBS, 4),
+                by_class_MB_per_step={k: round(v / steps / 1e6, 1) for k, v in cls.items()},
+                counts_per_step=dict(vcycles=round(tm["vcycles"] / steps, 2) if tm["vcycles"] >= 0 else None,
+                                     ru_passes=round(tm["ru_passes"] / steps, 2),
+                                     corrections=round(tm["corrections"] / steps, 2)))
+
+
 def form_errors(r, ref):
     """Errors of a run against the FP64 run over the same field steps."""
     import numpy as np
@@ -647,6 +701,9 @@ def main():
             fp64_per_step={k: round(v / steps, 2) for k, v in ref["stats"].items()},
             errors=form_errors(res, ref))),
         "roofline": roof,
+        "whole_step": dict(chosen=whole_step(res, args.precision, steps),
+                           fp64=None if ref is None else whole_step(ref, "fp64", steps),
+                           mixed_jacobi=None if jacobi is None else whole_step(jacobi, "mixed", steps)),
         "cpu_baseline": cpu_res,
     }
     print(json.dumps(out), flush=True)

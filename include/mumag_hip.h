@@ -306,6 +306,12 @@ typedef struct {
     int64_t mg_smooth_launches;
     double mg_smooth_ms;
     int64_t mg_smooth_bytes;
+    /* counts since mm_timing_enable: multigrid V-cycles that ran (-1: no coarse tail
+     * kernel to count them), true FP64 residual passes and corrections x += dx of the
+     * mixed-precision PCG */
+    int64_t vcycles;
+    int64_t ru_passes;
+    int64_t corrections;
 } mm_kernel_timing;
 int mm_timing_enable(mm_ctx *ctx, int32_t on);
 int mm_timing_get(mm_ctx *ctx, mm_kernel_timing *out);

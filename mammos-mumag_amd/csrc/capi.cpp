@@ -95,6 +95,10 @@ struct mm_ctx {
     // event timing of the PCG kernels (K apply and vector update)
     bool timing = false;
     KernelTimer t_spmv, t_upd, t_mg;   // t_mg: a level-0 smoothing step of the V-cycle
+    // counts since the timing was enabled: true residual passes and corrections of the
+    // mixed PCG (host-counted: they always run), V-cycles (device-counted) at enable time
+    int64_t n_ru = 0, n_corr = 0;
+    long long vcyc0 = 0;
 
     // multigrid preconditioner of the FP32 inner solve (structured grids)
     std::vector<double> axis[3];        // node coordinates per axis (host)
@@ -430,6 +434,7 @@ int pcg_mixed(mm_ctx *c, const double *b, double *x, double tol, double atol, in
     // r = b - K x in FP64 (one pass on the structured grid), its FP32 copy and z
     const bool fused = c->structured && grid_ru_residual_fits(c->P);
     auto true_residual = [&](bool init) {
+        c->n_ru++;
         if (fused) {
             launch_grid_ru_residual(L, c->P, x, b, c->r32, c->z32, c->p32, invD, tol, atol, maxiter, init, seg2);
         } else {
@@ -506,6 +511,7 @@ int pcg_mixed(mm_ctx *c, const double *b, double *x, double tol, double atol, in
         }
         // reliable update: fold dx into x, replace r by the true residual
         launch_add_correction(L, n, x, c->dx32);
+        c->n_corr++;
         true_residual(false);
         if ((rc = sync_scalars(c))) return rc;
         if (c->hsc->ir_done || i >= maxiter) break;
@@ -1543,6 +1549,7 @@ int mm_multigrid(mm_ctx *c, int32_t max_levels) {
         }
         a.b0 = c->mg_b[t];
         a.x0 = c->mg_xa[t];
+        a.count = &c->dsc->mg_vcycles;
         mg_tail_layout(a);
         return a;
     };
@@ -1908,6 +1915,10 @@ int mm_timing_enable(mm_ctx *c, int32_t on) {
         t->launches = 0;
         t->ms = 0.0;
     }
+    c->n_ru = c->n_corr = 0;
+    int rc = sync_scalars(c);
+    if (rc) return rc;
+    c->vcyc0 = c->hsc->mg_vcycles;
     return 0;
 }
 
@@ -1922,6 +1933,11 @@ int mm_timing_get(mm_ctx *c, mm_kernel_timing *o) {
     o->mg_smooth_launches = c->t_mg.launches;
     o->mg_smooth_ms = c->t_mg.ms;
     o->mg_smooth_bytes = c->t_mg.bytes;
+    int rc = sync_scalars(c);
+    if (rc) return rc;
+    o->vcycles = c->mg_nl > 0 && c->mg_tail < c->mg_nl ? (int64_t)(c->hsc->mg_vcycles - c->vcyc0) : -1;
+    o->ru_passes = c->n_ru;
+    o->corrections = c->n_corr;
     return 0;
 }
 

@@ -29,6 +29,8 @@ struct DevScalars {
     // generic reduction slots and the energy of the last evaluation
     double red[8];
     double energy;
+    // multigrid V-cycles that ran (the coarse tail kernel counts itself when not skipped)
+    long long mg_vcycles;
 };
 
 // Per-material coefficients, scaled as the reference scales them

@@ -47,6 +47,7 @@ struct MgTailArgs {
     const float *b0 = nullptr;
     float *x0 = nullptr;
     float om[kMgMaxDeg] = {}, omc[kMgMaxDeg] = {};
+    long long *count = nullptr;   // incremented once per launch that runs (V-cycles)
 };
 int mg_tail_layout(MgTailArgs &a);
 // the laid-out tail fits the kernel: its LDS, and at most one z column per thread per level

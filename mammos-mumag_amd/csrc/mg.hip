@@ -701,6 +701,7 @@ __global__ __launch_bounds__(kTailThreads) void k_mg_tail(MgTailArgs a, const in
     const int sk = *(skip ? skip : &kNoSkip);
     asm volatile("" : "+v"(v.x), "+v"(v.y), "+v"(v.z), "+v"(v.w));
     if (sk) return;
+    if (threadIdx.x == 0 && a.count) *a.count += 1;
     if ((int)threadIdx.x < n4) dst[threadIdx.x] = v;
     for (int t = threadIdx.x + kTailThreads; t < n4; t += kTailThreads) dst[t] = src[t];
     int cur[NT], oth[NT], ob[NT];

@@ -74,7 +74,8 @@ class KernelTiming(ctypes.Structure):
                 ("spmv_bytes", ctypes.c_int64), ("update_launches", ctypes.c_int64),
                 ("update_ms", ctypes.c_double), ("update_bytes", ctypes.c_int64),
                 ("mg_smooth_launches", ctypes.c_int64), ("mg_smooth_ms", ctypes.c_double),
-                ("mg_smooth_bytes", ctypes.c_int64)]
+                ("mg_smooth_bytes", ctypes.c_int64), ("vcycles", ctypes.c_int64),
+                ("ru_passes", ctypes.c_int64), ("corrections", ctypes.c_int64)]
 
 
 VP = ctypes.c_void_p

@@ -45,16 +45,37 @@ def test_external(sim, tmp_path, golden):
     assert _rel(out, ref) < 1e-12
 
 
-def test_hmag(sim, tmp_path, golden):
+def test_hmag(sim, tmp_path, golden, oracle_cube):
     print("[gpu] run_hmag", flush=True)
     sim.run_hmag(outdir=tmp_path, name="cube")
     ref = read_csv_values(golden / "hmag_cube.csv")
     out = read_csv_values(tmp_path / "cube.csv")
     assert np.allclose(ref, out)
     assert _rel(out, ref) < 1e-9
-    from mammos_mumag_amd.fileio import read_vtu_array
-    u = read_vtu_array(tmp_path / "cube_hmag.vtu", "U")
-    assert np.all(np.isfinite(u)) and np.max(np.abs(u)) > 0
+    # the VTU's fields (hmag.py:153-165) against the oracle: U solves K U = D m, h_nodes =
+    # -V (G U) / meas (meas 0 -> 1), cell h = -grad U; written with 7 significant digits
+    from mammos_mumag_amd.fileio import read_vtu_array, vtu_cell_order
+    from scipy.sparse.linalg import spsolve
+    O = oracle_cube
+    m = O.uniform_m([0.0, 0.0, 1.0])
+    b = sum(O.Dxyz[a] @ m[a::3] for a in range(3))
+    u_ref = spsolve(O.K.tocsc(), b)
+    g_ref = np.stack([O.Gxyz[a] @ u_ref for a in range(3)], axis=1)
+    meas = np.where(O.meas == 0, 1.0, O.meas)
+    hn_ref = -O.V * g_ref / meas[:, None]
+    X = O.coords[O.tets]
+    du = u_ref[O.tets[:, 1:]] - u_ref[O.tets[:, 0:1]]
+    hc_ref = -np.linalg.solve(X[:, 1:, :] - X[:, 0:1, :], du[..., None])[..., 0]
+    vtu = tmp_path / "cube_hmag.vtu"
+    u = read_vtu_array(vtu, "U")
+    hn = read_vtu_array(vtu, "h_nodes")
+    hc = read_vtu_array(vtu, "h")
+    mv = read_vtu_array(vtu, "m")
+    order = vtu_cell_order(O.tets)
+    assert np.max(np.abs(u - u_ref)) <= 2e-6 * np.max(np.abs(u_ref))
+    assert np.max(np.abs(hn - hn_ref)) <= 2e-6 * np.max(np.abs(hn_ref))
+    assert np.max(np.abs(hc - hc_ref[order])) <= 2e-6 * np.max(np.abs(hc_ref))
+    assert np.array_equal(mv, m.reshape(-1, 3))
 
 
 def test_loop(sim, tmp_path, golden):

@@ -161,3 +161,42 @@ def test_coarse_intervals_span_at_most_two_fine():
             axes = [a[k] for a, k in zip(axes, keeps)]
             levels += 1
         assert levels >= 3
+
+
+@pytest.mark.parametrize("cfg", ["voronoi8_100nm", "voronoi64_250nm"])
+def test_interpolation_reproduces_linear_functions(cfg):
+    """Per-axis linear interpolation (oracle mg.interp) on every level of the benchmark
+    grids: kept nodes are copied (identity rows), every other row is a convex pair of the
+    two enclosing coarse nodes, and a + b x sampled on the coarse nodes is reproduced
+    exactly on the fine nodes (the closed form of linear interpolation: a wrong neighbour,
+    swapped weights or an off-by-one coarse index all break it); the 3-D prolongation
+    (tensor product) reproduces a + b x + c y + d z."""
+    from mammos_mumag_amd.generate import build_config
+    g = build_config(cfg)
+    V = mg.VCycle(g.xs, g.ys, g.zs)
+    assert len(V.levels) >= 4
+    rng = np.random.default_rng(3)
+    for lev, (axes, nxt) in enumerate(zip(V.axes[:-1], V.axes[1:])):
+        hmin = min(np.diff(a).min() for a in axes)
+        for a, ac in zip(axes, nxt):
+            keep = mg.coarsen(a, hmin)
+            assert np.array_equal(a[keep], ac)
+            P = mg.interp(a, keep).toarray()
+            assert np.array_equal(P[keep], np.eye(len(keep)))
+            assert np.all(P >= 0.0) and np.all((P > 0).sum(axis=1) <= 2)
+            assert np.allclose(P.sum(axis=1), 1.0, rtol=0, atol=1e-14)
+            for _ in range(3):
+                c0, c1 = rng.normal(size=2)
+                f_c = c0 + c1 * ac
+                err = np.abs(P @ f_c - (c0 + c1 * a))
+                assert np.max(err) <= 1e-12 * (abs(c0) + abs(c1) * np.max(np.abs(a)))
+        # the 3-D prolongation of this level
+        Pl = V.levels[lev]["P"]
+        X, Y, Z = (np.asarray(t) for t in np.meshgrid(*nxt, indexing="ij"))
+        Xf, Yf, Zf = (np.asarray(t) for t in np.meshgrid(*axes, indexing="ij"))
+        c = rng.normal(size=4)
+        # node order x fastest (index i + nx (j + ny k)): transpose the ij-meshes to (z, y, x)
+        fc = (c[0] + c[1] * X + c[2] * Y + c[3] * Z).transpose(2, 1, 0).ravel()
+        ff = (c[0] + c[1] * Xf + c[2] * Yf + c[3] * Zf).transpose(2, 1, 0).ravel()
+        scale = np.abs(c[0]) + np.abs(c[1:]).sum() * max(np.max(np.abs(t)) for t in axes)
+        assert np.max(np.abs(Pl @ fc - ff)) <= 1e-12 * scale
