@@ -33,12 +33,13 @@ HBM_PEAK_GBS = 8000.0
 # HBM bytes per launch of the two PCG kernels and of the level-0 multigrid smoothing step
 # on the voronoi64_250nm grid (structured path), from rocprofv3 --pmc FETCH_SIZE and
 # WRITE_SIZE (median KiB per dispatch) scaled by the same counters on the 512 MiB device
-# copy of --traffic-probe (factors 0.5000 and 1.0000); see
+# copy and fill of --traffic-probe (FETCH_SIZE reads half the copied bytes: x 2;
+# WRITE_SIZE the filled bytes: x 1); see
 # profiles/pcg_kernels_{fp64,mixed_jacobi}_voronoi125.md and profiles/mg_vcycle_mixed_voronoi125.md.
 TRAFFIC = {
-    ("fp64", "spmv"): 132804214, ("fp64", "update"): 233823056,
-    ("mixed", "spmv"): 67652230, ("mixed", "update"): 116963400,
-    ("mixed", "mg_smooth"): 45808470,
+    ("fp64", "spmv"): 127833702, ("fp64", "update"): 233822003,
+    ("mixed", "spmv"): 66500096, ("mixed", "update"): 116983808,
+    ("mixed", "mg_smooth"): 45730406,
 }
 
 
@@ -171,8 +172,8 @@ def gpu_run(args, precision, traffic_probe=False, learned=None, multigrid=0):
     log(f"[bench] {tag}: timed {args.steps} steps: {elapsed:.3f} s, last h={hext - p['hstep']:.2f} T, "
         f"M.h={mh:.6f} T, E={energies[-1]:.6e}, {st.as_dict()}, {timing}")
     if traffic_probe:
-        # reference dispatch for the HBM byte counters: a device copy of 512 MiB
-        # (reads and writes 524288 KiB)
+        # reference dispatches for the HBM byte counters: a 512 MiB fill (torch.ones,
+        # writes 524288 KiB) and a 512 MiB device copy (reads 524288 KiB)
         src = torch.ones(64 << 20, dtype=torch.float64, device="cuda")
         dst = torch.empty_like(src)
         torch.cuda.synchronize()
