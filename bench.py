@@ -362,6 +362,93 @@ def avalanche_main(args, learned):
     return 0
 
 
+def reversal_check_main(args):
+    """--reversal-check: the chosen form (FP32 inner solve with multigrid, no network) sweeps
+    the field of --config from --rc-hstart by --rc-hstep until the first step with M.h < 0
+    (the reversal, step K) or the --rc-time-limit; the states after the last steps are kept
+    on the device and plain FP64 then re-solves step K - 1 from the chosen form's state after
+    K - 2 and step K from its state after K - 1 (--rc-fp64): the switching step of FP64 from
+    the same history, and |dM|/Ms at those steps.  One JSON line."""
+    import numpy as np
+    import torch
+
+    from mammos_mumag_amd import Model, _lib
+    from mammos_mumag_amd.generate import build_config, loop_params
+
+    g = build_config(args.config, seed=args.seed)
+    p = loop_params(args.config)
+    h0 = p["hstart"] if args.rc_hstart is None else args.rc_hstart
+    dh = p["hstep"] if args.rc_hstep is None else args.rc_hstep
+    t_set = time.perf_counter()
+    mix = Model.from_structured(g, precision="mixed", eval_precision=args.eval_precision)
+    mix.multigrid(args.multigrid)
+    log(f"[reversal] {args.config}: {mix.info} multigrid {mix.mg_levels()} ({time.perf_counter() - t_set:.1f} s)")
+    m = mix.to_device(mix.uniform_m(p["m"]))
+    m_ref, u, gF, st = m.clone(), mix.empty1(), mix.empty3(), _lib.Stats()
+    ms_sat = float(mix.meas.sum()) / mix.V
+    keep = []   # (k, h, m, u, gF) before each of the last steps
+    rec, K = [], None
+    hext, k = h0, 0
+    t_loop = time.perf_counter()
+    while True:
+        keep.append((k, hext, m.clone(), u.clone(), gF.clone()))
+        keep = keep[-2:]
+        prev = st.as_dict()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        E = mix.minimize(m, hext, p["h"], p, m_ref, u, gF, st)
+        mh = mix.compute_mh(m, p["h"])
+        dt = time.perf_counter() - t0
+        cur = st.as_dict()
+        d = {key: cur[key] - prev[key] for key in cur}
+        rec.append(dict(step=k, h=round(hext, 6), mh=mh, E=E, s=round(dt, 3), **d))
+        log(f"[reversal] chosen form step {k} h={hext:+.3f} T M.h={mh:+.6f} T {dt:.2f} s ncg {d['ncg_iters']} "
+            f"pcg {d['pcg_iters']} backtracks {d['backtracks']}")
+        if mh < 0:
+            K = k
+            break
+        if time.perf_counter() - t_loop > args.rc_time_limit:
+            break
+        hext += dh if args.rc_fine_from is None or hext + dh > args.rc_fine_from else args.rc_hstep_fine
+        k += 1
+    loop_s = time.perf_counter() - t_loop
+    out = dict(reversal_check=args.config, hstart=h0, hstep=dh, fine_from=args.rc_fine_from,
+               hstep_fine=args.rc_hstep_fine if args.rc_fine_from is not None else None, js_sat=ms_sat, steps=len(rec),
+               reversal_step=K, reversal_field_T=None if K is None else rec[K]["h"], loop_seconds=round(loop_s, 1),
+               chosen=dict(mean_s_per_step=round(loop_s / len(rec), 3),
+                           ncg_per_step=round(float(np.mean([r["ncg_iters"] for r in rec])), 2),
+                           pcg_per_step=round(float(np.mean([r["pcg_iters"] for r in rec])), 2),
+                           slowest_step_s=max(r["s"] for r in rec)),
+               records=rec)
+    mix.close()
+    del mix
+    torch.cuda.empty_cache()
+    print(json.dumps(dict(out, partial=True)), flush=True)
+    if args.rc_fp64 and K is not None and len(keep) == 2:
+        f64 = Model.from_structured(g)
+        cmp_ = []
+        for (kk, hh, m0, u0, g0) in keep:   # steps K - 1 and K from the chosen form's states
+            st2 = _lib.Stats()
+            mm, uu, gg = m0.clone(), u0.clone(), g0.clone()
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            E2 = f64.minimize(mm, hh, p["h"], p, m_ref, uu, gg, st2)
+            dt = time.perf_counter() - t0
+            mh2 = f64.compute_mh(mm, p["h"])
+            r = rec[kk]
+            c = dict(step=kk, h=hh, mh_fp64=mh2, mh_chosen=r["mh"], dM_over_Ms=abs(mh2 - r["mh"]) / ms_sat,
+                     E_fp64=E2, E_chosen=r["E"], seconds_fp64=round(dt, 1), seconds_chosen=r["s"],
+                     fp64_counts=st2.as_dict())
+            log(f"[reversal] fp64 re-solve {c}")
+            print(json.dumps(dict(partial=True, fp64_resolve=c)), flush=True)
+            cmp_.append(c)
+        out["fp64_resolves"] = cmp_
+        out["fp64_switches_at_same_step"] = bool(cmp_[0]["mh_fp64"] >= 0 and cmp_[1]["mh_fp64"] < 0)
+        f64.close()
+    print(json.dumps(out), flush=True)
+    return 0
+
+
 def precond_eval_main(args):
     """--precond-eval: the learned preconditioner on the benchmark geometry with a
     microstructure no network saw (--seed): potential solves b = D m of converged field
@@ -592,6 +679,16 @@ def main():
     ap.add_argument("--precond-every", type=int, nargs="+", default=[16, 64, 256, 1000000],
                     help="--precond-eval: iterations between the network's Galerkin steps (a large value: "
                          "at the start only)")
+    ap.add_argument("--reversal-check", action="store_true",
+                    help="the chosen form sweeps until the first step with M.h < 0; plain FP64 re-solves that step "
+                         "and the one before from the chosen form's states")
+    ap.add_argument("--rc-hstart", type=float, default=None, help="--reversal-check: first field (T)")
+    ap.add_argument("--rc-hstep", type=float, default=None, help="--reversal-check: field step (T)")
+    ap.add_argument("--rc-time-limit", type=float, default=700.0, help="--reversal-check: seconds of sweep at most")
+    ap.add_argument("--rc-fine-from", type=float, default=None,
+                    help="--reversal-check: below this field (T) the sweep steps by --rc-hstep-fine")
+    ap.add_argument("--rc-hstep-fine", type=float, default=-0.02, help="--reversal-check: the fine field step (T)")
+    ap.add_argument("--rc-fp64", action="store_true", help="--reversal-check: the FP64 re-solves")
     ap.add_argument("--avalanche-eps", default="1e-15",
                     help="comma-separated relative sizes of the perturbations of m (each with every seed)")
     args = ap.parse_args()
@@ -613,7 +710,7 @@ def main():
               file=sys.stderr)
         return 2
     cpu = None
-    if not args.no_cpu_baseline and not args.loop and not args.avalanche and not args.precond_eval:
+    if not (args.no_cpu_baseline or args.loop or args.avalanche or args.precond_eval or args.reversal_check):
         cpu = start_cpu_leg(args.config, args.seed)
     learned = None
     if args.learned != "none":
@@ -627,6 +724,8 @@ def main():
         return loop_main(args, learned)
     if args.avalanche:
         return avalanche_main(args, learned)
+    if args.reversal_check:
+        return reversal_check_main(args)
     if args.precond_eval:
         if not args.precond:
             args.precond = str(ROOT / "mammos-mumag_amd" / "mammos_mumag_amd" / "data" / f"precond_{args.config}.npz")
