@@ -386,13 +386,15 @@ def reversal_check_main(args):
     m = mix.to_device(mix.uniform_m(p["m"]))
     m_ref, u, gF, st = m.clone(), mix.empty1(), mix.empty3(), _lib.Stats()
     ms_sat = float(mix.meas.sum()) / mix.V
-    keep = []   # (k, h, m, u, gF) before each of the last steps
+    keep = []   # (k, h, m, u, gF) before each of the last steps, or before the --rc-resolve steps
     rec, K = [], None
     hext, k = h0, 0
     t_loop = time.perf_counter()
     while True:
-        keep.append((k, hext, m.clone(), u.clone(), gF.clone()))
-        keep = keep[-2:]
+        if args.rc_resolve is None or k in args.rc_resolve:
+            keep.append((k, hext, m.clone(), u.clone(), gF.clone()))
+        if args.rc_resolve is None:
+            keep = keep[-2:]
         prev = st.as_dict()
         torch.cuda.synchronize()
         t0 = time.perf_counter()
@@ -407,7 +409,7 @@ def reversal_check_main(args):
         if mh < 0:
             K = k
             break
-        if time.perf_counter() - t_loop > args.rc_time_limit:
+        if time.perf_counter() - t_loop > args.rc_time_limit or (args.rc_stop_step is not None and k >= args.rc_stop_step):
             break
         hext += dh if args.rc_fine_from is None or hext + dh > args.rc_fine_from else args.rc_hstep_fine
         k += 1
@@ -424,10 +426,10 @@ def reversal_check_main(args):
     del mix
     torch.cuda.empty_cache()
     print(json.dumps(dict(out, partial=True)), flush=True)
-    if args.rc_fp64 and K is not None and len(keep) == 2:
+    if args.rc_fp64 and (args.rc_resolve is not None or (K is not None and len(keep) == 2)):
         f64 = Model.from_structured(g)
         cmp_ = []
-        for (kk, hh, m0, u0, g0) in keep:   # steps K - 1 and K from the chosen form's states
+        for (kk, hh, m0, u0, g0) in keep:   # steps K - 1 and K (or the --rc-resolve steps)
             st2 = _lib.Stats()
             mm, uu, gg = m0.clone(), u0.clone(), g0.clone()
             torch.cuda.synchronize()
@@ -443,7 +445,8 @@ def reversal_check_main(args):
             print(json.dumps(dict(partial=True, fp64_resolve=c)), flush=True)
             cmp_.append(c)
         out["fp64_resolves"] = cmp_
-        out["fp64_switches_at_same_step"] = bool(cmp_[0]["mh_fp64"] >= 0 and cmp_[1]["mh_fp64"] < 0)
+        if args.rc_resolve is None:
+            out["fp64_switches_at_same_step"] = bool(cmp_[0]["mh_fp64"] >= 0 and cmp_[1]["mh_fp64"] < 0)
         f64.close()
     print(json.dumps(out), flush=True)
     return 0
@@ -688,6 +691,10 @@ def main():
     ap.add_argument("--rc-fine-from", type=float, default=None,
                     help="--reversal-check: below this field (T) the sweep steps by --rc-hstep-fine")
     ap.add_argument("--rc-hstep-fine", type=float, default=-0.02, help="--reversal-check: the fine field step (T)")
+    ap.add_argument("--rc-stop-step", type=int, default=None, help="--reversal-check: last step of the sweep")
+    ap.add_argument("--rc-resolve", type=int, nargs="+", default=None,
+                    help="--reversal-check: the steps plain FP64 re-solves from the chosen form's states (instead of "
+                         "the reversal step and the one before)")
     ap.add_argument("--rc-fp64", action="store_true", help="--reversal-check: the FP64 re-solves")
     ap.add_argument("--avalanche-eps", default="1e-15",
                     help="comma-separated relative sizes of the perturbations of m (each with every seed)")

@@ -42,6 +42,12 @@ constexpr int kFB = 1024;  // threads per block of the streaming fused kernels (
 #ifndef MM_KA_TY32
 #define MM_KA_TY32 4
 #endif
+#ifndef MM_KA_TX
+#define MM_KA_TX 64
+#endif
+#ifndef MM_KA_TX32
+#define MM_KA_TX32 64
+#endif
 #ifndef MM_KA_BLOCKS32
 #define MM_KA_BLOCKS32 768
 #endif
@@ -57,11 +63,11 @@ constexpr int kFB = 1024;  // threads per block of the streaming fused kernels (
 // rows of a KA tile on the structured grid (64 x TY nodes) and the target block count
 template <typename T>
 struct KaCfg {
-    static constexpr int TY = MM_KA_TY, blocks = MM_KA_BLOCKS;
+    static constexpr int TX = MM_KA_TX, TY = MM_KA_TY, blocks = MM_KA_BLOCKS;
 };
 template <>
 struct KaCfg<float> {
-    static constexpr int TY = MM_KA_TY32, blocks = MM_KA_BLOCKS32;
+    static constexpr int TX = MM_KA_TX32, TY = MM_KA_TY32, blocks = MM_KA_BLOCKS32;
 };
 
 __device__ __forceinline__ double fwave_sum(double v) {
@@ -224,25 +230,28 @@ struct Raw {
 // entries towards Dirichlet nodes (first / last index on an axis) are dropped.
 // Grid: 1D, gx * gy * gz blocks, logical tile = (z chunk fastest, then x tile, then y).
 constexpr int kKaChunk = 256;
-template <typename T, int TY, bool PZ>
-__global__ __launch_bounds__(64 * TY)
+template <typename T, int TX, int TY, bool PZ>
+__global__ __launch_bounds__(TX * TY)
 #if MM_KA_WAVES > 0
 __attribute__((amdgpu_waves_per_eu(MM_KA_WAVES)))
 #endif
 void k_ka_grid(PoissonGrid<T> P, PcgIterArgs a, int gx, int gz, const T *__restrict__ z,
                const T *__restrict__ pold, T *__restrict__ pnew, T *__restrict__ q, DevScalars *dsc) {
-    static_assert(TY >= 3 && TY <= 16, "halo lanes need 3 <= TY; blocks of at most 1024 threads");
-    constexpr int NT = 64 * TY;
+    static_assert(TX == 64 || TX == 32, "a wave holds one (TX = 64) or two (TX = 32) tile rows");
+    static_assert(TY >= 3 && 2 * TY <= TX && TX * TY <= 1024 && (TX * TY) % kWave == 0,
+                  "halo lanes need 3 <= TY <= TX / 2; blocks of whole waves, at most 1024 threads");
+    constexpr int NT = TX * TY;
     __shared__ double sh[3 * NT / kWave];
-    __shared__ T sp[2][TY + 2][66];
+    __shared__ T sp[2][TY + 2][TX + 2];
     __shared__ T szc[2][kKaChunk + 1];   // hz, 1 / (6 hz) of the chunk's planes k0 .. k1
     __shared__ int flag;
     const int nx = P.nx, ny = P.ny, nz = P.nz;
     const int t = xcd_tile(blockIdx.x, gridDim.x);
     const int cz = t % gz, bx = (t / gz) % gx, by = t / (gz * gx);
-    const int tx = threadIdx.x & 63;
-    const int ty = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);   // one row per wave
-    const int i = bx * 64 + tx, j = by * TY + ty;
+    const int tx = threadIdx.x % TX;
+    // one row per wave (uniform) with 64-wide tiles, two rows per wave with 32-wide ones
+    const int ty = TX == 64 ? __builtin_amdgcn_readfirstlane(threadIdx.x >> 6) : (int)(threadIdx.x / TX);
+    const int i = bx * TX + tx, j = by * TY + ty;
     const int chunk = (nz + gz - 1) / gz;
     const int k0 = cz * chunk, k1 = min(nz, k0 + chunk);   // block-uniform; empty when k0 >= nz
     const uint32_t es = sizeof(T);
@@ -251,16 +260,16 @@ void k_ka_grid(PoissonGrid<T> P, PcgIterArgs a, int gx, int gz, const T *__restr
     const bool run = k0 < k1;
     constexpr bool pz = PZ;   // p = z (FIRST, RESTART, RESTART_PARTS), else z + beta p_old
     // halo node of this thread (at most one): south row (ty == 0), north row
-    // (ty == TY - 1), west / east columns (ty == 1, lanes [0, TY) and [32, 32 + TY))
+    // (ty == TY - 1), west / east columns (ty == 1, lanes [0, TY) and [TX / 2, TX / 2 + TY))
     int hi = 0, hj = 0, hly = -1, hlx = 0;
     if (ty == 0) {
         hi = i; hj = by * TY - 1; hly = 0; hlx = tx + 1;
     } else if (ty == TY - 1) {
         hi = i; hj = by * TY + TY; hly = TY + 1; hlx = tx + 1;
     } else if (ty == 1 && tx < TY) {
-        hi = bx * 64 - 1; hj = by * TY + tx; hly = tx + 1; hlx = 0;
-    } else if (ty == 1 && tx >= 32 && tx < 32 + TY) {
-        hi = bx * 64 + 64; hj = by * TY + tx - 32; hly = tx - 31; hlx = 65;
+        hi = bx * TX - 1; hj = by * TY + tx; hly = tx + 1; hlx = 0;
+    } else if (ty == 1 && tx >= TX / 2 && tx < TX / 2 + TY) {
+        hi = bx * TX + TX; hj = by * TY + tx - TX / 2; hly = tx - TX / 2 + 1; hlx = TX + 1;
     }
     const bool hin = hly >= 0 && hi >= 0 && hi < nx && hj >= 0 && hj < ny;
     const uint32_t ob = in ? (uint32_t)(i + nx * j) * es : 0u;
@@ -306,10 +315,11 @@ void k_ka_grid(PoissonGrid<T> P, PcgIterArgs a, int gx, int gz, const T *__restr
     double dot = 0.0;
     if (run) {
         const bool edge_ij = !in || i == 0 || j == 0 || i == nx - 1 || j == ny - 1;
-        // y factors are wave-uniform (scalar path), x factors per lane
+        // y factors: wave-uniform with 64-wide tiles (scalar path), x factors per lane
         const bool jin = j > 0 && j < ny - 1;
-        const T hyl = jin ? uload(P.hyp, j) : (T)0, hyu = jin ? uload(P.hyp, j + 1) : (T)0;
-        const T b0 = jin ? uload(P.iyp, j) : (T)0, b1 = jin ? uload(P.iyp, j + 1) : (T)0;
+        auto yload = [&](const T *a, int k) -> T { return TX == 64 ? uload(a, k) : a[k]; };
+        const T hyl = jin ? yload(P.hyp, j) : (T)0, hyu = jin ? yload(P.hyp, j + 1) : (T)0;
+        const T b0 = jin ? yload(P.iyp, j) : (T)0, b1 = jin ? yload(P.iyp, j + 1) : (T)0;
         const T iys = b0 + b1, iym = j - 1 > 0 ? b0 : (T)0, iyp = j + 1 < ny - 1 ? b1 : (T)0;
         T hxl = 0, hxu = 0, ixs = 0, ixm = 0, ixp = 0;
         if (!edge_ij) {
@@ -329,7 +339,7 @@ void k_ka_grid(PoissonGrid<T> P, PcgIterArgs a, int gx, int gz, const T *__restr
             const T hv = comb(hc);
             on = raw(ob + plane(k + 1 + R), true);
             hn = raw(hb_off + plane(k + R), true);
-            T(*buf)[66] = sp[k & 1];
+            T(*buf)[TX + 2] = sp[k & 1];
             buf[ty + 1][tx + 1] = pc;
             if (hly >= 0) buf[hly][hlx] = hin ? hv : (T)0;
             __syncthreads();
@@ -505,7 +515,7 @@ struct KaDims {
 template <typename T>
 KaDims ka_grid_dims(const PoissonGrid<T> &P) {
     KaDims d;
-    d.gx = (P.nx + 63) / 64;
+    d.gx = (P.nx + KaCfg<T>::TX - 1) / KaCfg<T>::TX;
     d.gy = (P.ny + KaCfg<T>::TY - 1) / KaCfg<T>::TY;
     const int per_plane = d.gx * d.gy;
     d.gz = std::max(1, std::min(P.nz, KaCfg<T>::blocks / per_plane));
@@ -531,8 +541,9 @@ void launch_timed(const Launch &L, K kernel, dim3 grid, dim3 block, Args... args
 bool ka_grid_fits(int nx, int ny, int nz) {
     const int64_t gz = std::max<int64_t>(1, (nz + kKaChunk - 1) / kKaChunk);
     const int64_t n = (int64_t)nx * ny * nz;
-    for (const int ty : {KaCfg<double>::TY, KaCfg<float>::TY}) {
-        const int64_t per_plane = (int64_t)((nx + 63) / 64) * ((ny + ty - 1) / ty);
+    for (const int tt : {0, 1}) {
+        const int tx = tt ? KaCfg<float>::TX : KaCfg<double>::TX, ty = tt ? KaCfg<float>::TY : KaCfg<double>::TY;
+        const int64_t per_plane = (int64_t)((nx + tx - 1) / tx) * ((ny + ty - 1) / ty);
         if (per_plane * gz > kMaxPartials) return false;
     }
     return n * (int64_t)sizeof(double) < ((int64_t)1 << 32);
@@ -552,11 +563,11 @@ int launch_ka_grid(const Launch &L, const PoissonGrid<T> &P, const PcgIterArgs &
     const KaDims d = ka_grid_dims(P);
     const int nb = d.gx * d.gy * d.gz;
     if (a.mode == PCG_FIRST || a.mode == PCG_RESTART || a.mode == PCG_RESTART_PARTS)
-        launch_timed(L, k_ka_grid<T, KaCfg<T>::TY, true>, dim3(nb), dim3(64 * KaCfg<T>::TY), P, a, d.gx, d.gz, z,
+        launch_timed(L, k_ka_grid<T, KaCfg<T>::TX, KaCfg<T>::TY, true>, dim3(nb), dim3(KaCfg<T>::TX * KaCfg<T>::TY), P, a, d.gx, d.gz, z,
                      pold, pnew, q,
                      L.dsc);
     else
-        launch_timed(L, k_ka_grid<T, KaCfg<T>::TY, false>, dim3(nb), dim3(64 * KaCfg<T>::TY), P, a, d.gx, d.gz, z,
+        launch_timed(L, k_ka_grid<T, KaCfg<T>::TX, KaCfg<T>::TY, false>, dim3(nb), dim3(KaCfg<T>::TX * KaCfg<T>::TY), P, a, d.gx, d.gz, z,
                      pold, pnew, q,
                      L.dsc);
     return nb;
