@@ -1,6 +1,7 @@
 // Per-node MLP of the learned warm start (no counterpart in the reference): features
-// of a node and its 1-ring -> 128 -> 128 -> 128 -> 1, ReLU, bf16 MFMA
-// (v_mfma_f32_16x16x16_bf16) with FP32 accumulation, FP32 master weights and Adam.
+// of a node and its 1-ring -> 128 -> 128 -> 128 -> 1, ReLU, bf16 MFMA with FP32
+// accumulation (the forward pass in v_mfma_f32_16x16x32_bf16, the backward and weight-
+// gradient products in v_mfma_f32_16x16x16_bf16), FP32 master weights and Adam.
 //
 // Products are formed transposed, H_out^T = W^T H_in^T (features x nodes), so that the
 // 16x16 output fragment of a layer (lane l: column = node l % 16, rows = features
@@ -40,6 +41,21 @@ __device__ __forceinline__ f32x4 mfma(bf16x4 a, bf16x4 b, f32x4 c) {
     return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, c, 0, 0, 0);
 }
 
+// Two consecutive 16-wide k-steps in one v_mfma_f32_16x16x32_bf16 (gfx950: twice the
+// products per instruction).  Its operands hold 8 k-values per lane; a lane's elements
+// 0-3 take the first k-step's fragment (k = 4 g + e) and 4-7 the second's (k = 16 + 4 g + e),
+// for A and B alike: the same permutation of the 32 summed k-values on both sides, so the
+// 16x16x16 fragments above combine without moving data (checked lane by lane on the
+// device against a CPU product).
+typedef short s16x8 __attribute__((ext_vector_type(8)));
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef __bf16 nbf16x4 __attribute__((ext_vector_type(4)));
+__device__ __forceinline__ f32x4 mfma2(bf16x4 a0, bf16x4 a1, bf16x4 b0, bf16x4 b1, f32x4 c) {
+    const bf16x8 a = __builtin_bit_cast(bf16x8, (s16x8)__builtin_shufflevector(a0, a1, 0, 1, 2, 3, 4, 5, 6, 7));
+    const bf16x8 b = __builtin_bit_cast(bf16x8, (s16x8)__builtin_shufflevector(b0, b1, 0, 1, 2, 3, 4, 5, 6, 7));
+    return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
+}
+
 // Cooperative copy of `bytes` (multiple of 16) from global to LDS.
 __device__ __forceinline__ void copy_lds(void *dst, const void *src, int bytes) {
     const uint4 *s = (const uint4 *)src;
@@ -63,8 +79,8 @@ __global__ __launch_bounds__(kNT) void k_mlp_fwd(MlpDev w, int64_t n, const uint
     __shared__ uint16_t s1[kMlpW * kMlpP1];
     __shared__ uint16_t s2[kMlpW * kMlpPW];
     __shared__ uint16_t s3[kMlpW * kMlpPW];
-    __shared__ float sb[3][kMlpW];
-    __shared__ float sw4[kMlpW];
+    __shared__ __attribute__((aligned(16))) float sb[3][kMlpW];
+    __shared__ __attribute__((aligned(16))) float sw4[kMlpW];
     copy_lds(s1, w.w1t, kMlpW * kMlpP1 * 2);
     copy_lds(s2, w.w2t, kMlpW * kMlpPW * 2);
     copy_lds(s3, w.w3t, kMlpW * kMlpPW * 2);
@@ -105,38 +121,49 @@ __global__ __launch_bounds__(kNT) void k_mlp_fwd(MlpDev w, int64_t n, const uint
             const uint16_t *sw = layer == 0 ? s1 : layer == 1 ? s2 : s3;
             const int ld = layer == 0 ? kMlpP1 : kMlpPW;
             const int nk = layer == 0 ? kMlpIn / 16 : kMlpW / 16;
+            // the accumulators start at the bias (features 16 t + 4 grp + 0..3 of this lane)
             f32x4 acc[8][2];
 #pragma unroll
-            for (int t = 0; t < 8; ++t)
+            for (int t = 0; t < 8; ++t) {
+                const f32x4 bias = *(const f32x4 *)&sb[layer][16 * t + 4 * grp];
 #pragma unroll
-                for (int c = 0; c < 2; ++c) acc[t][c] = f32x4{0.f, 0.f, 0.f, 0.f};
+                for (int c = 0; c < 2; ++c) acc[t][c] = bias;
+            }
 #pragma unroll
-            for (int f = 0; f < nk; ++f) {
+            for (int f = 0; f < nk; f += 2) {
 #pragma unroll
                 for (int t = 0; t < 8; ++t) {
-                    const bf16x4 a = afrag(sw, ld, t, f, col, grp);
+                    const bf16x4 a0 = afrag(sw, ld, t, f, col, grp), a1 = afrag(sw, ld, t, f + 1, col, grp);
 #pragma unroll
-                    for (int c = 0; c < 2; ++c) acc[t][c] = mfma(a, hb[f][c], acc[t][c]);
+                    for (int c = 0; c < 2; ++c) acc[t][c] = mfma2(a0, a1, hb[f][c], hb[f + 1][c], acc[t][c]);
                 }
-                // keep the weight reads of one k-step in flight at a time (registers)
+                // keep the weight reads of one k-step pair in flight at a time (registers)
                 __builtin_amdgcn_sched_barrier(0);
             }
-            // bias + ReLU + bf16: the next layer's B operands (k-step t)
+            // ReLU + bf16 (round to nearest even, two values per conversion instruction): the
+            // next layer's B operands (k-step t)
 #pragma unroll
-            for (int t = 0; t < 8; ++t)
+            for (int t = 0; t < 8; ++t) {
+                const f32x4 w4 = layer == 2 ? *(const f32x4 *)&sw4[16 * t + 4 * grp] : f32x4{0.f, 0.f, 0.f, 0.f};
 #pragma unroll
                 for (int c = 0; c < 2; ++c) {
-                    bf16x4 v;
+                    f32x4 z = acc[t][c];
 #pragma unroll
-                    for (int e = 0; e < 4; ++e) {
-                        const int feat = 16 * t + 4 * grp + e;
-                        const float z = acc[t][c][e] + sb[layer][feat];
-                        v[e] = (short)f2bf(fmaxf(z, 0.f));
-                        if (layer == 2) yacc[c] += sw4[feat] * bf2f((uint16_t)v[e]);
-                        if (TRAIN && st[c]) hs[layer][(int64_t)feat * ldh + node[c]] = (uint16_t)v[e];
+                    for (int e = 0; e < 4; ++e) z[e] = fmaxf(z[e], 0.f);
+                    const nbf16x4 hv = __builtin_convertvector(z, nbf16x4);
+                    const bf16x4 v = __builtin_bit_cast(bf16x4, hv);
+                    if (layer == 2) {
+                        const f32x4 r = __builtin_convertvector(hv, f32x4);
+                        yacc[c] += w4[0] * r[0] + w4[1] * r[1] + w4[2] * r[2] + w4[3] * r[3];
+                    }
+                    if (TRAIN && st[c]) {
+#pragma unroll
+                        for (int e = 0; e < 4; ++e)
+                            hs[layer][(int64_t)(16 * t + 4 * grp + e) * ldh + node[c]] = (uint16_t)v[e];
                     }
                     hb[t][c] = v;
                 }
+            }
         }
         // layer 4: sum over the 4 lane groups holding the node's 128 features
 #pragma unroll
@@ -174,7 +201,7 @@ __global__ __launch_bounds__(kNT) void k_mlp_bwd(MlpDev w, int64_t n, const floa
                                                  int64_t ldh) {
     __shared__ uint16_t s2[kMlpW * kMlpPW];   // W2 row-major [in][out]
     __shared__ uint16_t s3[kMlpW * kMlpPW];   // W3
-    __shared__ float sw4[kMlpW];
+    __shared__ __attribute__((aligned(16))) float sw4[kMlpW];
     copy_lds(s2, w.w2n, kMlpW * kMlpPW * 2);
     copy_lds(s3, w.w3n, kMlpW * kMlpPW * 2);
     for (int i = threadIdx.x; i < kMlpW; i += blockDim.x) sw4[i] = w.w4[i];
