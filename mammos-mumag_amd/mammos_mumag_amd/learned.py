@@ -209,6 +209,93 @@ def collect_both(model, params, steps, hstep, nodes_ws, nodes_lp, lp_iters=(0, 8
     return torch.cat(Xw), torch.cat(uw), torch.cat(Xl), torch.cat(tl)
 
 
+def collect_dense(config, seed, params, steps, hstep, nodes, log=None, targets="fp64"):
+    """(X, u) warm-start samples of one microstructure at every one of ``steps`` field
+    steps of ``hstep`` from hstart: the states m come from a sweep of the mixed form
+    (FP32 inner solve with the multigrid preconditioner, FP64 outer residual; its field
+    steps agree with plain FP64 to ~1e-8), and each target u is the potential of that m
+    at ``nodes`` random nodes: targets "fp64", the plain FP64 Jacobi PCG's
+    (scripts/solvers.py:221-231 at the loop's tolerance, started from the previous target);
+    "mixed", the sweep's own last potential solve (the same FP64 residual tolerance).
+    Returns (X, u)."""
+    torch = _torch()
+    from . import _lib as L
+    from .generate import build_config
+    from .model import Model
+    g = build_config(config, seed=seed)
+    mix = Model.from_structured(g, precision="mixed")
+    mix.multigrid(8)
+    f64 = Model.from_structured(g) if targets == "fp64" else mix
+    dev = f"cuda:{mix.device}"
+    gen = torch.Generator(device=dev)
+    gen.manual_seed(int(seed))
+    m = mix.to_device(mix.uniform_m(params["m"]))
+    m_ref, u, gF, st = m.clone(), mix.empty1(), mix.empty3(), L.Stats()
+    u64 = f64.empty1()
+    X = torch.empty((f64.N, N_IN), dtype=torch.int16, device=dev)
+    Xs, us = [], []
+    hext = params["hstart"]
+    for k in range(steps):
+        mix.minimize(m, hext, params["h"], params, m_ref, u, gF, st)
+        if targets == "fp64":
+            b = f64.divergence(m)
+            f64.pcg(b, u64, params["tol_u"])
+        else:
+            u64 = u
+        features(f64, m, X)
+        idx = torch.randint(0, f64.N, (nodes,), generator=gen, device=dev)
+        Xs.append(X[idx].clone())
+        us.append(u64[idx].float())
+        if log and (k % 100 == 0 or k == steps - 1):
+            log(f"[collect] seed {seed} step {k}: h = {hext:.3f} T, M.h = {mix.compute_mh(m, params['h']):.4f}")
+        hext += hstep
+    mix.close()
+    if f64 is not mix:
+        f64.close()
+    return torch.cat(Xs), torch.cat(us)
+
+
+def train_dense(config, seeds, steps, hstep, nodes, train_steps, batch, lr, out,
+                log=lambda msg: print(msg, flush=True), budget_s=None, targets="fp64"):
+    """The warm-start network on dense snapshots (collect_dense) of the microstructures
+    ``seeds``: len(seeds) * steps (m, u) pairs; with ``budget_s``, the collection stops
+    before a seed that would end past that many seconds (at the mean time per seed so far),
+    and the saved provenance lists the seeds used."""
+    torch = _torch()
+    from .generate import build_config, loop_params
+    from .model import Model
+    p = loop_params(config)
+    parts = []
+    t0 = time.perf_counter()
+    used = []
+    for sd in seeds:
+        el = time.perf_counter() - t0
+        if budget_s is not None and used and el + el / len(used) > budget_s:
+            log(f"[learned] time budget: {len(used)} of {len(seeds)} seeds collected ({el:.1f} s)")
+            break
+        parts.append(collect_dense(config, sd, p, steps, hstep, nodes,
+                                   log=lambda msg: log(f"{msg} ({time.perf_counter() - t0:.1f} s)"), targets=targets))
+        used.append(sd)
+        log(f"[learned] seed {sd}: {parts[-1][0].shape[0]} samples ({time.perf_counter() - t0:.1f} s)")
+    seeds = used
+    X = torch.cat([q[0] for q in parts])
+    u = torch.cat([q[1] for q in parts])
+    del parts
+    model = Model.from_structured(build_config(config, seed=seeds[0]))
+    net = Network(model, max_batch=batch, seed=0)
+    hist = train(net, X, u, steps=train_steps, batch=batch, lr=lr, log=log, log_every=max(train_steps // 10, 1))
+    net.save(out, config=config, seeds=np.asarray(seeds), steps=steps, hstep=hstep, nodes_per_step=nodes,
+             snapshots=np.int64(len(seeds) * steps), samples=np.int64(X.shape[0]), states="mixed-form sweep",
+             targets="plain FP64 Jacobi PCG" if targets == "fp64" else "mixed form, FP64 residual tolerance",
+             train_steps=train_steps, batch=batch, lr=lr,
+             loss_first=hist[0][1], loss_last=hist[-1][1])
+    log(f"[learned] warm start: {X.shape[0]} samples from {len(seeds) * steps} snapshots, loss {hist[0][1]:.3e} -> "
+        f"{hist[-1][1]:.3e}, saved {out} ({time.perf_counter() - t0:.1f} s)")
+    net.close()
+    model.close()
+    return hist
+
+
 def train(net, X, u, steps=2000, batch=16384, lr=1e-3, lr_min=1e-5, seed=0, log=None, log_every=200,
           scale=None):
     """Adam on mean((net(X) - u / scale)^2) with a cosine learning-rate schedule;
@@ -328,7 +415,18 @@ if __name__ == "__main__":
                          "data/precond_*.npz)")
     ap.add_argument("--nodes-lp", type=int, default=4096, help="--both: sampled nodes per residual snapshot")
     ap.add_argument("--out-dir", default=None, help="--both: directory of the two .npz files (default: the package's)")
+    ap.add_argument("--budget", type=float, default=None, help="--dense: seconds of data collection at most")
+    ap.add_argument("--targets", default="fp64", choices=["fp64", "mixed"],
+                    help="--dense: the potential of each snapshot from plain FP64 or from the sweep's mixed form")
+    ap.add_argument("--dense", action="store_true",
+                    help="the warm start only, on a snapshot at every field step of mixed-form sweeps with plain "
+                         "FP64 targets (collect_dense)")
     a = ap.parse_args()
+    if a.dense:
+        out = a.out or str(pathlib.Path(__file__).resolve().parent / "data" / f"warmstart_{a.config}.npz")
+        train_dense(a.config, a.seeds, a.steps, a.hstep, a.nodes, a.train_steps, a.batch, a.lr, out,
+                    budget_s=a.budget, targets=a.targets)
+        raise SystemExit(0)
     if a.both:
         d = pathlib.Path(a.out_dir) if a.out_dir else pathlib.Path(__file__).resolve().parent / "data"
         d.mkdir(exist_ok=True)
