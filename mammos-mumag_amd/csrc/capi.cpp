@@ -86,6 +86,7 @@ struct mm_ctx {
     double *b = nullptr, *r = nullptr, *z = nullptr, *p = nullptr, *p2 = nullptr, *q = nullptr;
     float *r32 = nullptr, *z32 = nullptr, *p32 = nullptr, *p32b = nullptr, *q32 = nullptr, *dx32 = nullptr;
     double *partsA = nullptr, *partsB = nullptr;  // partials of the fused PCG kernels
+    double *ncg_parts = nullptr;                  // partials of the NCG scan (NCG_NV slots)
     // workspace (3N)
     double *gC = nullptr, *gd = nullptr;
     double *tp = nullptr, *tr = nullptr, *tz = nullptr, *td = nullptr, *tCd = nullptr, *tHd = nullptr;
@@ -605,11 +606,21 @@ int potential_solve(mm_ctx *c, const double *m, double *u, double tol, int *it) 
     return rc;
 }
 
+// A reduction over the NCG's interleaved 3-vectors queued behind an evaluation, so that
+// its totals arrive with the energy in one host round trip (into red[0 .. nout)).
+struct PostReduce {
+    RedOp op;
+    const double *a, *b, *cc, *d;
+    int nout;
+    double *out;
+};
+void launch_reduce3(mm_ctx *c, RedOp op, const double *a, const double *b, const double *cc, const double *d);
+
 // E, g_proj, g_full for magnetisation m (energies.py:67-99).  off_zero: on a structured
 // grid, m is zero off the magnet block and gproj, gfull already hold zero there.
 int total_eg(mm_ctx *c, const double *m, double hext, const double *hdir, double tol_u,
              int hmag_on, double *u, double *E, double *gproj, double *gfull, int *pcg_it,
-             bool off_zero = false) {
+             bool off_zero = false, const PostReduce *post = nullptr) {
     Launch L = c->L();
     op_C(c, m, c->gC, hmag_on > 0 ? c->b : nullptr, nullptr);
     int it = 0, rc = 0;
@@ -622,9 +633,12 @@ int total_eg(mm_ctx *c, const double *m, double hext, const double *hdir, double
                  zc2 = -hext * hdir[2] / c->V;
     launch_finish_eg(L, c->N, c->structured ? &c->G : nullptr, m, c->meas, zc0, zc1, zc2, c->gC, c->gd, gfull,
                      gproj, &c->dsc->energy, hmag_on > 0, !(off_zero && c->structured));
+    if (post) launch_reduce3(c, post->op, post->a, post->b, post->cc, post->d);
     int rc2 = sync_scalars(c);
     if (rc2) return rc2;
     *E = c->hsc->energy;
+    if (post)
+        for (int k = 0; k < post->nout; ++k) post->out[k] = c->hsc->red[k];
     if (pcg_it) *pcg_it = it;
     return rc;
 }
@@ -665,9 +679,12 @@ int reduce(mm_ctx *c, RedOp op, int64_t n, const double *a, const double *b, con
 }
 // reduce over the NCG's interleaved 3-vectors; on the magnet block only when the field
 // step's m is zero off it (c->ncg_block: then so are the gradient, direction and iterates)
+void launch_reduce3(mm_ctx *c, RedOp op, const double *a, const double *b, const double *cc, const double *d) {
+    launch_reduce(c->L(), op, 3 * c->N, a, b, cc, d, 0, 0, 0, c->dsc->red, c->ncg_block ? &c->G : nullptr, 3);
+}
 int reduce3(mm_ctx *c, RedOp op, const double *a, const double *b, const double *cc, const double *d,
             double *out, int nout) {
-    launch_reduce(c->L(), op, 3 * c->N, a, b, cc, d, 0, 0, 0, c->dsc->red, c->ncg_block ? &c->G : nullptr, 3);
+    launch_reduce3(c, op, a, b, cc, d);
     int rc = sync_scalars(c);
     if (rc) return rc;
     for (int k = 0; k < nout; ++k) out[k] = c->hsc->red[k];
@@ -979,7 +996,8 @@ int mm_ctx_create(const mm_mesh_desc *d, const mm_options *opts, mm_ctx **out) {
     if ((rc = c->alloc(&c->red.partials, 4 * (size_t)kMaxPartials))) return bail(rc);
 
     const size_t n = c->N, n3 = 3 * n;
-    if ((rc = c->alloc(&c->partsA, 3 * (size_t)kMaxPartials)) || (rc = c->alloc(&c->partsB, 3 * (size_t)kMaxPartials)))
+    if ((rc = c->alloc(&c->partsA, 3 * (size_t)kMaxPartials)) || (rc = c->alloc(&c->partsB, 3 * (size_t)kMaxPartials)) ||
+        (rc = c->alloc(&c->ncg_parts, NCG_NV * (size_t)kMaxPartials)))
         return bail(rc);
     double **vN[] = {&c->b, &c->r, &c->z, &c->p, &c->p2, &c->q};
     for (double **v : vN)
@@ -1267,10 +1285,10 @@ int mm_minimize(mm_ctx *c, double *m_inout, double hext, const double hdir[3],
     };
     auto blk = [&](const double *out) { return off_zero(out) ? &c->G : nullptr; };
 
-    auto feval = [&](const double *x, double *E, double *g) -> int {
+    auto feval = [&](const double *x, double *E, double *g, const PostReduce *post = nullptr) -> int {
         int it = 0;
         const bool oz_g = off_zero(g), oz_f = off_zero(gF);
-        int r = total_eg(c, x, hext, hdir, P->tol_u, P->hmag_on, u, E, g, gF, &it, oz_g && oz_f);
+        int r = total_eg(c, x, hext, hdir, P->tol_u, P->hmag_on, u, E, g, gF, &it, oz_g && oz_f, post);
         loc.fcalls++;
         loc.pcg_iters += it;
         loc.pcg_calls += P->hmag_on > 0;
@@ -1280,6 +1298,15 @@ int mm_minimize(mm_ctx *c, double *m_inout, double hext, const double hdir[3],
     auto minv = [&](const double *g, double *z) -> int {
         loc.c_applies += P->precond_iter;
         return precond(c, m_ref, gF, g, P->precond_iter, z, off_zero(z));
+    };
+    const GridDev *gb = c->ncg_block ? &c->G : nullptr;
+    // The totals of every test from the end of one iteration to the next line search
+    // (the restart after the HS+ update, the stop test, the restart before the line search,
+    // the line search's first step) come from one scan (k_ncg_scan) and one host round
+    // trip, read at the top of the loop; the HS+ beta stays on the device.
+    auto scan = [&](const double *xo, const double *xn, const double *go, const double *gn, bool upd) {
+        if (upd) launch_reduce3(c, RED_YZD, go, gn, Z1, D);   // ((g1 - g).z, (g1 - g).d) into red
+        launch_ncg_scan(L, n3, gb, xo, xn, go, gn, Z1, D, upd ? c->dsc->red : nullptr, c->ncg_parts, c->dsc->ncg);
     };
 
     const double FMAX = DBL_MAX;
@@ -1291,36 +1318,51 @@ int mm_minimize(mm_ctx *c, double *m_inout, double hext, const double hdir[3],
     HIPC(hipMemcpyAsync(X0, X, bytes3, hipMemcpyDeviceToDevice, c->s));
     off_zero(X);    // copies of m in full
     off_zero(X0);
+    scan(X0, X, G, G, false);
+    bool after_update = false;   // the scan's d is an HS+ update (its restart test pending)
     const double sqrt_tol = std::sqrt(tol), cbrt_tol = std::cbrt(tol);
     const double sqrt_eps = std::sqrt(DBL_EPSILON);
     for (;;) {
-        double s3[3];
-        if ((rc = reduce3(c, RED_STOP3, X0, X, G, nullptr, s3, 3))) return rc;
+        if ((rc = sync_scalars(c))) return rc;
+        double v[NCG_NV];
+        for (int q = 0; q < NCG_NV; ++q) v[q] = c->hsc->ncg[q];
+        // restart if the HS+ direction is not a sufficient descent direction
+        // (minimizers.py:182-184; tested against the gradient it was formed with)
+        bool rz = false;
+        if (after_update && v[NCG_DG_OLD] > -0.001 * std::sqrt(v[NCG_DD]) * std::sqrt(v[NCG_GG_OLD])) {
+            launch_axpby(L, n3, -1.0, Z1, 0.0, nullptr, D, blk(D));
+            loc.restarts_z++;
+            rz = true;
+        }
+        // stop test (minimizers.py:164-166): max|x0 - x|, max|x|, max|g|
         const bool a = (f0 - f) > tol * (1 + std::fabs(f));
-        const bool b = (s3[0] > sqrt_tol * (1 + s3[1])) || k == 0;
-        const bool cc = s3[2] > cbrt_tol * (1 + std::fabs(f));
+        const bool b = (v[NCG_DX_MAX] > sqrt_tol * (1 + v[NCG_X_MAX])) || k == 0;
+        const bool cc = v[NCG_G_MAX] > cbrt_tol * (1 + std::fabs(f));
         if (!((a || b || cc) && k < iter_max)) break;
-        // restart if d is not a sufficient descent direction (minimizers.py:171-173)
-        double dg3[3];
-        if ((rc = reduce3(c, RED_DG3, D, G, nullptr, nullptr, dg3, 3))) return rc;
+        // restart if d is not a sufficient descent direction (minimizers.py:171-173):
+        // d.g, d.d, g.g for d as it stands (-z after the restart above)
+        const double dg3[3] = {rz ? -v[NCG_ZG] : v[NCG_DG], rz ? v[NCG_ZZ] : v[NCG_DD], v[NCG_GG]};
+        bool rg = false;
         if (dg3[0] > -0.001 * std::sqrt(dg3[1]) * std::sqrt(dg3[2])) {
             launch_axpby(L, n3, -1.0, G, 0.0, nullptr, D, blk(D));
             loc.restarts_g++;
+            rg = true;
         }
         // ---- line search (minimizers.py:92-137)
         double f1 = 0.0;
         {
             const double rho = 0.5, cA = 1e-4, delta = 0.1, eps = 1e-6;
             const int max_ls = 20;
-            double h3[3];
-            if ((rc = reduce3(c, RED_H3, X, D, nullptr, nullptr, h3, 3))) return rc;
+            // x.x, d.d, max|d| for the final d
+            const double h3[3] = {v[NCG_XX], rg ? v[NCG_GG] : rz ? v[NCG_ZZ] : v[NCG_DD],
+                                  rg ? v[NCG_G_MAX] : rz ? v[NCG_Z_MAX] : v[NCG_D_MAX]};
             const double t1 = 2.0 * sqrt_eps * (1.0 + std::sqrt(h3[0])) / std::sqrt(h3[1]);
             const double t2 = sqrt_eps / h3[2];
             const double h = t1 < t2 ? t1 : t2;
             launch_update_m(L, n, X, h, D, X1, blk(X1));
-            if ((rc = feval(X1, &f1, G1))) return rc;
             double gd2[2];
-            if ((rc = reduce3(c, RED_GD2, G, G1, D, nullptr, gd2, 2))) return rc;
+            const PostReduce pgd2{RED_GD2, G, G1, D, nullptr, 2, gd2};
+            if ((rc = feval(X1, &f1, G1, &pgd2))) return rc;
             const double gd = gd2[0];
             double alpha;
             {  // alpha_init (minimizers.py:34-57), c = 0.1, o = 0.99
@@ -1330,10 +1372,10 @@ int mm_minimize(mm_ctx *c, double *m_inout, double hext, const double hdir[3],
                 if (term2 < 0) term2 = 1.0e30;
                 alpha = term1 < 0.99 * term2 ? term1 : term2;
             }
-            launch_update_m(L, n, X, alpha, D, X1, blk(X1));
-            if ((rc = feval(X1, &f1, G1))) return rc;
             double gd1;
-            if ((rc = reduce3(c, RED_DOT, G1, D, nullptr, nullptr, &gd1, 1))) return rc;
+            const PostReduce pdot{RED_DOT, G1, D, nullptr, nullptr, 1, &gd1};
+            launch_update_m(L, n, X, alpha, D, X1, blk(X1));
+            if ((rc = feval(X1, &f1, G1, &pdot))) return rc;
             int i = 0;
             for (;;) {
                 const double eps_k = eps * std::fabs(f);
@@ -1342,23 +1384,16 @@ int mm_minimize(mm_ctx *c, double *m_inout, double hext, const double hdir[3],
                 if (done) break;
                 alpha *= rho;
                 launch_update_m(L, n, X, alpha, D, X1, blk(X1));
-                if ((rc = feval(X1, &f1, G1))) return rc;
-                if ((rc = reduce3(c, RED_DOT, G1, D, nullptr, nullptr, &gd1, 1))) return rc;
+                if ((rc = feval(X1, &f1, G1, &pdot))) return rc;
                 ++i;
                 loc.backtracks++;
             }
         }
-        // ---- HS+ direction (minimizers.py:175-184)
+        // ---- HS+ direction (minimizers.py:175-184): d = -z1 + beta d on the device, its
+        // restart test at the top of the next iteration
         if ((rc = minv(G1, Z1))) return rc;
-        double yzd[2];
-        if ((rc = reduce3(c, RED_YZD, G, G1, Z1, D, yzd, 2))) return rc;
-        const double beta = std::max(yzd[0] / yzd[1], 0.0);
-        launch_axpby(L, n3, -1.0, Z1, beta, D, D, blk(D));
-        if ((rc = reduce3(c, RED_DG3, D, G, nullptr, nullptr, dg3, 3))) return rc;
-        if (dg3[0] > -0.001 * std::sqrt(dg3[1]) * std::sqrt(dg3[2])) {
-            launch_axpby(L, n3, -1.0, Z1, 0.0, nullptr, D, blk(D));
-            loc.restarts_z++;
-        }
+        scan(X, X1, G, G1, true);
+        after_update = true;
         // shift state: x0 <- x, x <- x1, f0 <- f, f <- f1, g <- g1
         std::swap(X0, X);
         std::swap(X, X1);

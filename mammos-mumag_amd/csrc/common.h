@@ -31,6 +31,8 @@ struct DevScalars {
     double energy;
     // multigrid V-cycles that ran (the coarse tail kernel counts itself when not skipped)
     long long mg_vcycles;
+    // totals of the NCG scan between two iterations (NcgScan order)
+    double ncg[16];
 };
 
 // Per-material coefficients, scaled as the reference scales them
@@ -264,5 +266,33 @@ void launch_outside_max(const Launch &L, int64_t n, const GridDev &G, const doub
 void launch_reduce(const Launch &L, RedOp op, int64_t n, const double *a, const double *b,
                    const double *c, const double *d, double h0, double h1, double h2, double *out,
                    const GridDev *G = nullptr, int comp = 1);
+
+// Totals of the NCG scan between two iterations (k_ncg_scan): the old iterate xo and
+// gradient go, the new xn and gn, the preconditioned gradient z, the direction d.
+enum NcgScan {
+    NCG_DG_OLD = 0,   // d.go   (the restart test after the HS+ update)
+    NCG_DD = 1,       // d.d
+    NCG_GG_OLD = 2,   // go.go
+    NCG_DX_MAX = 3,   // max |xo - xn|   (the stop test)
+    NCG_X_MAX = 4,    // max |xn|
+    NCG_G_MAX = 5,    // max |gn|
+    NCG_DG = 6,       // d.gn   (the restart test before the line search)
+    NCG_GG = 7,       // gn.gn
+    NCG_ZG = 8,       // z.gn
+    NCG_ZZ = 9,       // z.z
+    NCG_XX = 10,      // xn.xn  (the line search's first step)
+    NCG_D_MAX = 11,   // max |d|
+    NCG_Z_MAX = 12,   // max |z|
+    NCG_NV = 13,
+};
+constexpr unsigned kNcgMaxMask = (1u << NCG_DX_MAX) | (1u << NCG_X_MAX) | (1u << NCG_G_MAX) |
+                                 (1u << NCG_D_MAX) | (1u << NCG_Z_MAX);
+// With yzd (device pointer to the totals (g1 - g).z, (g1 - g).d) first d = -z + beta d,
+// beta = max(yzd[0] / yzd[1], 0); then the NCG_NV totals into out (device).  G as for
+// launch_reduce over n3 = 3 N interleaved entries.  partials: NCG_NV * kMaxPartials
+// doubles.  Returns the block count.
+int launch_ncg_scan(const Launch &L, int64_t n3, const GridDev *G, const double *xo, const double *xn,
+                    const double *go, const double *gn, const double *z, double *d, const double *yzd,
+                    double *partials, double *out);
 
 }  // namespace mm

@@ -1776,6 +1776,106 @@ void launch_reduce(const Launch &L, RedOp op, int64_t n, const double *a, const 
     launch_fin(L, FinArgs::store(out, nv, mask, nb, nullptr));
 }
 
+// Totals of NV values' partials into out[0 .. NV), each summed in k_fin's order (per
+// thread over j = t, t + 256, ..., then the wave, then the waves in order).
+template <int NV>
+__global__ __launch_bounds__(kFinThreads) void k_fin_store(const double *__restrict__ partials, int nparts,
+                                                           unsigned maxmask, double *__restrict__ out) {
+    __shared__ double sh[NV][kFinThreads / kWave];
+    const int lane = threadIdx.x & (kWave - 1), w = threadIdx.x / kWave;
+    double acc[NV];
+#pragma unroll
+    for (int i = 0; i < NV; ++i) acc[i] = 0.0;
+    for (int j = threadIdx.x; j < nparts; j += kFinThreads) {
+#pragma unroll
+        for (int i = 0; i < NV; ++i) {
+            const double p = partials[i * kMaxPartials + j];
+            acc[i] = ((maxmask >> i) & 1u) ? fmax(acc[i], p) : acc[i] + p;
+        }
+    }
+#pragma unroll
+    for (int i = 0; i < NV; ++i) {
+        acc[i] = ((maxmask >> i) & 1u) ? wave_max(acc[i]) : wave_sum(acc[i]);
+        if (lane == 0) sh[i][w] = acc[i];
+    }
+    __syncthreads();
+    if (threadIdx.x >= NV) return;
+    const int i = threadIdx.x;
+    const bool mx = (maxmask >> i) & 1u;
+    double t = sh[i][0];
+    for (int k = 1; k < kFinThreads / kWave; ++k) t = mx ? fmax(t, sh[i][k]) : t + sh[i][k];
+    out[i] = t;
+}
+
+// One pass over the NCG state between two iterations.  With UPD, the HS+ direction
+// d = -z + beta d (minimizers.py:175-181), beta = max(yzd[0] / yzd[1], 0) from the totals
+// of (g1 - g).z and (g1 - g).d; then the totals every test up to the next line search may
+// need (NcgScan in common.h), for d as formed and for the two restart directions -z and
+// -g1.  The element loop, block count and block reduction are k_reduce's over the same
+// vectors, so each total is the value the separate reduction would give.
+template <bool UPD>
+__global__ __launch_bounds__(kBlock) void k_ncg_scan(int64_t n, GridDev G, const double *__restrict__ xo,
+                                                     const double *__restrict__ xn, const double *__restrict__ go,
+                                                     const double *__restrict__ gn, const double *__restrict__ z,
+                                                     double *__restrict__ d, const double *__restrict__ yzd,
+                                                     double *partials) {
+    double s[NCG_NV];
+#pragma unroll
+    for (int k = 0; k < NCG_NV; ++k) s[k] = 0.0;
+    double beta = 0.0;
+    if (UPD) {
+        const double q = yzd[0] / yzd[1];
+        beta = q < 0.0 ? 0.0 : q;
+    }
+    const int n1 = G.mn + 1;
+    const int64_t ne = G.nx > 0 ? (int64_t)n1 * n1 * n1 * 3 : n;
+    for (int64_t e = blockIdx.x * (int64_t)kBlock + threadIdx.x; e < ne; e += (int64_t)gridDim.x * kBlock) {
+        int64_t i = e;
+        if (G.nx > 0) {
+            const uint32_t eu = (uint32_t)e, n1u = (uint32_t)n1;
+            const uint32_t t = eu / 3u, r = eu - t * 3u;
+            const uint32_t tj = t / n1u, k = tj / n1u;
+            i = 3 * gnode(G, (int)(t - tj * n1u), (int)(tj - k * n1u), (int)k) + r;
+        }
+        const double zv = z[i], gov = go[i], gnv = gn[i], xnv = xn[i];
+        double dv = d[i];
+        if (UPD) {
+            dv = fma(beta, dv, -zv);
+            d[i] = dv;
+        }
+        s[NCG_DG_OLD] += dv * gov;
+        s[NCG_DD] += dv * dv;
+        s[NCG_GG_OLD] += gov * gov;
+        s[NCG_DX_MAX] = fmax(s[NCG_DX_MAX], fabs(xo[i] - xnv));
+        s[NCG_X_MAX] = fmax(s[NCG_X_MAX], fabs(xnv));
+        s[NCG_G_MAX] = fmax(s[NCG_G_MAX], fabs(gnv));
+        s[NCG_DG] += dv * gnv;
+        s[NCG_GG] += gnv * gnv;
+        s[NCG_ZG] += zv * gnv;
+        s[NCG_ZZ] += zv * zv;
+        s[NCG_XX] += xnv * xnv;
+        s[NCG_D_MAX] = fmax(s[NCG_D_MAX], fabs(dv));
+        s[NCG_Z_MAX] = fmax(s[NCG_Z_MAX], fabs(zv));
+    }
+    block_partials<NCG_NV>(s, kNcgMaxMask, partials);
+}
+
+int launch_ncg_scan(const Launch &L, int64_t n3, const GridDev *G, const double *xo, const double *xn,
+                    const double *go, const double *gn, const double *z, double *d, const double *yzd,
+                    double *partials, double *out) {
+    const GridDev gd = G ? *G : GridDev{};
+    const int64_t n1 = gd.mn + 1;
+    const int nb = grid_for(G ? n1 * n1 * n1 * 3 : n3, 8);
+    if (yzd)
+        hipLaunchKernelGGL(k_ncg_scan<true>, dim3(nb), dim3(kBlock), 0, L.s, n3, gd, xo, xn, go, gn, z, d, yzd,
+                           partials);
+    else
+        hipLaunchKernelGGL(k_ncg_scan<false>, dim3(nb), dim3(kBlock), 0, L.s, n3, gd, xo, xn, go, gn, z, d, yzd,
+                           partials);
+    hipLaunchKernelGGL(k_fin_store<NCG_NV>, dim3(1), dim3(kFinThreads), 0, L.s, partials, nb, kNcgMaxMask, out);
+    return nb;
+}
+
 __global__ void k_fill_zero(int64_t n, double *__restrict__ x, const int *skip) {
     if (skip && *skip) return;
     for (int64_t i = blockIdx.x * (int64_t)kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock)
