@@ -79,7 +79,8 @@ struct mm_ctx {
     MatCoef *mats = nullptr;
     int n_mats = 0;
     DevScalars *dsc = nullptr;
-    DevScalars *hsc = nullptr;  // pinned host mirror
+    DevScalars *hsc = nullptr;  // pinned host mirror (written by the device, k_publish)
+    unsigned long long *hseq = nullptr, seq = 0;   // the mirror's sequence number (same allocation)
     Reducer red;
 
     // workspace (N)
@@ -225,10 +226,25 @@ struct mm_mlp {
 
 namespace {
 
+// The device scalars into the host mirror: a one-wave kernel writes them into pinned host
+// memory and then a new sequence number behind a system-scope release; the host spins on
+// the number (a stream synchronize after a device-to-host copy costs ~5 us more per round
+// trip), polling the stream now and then so that a failed launch returns its error.
 int sync_scalars(mm_ctx *c) {
-    HIPC(hipMemcpyAsync(c->hsc, c->dsc, sizeof(DevScalars), hipMemcpyDeviceToHost, c->s));
-    HIPC(hipStreamSynchronize(c->s));
-    return 0;
+    const unsigned long long seq = ++c->seq;
+    launch_publish(c->L(), c->dsc, c->hsc, c->hseq, seq);
+    for (unsigned k = 1;; ++k) {
+        if (__atomic_load_n(c->hseq, __ATOMIC_ACQUIRE) == seq) return 0;
+        if ((k & 0x3fffu) == 0) {
+            const hipError_t e = hipStreamQuery(c->s);
+            if (e == hipErrorNotReady) continue;
+            if (e != hipSuccess) return fail(MM_E_HIP, std::string("stream: ") + hipGetErrorString(e));
+            // the stream has drained: the number is there unless the launch failed
+            if (__atomic_load_n(c->hseq, __ATOMIC_ACQUIRE) == seq) return 0;
+            const hipError_t le = hipGetLastError();
+            return fail(MM_E_HIP, std::string("scalar publish did not complete: ") + hipGetErrorString(le));
+        }
+    }
 }
 
 // ---------------------------------------------------------------- operators
@@ -991,8 +1007,16 @@ int mm_ctx_create(const mm_mesh_desc *d, const mm_options *opts, mm_ctx **out) {
         init.ir_done = 1;
         HIPC(hipMemcpy(c->dsc, &init, sizeof(DevScalars), hipMemcpyHostToDevice));
     }
-    if (hipHostMalloc((void **)&c->hsc, sizeof(DevScalars), hipHostMallocDefault) != hipSuccess)
-        return bail(fail(MM_E_HIP, "hipHostMalloc failed"));
+    {   // the mirror and, on its own 64-byte line after it, the sequence number; coherent, so
+        // that the device's writes bypass its caches
+        constexpr size_t kMirror = (sizeof(DevScalars) + 63) / 64 * 64;
+        void *hm = nullptr;
+        if (hipHostMalloc(&hm, kMirror + 64, hipHostMallocCoherent) != hipSuccess)
+            return bail(fail(MM_E_HIP, "hipHostMalloc failed"));
+        std::memset(hm, 0, kMirror + 64);
+        c->hsc = (DevScalars *)hm;
+        c->hseq = (unsigned long long *)((char *)hm + kMirror);
+    }
     if ((rc = c->alloc(&c->red.partials, 4 * (size_t)kMaxPartials))) return bail(rc);
 
     const size_t n = c->N, n3 = 3 * n;

@@ -149,6 +149,9 @@ struct Launch {
 
 // ---------------- kernel launchers (kernels.hip) ----------------
 void launch_fin(const Launch &L, const FinArgs &a);
+// The device scalars d into the pinned host mirror h, then seq into *hseq (host memory)
+void launch_publish(const Launch &L, const DevScalars *d, DevScalars *h, unsigned long long *hseq,
+                    unsigned long long seq);
 
 // y = A x, optional (x . y) into *dot_out (device); skipped when *skip_flag != 0.
 void launch_sell_spmv(const Launch &L, const SellDev<double> &A, const double *x, double *y,

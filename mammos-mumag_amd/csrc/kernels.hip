@@ -162,6 +162,23 @@ void launch_fin(const Launch &L, const FinArgs &a) {
     hipLaunchKernelGGL(k_fin, dim3(1), dim3(kFinThreads), 0, L.s, L.red.partials, a, L.dsc);
 }
 
+// The device scalars copied word by word into the host mirror by one wave; the wave's
+// system-scope fence makes the copy visible before lane 0 writes the sequence number.
+constexpr int kScalarWords = (int)(sizeof(DevScalars) / 8);
+static_assert(sizeof(DevScalars) % 8 == 0 && kScalarWords <= 2 * kWave, "two words per lane at most");
+__global__ __launch_bounds__(kWave) void k_publish(const unsigned long long *__restrict__ d,
+                                                   unsigned long long *h, unsigned long long *hseq,
+                                                   unsigned long long seq) {
+    for (int w = threadIdx.x; w < kScalarWords; w += kWave) h[w] = d[w];
+    __threadfence_system();
+    if (threadIdx.x == 0) __hip_atomic_store(hseq, seq, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+}
+void launch_publish(const Launch &L, const DevScalars *d, DevScalars *h, unsigned long long *hseq,
+                    unsigned long long seq) {
+    hipLaunchKernelGGL(k_publish, dim3(1), dim3(kWave), 0, L.s, (const unsigned long long *)d,
+                       (unsigned long long *)h, hseq, seq);
+}
+
 // ------------------------------------------------------------------ SELL SpMV
 
 template <typename T>
