@@ -662,11 +662,38 @@ int total_eg(mm_ctx *c, const double *m, double hext, const double *hdir, double
 // M_inv (loop.py:40-55): truncated CG on H with grad_fk = -g.  No host sync.
 // zout_off_zero: on a structured grid, g is zero off the magnet block and zout already
 // holds zero there.
+// The truncated CG's direction ring for at least `slots` directions (max_iter + 1); a
+// larger ring replaces the old one (freed once the stream has drained).
+constexpr int kTcgSlots = 11;   // precond_iter 10, the reference's default
+template <typename T>
+int tcg_ring(mm_ctx *c, TcgBlock<T> &B, int slots) {
+    if (B.slots >= slots) return 0;
+    const size_t n1 = (size_t)c->G.mn + 1, n3 = 3 * n1 * n1 * n1;
+    if (B.dring) {
+        HIPC(hipStreamSynchronize(c->s));
+        for (void *old : {(void *)B.dring, (void *)B.alpha}) {
+            c->allocs.erase(std::find(c->allocs.begin(), c->allocs.end(), old));
+            (void)hipFree(old);
+        }
+        c->bytes -= (size_t)B.slots * n3 * sizeof(T) + (size_t)B.slots * sizeof(double);
+        B.dring = nullptr;
+        B.alpha = nullptr;
+        B.slots = 0;
+    }
+    int rc;
+    if ((rc = c->alloc(&B.dring, (size_t)slots * n3)) || (rc = c->alloc(&B.alpha, (size_t)slots))) return rc;
+    B.slots = slots;
+    return 0;
+}
+
 int precond(mm_ctx *c, const double *m_ref, const double *gF, const double *g, int max_iter,
             double *zout, bool zout_off_zero = false) {
     Launch L = c->L();
     const int64_t n = c->N, n3 = 3 * c->N;
     if (c->structured) {   // on the magnet block only (FP32 vectors in the mixed form)
+        const int rc = c->precision == MM_PREC_MIXED ? tcg_ring(c, c->tb32, max_iter + 1)
+                                                     : tcg_ring(c, c->tb64, max_iter + 1);
+        if (rc) return rc;
         if (c->precision == MM_PREC_MIXED)
             tcgc_run<float>(L, c->G, n, c->tb32, m_ref, gF, g, c->Dex, max_iter, zout, !zout_off_zero);
         else
@@ -1039,9 +1066,9 @@ int mm_ctx_create(const mm_mesh_desc *d, const mm_options *opts, mm_ctx **out) {
         auto block = [&](auto &B) -> int {
             int r_;
             if ((r_ = c->alloc(&B.mg, nb))) return r_;
-            for (auto **v : {&B.m0, &B.D, &B.p, &B.r, &B.z, &B.Hd, &B.d[0], &B.d[1]})
+            for (auto **v : {&B.m0, &B.D, &B.p, &B.r, &B.z, &B.Hd})
                 if ((r_ = c->alloc(v, 3 * nb))) return r_;
-            return 0;
+            return tcg_ring(c, B, kTcgSlots);
         };
         if ((rc = c->precision == MM_PREC_MIXED ? block(c->tb32) : block(c->tb64))) return bail(rc);
     } else {

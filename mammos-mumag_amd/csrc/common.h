@@ -230,11 +230,16 @@ void launch_tcg_final(const Launch &L, int64_t n3, const double *p, const double
 template <typename T>
 struct TcgBlock {
     T *m0 = nullptr, *mg = nullptr, *D = nullptr, *p = nullptr, *r = nullptr, *z = nullptr, *Hd = nullptr;
-    T *d[2] = {nullptr, nullptr};
+    // the directions of one solve, n3 entries each: slot 0 the initial d, slot k + 1 the
+    // k-th iteration's d'; p = sum_k alpha[k] slot[k + 1] is formed once at the end
+    T *dring = nullptr;
+    int slots = 0;
+    double *alpha = nullptr;   // [slots - 1] step lengths (0: the iteration made no update)
 };
 // The magnet block's node tiles fit the partial slots (one partial per tile)
 bool tcg_tiles_fit(int mn);
-// write_off = false: zout already holds zero off the block and is written on it only
+// write_off = false: zout already holds zero off the block and is written on it only;
+// B.slots >= max_iter + 1
 template <typename T>
 void tcgc_run(const Launch &L, const GridDev &G, int64_t n, const TcgBlock<T> &B, const double *m0,
               const double *gF, const double *g, const double *D, int max_iter, double *zout,

@@ -1291,9 +1291,11 @@ void launch_tcg_final(const Launch &L, int64_t n3, const double *p, const double
 // (mn+1)^3 block nodes only, magnet index t = i + n1 (j + n1 k), vectors [3 t + a] in the
 // storage type T (FP32 for the mixed form, FP64 otherwise; dot products in FP64), with
 // d = -z + beta d, C d and H d fused into one kernel:
-//   k_tcgc_init   r = -g, z = r / D, d = -z, p = 0 (block copies of m0, m0.gF, D); (r.r, r.z)
+//   k_tcgc_init   r = -g, z = r / D, d = -z (block copies of m0, m0.gF, D); (r.r, r.z)
 //   k_tcgc_dh     d' = -z + beta d (27-neighbour gather), C d', H d'; d'.H d'
-//   k_tcgc_upd    p += alpha d', r += alpha H d', z = r / D; (r.r, r.z)
+//   k_tcgc_upd    r += alpha H d', z = r / D; (r.r, r.z); alpha kept per iteration
+//   k_tcgc_accum  p = sum_k alpha_k d'_k over the iterations' directions (kept in a ring
+//                 of slots, so no iteration reads and writes p); max |p|
 //   k_tcgc_final  zout = (max|p| == 0) ? g : p scattered to the grid (0 off the block)
 
 __device__ __forceinline__ void block_ijk(int n1, int64_t t, int &i, int &j, int &k) {
@@ -1308,11 +1310,13 @@ __global__ __launch_bounds__(kBlock) void k_tcgc_init(GridDev G, const double *_
                                                       const double *__restrict__ gF,
                                                       const double *__restrict__ m0,
                                                       const double *__restrict__ D, T *__restrict__ m0c,
-                                                      T *__restrict__ mgc, T *__restrict__ Dc, T *__restrict__ p,
-                                                      T *__restrict__ r, T *__restrict__ z, T *__restrict__ d,
-                                                      double *partials) {
+                                                      T *__restrict__ mgc, T *__restrict__ Dc, T *__restrict__ r,
+                                                      T *__restrict__ z, T *__restrict__ d,
+                                                      double *__restrict__ alpha, int nalpha, double *partials) {
     const int n1 = G.mn + 1;
     const int64_t nb = (int64_t)n1 * n1 * n1;
+    if (blockIdx.x == 0)
+        for (int q = threadIdx.x; q < nalpha; q += kBlock) alpha[q] = 0.0;
     double s[2] = {0.0, 0.0};
     for (int64_t t = blockIdx.x * (int64_t)kBlock + threadIdx.x; t < nb; t += (int64_t)gridDim.x * kBlock) {
         int i, j, k;
@@ -1328,7 +1332,6 @@ __global__ __launch_bounds__(kBlock) void k_tcgc_init(GridDev G, const double *_
             const T ri = (T)(-g[3 * gn + a]);
             const T zi = (T)((double)ri / (double)Da);
             Dc[3 * t + a] = Da;
-            p[3 * t + a] = (T)0;
             r[3 * t + a] = ri;
             z[3 * t + a] = zi;
             d[3 * t + a] = -zi;
@@ -1495,27 +1498,31 @@ __global__ __launch_bounds__(kBlock) void k_tcgc_dh(GridDev G, TcgTiles tl, cons
 }
 
 // den = d'.H d' from k_tcgc_dh's nden partials, summed in every block in k_fin's order
-// (so alpha is what a finalize launch would give); block 0 publishes it.
+// (so alpha is what a finalize launch would give); block 0 publishes it and the
+// iteration's step length (0 on negative curvature; an iteration skipped after the stop
+// keeps the 0 the init wrote).
 template <typename T>
-__global__ __launch_bounds__(kBlock) void k_tcgc_upd(int64_t n3, T *__restrict__ p, T *__restrict__ r,
-                                                     T *__restrict__ z, const T *__restrict__ d,
+__global__ __launch_bounds__(kBlock) void k_tcgc_upd(int64_t n3, T *__restrict__ r, T *__restrict__ z,
                                                      const T *__restrict__ Hd, const T *__restrict__ Dc,
                                                      DevScalars *dsc, const double *__restrict__ dparts,
-                                                     int nden, double *partials) {
+                                                     int nden, double *__restrict__ alpha_out,
+                                                     double *partials) {
     static_assert(kFinThreads == kBlock, "the finalize order needs the finalize's block width");
     __shared__ double sh[kBlock / kWave];
     const int done = dsc->tcg_done;   // block-uniform; tested after the barrier
     const double den = fin_order_sum(dparts, nden, sh);
     if (done) return;
-    if (blockIdx.x == 0 && threadIdx.x == 0) dsc->tcg_den = den;
     const bool neg = den <= 0.0;
     const double alpha = neg ? 0.0 : dsc->tcg_rz / den;
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        dsc->tcg_den = den;
+        *alpha_out = alpha;
+    }
     double s[2] = {0.0, 0.0};
     for (int64_t i = blockIdx.x * (int64_t)kBlock + threadIdx.x; i < n3; i += (int64_t)gridDim.x * kBlock) {
         const T rn = (T)((double)r[i] + alpha * (double)Hd[i]);
         const T zn = (T)((double)rn / (double)Dc[i]);
         if (!neg) {
-            p[i] = (T)((double)p[i] + alpha * (double)d[i]);
             r[i] = rn;
             z[i] = zn;
         }
@@ -1525,11 +1532,37 @@ __global__ __launch_bounds__(kBlock) void k_tcgc_upd(int64_t n3, T *__restrict__
     block_partials<2>(s, 0u, partials);
 }
 
+// p = sum_k alpha_k d'_k in iteration order (the sum an update p += alpha_k d'_k per
+// iteration forms, rounded to T once), skipping the iterations without an update
+// (alpha_k = 0); max |p|.  Only the iterations that ran (tcg_iter, counted by the
+// finalize) wrote a direction; their loads are issued four slots at a time.
 template <typename T>
-__global__ __launch_bounds__(kBlock) void k_tcgc_pmax(int64_t n3, const T *__restrict__ p, double *partials) {
+__global__ __launch_bounds__(kBlock) void k_tcgc_accum(int64_t n3, const T *__restrict__ dring,
+                                                       const double *__restrict__ alpha, int nit,
+                                                       const DevScalars *dsc, T *__restrict__ p, double *partials) {
+    const int nrun = min(dsc->tcg_iter, nit);   // uniform
     double s[1] = {0.0};
-    for (int64_t i = blockIdx.x * (int64_t)kBlock + threadIdx.x; i < n3; i += (int64_t)gridDim.x * kBlock)
-        s[0] = fmax(s[0], fabs((double)p[i]));
+    for (int64_t i = blockIdx.x * (int64_t)kBlock + threadIdx.x; i < n3; i += (int64_t)gridDim.x * kBlock) {
+        double acc = 0.0;
+        int k = 0;
+        for (; k + 4 <= nrun; k += 4) {
+            double dv[4], av[4];
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                av[q] = alpha[k + q];
+                dv[q] = (double)dring[(int64_t)(k + 1 + q) * n3 + i];
+            }
+#pragma unroll
+            for (int q = 0; q < 4; ++q) acc = av[q] != 0.0 ? acc + av[q] * dv[q] : acc;
+        }
+        for (; k < nrun; ++k) {
+            const double a = alpha[k], dv = (double)dring[(int64_t)(k + 1) * n3 + i];
+            acc = a != 0.0 ? acc + a * dv : acc;
+        }
+        const T pv = (T)acc;
+        p[i] = pv;
+        s[0] = fmax(s[0], fabs((double)pv));
+    }
     block_partials<1>(s, 1u, partials);
 }
 
@@ -1572,16 +1605,16 @@ void tcgc_run(const Launch &L, const GridDev &G, int64_t n, const TcgBlock<T> &B
     const int gi = grid_for(nb, 2), gu = grid_for(n3, 4);
     const TcgTiles tl = block_tiles(G);
     const int gd = tl.gx * tl.gy * tl.gz;
-    hipLaunchKernelGGL(k_tcgc_init<T>, dim3(gi), dim3(kBlock), 0, L.s, G, g, gF, m0, D, B.m0, B.mg, B.D, B.p, B.r,
-                       B.z, B.d[0], L.red.partials);
+    hipLaunchKernelGGL(k_tcgc_init<T>, dim3(gi), dim3(kBlock), 0, L.s, G, g, gF, m0, D, B.m0, B.mg, B.D, B.r, B.z,
+                       B.dring, B.alpha, max_iter, L.red.partials);
     FinArgs ai{};
     ai.op = FIN_TCG_INIT;
     ai.nv = 2;
     ai.nparts = gi;
     launch_fin(L, ai);
     for (int it = 0; it < max_iter; ++it) {
-        const T *din = B.d[it & 1];
-        T *dout = B.d[(it + 1) & 1];
+        const T *din = B.dring + (int64_t)it * n3;
+        T *dout = B.dring + (int64_t)(it + 1) * n3;
         // d'.H d' partials in slot 3, read by the update kernel while it writes slots 0, 1
         double *const dparts = L.red.partials + 3 * kMaxPartials;
         if (it == 0)
@@ -1590,8 +1623,8 @@ void tcgc_run(const Launch &L, const GridDev &G, int64_t n, const TcgBlock<T> &B
         else
             hipLaunchKernelGGL((k_tcgc_dh<T, false>), dim3(gd), dim3(kBlock), 0, L.s, G, tl, L.mats, B.z, din, dout,
                                B.m0, B.mg, B.Hd, L.dsc, dparts);
-        hipLaunchKernelGGL(k_tcgc_upd<T>, dim3(gu), dim3(kBlock), 0, L.s, n3, B.p, B.r, B.z, dout, B.Hd, B.D, L.dsc,
-                           dparts, gd, L.red.partials);
+        hipLaunchKernelGGL(k_tcgc_upd<T>, dim3(gu), dim3(kBlock), 0, L.s, n3, B.r, B.z, B.Hd, B.D, L.dsc, dparts, gd,
+                           B.alpha + it, L.red.partials);
         FinArgs a{};
         a.op = FIN_TCG_UPDATE;
         a.nv = 2;
@@ -1599,7 +1632,8 @@ void tcgc_run(const Launch &L, const GridDev &G, int64_t n, const TcgBlock<T> &B
         a.skip = &L.dsc->tcg_done;
         launch_fin(L, a);
     }
-    hipLaunchKernelGGL(k_tcgc_pmax<T>, dim3(gu), dim3(kBlock), 0, L.s, n3, B.p, L.red.partials);
+    hipLaunchKernelGGL(k_tcgc_accum<T>, dim3(gu), dim3(kBlock), 0, L.s, n3, B.dring, B.alpha, max_iter, L.dsc,
+                       B.p, L.red.partials);
     launch_fin(L, FinArgs::store(&L.dsc->tcg_pmax, 1, 1u, gu, nullptr));
     if (write_off)
         hipLaunchKernelGGL(k_tcgc_final<T>, dim3(grid_for(n, 2)), dim3(kBlock), 0, L.s, G, n, B.p, g, zout, L.dsc);
