@@ -605,8 +605,10 @@ def whole_step(res, precision, steps):
         # r, z written)
         cls["pcg_starts"] = st["pcg_calls"] * (16 + 40) * N
     # truncated CG (magnet block): d' = -z + beta d, C d', H d' (z, d, m0, m0.gF read; d', Hd
-    # written; voxel materials) and the update (r, Hd, diag, p, d read; p, r, z written)
-    cls["truncated_cg"] = st["tcg_iters"] * ((40 * s) * Nb + 2 * Nv)
+    # written; voxel materials), the update (r, Hd, diag read; r, z written) and, once per
+    # solve (one per NCG iteration), the solution p = sum alpha_k d'_k (each iteration's d'
+    # read, p written)
+    cls["truncated_cg"] = st["tcg_iters"] * ((34 * s) * Nb + 2 * Nv) + st["ncg_iters"] * 3 * s * Nb
     # evaluation (FP64): C m with D m (m read, gC and b written), G u (u read, g written),
     # finish (meas, m, gC, gd read; g_full, g_proj written)
     cls["evaluations"] = st["fcalls"] * (216 * Nb + 4 * Nv)

@@ -20,7 +20,7 @@ def bench():
 def _res(levels, vcycles, ru=0, corr=0):
     N, Nb, Nv = 1000, 343, 216
     return dict(info=dict(structured=1), n_nodes=N, n_mag_nodes=Nb, n_voxels=Nv, levels=levels,
-                stats=dict(pcg_iters=10, pcg_calls=2, tcg_iters=5, fcalls=2),
+                stats=dict(pcg_iters=10, pcg_calls=2, tcg_iters=5, fcalls=2, ncg_iters=3),
                 timing=dict(vcycles=vcycles, ru_passes=ru, corrections=corr), elapsed=2.0)
 
 
@@ -31,7 +31,7 @@ def test_mixed_multigrid(bench):
         "pcg_iterations": 10 * 12 * 4 * 1000,
         "v_cycles": 9 * (108 * 1000 + 72 * (216 + 64)),
         "true_residuals": 4 * 28 * 1000 + 2 * 24 * 1000,
-        "truncated_cg": 5 * (40 * 4 * 343 + 2 * 216),
+        "truncated_cg": 5 * (34 * 4 * 343 + 2 * 216) + 3 * 3 * 4 * 343,
         "evaluations": 2 * (216 * 343 + 4 * 216),
     }
     total = sum(cls.values())
@@ -46,7 +46,8 @@ def test_mixed_jacobi_and_fp64(bench):
     assert "v_cycles" not in w["by_class_MB_per_step"] and w["counts_per_step"]["vcycles"] is None
     f = bench.whole_step(_res([], vcycles=-1), "fp64", steps=1)
     cls = {"pcg_iterations": 10 * 12 * 8 * 1000, "pcg_starts": 2 * 56 * 1000,
-           "truncated_cg": 5 * (40 * 8 * 343 + 2 * 216), "evaluations": 2 * (216 * 343 + 4 * 216)}
+           "truncated_cg": 5 * (34 * 8 * 343 + 2 * 216) + 3 * 3 * 8 * 343,
+           "evaluations": 2 * (216 * 343 + 4 * 216)}
     assert f["bytes_per_step"] == sum(cls.values())
     r = _res([], vcycles=-1)
     r["info"]["structured"] = 0
