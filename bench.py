@@ -135,6 +135,8 @@ def gpu_run(args, precision, traffic_probe=False, learned=None, multigrid=0):
     if multigrid:
         model.multigrid(multigrid)
         levels = model.mg_levels()
+    if multigrid and not args.no_secant:
+        model.secant_start(True)
     tag = precision + (f" + multigrid ({len(levels)} levels)" if multigrid else "") + (" + learned" if learned else "")
     if learned:
         from mammos_mumag_amd.learned import Network
@@ -205,6 +207,8 @@ def loop_run(args, precision, multigrid=0, learned=None, perturb=0.0):
     model = Model.from_structured(g, precision=precision, eval_precision=args.eval_precision)
     if multigrid:
         model.multigrid(multigrid)
+        if not args.no_secant:
+            model.secant_start(True)
     tag = precision + (" + multigrid" if multigrid else "") + (" + learned" if learned else "")
     if precision == "mixed":
         tag += f" (evaluation {args.eval_precision})"
@@ -655,6 +659,8 @@ def main():
     ap.add_argument("--seed", type=int, default=1000)
     ap.add_argument("--general", action="store_true", help="unstructured kernels (SELL K, element loops)")
     ap.add_argument("--inner-tol", type=float, default=0.0)
+    ap.add_argument("--no-secant", action="store_true",
+                    help="chosen form without the secant start of the line search's later potential solves")
     ap.add_argument("--eval-precision", default="fp64", choices=["fp64", "mixed"],
                     help="mixed form: arithmetic of the energy evaluation's C m, D m and G u "
                          "(the inner PCG and truncated CG are FP32 either way)")
@@ -789,7 +795,9 @@ def main():
                    "precision": args.precision,
                    "form": (("learned warm start + " if learned else "")
                             + ("FP32 inner solve" + (f" (multigrid preconditioner, levels {res['mg_levels']})" if mgl else "")
-                               + " + FP64 outer" if args.precision == "mixed" else "FP64")),
+                               + " + FP64 outer" if args.precision == "mixed" else "FP64")
+                            + (" + secant start of the line search's later solves" if mgl and not args.no_secant
+                               else "")),
                    "learned": (None if not learned else dict(
                        network=os.path.relpath(learned, ROOT), policy=args.ws_policy,
                        solves=res["warm_start"]["solves"], warm_started_solves=res["warm_start"]["calls"],

@@ -263,6 +263,16 @@ int mm_warm_start_policy(mm_ctx *ctx, int32_t policy);
 int mm_warm_start_report(mm_ctx *ctx, int64_t *solves, int64_t *warm_solves, double *mean_s_plain,
                          double *mean_s_warm);
 
+/* Secant start of the line search's potential solves (no counterpart in the reference,
+ * whose solves start from the previous solution, solvers.py:222-231): in mm_minimize, every
+ * evaluation of a line search after its first starts from the previous solution moved by
+ * the Galerkin step along the change of the solution since the line search's first
+ * evaluation, gamma = d.(b - K u) / d.K d (d = u - u_first, on the device, no host sync).
+ * The PCG ends at the unchanged tolerance, so the answer is the plain start's to within
+ * it; only the iteration counts differ.  on = 0 (the default) restores the reference's
+ * start.  Allocates N doubles on the first enable. */
+int mm_secant_start(mm_ctx *ctx, int32_t on);
+
 /* ---- Learned preconditioner (no counterpart in the reference, whose potential solve is
  * Jacobi-preconditioned, solvers.py:222-231): a second network of the same architecture
  * (mm_mlp_create) maps features of the residual r over a node's neighbourhood to the error

@@ -125,6 +125,11 @@ struct mm_ctx {
     bool ws_have_prev = false;
     double *ws_red = nullptr;        // [0..2] mbar sums, [4] num, [5] unused, [6] den, [7] gamma
     int64_t ws_calls = 0;
+    // secant start of the line search's later potential solves (mm_secant_start): u at the
+    // line search's first evaluation; [0] num, [2] den, [3] gamma of the Galerkin step
+    bool sec_on = false, sec_next = false;
+    double *u_sec = nullptr, *sec_red = nullptr;
+    int64_t sec_calls = 0;
     // MM_WS_MEASURED: host-timed solves with and without the warm start (running means)
     int ws_policy = 0;
     int64_t ws_solves = 0, ws_n[2] = {0, 0};
@@ -611,7 +616,31 @@ void ws_record(mm_ctx *c, bool ws, double seconds) {
 
 // u = K^-1 b for b = D m (already in c->b), from the caller's u, with the learned warm
 // start when a network is enabled and the policy picks it.
+// Secant start (mm_secant_start): for b = D m of a later evaluation of the line search, u
+// (the previous evaluation's solution) moves by the Galerkin step along d = u - u_sec, the
+// change of the solution since the line search's first evaluation: gamma = d.(b - Ku) /
+// d.Kd (device-side, no host sync; d is zero on the Dirichlet nodes).  The PCG then ends
+// at the unchanged tolerance.
+void secant_start(mm_ctx *c, const double *b, double *u) {
+    Launch L = c->L();
+    const int64_t n = c->N;
+    double *d = c->z, *Ku = c->q, *Kd = c->r;   // PCG workspace, free before the solve
+    launch_axpby(L, n, 1.0, u, -1.0, c->u_sec, d);
+    op_K(c, u, Ku, nullptr, nullptr);
+    op_K(c, d, Kd, nullptr, nullptr);
+    launch_reduce(L, RED_YZD, n, Ku, b, d, d, 0, 0, 0, c->sec_red);   // (b - Ku).d
+    launch_reduce(L, RED_DOT, n, d, Kd, nullptr, nullptr, 0, 0, 0, c->sec_red + 2);
+    launch_mlp_galerkin(L, n, c->sec_red, c->sec_red + 2, d, u, c->sec_red + 3);
+    c->sec_calls++;
+}
+
 int potential_solve(mm_ctx *c, const double *m, double *u, double tol, int *it) {
+    if (c->sec_next) {
+        c->sec_next = false;
+        c->ws_have_prev = false;
+        secant_start(c, c->b, u);
+        return pcg(c, c->b, u, tol, 0.0, 10000, it);
+    }
     const bool ws = c->ws_net && ws_choose(c);
     const auto t0 = std::chrono::steady_clock::now();
     if (ws) learned_warm_start(c, m, c->b, u);
@@ -1317,6 +1346,7 @@ int mm_minimize(mm_ctx *c, double *m_inout, double hext, const double hdir[3],
     mm_stats loc{};
     int rc = 0;
     double *X = c->X, *X0 = c->X0, *X1 = c->X1, *G = c->Gp, *G1 = c->G1, *D = c->Dd, *Z1 = c->Z1;
+    c->sec_next = false;
     HIPC(hipMemcpyAsync(X, m_inout, bytes3, hipMemcpyDeviceToDevice, c->s));
     if (c->structured) launch_outside_max(L, n, c->G, X, &c->dsc->red[0]);
     if ((rc = sync_scalars(c))) return rc;
@@ -1410,6 +1440,9 @@ int mm_minimize(mm_ctx *c, double *m_inout, double hext, const double hdir[3],
             const double t1 = 2.0 * sqrt_eps * (1.0 + std::sqrt(h3[0])) / std::sqrt(h3[1]);
             const double t2 = sqrt_eps / h3[2];
             const double h = t1 < t2 ? t1 : t2;
+            // the secant start's reference solution: u at x, before the first evaluation
+            const bool sec = c->sec_on && P->hmag_on > 0;
+            if (sec) HIPC(hipMemcpyAsync(c->u_sec, u, (size_t)n * sizeof(double), hipMemcpyDeviceToDevice, c->s));
             launch_update_m(L, n, X, h, D, X1, blk(X1));
             double gd2[2];
             const PostReduce pgd2{RED_GD2, G, G1, D, nullptr, 2, gd2};
@@ -1426,6 +1459,7 @@ int mm_minimize(mm_ctx *c, double *m_inout, double hext, const double hdir[3],
             double gd1;
             const PostReduce pdot{RED_DOT, G1, D, nullptr, nullptr, 1, &gd1};
             launch_update_m(L, n, X, alpha, D, X1, blk(X1));
+            c->sec_next = sec;
             if ((rc = feval(X1, &f1, G1, &pdot))) return rc;
             int i = 0;
             for (;;) {
@@ -1435,6 +1469,7 @@ int mm_minimize(mm_ctx *c, double *m_inout, double hext, const double hdir[3],
                 if (done) break;
                 alpha *= rho;
                 launch_update_m(L, n, X, alpha, D, X1, blk(X1));
+                c->sec_next = sec;
                 if ((rc = feval(X1, &f1, G1, &pdot))) return rc;
                 ++i;
                 loc.backtracks++;
@@ -1951,6 +1986,17 @@ int mm_lp_pcg(mm_ctx *c, mm_mlp *net, double scale, const double *b, double *x, 
     if (iters_host) *iters_host = it;
     if (lp_apps_host) *lp_apps_host = apps;
     return rr > bound ? MM_W_MAXITER : 0;
+}
+
+int mm_secant_start(mm_ctx *c, int32_t on) {
+    if (!c) return fail(MM_E_ARG, "null context");
+    if (on && !c->u_sec) {
+        int rc;
+        if ((rc = c->alloc(&c->u_sec, (size_t)c->N)) || (rc = c->alloc(&c->sec_red, 8))) return rc;
+    }
+    c->sec_on = on != 0;
+    c->sec_next = false;
+    return 0;
 }
 
 int mm_warm_start_policy(mm_ctx *c, int32_t policy) {

@@ -122,6 +122,7 @@ SIGNATURES = {
     "mm_mg_levels": [VP, VP, VP],
     "mm_warm_start_stats": [VP, ctypes.POINTER(ctypes.c_int64), c_double_p],
     "mm_warm_start_policy": [VP, ctypes.c_int32],
+    "mm_secant_start": [VP, ctypes.c_int32],
     "mm_warm_start_report": [VP, VP, VP, VP, VP],
     "mm_lp_setup": [VP],
     "mm_lp_features": [VP, VP, VP],

@@ -404,6 +404,12 @@ class Model:
                    "mm_mg_apply")
         return z
 
+    def secant_start(self, on=True):
+        """Start every line-search potential solve after the first from the Galerkin step
+        along the change of the solution since that first one (``mm_secant_start``); the
+        answer is the plain start's to within the solve tolerance."""
+        _lib.check(self._lib.mm_secant_start(self._ctx, 1 if on else 0), "mm_secant_start")
+
     def warm_start(self, net=None):
         """Enable the learned warm start of every potential solve with ``net``
         (``learned.Network``), or disable it (None)."""

@@ -207,6 +207,43 @@ def test_mixed_multigrid_loop():
     assert out["mixed+mg"][1]["pcg_iters"] < out["mixed"][1]["pcg_iters"] / 10
 
 
+def test_secant_start_keeps_the_answer():
+    """mm_secant_start: the line search's later potential solves start from the Galerkin
+    step along the solution's change since its first one.  Plain FP64 (Jacobi) and the
+    mixed form with multigrid, each with and without it: the loop's energies and M.h agree
+    to within the solve tolerance's effect, with fewer PCG iterations."""
+    from mammos_mumag_amd import Model, _lib
+    g = generate.build(edge=39.0, n=39, layers=13, grains=2, cone=15.0, seed=3)
+    p = generate.loop_params("cube40_single")
+    out = {}
+    for form in ("fp64", "fp64+secant", "mixed+mg", "mixed+mg+secant"):
+        print(f"[gpu] 66^3 loop, {form}", flush=True)
+        model = Model.from_structured(g, precision="fp64" if form.startswith("fp64") else "mixed")
+        if "mg" in form:
+            model.multigrid()
+        if form.endswith("secant"):
+            model.secant_start(True)
+        m = model.to_device(model.uniform_m(p["m"]))
+        m_ref, u, gF = m.clone(), model.empty1(), model.empty3()
+        st = _lib.Stats()
+        hext, rec = p["hstart"], []
+        for _ in range(3):
+            E = model.minimize(m, hext, p["h"], p, m_ref, u, gF, st)
+            rec.append((E, model.compute_mh(m, p["h"])))
+            hext -= 0.4
+        out[form] = (np.array(rec), st.as_dict())
+        model.close()
+    for base in ("fp64", "mixed+mg"):
+        r0, s0 = out[base]
+        r1, s1 = out[base + "+secant"]
+        dmh = np.max(np.abs(r1[:, 1] - r0[:, 1]))
+        dE = np.max(np.abs(r1[:, 0] / r0[:, 0] - 1))
+        print(f"[gpu] {base}: secant |dM.h| {dmh:.2e} |dE/E| {dE:.2e} PCG {s0['pcg_iters']} -> {s1['pcg_iters']} "
+              f"NCG {s0['ncg_iters']} -> {s1['ncg_iters']}", flush=True)
+        assert dmh < 1e-9 and dE < 1e-11, base
+        assert s1["pcg_iters"] < s0["pcg_iters"], base
+
+
 def test_mixed_multigrid_pcg_limits():
     """mm_pcg in the mixed form with the multigrid preconditioner (its segment target and
     iteration cap are tested on the device between KB and the V-cycle): from zero the true
