@@ -754,14 +754,7 @@ int reduce(mm_ctx *c, RedOp op, int64_t n, const double *a, const double *b, con
 void launch_reduce3(mm_ctx *c, RedOp op, const double *a, const double *b, const double *cc, const double *d) {
     launch_reduce(c->L(), op, 3 * c->N, a, b, cc, d, 0, 0, 0, c->dsc->red, c->ncg_block ? &c->G : nullptr, 3);
 }
-int reduce3(mm_ctx *c, RedOp op, const double *a, const double *b, const double *cc, const double *d,
-            double *out, int nout) {
-    launch_reduce3(c, op, a, b, cc, d);
-    int rc = sync_scalars(c);
-    if (rc) return rc;
-    for (int k = 0; k < nout; ++k) out[k] = c->hsc->red[k];
-    return 0;
-}
+
 
 std::vector<MatCoef> material_table(const mm_mesh_desc *d, double V) {
     const double mu0 = 4.0e-7 * M_PI, size = d->length_scale > 0 ? d->length_scale : 1e-9;
