@@ -209,7 +209,8 @@ def loop_run(args, precision, multigrid=0, learned=None, perturb=0.0):
         model.multigrid(multigrid)
         if not args.no_secant:
             model.secant_start(True)
-    tag = precision + (" + multigrid" if multigrid else "") + (" + learned" if learned else "")
+    tag = (precision + (" + multigrid" if multigrid else "") + (" + secant" if multigrid and not args.no_secant else "")
+           + (" + learned" if learned else ""))
     if precision == "mixed":
         tag += f" (evaluation {args.eval_precision})"
     if learned:
@@ -352,6 +353,8 @@ def avalanche_main(args, learned):
     torch.cuda.empty_cache()
     mix = Model.from_structured(g, precision="mixed", eval_precision=args.eval_precision)
     mix.multigrid(args.multigrid)
+    if not args.no_secant:
+        mix.secant_start(True)
     if learned:
         from mammos_mumag_amd.learned import Network
         net = Network(mix, max_batch=32, seed=0)
@@ -359,8 +362,8 @@ def avalanche_main(args, learned):
         mix.warm_start(net)
         mix.warm_start_policy(args.ws_policy)
     mm, uu, gg = (x.clone() for x in saved)
-    one(mix, "chosen form (FP32 inner solve, multigrid" + (", learned warm start" if learned else "") + ")",
-        mm, uu, gg)
+    one(mix, "chosen form (FP32 inner solve, multigrid" + (", secant start" if not args.no_secant else "")
+        + (", learned warm start" if learned else "") + ")", mm, uu, gg)
     mix.close()
     print(json.dumps(out), flush=True)
     return 0
@@ -386,6 +389,8 @@ def reversal_check_main(args):
     t_set = time.perf_counter()
     mix = Model.from_structured(g, precision="mixed", eval_precision=args.eval_precision)
     mix.multigrid(args.multigrid)
+    if not args.no_secant:
+        mix.secant_start(True)
     log(f"[reversal] {args.config}: {mix.info} multigrid {mix.mg_levels()} ({time.perf_counter() - t_set:.1f} s)")
     m = mix.to_device(mix.uniform_m(p["m"]))
     m_ref, u, gF, st = m.clone(), mix.empty1(), mix.empty3(), _lib.Stats()
