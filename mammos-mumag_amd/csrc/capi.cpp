@@ -1389,10 +1389,8 @@ int mm_minimize(mm_ctx *c, double *m_inout, double hext, const double hdir[3],
     if ((rc = minv(G, Z1))) return rc;
     launch_axpby(L, n3, -1.0, Z1, 0.0, nullptr, D, blk(D));  // d = -M_inv(g)
     int64_t k = 1;
-    HIPC(hipMemcpyAsync(X0, X, bytes3, hipMemcpyDeviceToDevice, c->s));
-    off_zero(X);    // copies of m in full
-    off_zero(X0);
-    scan(X0, X, G, G, false);
+    off_zero(X);    // the copy of m, in full
+    scan(X, X, G, G, false);   // x0 = x: the first stop test's max|x0 - x| is 0
     bool after_update = false;   // the scan's d is an HS+ update (its restart test pending)
     const double sqrt_tol = std::sqrt(tol), cbrt_tol = std::cbrt(tol);
     const double sqrt_eps = std::sqrt(DBL_EPSILON);

@@ -133,6 +133,46 @@ def test_mixed_precision_errors(gpu_mixed, oracle):
         assert abs(rg[3] / ro[3] - 1) < 1e-6
 
 
+@pytest.fixture(scope="module")
+def near_equilibrium(oracle):
+    """The oracle's minimised state of the first field step (+1.5 T): its projected
+    gradient is ~4e-9, so the truncated CG's stop test (r <= min(0.5, sqrt|g|) |g|) lets
+    it run 14 iterations."""
+    from oracle.energies import total_eg
+    from oracle.loop import Minimizer
+    O = oracle
+    p = generate.loop_params("cube40_single")
+    mini = Minimizer(O, p, O.uniform_m(p["m"]))
+    _, x, _ = mini.minimize(O.uniform_m(p["m"]), 1.5)
+    _, gp, _, gf = total_eg(O, x, 1e-10, 1.5, p["h"], 1, np.zeros(O.N))
+    return gp, gf
+
+
+@pytest.mark.parametrize("max_iter", [3, 10, 17])
+def test_truncated_cg_iteration_caps(gpu_s, gpu_mixed, oracle, near_equilibrium, max_iter):
+    """M_inv on the magnet block (k_tcgc_*) with iteration caps below, at and above the
+    direction ring's initial 11 slots (17 grows it and, with 14 iterations, uses slots past
+    the initial ones), against the oracle's truncated CG: the FP64 structured context with
+    the oracle's iteration count to 1e-12, the FP32 form within one iteration to 1e-6."""
+    from oracle.solvers import truncated_cg
+    from oracle.loop import Minimizer
+    O = oracle
+    gp_ref, gf_ref = near_equilibrium
+    p = dict(generate.loop_params("cube40_single"), precond_iter=max_iter)
+    m0 = O.uniform_m(p["m"])
+    mini = Minimizer(O, p, m0)
+    mini.gF = gf_ref
+    z_ref, it_ref = truncated_cg(mini.H, -gp_ref, O.Dex, max_iter=max_iter)
+    assert it_ref == min(max_iter, 14)
+    for model, tol, dit in ((gpu_s, 1e-12, 0), (gpu_mixed, 1e-6, 1)):
+        z, it = model.M_inv(model.to_device(m0), model.to_device(gf_ref), model.to_device(gp_ref), max_iter)
+        err = rel(z.cpu().numpy(), z_ref)
+        print(f"[gpu] M_inv cap {max_iter} ({model.precision}): iterations {it} (oracle {it_ref}), "
+              f"rel {err:.2e}", flush=True)
+        assert abs(it - it_ref) <= dit
+        assert err < tol
+
+
 def test_poisson_stencil_chunked_grid():
     """A 66^3-node grid, where the z-march leaves the last block chunks empty: the stencil
     must touch nothing outside the grid and cover every row once (regression)."""
