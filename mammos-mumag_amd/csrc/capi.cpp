@@ -79,10 +79,6 @@ struct mm_ctx {
     MatCoef *mats = nullptr;
     int n_mats = 0;
     DevScalars *dsc = nullptr;
-    // side stream of the energy evaluation (structured grids): C m runs there while the
-    // main stream forms D m and solves for u; ev_m orders it after m, ev_c joins it
-    hipStream_t s2 = nullptr;
-    hipEvent_t ev_m = nullptr, ev_c = nullptr;
     DevScalars *hsc = nullptr;  // pinned host mirror (written by the device, k_publish)
     unsigned long long *hseq = nullptr, seq = 0;   // the mirror's sequence number (same allocation)
     Reducer red;
@@ -179,7 +175,6 @@ struct mm_ctx {
         return 0;
     }
     ~mm_ctx() {
-        if (s2) (void)hipStreamSynchronize(s2);
         if (s) (void)hipStreamSynchronize(s);
         for (void *a : allocs) (void)hipFree(a);
         if (hsc) (void)hipHostFree(hsc);
@@ -187,9 +182,6 @@ struct mm_ctx {
         for (hipEvent_t e : t_upd.ev) (void)hipEventDestroy(e);
         for (hipEvent_t e : t_mg.ev) (void)hipEventDestroy(e);
         if (s) (void)hipStreamDestroy(s);
-        if (s2) (void)hipStreamDestroy(s2);
-        for (hipEvent_t e : {ev_m, ev_c})
-            if (e) (void)hipEventDestroy(e);
     }
 };
 
@@ -675,31 +667,13 @@ int total_eg(mm_ctx *c, const double *m, double hext, const double *hdir, double
              int hmag_on, double *u, double *E, double *gproj, double *gfull, int *pcg_it,
              bool off_zero = false, const PostReduce *post = nullptr) {
     Launch L = c->L();
-    // on a structured grid with the potential, C m (needed only by the finish) runs on the
-    // side stream, overlapping the latency-bound potential solve; D m stays on the main
-    // stream, so the solve starts after the lighter divergence pass
-    const bool side = c->structured && hmag_on > 0 && c->s2;
-    if (side) {
-        HIPC(hipEventRecord(c->ev_m, c->s));   // m written; the previous finish has read gC
-        HIPC(hipStreamWaitEvent(c->s2, c->ev_m, 0));
-        Launch L2 = L;
-        L2.s = c->s2;
-        launch_grid_exani(L2, c->G, c->eval32, m, c->gC, nullptr, nullptr);
-        HIPC(hipEventRecord(c->ev_c, c->s2));
-        launch_grid_exani(L, c->G, c->eval32, m, nullptr, c->b, nullptr);
-    } else {
-        op_C(c, m, c->gC, hmag_on > 0 ? c->b : nullptr, nullptr);
-    }
+    op_C(c, m, c->gC, hmag_on > 0 ? c->b : nullptr, nullptr);
     int it = 0, rc = 0;
     if (hmag_on > 0) {
         rc = potential_solve(c, m, u, tol_u, &it);
-        if (rc < 0) {
-            if (side) (void)hipStreamSynchronize(c->s2);
-            return rc;
-        }
+        if (rc < 0) return rc;
         op_grad(c, u, c->gd);
     }
-    if (side) HIPC(hipStreamWaitEvent(c->s, c->ev_c, 0));
     const double zc0 = -hext * hdir[0] / c->V, zc1 = -hext * hdir[1] / c->V,
                  zc2 = -hext * hdir[2] / c->V;
     launch_finish_eg(L, c->N, c->structured ? &c->G : nullptr, m, c->meas, zc0, zc1, zc2, c->gC, c->gd, gfull,
@@ -922,10 +896,6 @@ int mm_ctx_create(const mm_mesh_desc *d, const mm_options *opts, mm_ctx **out) {
     };
     if (hipStreamCreateWithFlags(&c->s, hipStreamNonBlocking) != hipSuccess)
         return bail(fail(MM_E_HIP, "hipStreamCreate failed"));
-    if (structured && (hipStreamCreateWithFlags(&c->s2, hipStreamNonBlocking) != hipSuccess ||
-                       hipEventCreateWithFlags(&c->ev_m, hipEventDisableTiming) != hipSuccess ||
-                       hipEventCreateWithFlags(&c->ev_c, hipEventDisableTiming) != hipSuccess))
-        return bail(fail(MM_E_HIP, "side stream / events"));
 
     const std::vector<double> &meas = structured ? S.meas : H.meas;
     const std::vector<double> &Dex = structured ? S.Dex : H.Dex;

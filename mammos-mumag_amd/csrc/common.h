@@ -204,8 +204,7 @@ void launch_ebe_exani(const Launch &L, const ElemDev &E, const double *v, double
                       const int *skip_flag);
 void launch_ebe_grad(const Launch &L, const ElemDev &E, const double *u, double *g, double invV);
 
-// Structured magnet operators (node-centric, overwrite the magnet entries only).  y = C v
-// and/or b = D v: y == nullptr forms b alone, b == nullptr y alone.
+// Structured magnet operators (node-centric, overwrite the magnet entries only).
 void launch_grid_exani(const Launch &L, const GridDev &G, bool fp32, const double *v, double *y,
                        double *b, const int *skip_flag);
 void launch_grid_grad(const Launch &L, const GridDev &G, bool fp32, const double *u, double *g);

@@ -697,9 +697,8 @@ struct VoxCoef {
 // (C v)_node from the values nv[(dx+1) + 3 (dy+1) + 9 (dz+1)][a] of v at the node's 27
 // neighbours (clamped to the block: clamped entries only meet zero-material voxels) and
 // vox(w), the coefficients of voxel (i-1+vx, j-1+vy, k-1+vz), w = vx + 2 vy + 4 vz; with
-// DIV also the node's divergence term sum Js (B v); without CM the divergence only (y is
-// not written).  Unscaled: y = fS ex + fM an.
-template <typename TC, bool DIV, bool CM = true, typename Vox>
+// DIV also the node's divergence term sum Js (B v).  Unscaled: y = fS ex + fM an.
+template <typename TC, bool DIV, typename Vox>
 __device__ __forceinline__ void grid_c_core(const GridDev &G, const TC (&nv)[27][3], Vox vox, TC (&y)[3],
                                             TC &bdiv) {
     TC ex[3] = {0, 0, 0}, an[3] = {0, 0, 0}, bd = 0;
@@ -715,11 +714,11 @@ __device__ __forceinline__ void grid_c_core(const GridDev &G, const TC (&nv)[27]
             const int cx = c & 1, cy = (c >> 1) & 1, cz = (c >> 2) & 1;
             const int s = (cx - (1 - vx) + 1) + 3 * (cy - (1 - vy) + 1) + 9 * (cz - (1 - vz) + 1);
             const int KS = kKS[p][c], KM = kKM[p][c];
-            if (CM && KS != 0) {
+            if (KS != 0) {
 #pragma unroll
                 for (int a = 0; a < 3; ++a) sS[a] += (TC)KS * nv[s][a];
             }
-            if (CM && KM != 0) sM += (TC)KM * (u0 * nv[s][0] + u1 * nv[s][1] + u2 * nv[s][2]);
+            if (KM != 0) sM += (TC)KM * (u0 * nv[s][0] + u1 * nv[s][1] + u2 * nv[s][2]);
             if (DIV) {
                 const int B0 = kKB[p][c][0], B1 = kKB[p][c][1], B2 = kKB[p][c][2];
                 if (B0 != 0) sB += (TC)B0 * nv[s][0];
@@ -727,20 +726,16 @@ __device__ __forceinline__ void grid_c_core(const GridDev &G, const TC (&nv)[27]
                 if (B2 != 0) sB += (TC)B2 * nv[s][2];
             }
         }
-        if (CM) {
 #pragma unroll
-            for (int a = 0; a < 3; ++a) ex[a] += mc.cex * sS[a];
-            an[0] += mc.cK * sM * u0;
-            an[1] += mc.cK * sM * u1;
-            an[2] += mc.cK * sM * u2;
-        }
+        for (int a = 0; a < 3; ++a) ex[a] += mc.cex * sS[a];
+        an[0] += mc.cK * sM * u0;
+        an[1] += mc.cK * sM * u1;
+        an[2] += mc.cK * sM * u2;
         if (DIV) bd += mc.Js * sB;
     }
     const TC fS = (TC)G.fS, fM = (TC)G.fM;
-    if (CM) {
 #pragma unroll
-        for (int a = 0; a < 3; ++a) y[a] = fS * ex[a] + fM * an[a];
-    }
+    for (int a = 0; a < 3; ++a) y[a] = fS * ex[a] + fM * an[a];
     if (DIV) bdiv = (TC)G.fB * bd;
 }
 
@@ -785,15 +780,13 @@ bool tcg_tiles_fit(int mn) {   // the truncated CG's d'.H d' writes one partial 
     const TcgTiles tl = block_tiles(G);
     return (int64_t)tl.gx * tl.gy * tl.gz <= kMaxPartials;
 }
-// MODE 0: y = C v; 1: y = C v and b = D v; 2: b = D v only (the voxels' Js alone staged).
-template <typename TC, int MODE>
+template <typename TC, bool DIV>
 __global__ __launch_bounds__(kBlock) void k_grid_exani(GridDev G, TcgTiles tl, const MatCoef *__restrict__ mats,
                                                        const double *__restrict__ v,
                                                        double *__restrict__ y, double *__restrict__ b,
                                                        const int *skip) {
     static_assert(kCTX * kCTY == kBlock, "one node per thread and plane");
-    constexpr bool DIV = MODE >= 1, CM = MODE <= 1;
-    constexpr int NV = MODE == 0 ? 5 : MODE == 1 ? 6 : 1;
+    constexpr int NV = DIV ? 6 : 5;
     __shared__ TC nd[3][kCTY + 2][kCTX + 2][3];
     __shared__ TC vc[2][kCTY + 1][kCTX + 1][NV];
     if (skip && *skip) return;   // block-uniform
@@ -865,13 +858,11 @@ __global__ __launch_bounds__(kBlock) void k_grid_exani(GridDev G, TcgTiles tl, c
             const int ex = e % (kCTX + 1), ey = e / (kCTX + 1);
             const VoxCoef<TC> c = vox_coef<TC>(mats[r.mat[q]]);
             TC *s = vc[slot][ey][ex];
-            if (CM) {
-                s[0] = c.cex;
-                s[1] = c.cK;
-                s[2] = c.u0;
-                s[3] = c.u1;
-                s[4] = c.u2;
-            }
+            s[0] = c.cex;
+            s[1] = c.cK;
+            s[2] = c.u0;
+            s[3] = c.u1;
+            s[4] = c.u2;
             if (DIV) s[NV - 1] = c.Js;
         }
     };
@@ -904,16 +895,13 @@ __global__ __launch_bounds__(kBlock) void k_grid_exani(GridDev G, TcgTiles tl, c
             auto vox = [&](int w) {
                 const int vx = w & 1, vy = (w >> 1) & 1, sl = (k - 1 + ((w >> 2) & 1)) & 1;
                 const TC *q = vc[sl][ty + vy][tx + vx];
-                if (CM) return VoxCoef<TC>{q[0], q[1], DIV ? q[NV - 1] : (TC)0, q[2], q[3], q[4]};
-                return VoxCoef<TC>{(TC)0, (TC)0, q[NV - 1], (TC)0, (TC)0, (TC)0};
+                return VoxCoef<TC>{q[0], q[1], DIV ? q[NV - 1] : (TC)0, q[2], q[3], q[4]};
             };
             TC yv[3], bd = 0;
-            grid_c_core<TC, DIV, CM>(G, nv, vox, yv, bd);
+            grid_c_core<TC, DIV>(G, nv, vox, yv, bd);
             const int64_t g = gnode(G, i, j, k);
-            if (CM) {
 #pragma unroll
-                for (int a = 0; a < 3; ++a) y[3 * g + a] = (double)yv[a];
-            }
+            for (int a = 0; a < 3; ++a) y[3 * g + a] = (double)yv[a];
             if (DIV) b[g] = (double)bd;
         }
         __syncthreads();
@@ -1046,19 +1034,17 @@ void launch_grid_exani(const Launch &L, const GridDev &G, bool fp32, const doubl
                        double *b, const int *skip_flag) {
     const TcgTiles tl = block_tiles(G);
     const dim3 g(tl.gx * tl.gy * tl.gz), blk(kBlock);
-    const int mode = !b ? 0 : y ? 1 : 2;
-#define MM_EXANI(TC, MODE) \
-    hipLaunchKernelGGL((k_grid_exani<TC, MODE>), g, blk, 0, L.s, G, tl, L.mats, v, y, b, skip_flag)
     if (fp32) {
-        if (mode == 0) MM_EXANI(float, 0);
-        else if (mode == 1) MM_EXANI(float, 1);
-        else MM_EXANI(float, 2);
+        if (b)
+            hipLaunchKernelGGL((k_grid_exani<float, true>), g, blk, 0, L.s, G, tl, L.mats, v, y, b, skip_flag);
+        else
+            hipLaunchKernelGGL((k_grid_exani<float, false>), g, blk, 0, L.s, G, tl, L.mats, v, y, b, skip_flag);
     } else {
-        if (mode == 0) MM_EXANI(double, 0);
-        else if (mode == 1) MM_EXANI(double, 1);
-        else MM_EXANI(double, 2);
+        if (b)
+            hipLaunchKernelGGL((k_grid_exani<double, true>), g, blk, 0, L.s, G, tl, L.mats, v, y, b, skip_flag);
+        else
+            hipLaunchKernelGGL((k_grid_exani<double, false>), g, blk, 0, L.s, G, tl, L.mats, v, y, b, skip_flag);
     }
-#undef MM_EXANI
 }
 void launch_grid_grad(const Launch &L, const GridDev &G, bool fp32, const double *u, double *g) {
     const TcgTiles tl = block_tiles(G);
