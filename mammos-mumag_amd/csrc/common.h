@@ -23,6 +23,7 @@ struct DevScalars {
     double tcg_eps, tcg_rz, tcg_den, tcg_beta, tcg_pmax;
     int tcg_done, tcg_neg, tcg_iter, tcg_pad;
     long long tcg_total;
+    double tcg_rzp[2];   // r.z by iteration parity (magnet-block kernels: read and written in one launch)
     // true (FP64) residual of the mixed-precision reliable-update PCG
     double ir_rr, ir_bound, ir_rmax2, ru_gamma;
     int ir_done, ir_outer, ru_maxiter, ru_pad;

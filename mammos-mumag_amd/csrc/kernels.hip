@@ -118,6 +118,7 @@ __global__ __launch_bounds__(kFinThreads) void k_fin(const double *__restrict__ 
             const double nrm = sqrt(tot[0]);
             dsc->tcg_eps = fmin(0.5, sqrt(nrm) * nrm);
             dsc->tcg_rz = tot[1];
+            dsc->tcg_rzp[0] = tot[1];
             dsc->tcg_done = 0;
             dsc->tcg_neg = 0;
             dsc->tcg_iter = 0;
@@ -1354,11 +1355,35 @@ __global__ __launch_bounds__(kBlock) void k_tcgc_dh(GridDev G, TcgTiles tl, cons
                                                     const T *__restrict__ z, const T *__restrict__ din,
                                                     T *__restrict__ dout, const T *__restrict__ m0c,
                                                     const T *__restrict__ mgc, T *__restrict__ Hd,
-                                                    const DevScalars *dsc, double *partials) {
+                                                    DevScalars *dsc, int it, const double *__restrict__ uparts,
+                                                    int nuparts, double *partials) {
     static_assert(kCTX * kCTY == kBlock, "one node per thread and plane");
+    static_assert(kFinThreads == kBlock, "the finalize order needs the finalize's block width");
     __shared__ T nd[3][kCTY + 2][kCTX + 2][3];
     __shared__ T vc[2][kCTY + 1][kCTX + 1][5];
+    __shared__ double shf[kBlock / kWave];
     if (dsc->tcg_done) return;   // block-uniform
+    // The previous iteration's finalize (FIN_TCG_UPDATE), formed in every block from the
+    // update's partials (r'.r', r'.z') in k_fin's order: stop test and beta; block 0
+    // publishes them, writing r.z into the other parity slot than the one read here.
+    double beta_d = 0.0;
+    if (!FIRST) {
+        const double rr = fin_order_sum(uparts, nuparts, shf);
+        __syncthreads();   // shf is reused by the second sum
+        const double rz = fin_order_sum(uparts + kMaxPartials, nuparts, shf);
+        const bool neg = dsc->tcg_den <= 0.0;
+        const bool conv = sqrt(rr) <= dsc->tcg_eps;
+        beta_d = (!neg && !conv) ? rz / (dsc->tcg_rzp[(it - 1) & 1] + 1e-16) : 0.0;
+        if (blockIdx.x == 0 && threadIdx.x == 0) {
+            if (!neg) dsc->tcg_rzp[it & 1] = rz;
+            dsc->tcg_beta = beta_d;
+            dsc->tcg_neg = neg;
+            dsc->tcg_done = neg || conv;
+            dsc->tcg_iter += 1;
+            dsc->tcg_total += 1;
+        }
+        if (neg || conv) return;   // block-uniform
+    }
     const int mn = G.mn, n1 = mn + 1;
     const int b = blockIdx.x;
     const int cz = b % tl.gz, bx = (b / tl.gz) % tl.gx, by = b / (tl.gz * tl.gx);
@@ -1366,7 +1391,7 @@ __global__ __launch_bounds__(kBlock) void k_tcgc_dh(GridDev G, TcgTiles tl, cons
     const int i0 = bx * kCTX, j0 = by * kCTY, i = i0 + tx, j = j0 + ty;
     const int chunk = (n1 + tl.gz - 1) / tl.gz;
     const int k0 = cz * chunk, k1 = min(n1, k0 + chunk);
-    const T beta = (T)dsc->tcg_beta;
+    const T beta = (T)beta_d;
     auto ring3 = [](int p) { return ((p % 3) + 3) % 3; };
     // A thread's ring entries: nodes e = t, t + kBlock < (kCTY + 2)(kCTX + 2), voxels
     // e = t, t + kBlock < (kCTY + 1)(kCTX + 1).  Plane k + 2's raw node values (d, and z
@@ -1505,7 +1530,7 @@ template <typename T>
 __global__ __launch_bounds__(kBlock) void k_tcgc_upd(int64_t n3, T *__restrict__ r, T *__restrict__ z,
                                                      const T *__restrict__ Hd, const T *__restrict__ Dc,
                                                      DevScalars *dsc, const double *__restrict__ dparts,
-                                                     int nden, double *__restrict__ alpha_out,
+                                                     int nden, double *__restrict__ alpha_out, int it,
                                                      double *partials) {
     static_assert(kFinThreads == kBlock, "the finalize order needs the finalize's block width");
     __shared__ double sh[kBlock / kWave];
@@ -1513,7 +1538,7 @@ __global__ __launch_bounds__(kBlock) void k_tcgc_upd(int64_t n3, T *__restrict__
     const double den = fin_order_sum(dparts, nden, sh);
     if (done) return;
     const bool neg = den <= 0.0;
-    const double alpha = neg ? 0.0 : dsc->tcg_rz / den;
+    const double alpha = neg ? 0.0 : dsc->tcg_rzp[it & 1] / den;
     if (blockIdx.x == 0 && threadIdx.x == 0) {
         dsc->tcg_den = den;
         *alpha_out = alpha;
@@ -1602,7 +1627,9 @@ template <typename T>
 void tcgc_run(const Launch &L, const GridDev &G, int64_t n, const TcgBlock<T> &B, const double *m0,
               const double *gF, const double *g, const double *D, int max_iter, double *zout, bool write_off) {
     const int64_t n1 = G.mn + 1, nb = n1 * n1 * n1, n3 = 3 * nb;
-    const int gi = grid_for(nb, 2), gu = grid_for(n3, 4);
+    // the update's partials are summed by every block of the next stencil launch: at most
+    // 1024 of them
+    const int gi = grid_for(nb, 2), gu = std::min(grid_for(n3, 4), 1024);
     const TcgTiles tl = block_tiles(G);
     const int gd = tl.gx * tl.gy * tl.gz;
     hipLaunchKernelGGL(k_tcgc_init<T>, dim3(gi), dim3(kBlock), 0, L.s, G, g, gF, m0, D, B.m0, B.mg, B.D, B.r, B.z,
@@ -1617,14 +1644,18 @@ void tcgc_run(const Launch &L, const GridDev &G, int64_t n, const TcgBlock<T> &B
         T *dout = B.dring + (int64_t)(it + 1) * n3;
         // d'.H d' partials in slot 3, read by the update kernel while it writes slots 0, 1
         double *const dparts = L.red.partials + 3 * kMaxPartials;
+        // from the second iteration on, the stencil launch also finalizes the previous
+        // update (the update's partials in slots 0, 1)
         if (it == 0)
             hipLaunchKernelGGL((k_tcgc_dh<T, true>), dim3(gd), dim3(kBlock), 0, L.s, G, tl, L.mats, B.z, din, dout,
-                               B.m0, B.mg, B.Hd, L.dsc, dparts);
+                               B.m0, B.mg, B.Hd, L.dsc, it, L.red.partials, gu, dparts);
         else
             hipLaunchKernelGGL((k_tcgc_dh<T, false>), dim3(gd), dim3(kBlock), 0, L.s, G, tl, L.mats, B.z, din, dout,
-                               B.m0, B.mg, B.Hd, L.dsc, dparts);
+                               B.m0, B.mg, B.Hd, L.dsc, it, L.red.partials, gu, dparts);
         hipLaunchKernelGGL(k_tcgc_upd<T>, dim3(gu), dim3(kBlock), 0, L.s, n3, B.r, B.z, B.Hd, B.D, L.dsc, dparts, gd,
-                           B.alpha + it, L.red.partials);
+                           B.alpha + it, it, L.red.partials);
+    }
+    if (max_iter > 0) {   // the last update's finalize (its r.z is not needed)
         FinArgs a{};
         a.op = FIN_TCG_UPDATE;
         a.nv = 2;
