@@ -1663,9 +1663,10 @@ void tcgc_run(const Launch &L, const GridDev &G, int64_t n, const TcgBlock<T> &B
         a.skip = &L.dsc->tcg_done;
         launch_fin(L, a);
     }
-    hipLaunchKernelGGL(k_tcgc_accum<T>, dim3(gu), dim3(kBlock), 0, L.s, n3, B.dring, B.alpha, max_iter, L.dsc,
+    const int ga = grid_for(n3, 4);
+    hipLaunchKernelGGL(k_tcgc_accum<T>, dim3(ga), dim3(kBlock), 0, L.s, n3, B.dring, B.alpha, max_iter, L.dsc,
                        B.p, L.red.partials);
-    launch_fin(L, FinArgs::store(&L.dsc->tcg_pmax, 1, 1u, gu, nullptr));
+    launch_fin(L, FinArgs::store(&L.dsc->tcg_pmax, 1, 1u, ga, nullptr));
     if (write_off)
         hipLaunchKernelGGL(k_tcgc_final<T>, dim3(grid_for(n, 2)), dim3(kBlock), 0, L.s, G, n, B.p, g, zout, L.dsc);
     else
