@@ -209,6 +209,23 @@ int mm_mg_levels(mm_ctx *ctx, int32_t *levels, int32_t *dims);
 /* z = M r with the multigrid V-cycle (device f32[N]; r zero on the Dirichlet faces). */
 int mm_mg_apply(mm_ctx *ctx, const float *r, float *z);
 
+/* Algebraic multigrid for the FP32 inner solve of a general (unstructured) MM_PREC_MIXED
+ * context (no counterpart in the reference, whose potential solve is Jacobi-preconditioned,
+ * solvers.py:222-231): smoothed aggregation on K (every stored coupling between free
+ * nodes, three-pass aggregation, one damped-Jacobi smoothing of the piecewise-constant
+ * prolongation, Galerkin coarse operators) down to at most 500 unknowns, solved by their
+ * dense inverse; degree-2 Chebyshev smoothing; up to max_levels levels (2..10; 0 restores
+ * Jacobi).  Set up on the host once per context (a later call with the same bound
+ * re-enables it).  The stop test on the FP64 true residual is unchanged, so only the
+ * iteration count changes. */
+#define MM_AMG_MAX_LEVELS 10
+int mm_amg(mm_ctx *ctx, int32_t max_levels);
+/* Levels of the AMG (0: none) and, if sizes is not null, the unknowns of each level
+ * (sizes holds MM_AMG_MAX_LEVELS). */
+int mm_amg_levels(mm_ctx *ctx, int32_t *levels, int64_t *sizes);
+/* z = M r with the AMG V-cycle (device f32[N]; r zero on the Dirichlet nodes). */
+int mm_amg_apply(mm_ctx *ctx, const float *r, float *z);
+
 /* ---- Learned warm start (no counterpart in the reference: the learned part of the
  * chosen form, for callers that run many potential solves on one structured grid).
  *

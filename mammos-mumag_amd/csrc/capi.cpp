@@ -16,6 +16,7 @@
 #include <vector>
 
 #include "../../include/mumag_hip.h"
+#include "amg.h"
 #include "assemble.h"
 #include "common.h"
 #include "mg.h"
@@ -104,6 +105,13 @@ struct mm_ctx {
 
     // multigrid preconditioner of the FP32 inner solve (structured grids)
     std::vector<double> axis[3];        // node coordinates per axis (host)
+    // smoothed-aggregation AMG of the general meshes' inner solve (mm_amg): K on the host
+    // (general contexts), the levels (level 0's matrix is K32), the coarsest inverse, the
+    // first smoothing step's weights of level 0 and its residual vector
+    CsrHost hK;
+    int amg_nl = 0, amg_built = 0, amg_built_max = 0, amg_nc = 0;
+    AmgLevelDev amg[kAmgMaxLevels];
+    float *amg_inv = nullptr, *amg_invw0 = nullptr, *amg_r0 = nullptr;
     int mg_nl = 0;                      // levels (0: Jacobi)
     int mg_batch = 8;                   // most inner iterations between host checks with multigrid
     double mg_rate2 = 0.05;             // running r.r reduction per multigrid-PCG iteration
@@ -434,6 +442,50 @@ float *mg_vcycle(mm_ctx *c, const float *r, float *z, float *alt, const int *ski
     return cur[0];
 }
 
+// AMG V-cycle M r on the FP32 vectors of the inner solve (amg.hip), the CPU restatement's
+// order (oracle/amg.py): on each level but the last, the first smoothing step from zero
+// (on level 0 formed by the kernel that produced r when first_done), the other steps, the
+// residual restricted with the next level's first step fused, the coarse correction
+// added, the steps in reverse order; the last level by its dense inverse.  Level 0
+// ping-pongs between z and alt; the returned buffer holds M r.
+float *amg_vcycle(mm_ctx *c, const float *r, float *z, float *alt, const int *skip, bool first_done,
+                  double *parts = nullptr, int *nparts = nullptr) {
+    Launch L = c->L();
+    const int nl = c->amg_nl;
+    float *cur[kAmgMaxLevels], *oth[kAmgMaxLevels];
+    const float *B[kAmgMaxLevels];
+    for (int l = 0; l < nl; ++l) {
+        B[l] = l == 0 ? r : c->amg[l].b;
+        cur[l] = l == 0 ? z : c->amg[l].xa;
+        oth[l] = l == 0 ? alt : c->amg[l].xb;
+    }
+    const AmgLevelDev *A = c->amg;
+    if (!first_done) launch_amg_smooth(L, A[0].A, A[0].dinv, r, nullptr, z, A[0].om[0], skip);
+    for (int l = 0; l + 1 < nl; ++l) {
+        for (int k = 1; k < A[l].deg; ++k) {
+            launch_amg_smooth(L, A[l].A, A[l].dinv, B[l], cur[l], oth[l], A[l].om[k], skip);
+            std::swap(cur[l], oth[l]);
+        }
+        float *res = l == 0 ? c->amg_r0 : A[l].r;
+        launch_amg_resid(L, A[l].A, B[l], cur[l], res, skip);
+        const bool smooth_next = l + 2 < nl;   // the last level is solved, not smoothed
+        launch_amg_restrict(L, A[l].R, res, A[l + 1].b, smooth_next ? cur[l + 1] : nullptr, A[l + 1].dinv,
+                            A[l + 1].om[0], skip);
+    }
+    launch_amg_dense(L, c->amg_inv, c->amg_nc, B[nl - 1], cur[nl - 1], skip);
+    for (int l = nl - 2; l >= 0; --l) {
+        launch_amg_prolong(L, A[l].P, cur[l + 1], cur[l], skip);
+        for (int k = A[l].deg - 1; k >= 0; --k) {
+            launch_amg_smooth(L, A[l].A, A[l].dinv, B[l], cur[l], oth[l], A[l].om[k], skip);
+            std::swap(cur[l], oth[l]);
+        }
+    }
+    int np = 0;
+    if (parts) np = launch_dot2<float>(L, c->N, r, cur[0], parts);
+    if (nparts) *nparts = np;
+    return cur[0];
+}
+
 // Mixed precision with reliable updates: the Jacobi-PCG recursion (alpha, beta, r, z, p
 // and the correction dx) runs in FP32 with FP64 dot products; whenever the FP32
 // recursive residual has dropped by ru_delta since the last replacement, or claims
@@ -449,10 +501,10 @@ int pcg_mixed(mm_ctx *c, const double *b, double *x, double tol, double atol, in
     Launch L = c->L();
     const int64_t n = c->N;
     const double delta2 = c->ru_delta * c->ru_delta;
-    const bool mg = c->mg_nl > 0;
+    const bool mg = c->mg_nl > 0 || c->amg_nl > 0;
     const double seg2 = mg ? delta2 : 0.0;
     // with multigrid, every kernel that forms z from r writes the V-cycle's first step
-    const float *invD = mg ? c->mg[0].invw : c->invDK32;
+    const float *invD = c->mg_nl > 0 ? c->mg[0].invw : c->amg_nl > 0 ? c->amg_invw0 : c->invDK32;
     // r = b - K x in FP64 (one pass on the structured grid), its FP32 copy and z
     const bool fused = c->structured && grid_ru_residual_fits(c->P);
     auto true_residual = [&](bool init) {
@@ -473,6 +525,11 @@ int pcg_mixed(mm_ctx *c, const double *b, double *x, double tol, double atol, in
     int *const done = &c->dsc->pcg_done;
     // z = M r; the V-cycle's result buffer becomes z32 (the other one its scratch)
     auto precondition = [&](int it) {
+        if (c->amg_nl > 0) {
+            float *res = amg_vcycle(c, c->r32, c->z32, c->amg[0].xb, done, true, c->partsB, &nB);
+            if (res != c->z32) std::swap(c->z32, c->amg[0].xb);
+            return;
+        }
         float *res = mg_vcycle(c, c->r32, c->z32, c->mg_xb[0], done, true, c->partsB, &nB, it);
         if (res != c->z32) std::swap(c->z32, c->mg_xb[0]);
     };
@@ -988,6 +1045,23 @@ int mm_ctx_create(const mm_mesh_desc *d, const mm_options *opts, mm_ctx **out) {
             if ((rc = c->alloc(&c->ws_red, 8))) return bail(rc);
         }
     } else {
+        {   // K on the host for the AMG set-up, columns ascending within each row
+            CsrHost &K = c->hK;
+            K.n = K.m = (int)c->N;
+            K.ptr = H.k_rowptr;
+            K.col = H.k_col;
+            K.val = H.k_val;
+            std::vector<std::pair<int, double>> row;
+            for (int i = 0; i < K.n; ++i) {
+                row.clear();
+                for (int q = K.ptr[i]; q < K.ptr[i + 1]; ++q) row.emplace_back(K.col[q], K.val[q]);
+                std::sort(row.begin(), row.end(), [](const auto &a, const auto &b) { return a.first < b.first; });
+                for (size_t t = 0; t < row.size(); ++t) {
+                    K.col[K.ptr[i] + t] = row[t].first;
+                    K.val[K.ptr[i] + t] = row[t].second;
+                }
+            }
+        }
         std::vector<int> sp, scol;
         std::vector<double> sval;
         csr_to_sell((int)c->N, H.k_rowptr, H.k_col, H.k_val, sp, scol, sval);
@@ -1684,6 +1758,118 @@ int mm_multigrid(mm_ctx *c, int32_t max_levels) {
     c->mg_nl = nl;   // published last: a set-up that failed before leaves the Jacobi form in use
     c->mg_built = nl;
     c->mg_built_max = max_levels;
+    return 0;
+}
+
+namespace {
+// SELL-64 of a CSR matrix (rectangular allowed: padding entries take column 0), FP32
+int upload_sell32(mm_ctx *c, const CsrHost &M, SellDev<float> &S) {
+    const int C = kWave, ns = (M.n + C - 1) / C;
+    std::vector<int> sp(ns + 1, 0);
+    for (int s = 0; s < ns; ++s) {
+        int w = 0;
+        for (int r = s * C; r < std::min(M.n, (s + 1) * C); ++r) w = std::max(w, M.ptr[r + 1] - M.ptr[r]);
+        sp[s + 1] = sp[s] + w * C;
+    }
+    std::vector<int> col(sp[ns], 0);
+    std::vector<float> val(sp[ns], 0.f);
+    for (int s = 0; s < ns; ++s) {
+        const int w = (sp[s + 1] - sp[s]) / C;
+        for (int lane = 0; lane < C; ++lane) {
+            const int r = s * C + lane;
+            const int len = r < M.n ? M.ptr[r + 1] - M.ptr[r] : 0;
+            for (int j = 0; j < len && j < w; ++j) {
+                col[sp[s] + (size_t)j * C + lane] = M.col[M.ptr[r] + j];
+                val[sp[s] + (size_t)j * C + lane] = (float)M.val[M.ptr[r] + j];
+            }
+        }
+    }
+    int *dsp, *dcol;
+    float *dval;
+    int rc;
+    if ((rc = c->upload(&dsp, sp)) || (rc = c->upload(&dcol, col)) || (rc = c->upload(&dval, val))) return rc;
+    S.n_rows = M.n;
+    S.n_slices = ns;
+    S.slice_ptr = dsp;
+    S.col = dcol;
+    S.val = dval;
+    return 0;
+}
+}  // namespace
+
+int mm_amg(mm_ctx *c, int32_t max_levels) {
+    if (!c) return fail(MM_E_ARG, "null context");
+    if (max_levels == 0) {   // back to Jacobi, with Jacobi's reliable-update interval
+        c->amg_nl = 0;
+        if (!c->ru_delta_set) c->ru_delta = 0.1;
+        return 0;
+    }
+    if (c->structured || c->precision != MM_PREC_MIXED)
+        return fail(MM_E_STATE, "the AMG preconditioner needs a general (unstructured) mixed-precision context");
+    if (max_levels < 2 || max_levels > kAmgMaxLevels) return fail(MM_E_ARG, "levels outside [2, 10]");
+    if (c->amg_built > 0) {   // built once per context (a later call re-enables it)
+        if (max_levels != c->amg_built_max)
+            return fail(MM_E_STATE, "AMG already set up with another level bound; use a new context");
+        c->amg_nl = c->amg_built;
+        if (!c->ru_delta_set) c->ru_delta = 1e-3;
+        return 0;
+    }
+    constexpr int kCoarse = 500, kDeg = 2;
+    constexpr double kAlpha = 30.0;
+    AmgHost H;
+    const std::string err = amg_setup(c->hK, max_levels, kCoarse, kDeg, kAlpha, H);
+    if (!err.empty()) return fail(MM_E_STATE, err);
+    const int nl = (int)H.levels.size();
+    if (nl < 2) return fail(MM_E_STATE, "amg: the matrix is already at the coarsest size");
+    int rc;
+    for (int l = 0; l < nl; ++l) {
+        const AmgLevelHost &h = H.levels[l];
+        AmgLevelDev &d = c->amg[l];
+        d.n = h.A.n;
+        d.deg = (int)h.om.size();
+        for (int k = 0; k < d.deg; ++k) d.om[k] = (float)h.om[k];
+        if (l == 0) {
+            d.A = c->K32;
+            d.dinv = c->invDK32;
+            if ((rc = c->alloc(&d.xb, (size_t)d.n)) || (rc = c->alloc(&c->amg_r0, (size_t)d.n))) return rc;
+            std::vector<float> w0(d.n);
+            for (int i = 0; i < d.n; ++i) w0[i] = (float)(h.om[0] * h.dinv[i]);
+            if ((rc = c->upload(&c->amg_invw0, w0))) return rc;
+        } else {
+            float *dv;
+            std::vector<float> di(h.dinv.begin(), h.dinv.end());
+            if ((rc = c->upload(&dv, di))) return rc;
+            d.dinv = dv;
+            if ((rc = upload_sell32(c, h.A, d.A))) return rc;
+            for (float **v : {&d.b, &d.xa, &d.xb, &d.r})
+                if ((rc = c->alloc(v, (size_t)d.n))) return rc;
+        }
+        if (l + 1 < nl && ((rc = upload_sell32(c, h.P, d.P)) || (rc = upload_sell32(c, h.R, d.R)))) return rc;
+    }
+    c->amg_nc = H.levels.back().A.n;
+    std::vector<float> inv(H.inv.begin(), H.inv.end());
+    if ((rc = c->upload(&c->amg_inv, inv))) return rc;
+    c->amg_built = c->amg_nl = nl;
+    c->amg_built_max = max_levels;
+    if (!c->ru_delta_set) c->ru_delta = 1e-3;
+    HIPC(hipDeviceSynchronize());
+    return 0;
+}
+
+int mm_amg_levels(mm_ctx *c, int32_t *levels, int64_t *sizes) {
+    if (!c || !levels) return fail(MM_E_ARG, "null argument");
+    *levels = c->amg_nl;
+    if (sizes)
+        for (int l = 0; l < c->amg_nl; ++l) sizes[l] = c->amg[l].n;
+    return 0;
+}
+
+int mm_amg_apply(mm_ctx *c, const float *r, float *z) {
+    if (!c || !r || !z) return fail(MM_E_ARG, "null argument");
+    if (c->amg_nl <= 0) return fail(MM_E_STATE, "AMG not set up (mm_amg)");
+    float *res = amg_vcycle(c, r, z, c->amg[0].xb, nullptr, false);
+    if (res != z) HIPC(hipMemcpyAsync(z, res, (size_t)c->N * sizeof(float), hipMemcpyDeviceToDevice, c->s));
+    HIPC(hipStreamSynchronize(c->s));
     return 0;
 }
 

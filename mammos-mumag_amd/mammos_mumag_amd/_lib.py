@@ -120,6 +120,9 @@ SIGNATURES = {
     "mm_multigrid": [VP, ctypes.c_int32],
     "mm_mg_apply": [VP, VP, VP],
     "mm_mg_levels": [VP, VP, VP],
+    "mm_amg": [VP, ctypes.c_int32],
+    "mm_amg_apply": [VP, VP, VP],
+    "mm_amg_levels": [VP, VP, VP],
     "mm_warm_start_stats": [VP, ctypes.POINTER(ctypes.c_int64), c_double_p],
     "mm_warm_start_policy": [VP, ctypes.c_int32],
     "mm_secant_start": [VP, ctypes.c_int32],

@@ -14,7 +14,8 @@ INCLUDE = PKG.parent.parent / "include"
 LIBDIR = PKG / "_lib"
 LIB = LIBDIR / "libmumag_hip.so"
 
-SOURCES = ["kernels.hip", "pcg_fused.hip", "mlp.hip", "mg.hip", "assemble.cpp", "structured.cpp", "capi.cpp"]
+SOURCES = ["kernels.hip", "pcg_fused.hip", "mlp.hip", "mg.hip", "amg.hip", "assemble.cpp", "structured.cpp",
+           "amg.cpp", "capi.cpp"]
 ARCH = os.environ.get("MUMAG_OFFLOAD_ARCH", "gfx950")
 
 

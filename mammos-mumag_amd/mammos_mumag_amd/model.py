@@ -388,6 +388,27 @@ class Model:
         contexts) with at most ``levels`` levels; 0 restores Jacobi."""
         _lib.check(self._lib.mm_multigrid(self._ctx, int(levels)), "mm_multigrid")
 
+    def amg(self, levels=10):
+        """Smoothed-aggregation AMG preconditioner for the FP32 inner solve (general mixed
+        contexts) with at most ``levels`` levels; 0 restores Jacobi."""
+        _lib.check(self._lib.mm_amg(self._ctx, int(levels)), "mm_amg")
+
+    def amg_levels(self):
+        """Unknowns of every AMG level, finest first."""
+        nl = ctypes.c_int32()
+        sizes = (ctypes.c_int64 * 10)()
+        _lib.check(self._lib.mm_amg_levels(self._ctx, ctypes.byref(nl), sizes), "mm_amg_levels")
+        return [int(sizes[l]) for l in range(nl.value)]
+
+    def amg_apply(self, r, z=None):
+        """z = M r with the AMG V-cycle (float32 device vectors)."""
+        torch = _torch()
+        z = torch.empty(self.N, dtype=torch.float32, device=r.device) if z is None else z
+        self._sync()
+        _lib.check(self._lib.mm_amg_apply(self._ctx, _ptr(r, self.N, "float32"), _ptr(z, self.N, "float32")),
+                   "mm_amg_apply")
+        return z
+
     def mg_levels(self):
         """Node counts per axis (nx, ny, nz) of every V-cycle level, finest first."""
         nl = ctypes.c_int32()
