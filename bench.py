@@ -115,7 +115,7 @@ def start_cpu_leg(cfg, seed):
 
 
 # --------------------------------------------------------------------------- GPU leg
-def gpu_run(args, precision, traffic_probe=False, learned=None, multigrid=0):
+def gpu_run(args, precision, traffic_probe=False, learned=None, multigrid=0, amg=False):
     """Warm-up and timed field steps of one form; returns throughput, counters, kernel
     timing and the per-step records (M.h, E) plus the final potential and demag field."""
     import numpy as np
@@ -135,9 +135,17 @@ def gpu_run(args, precision, traffic_probe=False, learned=None, multigrid=0):
     if multigrid:
         model.multigrid(multigrid)
         levels = model.mg_levels()
+    amg_sizes, amg_setup_s = [], None
+    if amg:
+        t1 = time.perf_counter()
+        model.amg()
+        amg_setup_s = time.perf_counter() - t1
+        amg_sizes = model.amg_levels()
+        log(f"[bench] AMG set-up: levels {amg_sizes} ({amg_setup_s:.2f} s)")
     if multigrid and not args.no_secant:
         model.secant_start(True)
-    tag = precision + (f" + multigrid ({len(levels)} levels)" if multigrid else "") + (" + learned" if learned else "")
+    tag = (precision + (f" + multigrid ({len(levels)} levels)" if multigrid else "")
+           + (f" + AMG ({len(amg_sizes)} levels)" if amg else "") + (" + learned" if learned else ""))
     if learned:
         from mammos_mumag_amd.learned import Network
         net = Network(model, max_batch=32, seed=0)
@@ -185,6 +193,7 @@ def gpu_run(args, precision, traffic_probe=False, learned=None, multigrid=0):
     ws = dict(model.warm_start_stats(), **model.warm_start_report()) if learned else None
     res = dict(elapsed=elapsed, stats=st.as_dict(), timing=timing, info=model.info, warm_start=ws,
                mg_levels=" -> ".join("x".join(map(str, d)) for d in levels), levels=levels,
+               amg_levels=amg_sizes, amg_setup_s=amg_setup_s,
                n_voxels=int(CONFIGS_N(args.config)) ** 3,
                n_nodes=n_nodes, n_elems=n_elems, n_mag_nodes=int(np.sum(model.meas > 0)),
                mh=mh, energies=energies, u=u.cpu().numpy(), hd=model.gradient(u).cpu().numpy())
@@ -550,11 +559,16 @@ def roofline(res, precision, args):
     kname = ("k_ka_grid (7-point Poisson stencil fused with p = z + beta p, p.q partials)"
              if res["info"]["structured"] else "k_ka_sell (SELL-64 K apply fused with p = z + beta p, p.q partials)")
     kernels = []
+    # level-0 smoothing launches per PCG iteration: geometric V-cycle 6, AMG 3 (degree 2:
+    # one step down, the first fused into the update, two up)
+    smooth = (("mg_smooth", "k_amg_smooth<1> (Chebyshev smoothing step of the AMG V-cycle's finest level, "
+                            "SELL-64 K), FP32 inner solve", 3) if res.get("amg_levels") else
+              ("mg_smooth", "k_mg_stencil<SMOOTH> (Chebyshev smoothing step of the multigrid "
+                            "V-cycle's finest level), FP32 inner solve", 6))
     # (timer, kernel, launches of that kernel per PCG iteration)
     for key, name, per_it in (("spmv", f"{kname}, {prec}", 1),
                               ("update", f"k_kb (x, r, z update, r.z and r.r partials), {prec}", 1),
-                              ("mg_smooth", "k_mg_stencil<SMOOTH> (Chebyshev smoothing step of the multigrid "
-                                            "V-cycle's finest level), FP32 inner solve", 6)):
+                              smooth):
         if tm[f"{key}_launches"] > 0:
             avg_s = tm[f"{key}_ms"] / tm[f"{key}_launches"] / 1e3
             kernels.append(dict(key=key, kernel=name, total_ms=tm[f"{key}_ms"], avg_s=avg_s, per_it=per_it,
@@ -664,6 +678,8 @@ def main():
     ap.add_argument("--seed", type=int, default=1000)
     ap.add_argument("--general", action="store_true", help="unstructured kernels (SELL K, element loops)")
     ap.add_argument("--inner-tol", type=float, default=0.0)
+    ap.add_argument("--amg", action="store_true",
+                    help="--general --precision mixed: smoothed-aggregation AMG preconditioner of the FP32 inner solve")
     ap.add_argument("--no-secant", action="store_true",
                     help="chosen form without the secant start of the line search's later potential solves")
     ap.add_argument("--eval-precision", default="fp64", choices=["fp64", "mixed"],
@@ -751,12 +767,13 @@ def main():
             args.precond = str(ROOT / "mammos-mumag_amd" / "mammos_mumag_amd" / "data" / f"precond_{args.config}.npz")
         return precond_eval_main(args)
     mgl = args.multigrid if args.precision == "mixed" and not args.general else 0
-    res = gpu_run(args, args.precision, traffic_probe=args.traffic_probe, learned=learned, multigrid=mgl)
+    amg = bool(args.amg and args.general and args.precision == "mixed")
+    res = gpu_run(args, args.precision, traffic_probe=args.traffic_probe, learned=learned, multigrid=mgl, amg=amg)
     ablate = None
     if learned and not args.no_compare:
         ablate = gpu_run(args, args.precision, multigrid=mgl)   # the same form without the network
     jacobi = None
-    if args.precision == "mixed" and mgl and not args.no_compare:
+    if args.precision == "mixed" and (mgl or amg) and not args.no_compare:
         jacobi = gpu_run(args, "mixed")   # FP32 inner solve alone: the Jacobi preconditioner
     ref = None
     if (args.precision != "fp64" or learned) and not args.no_compare:
@@ -800,6 +817,8 @@ def main():
                    "precision": args.precision,
                    "form": (("learned warm start + " if learned else "")
                             + ("FP32 inner solve" + (f" (multigrid preconditioner, levels {res['mg_levels']})" if mgl else "")
+                               + (f" (smoothed-aggregation AMG preconditioner, levels {res['amg_levels']}, set-up "
+                                  f"{res['amg_setup_s']:.2f} s outside the timed steps)" if amg else "")
                                + " + FP64 outer" if args.precision == "mixed" else "FP64")
                             + (" + secant start of the line search's later solves" if mgl and not args.no_secant
                                else "")),

@@ -2,6 +2,7 @@
 // 64-row slice, one lane per row (the slice's entries column-major, so every load of
 // values and column indices is coalesced; padding entries hold 0 and a valid column).
 // Each kernel returns at once when *skip is set (iterations queued past convergence).
+#include <hip/hip_ext.h>
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -89,7 +90,9 @@ int slice_blocks(int n_slices) {
 void launch_amg_smooth(const Launch &L, const SellDev<float> &A, const float *dinv, const float *b,
                        const float *x, float *out, float w, const int *skip) {
     const dim3 g(slice_blocks(A.n_slices)), blk(kBlock);
-    if (x)
+    if (x && L.t0)   // timed: events recorded on the dispatch itself
+        hipExtLaunchKernelGGL(k_amg_smooth<1>, g, blk, 0, L.s, L.t0, L.t1, 0, A, dinv, b, x, out, w, skip);
+    else if (x)
         hipLaunchKernelGGL(k_amg_smooth<1>, g, blk, 0, L.s, A, dinv, b, x, out, w, skip);
     else
         hipLaunchKernelGGL(k_amg_smooth<0>, g, blk, 0, L.s, A, dinv, b, x, out, w, skip);

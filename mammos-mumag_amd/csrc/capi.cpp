@@ -449,7 +449,7 @@ float *mg_vcycle(mm_ctx *c, const float *r, float *z, float *alt, const int *ski
 // added, the steps in reverse order; the last level by its dense inverse.  Level 0
 // ping-pongs between z and alt; the returned buffer holds M r.
 float *amg_vcycle(mm_ctx *c, const float *r, float *z, float *alt, const int *skip, bool first_done,
-                  double *parts = nullptr, int *nparts = nullptr) {
+                  double *parts = nullptr, int *nparts = nullptr, int it = -1) {
     Launch L = c->L();
     const int nl = c->amg_nl;
     float *cur[kAmgMaxLevels], *oth[kAmgMaxLevels];
@@ -476,7 +476,9 @@ float *amg_vcycle(mm_ctx *c, const float *r, float *z, float *alt, const int *sk
     for (int l = nl - 2; l >= 0; --l) {
         launch_amg_prolong(L, A[l].P, cur[l + 1], cur[l], skip);
         for (int k = A[l].deg - 1; k >= 0; --k) {
-            launch_amg_smooth(L, A[l].A, A[l].dinv, B[l], cur[l], oth[l], A[l].om[k], skip);
+            // the PCG iteration's timed level-0 step (HIP events on its dispatch)
+            const Launch LS = l == 0 && k == A[l].deg - 1 && it >= 0 ? timed(c, c->t_mg, it) : L;
+            launch_amg_smooth(LS, A[l].A, A[l].dinv, B[l], cur[l], oth[l], A[l].om[k], skip);
             std::swap(cur[l], oth[l]);
         }
     }
@@ -526,7 +528,7 @@ int pcg_mixed(mm_ctx *c, const double *b, double *x, double tol, double atol, in
     // z = M r; the V-cycle's result buffer becomes z32 (the other one its scratch)
     auto precondition = [&](int it) {
         if (c->amg_nl > 0) {
-            float *res = amg_vcycle(c, c->r32, c->z32, c->amg[0].xb, done, true, c->partsB, &nB);
+            float *res = amg_vcycle(c, c->r32, c->z32, c->amg[0].xb, done, true, c->partsB, &nB, it);
             if (res != c->z32) std::swap(c->z32, c->amg[0].xb);
             return;
         }
@@ -1811,6 +1813,7 @@ int mm_amg(mm_ctx *c, int32_t max_levels) {
         if (max_levels != c->amg_built_max)
             return fail(MM_E_STATE, "AMG already set up with another level bound; use a new context");
         c->amg_nl = c->amg_built;
+        c->t_mg.bytes = c->k_nnz * 8 + 4 * (c->N + 1) + 16 * (int64_t)c->N;
         if (!c->ru_delta_set) c->ru_delta = 1e-3;
         return 0;
     }
@@ -1851,6 +1854,9 @@ int mm_amg(mm_ctx *c, int32_t max_levels) {
     if ((rc = c->upload(&c->amg_inv, inv))) return rc;
     c->amg_built = c->amg_nl = nl;
     c->amg_built_max = max_levels;
+    // a level-0 smoothing step reads the SELL matrix (value + column per stored entry, the
+    // slice pointers), 1 / diag, b and x, and writes the new x
+    c->t_mg.bytes = c->k_nnz * 8 + 4 * (c->N + 1) + 16 * (int64_t)c->N;
     if (!c->ru_delta_set) c->ru_delta = 1e-3;
     HIPC(hipDeviceSynchronize());
     return 0;
