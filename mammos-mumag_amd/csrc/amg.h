@@ -38,10 +38,21 @@ struct AmgHost {
 // smoothing of degree `deg` on [rho / alpha, rho].  Returns "" or an error message.
 std::string amg_setup(const CsrHost &A, int max_levels, int coarse, int deg, double alpha, AmgHost &out);
 
-// Device side of one level (FP32): the matrix (level 0: the context's K), 1 / diag, the
-// transfers to the next level and the level's vectors.
+// CSR on the device (FP32) for rows with many entries (the restrictions, the matrices of
+// the coarse levels): `group` lanes per row (a power of two, 4..64), chosen from the mean
+// row length.
+struct CsrDevF {
+    int n_rows = 0, group = 4;
+    const int *ptr = nullptr, *col = nullptr;
+    const float *val = nullptr;
+};
+
+// Device side of one level (FP32): the matrix (level 0: the context's K in SELL-64, A;
+// coarser levels: CSR, Ac), 1 / diag, the transfers to the next level (P in SELL-64, its
+// rows hold a few entries; R = P^T in CSR) and the level's vectors.
 struct AmgLevelDev {
-    SellDev<float> A, P, R;
+    SellDev<float> A, P;
+    CsrDevF Ac, R;
     int n = 0;
     const float *dinv = nullptr;
     float om[8] = {};
@@ -54,8 +65,13 @@ void launch_amg_smooth(const Launch &L, const SellDev<float> &A, const float *di
                        const float *x, float *out, float w, const int *skip);
 void launch_amg_resid(const Launch &L, const SellDev<float> &A, const float *b, const float *x, float *out,
                       const int *skip);
+// the same on a coarse level's CSR matrix (x given)
+void launch_amg_smooth(const Launch &L, const CsrDevF &A, const float *dinv, const float *b, const float *x,
+                       float *out, float w, const int *skip);
+void launch_amg_resid(const Launch &L, const CsrDevF &A, const float *b, const float *x, float *out,
+                      const int *skip);
 // bc = R r and, when x1c is given, x1c = w dinvc bc (the coarse level's first smoothing step)
-void launch_amg_restrict(const Launch &L, const SellDev<float> &R, const float *r, float *bc, float *x1c,
+void launch_amg_restrict(const Launch &L, const CsrDevF &R, const float *r, float *bc, float *x1c,
                          const float *dinvc, float w, const int *skip);
 // x += P xc
 void launch_amg_prolong(const Launch &L, const SellDev<float> &P, const float *xc, float *x, const int *skip);

@@ -460,14 +460,18 @@ float *amg_vcycle(mm_ctx *c, const float *r, float *z, float *alt, const int *sk
         oth[l] = l == 0 ? alt : c->amg[l].xb;
     }
     const AmgLevelDev *A = c->amg;
+    // level 0 in SELL-64, the coarser levels in CSR
+    auto smooth = [&](const Launch &LS, int l, float w) {
+        if (l == 0) launch_amg_smooth(LS, A[0].A, A[0].dinv, B[0], cur[0], oth[0], w, skip);
+        else launch_amg_smooth(LS, A[l].Ac, A[l].dinv, B[l], cur[l], oth[l], w, skip);
+        std::swap(cur[l], oth[l]);
+    };
     if (!first_done) launch_amg_smooth(L, A[0].A, A[0].dinv, r, nullptr, z, A[0].om[0], skip);
     for (int l = 0; l + 1 < nl; ++l) {
-        for (int k = 1; k < A[l].deg; ++k) {
-            launch_amg_smooth(L, A[l].A, A[l].dinv, B[l], cur[l], oth[l], A[l].om[k], skip);
-            std::swap(cur[l], oth[l]);
-        }
+        for (int k = 1; k < A[l].deg; ++k) smooth(L, l, A[l].om[k]);
         float *res = l == 0 ? c->amg_r0 : A[l].r;
-        launch_amg_resid(L, A[l].A, B[l], cur[l], res, skip);
+        if (l == 0) launch_amg_resid(L, A[0].A, B[0], cur[0], res, skip);
+        else launch_amg_resid(L, A[l].Ac, B[l], cur[l], res, skip);
         const bool smooth_next = l + 2 < nl;   // the last level is solved, not smoothed
         launch_amg_restrict(L, A[l].R, res, A[l + 1].b, smooth_next ? cur[l + 1] : nullptr, A[l + 1].dinv,
                             A[l + 1].om[0], skip);
@@ -475,12 +479,8 @@ float *amg_vcycle(mm_ctx *c, const float *r, float *z, float *alt, const int *sk
     launch_amg_dense(L, c->amg_inv, c->amg_nc, B[nl - 1], cur[nl - 1], skip);
     for (int l = nl - 2; l >= 0; --l) {
         launch_amg_prolong(L, A[l].P, cur[l + 1], cur[l], skip);
-        for (int k = A[l].deg - 1; k >= 0; --k) {
-            // the PCG iteration's timed level-0 step (HIP events on its dispatch)
-            const Launch LS = l == 0 && k == A[l].deg - 1 && it >= 0 ? timed(c, c->t_mg, it) : L;
-            launch_amg_smooth(LS, A[l].A, A[l].dinv, B[l], cur[l], oth[l], A[l].om[k], skip);
-            std::swap(cur[l], oth[l]);
-        }
+        for (int k = A[l].deg - 1; k >= 0; --k)   // the PCG iteration's timed level-0 step
+            smooth(l == 0 && k == A[l].deg - 1 && it >= 0 ? timed(c, c->t_mg, it) : L, l, A[l].om[k]);
     }
     int np = 0;
     if (parts) np = launch_dot2<float>(L, c->N, r, cur[0], parts);
@@ -1797,6 +1797,24 @@ int upload_sell32(mm_ctx *c, const CsrHost &M, SellDev<float> &S) {
     S.val = dval;
     return 0;
 }
+
+// CSR (FP32) with lanes per row from the mean row length: about four entries per lane
+int upload_csr32(mm_ctx *c, const CsrHost &M, CsrDevF &S) {
+    const double mean = M.n > 0 ? (double)M.ptr[M.n] / M.n : 0.0;
+    int g = 4;
+    while (g < kWave && 4.0 * g < mean) g *= 2;
+    std::vector<float> val(M.val.begin(), M.val.end());
+    int *dptr, *dcol;
+    float *dval;
+    int rc;
+    if ((rc = c->upload(&dptr, M.ptr)) || (rc = c->upload(&dcol, M.col)) || (rc = c->upload(&dval, val))) return rc;
+    S.n_rows = M.n;
+    S.group = g;
+    S.ptr = dptr;
+    S.col = dcol;
+    S.val = dval;
+    return 0;
+}
 }  // namespace
 
 int mm_amg(mm_ctx *c, int32_t max_levels) {
@@ -1843,11 +1861,11 @@ int mm_amg(mm_ctx *c, int32_t max_levels) {
             std::vector<float> di(h.dinv.begin(), h.dinv.end());
             if ((rc = c->upload(&dv, di))) return rc;
             d.dinv = dv;
-            if ((rc = upload_sell32(c, h.A, d.A))) return rc;
+            if ((rc = upload_csr32(c, h.A, d.Ac))) return rc;
             for (float **v : {&d.b, &d.xa, &d.xb, &d.r})
                 if ((rc = c->alloc(v, (size_t)d.n))) return rc;
         }
-        if (l + 1 < nl && ((rc = upload_sell32(c, h.P, d.P)) || (rc = upload_sell32(c, h.R, d.R)))) return rc;
+        if (l + 1 < nl && ((rc = upload_sell32(c, h.P, d.P)) || (rc = upload_csr32(c, h.R, d.R)))) return rc;
     }
     c->amg_nc = H.levels.back().A.n;
     std::vector<float> inv(H.inv.begin(), H.inv.end());
