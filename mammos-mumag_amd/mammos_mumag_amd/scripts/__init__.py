@@ -39,9 +39,16 @@ def _name(argv, script):
 
 def _model(name):
     """The device context of ``<name>``; ``MAMMOS_MUMAG_PRECISION`` = ``fp64`` (default) or
-    ``mixed`` selects the arithmetic."""
+    ``mixed`` selects the arithmetic.  The mixed form preconditions its FP32 potential solve
+    with the smoothed-aggregation AMG when the mesh has more nodes than the AMG's coarsest
+    level (500); ``MAMMOS_MUMAG_AMG=0`` keeps Jacobi.  Either changes iteration counts, not
+    the solve's tolerance."""
     from ..model import Model
-    return Model.from_files(name, precision=os.environ.get("MAMMOS_MUMAG_PRECISION", "fp64"))
+    precision = os.environ.get("MAMMOS_MUMAG_PRECISION", "fp64")
+    model = Model.from_files(name, precision=precision)
+    if precision == "mixed" and os.environ.get("MAMMOS_MUMAG_AMG", "1") != "0" and model.N > 500:
+        model.amg()
+    return model
 
 
 def _krn_first(name):

@@ -94,6 +94,23 @@ def test_loop(sim, tmp_path, golden):
     assert np.allclose(ref["m"].reshape(m.shape), m)
 
 
+@pytest.mark.parametrize("amg", ["1", "0"])
+def test_loop_mixed(sim, tmp_path, golden, monkeypatch, amg):
+    """The loop script in the mixed form (MAMMOS_MUMAG_PRECISION=mixed), with the AMG
+    preconditioner (the default of that form) and with Jacobi: the reference's records to
+    its own criterion (np.allclose) and M.h to 1e-6."""
+    monkeypatch.setenv("MAMMOS_MUMAG_PRECISION", "mixed")
+    monkeypatch.setenv("MAMMOS_MUMAG_AMG", amg)
+    print(f"[gpu] run_loop mixed, MAMMOS_MUMAG_AMG={amg}", flush=True)
+    sim.run_loop(outdir=tmp_path, name="cube")
+    data_loop = np.loadtxt(golden / "loop_cube.dat")
+    sim_loop = np.loadtxt(tmp_path / "cube.dat")
+    assert sim_loop.shape == data_loop.shape
+    assert np.array_equal(sim_loop[:, :2], data_loop[:, :2])
+    assert np.allclose(data_loop, sim_loop)
+    assert np.max(np.abs(sim_loop[:, 2] - data_loop[:, 2])) < 1e-6      # M.h
+
+
 def test_mapping(sim, tmp_path, golden):
     """mapping.py evaluates in float32 (scripts/mapping.py:22-118): the fixtures hold
     float32 values, compared as the reference's test_mapping does."""
