@@ -48,16 +48,18 @@ std::vector<double> omegas(int deg, double lo, double hi) {
 }
 
 // Aggregate per node (-1 off the free set) and the aggregate count; strong neighbours of
-// i are the free j != i with s_ij = |a_ij| / sqrt(a_ii a_jj) > 1e-10 (couplings that cancel
-// to rounding level never count), visited in ascending column order; in pass 2 values
-// within 1e-9 relative are ties, which go to the lower index.
-int aggregate(const CsrHost &A, const std::vector<char> &fr, const std::vector<double> &d, std::vector<int> &agg) {
+// i are the free j != i with s_ij = |a_ij| / sqrt(a_ii a_jj) > max(theta, 1e-10) (couplings
+// that cancel to rounding level never count), visited in ascending column order; in pass 2
+// values within 1e-9 relative are ties, which go to the lower index.
+int aggregate(const CsrHost &A, const std::vector<char> &fr, const std::vector<double> &d, double theta,
+              std::vector<int> &agg) {
     const int n = A.n;
     agg.assign(n, -1);
     std::vector<double> sd(n);
     for (int i = 0; i < n; ++i) sd[i] = std::sqrt(std::fabs(d[i]));
     auto sval = [&](int i, int q) { return std::fabs(A.val[q]) / (sd[i] * sd[A.col[q]]); };
-    auto strong = [&](int i, int q) { const int j = A.col[q]; return j != i && fr[j] && sval(i, q) > 1e-10; };
+    const double th = std::max(theta, 1e-10);
+    auto strong = [&](int i, int q) { const int j = A.col[q]; return j != i && fr[j] && sval(i, q) > th; };
     int na = 0;
     for (int i = 0; i < n; ++i) {   // pass 1
         if (!fr[i] || agg[i] >= 0) continue;
@@ -210,7 +212,8 @@ bool spd_inverse(const CsrHost &A, std::vector<double> &inv) {
 
 }  // namespace
 
-std::string amg_setup(const CsrHost &A0, int max_levels, int coarse, int deg, double alpha, AmgHost &out) {
+std::string amg_setup(const CsrHost &A0, int max_levels, int coarse, int deg, double alpha, double theta,
+                      AmgHost &out) {
     out.levels.clear();
     CsrHost A = A0;
     std::vector<char> fr = free_rows(A);
@@ -230,7 +233,7 @@ std::string amg_setup(const CsrHost &A0, int max_levels, int coarse, int deg, do
             break;
         }
         std::vector<int> agg;
-        const int na = aggregate(A, fr, d, agg);
+        const int na = aggregate(A, fr, d, theta, agg);
         if (na > 0.8 * n) {
             lev.A = std::move(A);
             out.levels.push_back(std::move(lev));

@@ -32,11 +32,12 @@ struct AmgHost {
     std::vector<double> inv;    // the last level's inverse, row-major
 };
 
-// The hierarchy of A (square CSR with Dirichlet rows as identity rows): strength "every
-// stored non-zero coupling between free nodes", three-pass aggregation in index order,
+// The hierarchy of A (square CSR with Dirichlet rows as identity rows): strong couplings
+// |a_ij| > theta sqrt(a_ii a_jj) between free nodes, three-pass aggregation in index order,
 // P = (I - 4 / (3 rho) D^-1 A) T, A_c = P^T A P, down to at most `coarse` unknowns;
 // smoothing of degree `deg` on [rho / alpha, rho].  Returns "" or an error message.
-std::string amg_setup(const CsrHost &A, int max_levels, int coarse, int deg, double alpha, AmgHost &out);
+std::string amg_setup(const CsrHost &A, int max_levels, int coarse, int deg, double alpha, double theta,
+                      AmgHost &out);
 
 // CSR on the device (FP32) for rows with many entries (the restrictions, the matrices of
 // the coarse levels): `group` lanes per row (a power of two, 4..64), chosen from the mean

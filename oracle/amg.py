@@ -8,8 +8,11 @@ scripts/solvers.py:222-231).  Set-up, per level (matrix A, diagonal D):
 * Dirichlet rows of the finest level (no non-zero off-diagonal entry) take no part: they join no aggregate and their rows of P are zero.
 * Strength: j is a strong neighbour of i (both free, j != i) when
   s_ij = |a_ij| / sqrt(a_ii a_jj) > max(theta, 1e-10) (entries that cancel to rounding
-  level in a Galerkin product never count, whichever way they rounded); theta = 0 (every stored coupling) by default, which on
-  the graded air-box meshes coarsens ~13x per level where theta = 0.08 stalls.
+  level in a Galerkin product never count, whichever way they rounded); theta = 0.02 by
+  default.  PCG iterations to 1e-10 from zero, theta 0 / 0.02 / 0.05: 29 / 19 / 17 on a
+  graded air-box mesh of 117,649 nodes (operator complexity 1.57 / 1.75 / 2.22), 113 / 29 /
+  20 on the 103,823-node spherical-shell mesh (1.20 / 1.66 / 2.07); theta = 0.1 grows the
+  coarse operators 10-14x.
 * Aggregation in three passes over the nodes in index order: (1) a node whose strong
   neighbours all are still unaggregated starts an aggregate with them; (2) a node left
   over joins the pass-1 aggregate of its strongest aggregated neighbour (largest s_ij;
@@ -21,8 +24,7 @@ scripts/solvers.py:222-231).  Set-up, per level (matrix A, diagonal D):
   (max_i sum_j |a_ij| / a_ii, which bounds its spectrum from above).
 * A_c = P^T A P (exact zeros dropped).  Down to at most `coarse` unknowns (or until the
   coarsening stalls, fewer than 20% removed); the last level's matrix is inverted.
-  Measured on the generated meshes' K (degree 2 / 3): 20 / 15 PCG iterations against 166
-  with Jacobi at 24 k nodes, 31 / 24 against 389 at 287 k.
+  Jacobi takes 313 and 550 iterations on the two meshes above.
 
 V-cycle: on each level but the last, `deg` Chebyshev-Richardson steps
 x += w_k D^-1 (b - A x), w_k the reciprocal Chebyshev nodes of [rho / alpha, rho] (largest
@@ -111,7 +113,7 @@ def omegas(deg, lo, hi):
 
 
 class AMG:
-    def __init__(self, A, theta=0.0, coarse=500, max_levels=10, deg=2, alpha=30.0):
+    def __init__(self, A, theta=0.02, coarse=500, max_levels=10, deg=2, alpha=30.0):
         A = sp.csr_matrix(A, dtype=np.float64)
         A.sort_indices()
         free = ~dirichlet_rows(A)
