@@ -63,6 +63,7 @@ struct mm_ctx {
     bool eval32 = false;
     int mg_built = 0, mg_built_max = 0;   // levels built by mm_multigrid and the bound they were built for
     int64_t k_nnz = 0, k_entries = 0;
+    int64_t pc_entries = 0;   // entries of the assembled C (general meshes)
     size_t bytes = 0;
     std::vector<void *> allocs;
 
@@ -71,6 +72,7 @@ struct mm_ctx {
     SellDev<double> K;
     SellDev<float> K32;
     ElemDev E;
+    PairCDev PC;   // assembled C (general meshes) for the applies without D v
     GridDev G;
     TcgBlock<float> tb32;    // truncated CG on the magnet block (structured grids)
     bool ncg_block = false;  // the NCG's reductions visit the magnet block only (this call)
@@ -275,7 +277,10 @@ void op_C(mm_ctx *c, const double *v, double *y, double *b, const int *skip) {
     if (c->structured) {
         launch_grid_exani(L, c->G, c->eval32, v, y, b, skip);
     } else {
-        launch_ebe_exani(L, c->E, v, y, b, skip);   // writes every node of y (and b)
+        if (!b && c->PC.nn > 0)
+            launch_pair_c(L, c->PC, v, y, skip);      // writes every node of y
+        else
+            launch_ebe_exani(L, c->E, v, y, b, skip);   // writes every node of y (and b)
     }
 }
 void op_grad(mm_ctx *c, const double *u, double *g) {
@@ -815,6 +820,108 @@ void launch_reduce3(mm_ctx *c, RedOp op, const double *a, const double *b, const
 }
 
 
+struct PairEntry {
+    int mat, col;
+    double S, M;
+};
+
+// The rows of node i of the assembled C (PairCDev) from its slots 4 q + k (ascending): per
+// magnet element q and local node k, for each local node l (node j) the exchange value
+// 2 A_eff / V vol_q (grad phi_k . grad phi_l) and the consistent-mass value
+// -2 K / V vol_q (1 + [k == l]) / 20 of material m, summed per (m, j) in slot order.
+void pair_rows(const HostOps &H, const std::vector<MatCoef> &mc, const int *slot, int nslot,
+               std::vector<PairEntry> &out) {
+    out.clear();
+    const int64_t n = (int64_t)H.mag_vol.size();
+    for (int t = 0; t < nslot; ++t) {
+        const int q = slot[t] / 4, k = slot[t] % 4;
+        const int m = H.mag_mat[q];
+        const double vol = H.mag_vol[q];
+        double G[4][3];
+        for (int l = 1; l < 4; ++l)
+            for (int a = 0; a < 3; ++a) G[l][a] = H.mag_G[(3 * (l - 1) + a) * n + q];
+        for (int a = 0; a < 3; ++a) G[0][a] = -(G[1][a] + G[2][a] + G[3][a]);
+        const double cex = mc[m].cex * vol, ck = mc[m].cK * vol / 20.0;
+        for (int l = 0; l < 4; ++l)
+            out.push_back({m, H.mag_tets[4 * q + l],
+                           cex * (G[k][0] * G[l][0] + G[k][1] * G[l][1] + G[k][2] * G[l][2]),
+                           ck * (k == l ? 2.0 : 1.0)});
+    }
+    std::stable_sort(out.begin(), out.end(), [](const PairEntry &a, const PairEntry &b) {
+        return a.mat != b.mat ? a.mat < b.mat : a.col < b.col;
+    });
+    size_t w = 0;
+    for (size_t t = 0; t < out.size(); ++t) {
+        if (w > 0 && out[w - 1].mat == out[t].mat && out[w - 1].col == out[t].col) {
+            out[w - 1].S += out[t].S;
+            out[w - 1].M += out[t].M;
+        } else {
+            out[w++] = out[t];
+        }
+    }
+    out.resize(w);
+}
+
+// Assembled C of a general context (two passes over the nodes: sizes, then the rows).
+int build_pair_c(mm_ctx *c, const HostOps &H, const std::vector<MatCoef> &mc, const std::vector<int> &iptr,
+                 const std::vector<int> &islot) {
+    const int64_t N = c->N;
+    std::vector<int> nrows(N, 0), nent(N, 0);
+#pragma omp parallel
+    {
+        std::vector<PairEntry> buf;
+#pragma omp for schedule(dynamic, 1024)
+        for (int64_t i = 0; i < N; ++i) {
+            pair_rows(H, mc, islot.data() + iptr[i], iptr[i + 1] - iptr[i], buf);
+            nent[i] = (int)buf.size();
+            for (size_t t = 0; t < buf.size(); ++t) nrows[i] += t == 0 || buf[t].mat != buf[t - 1].mat;
+        }
+    }
+    std::vector<int> node_ptr(N + 1, 0);
+    std::vector<int64_t> ent0(N + 1, 0);
+    for (int64_t i = 0; i < N; ++i) {
+        node_ptr[i + 1] = node_ptr[i] + nrows[i];
+        ent0[i + 1] = ent0[i] + nent[i];
+    }
+    if (ent0[N] >= INT32_MAX) return fail(MM_E_STATE, "assembled C: more than 2^31 entries");
+    const int R = node_ptr[N];
+    std::vector<int> row_mat(R), row_ptr(R + 1), col(ent0[N]);
+    std::vector<double2> sm(ent0[N]);
+    row_ptr[R] = (int)ent0[N];
+#pragma omp parallel
+    {
+        std::vector<PairEntry> buf;
+#pragma omp for schedule(dynamic, 1024)
+        for (int64_t i = 0; i < N; ++i) {
+            pair_rows(H, mc, islot.data() + iptr[i], iptr[i + 1] - iptr[i], buf);
+            int r = node_ptr[i];
+            for (size_t t = 0; t < buf.size(); ++t) {
+                const int64_t e = ent0[i] + (int64_t)t;
+                if (t == 0 || buf[t].mat != buf[t - 1].mat) {
+                    row_mat[r] = buf[t].mat;
+                    row_ptr[r++] = (int)e;
+                }
+                col[e] = buf[t].col;
+                sm[e] = make_double2(buf[t].S, buf[t].M);
+            }
+        }
+    }
+    int *dnp, *drm, *drp, *dcol;
+    double2 *dsm;
+    int rc;
+    if ((rc = c->upload(&dnp, node_ptr)) || (rc = c->upload(&drm, row_mat)) || (rc = c->upload(&drp, row_ptr)) ||
+        (rc = c->upload(&dcol, col)) || (rc = c->upload(&dsm, sm)))
+        return rc;
+    c->PC.nn = N;
+    c->PC.node_ptr = dnp;
+    c->PC.row_mat = drm;
+    c->PC.row_ptr = drp;
+    c->PC.col = dcol;
+    c->PC.sm = dsm;
+    c->pc_entries = ent0[N];
+    return 0;
+}
+
 std::vector<MatCoef> material_table(const mm_mesh_desc *d, double V) {
     const double mu0 = 4.0e-7 * M_PI, size = d->length_scale > 0 ? d->length_scale : 1e-9;
     std::vector<MatCoef> mc(d->n_mats + 1);
@@ -1112,6 +1219,7 @@ int mm_ctx_create(const mm_mesh_desc *d, const mm_options *opts, mm_ctx **out) {
             c->E.inc_slot = dsl;
             c->E.cy = dcy;
             c->E.cb = dcb;
+            if ((rc = build_pair_c(c, H, material_table(d, c->V), iptr, islot))) return bail(rc);
         }
     }
     // algorithmic bytes of one KA launch of the PCG loop: z and p_old read once, p and

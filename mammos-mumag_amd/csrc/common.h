@@ -70,6 +70,20 @@ struct ElemDev {
     double *cy = nullptr, *cb = nullptr;
 };
 
+// C of the general meshes assembled per node and material, for the applies that need no
+// D v (the truncated CG's C d): node i has one row per material of its magnet elements
+// (ascending), each row its couplings within that material (ascending column) as the
+// exchange value S and the anisotropy's consistent-mass value M;
+// (C v)_i = sum over the rows (S v)_i + u_m (u_m . (M v)_i), u_m the material's axis.
+struct PairCDev {
+    int64_t nn = 0;
+    const int *node_ptr = nullptr;   // [nn + 1] rows of each node
+    const int *row_mat = nullptr;    // [rows] material index
+    const int *row_ptr = nullptr;    // [rows + 1] entry offsets
+    const int *col = nullptr;        // [entries] node index
+    const double2 *sm = nullptr;     // [entries] (S, M)
+};
+
 // Structured Kuhn grid: the magnet block of mn^3 uniform voxels (edge h) starting at
 // grid node (m0x, m0y, m0z) of a rectilinear grid with nx * ny node rows.
 struct GridDev {
@@ -203,6 +217,8 @@ void launch_add_correction(const Launch &L, int64_t n, double *x, float *dx);
 // Element-by-element operators (general meshes), accumulated into zeroed outputs.
 void launch_ebe_exani(const Launch &L, const ElemDev &E, const double *v, double *y, double *b,
                       const int *skip_flag);
+// y = C v for every node (zero rows: 0), from the assembled rows
+void launch_pair_c(const Launch &L, const PairCDev &P, const double *v, double *y, const int *skip);
 void launch_ebe_grad(const Launch &L, const ElemDev &E, const double *u, double *g, double invV);
 
 // Structured magnet operators (node-centric, overwrite the magnet entries only).

@@ -647,6 +647,52 @@ __global__ __launch_bounds__(kBlock) void k_ebe_gather(int64_t n, const int *__r
     }
 }
 
+// Assembled C: kPairLanes lanes share a node; per row of the node the lanes stride over
+// its entries (S v and M v in registers), a butterfly adds the lanes' sums, and every lane
+// adds S v + u (u . M v) in row order (so lane 0 holds the node's sum in a fixed order).
+constexpr int kPairLanes = 8;
+
+__global__ __launch_bounds__(kBlock) void k_pair_c(PairCDev P, const MatCoef *__restrict__ mats,
+                                                   const double *__restrict__ v, double *__restrict__ y,
+                                                   const int *skip) {
+    if (skip && *skip) return;
+    const int lane = threadIdx.x & (kPairLanes - 1);
+    const int64_t stride = (int64_t)gridDim.x * (kBlock / kPairLanes);
+    for (int64_t i = blockIdx.x * (int64_t)(kBlock / kPairLanes) + threadIdx.x / kPairLanes; i < P.nn;
+         i += stride) {
+        double y3[3] = {0.0, 0.0, 0.0};
+        const int r1 = P.node_ptr[i + 1];
+        for (int r = P.node_ptr[i]; r < r1; ++r) {   // the same rows for every lane of the group
+            double sv[3] = {0.0, 0.0, 0.0}, mv[3] = {0.0, 0.0, 0.0};
+            const int q1 = P.row_ptr[r + 1];
+            for (int q = P.row_ptr[r] + lane; q < q1; q += kPairLanes) {
+                const int64_t j = P.col[q];
+                const double2 w = P.sm[q];
+#pragma unroll
+                for (int a = 0; a < 3; ++a) {
+                    const double vj = v[3 * j + a];
+                    sv[a] += w.x * vj;
+                    mv[a] += w.y * vj;
+                }
+            }
+#pragma unroll
+            for (int off = kPairLanes / 2; off > 0; off >>= 1)
+#pragma unroll
+                for (int a = 0; a < 3; ++a) {
+                    sv[a] += __shfl_xor(sv[a], off, kPairLanes);
+                    mv[a] += __shfl_xor(mv[a], off, kPairLanes);
+                }
+            const double *u = mats[P.row_mat[r]].u;
+            const double um = u[0] * mv[0] + u[1] * mv[1] + u[2] * mv[2];
+#pragma unroll
+            for (int a = 0; a < 3; ++a) y3[a] += sv[a] + u[a] * um;
+        }
+        if (lane == 0)
+#pragma unroll
+            for (int a = 0; a < 3; ++a) y[3 * i + a] = y3[a];
+    }
+}
+
 static inline int gather_grid(int64_t nodes) {
     const int64_t b = (nodes * kGatherLanes + kBlock - 1) / kBlock;
     return (int)std::max<int64_t>(1, std::min<int64_t>(b, 16384));
@@ -663,6 +709,11 @@ void launch_ebe_exani(const Launch &L, const ElemDev &E, const double *v, double
     if (b)
         hipLaunchKernelGGL((k_ebe_gather<1, 1>), g, dim3(kBlock), 0, L.s, E.nn, E.inc_ptr, E.inc_slot, E.cb, b,
                            skip_flag);
+}
+void launch_pair_c(const Launch &L, const PairCDev &P, const double *v, double *y, const int *skip) {
+    const int64_t b = (P.nn * kPairLanes + kBlock - 1) / kBlock;
+    const int g = (int)std::max<int64_t>(1, std::min<int64_t>(b, 16384));
+    hipLaunchKernelGGL(k_pair_c, dim3(g), dim3(kBlock), 0, L.s, P, L.mats, v, y, skip);
 }
 void launch_ebe_grad(const Launch &L, const ElemDev &E, const double *u, double *g, double invV) {
     if (E.n > 0)
