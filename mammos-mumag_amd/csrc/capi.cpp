@@ -277,10 +277,7 @@ void op_C(mm_ctx *c, const double *v, double *y, double *b, const int *skip) {
     if (c->structured) {
         launch_grid_exani(L, c->G, c->eval32, v, y, b, skip);
     } else {
-        if (!b && c->PC.nn > 0)
-            launch_pair_c(L, c->PC, v, y, skip);      // writes every node of y
-        else
-            launch_ebe_exani(L, c->E, v, y, b, skip);   // writes every node of y (and b)
+        launch_pair_c(L, c->PC, v, y, b, skip);   // writes every node of y (and b)
     }
 }
 void op_grad(mm_ctx *c, const double *u, double *g) {
@@ -822,13 +819,14 @@ void launch_reduce3(mm_ctx *c, RedOp op, const double *a, const double *b, const
 
 struct PairEntry {
     int mat, col;
-    double S, M;
+    double S, M, D[3];
 };
 
-// The rows of node i of the assembled C (PairCDev) from its slots 4 q + k (ascending): per
-// magnet element q and local node k, for each local node l (node j) the exchange value
-// 2 A_eff / V vol_q (grad phi_k . grad phi_l) and the consistent-mass value
-// -2 K / V vol_q (1 + [k == l]) / 20 of material m, summed per (m, j) in slot order.
+// The rows of node i of the assembled C and D (PairCDev) from its slots 4 q + k
+// (ascending): per magnet element q and local node k, for each local node l (node j) the
+// exchange value 2 A_eff / V vol_q (grad phi_k . grad phi_l), the consistent-mass value
+// -2 K / V vol_q (1 + [k == l]) / 20 and the divergence's Js vol_q grad phi_k / 4 of
+// material m, summed per (m, j) in slot order.
 void pair_rows(const HostOps &H, const std::vector<MatCoef> &mc, const int *slot, int nslot,
                std::vector<PairEntry> &out) {
     out.clear();
@@ -841,11 +839,12 @@ void pair_rows(const HostOps &H, const std::vector<MatCoef> &mc, const int *slot
         for (int l = 1; l < 4; ++l)
             for (int a = 0; a < 3; ++a) G[l][a] = H.mag_G[(3 * (l - 1) + a) * n + q];
         for (int a = 0; a < 3; ++a) G[0][a] = -(G[1][a] + G[2][a] + G[3][a]);
-        const double cex = mc[m].cex * vol, ck = mc[m].cK * vol / 20.0;
+        const double cex = mc[m].cex * vol, ck = mc[m].cK * vol / 20.0, cd = 0.25 * mc[m].Js * vol;
         for (int l = 0; l < 4; ++l)
             out.push_back({m, H.mag_tets[4 * q + l],
                            cex * (G[k][0] * G[l][0] + G[k][1] * G[l][1] + G[k][2] * G[l][2]),
-                           ck * (k == l ? 2.0 : 1.0)});
+                           ck * (k == l ? 2.0 : 1.0),
+                           {cd * G[k][0], cd * G[k][1], cd * G[k][2]}});
     }
     std::stable_sort(out.begin(), out.end(), [](const PairEntry &a, const PairEntry &b) {
         return a.mat != b.mat ? a.mat < b.mat : a.col < b.col;
@@ -855,6 +854,7 @@ void pair_rows(const HostOps &H, const std::vector<MatCoef> &mc, const int *slot
         if (w > 0 && out[w - 1].mat == out[t].mat && out[w - 1].col == out[t].col) {
             out[w - 1].S += out[t].S;
             out[w - 1].M += out[t].M;
+            for (int a = 0; a < 3; ++a) out[w - 1].D[a] += out[t].D[a];
         } else {
             out[w++] = out[t];
         }
@@ -887,6 +887,7 @@ int build_pair_c(mm_ctx *c, const HostOps &H, const std::vector<MatCoef> &mc, co
     const int R = node_ptr[N];
     std::vector<int> row_mat(R), row_ptr(R + 1), col(ent0[N]);
     std::vector<double2> sm(ent0[N]);
+    std::vector<double> d3(3 * ent0[N]);
     row_ptr[R] = (int)ent0[N];
 #pragma omp parallel
     {
@@ -903,14 +904,16 @@ int build_pair_c(mm_ctx *c, const HostOps &H, const std::vector<MatCoef> &mc, co
                 }
                 col[e] = buf[t].col;
                 sm[e] = make_double2(buf[t].S, buf[t].M);
+                for (int a = 0; a < 3; ++a) d3[3 * e + a] = buf[t].D[a];
             }
         }
     }
     int *dnp, *drm, *drp, *dcol;
     double2 *dsm;
+    double *dd3;
     int rc;
     if ((rc = c->upload(&dnp, node_ptr)) || (rc = c->upload(&drm, row_mat)) || (rc = c->upload(&drp, row_ptr)) ||
-        (rc = c->upload(&dcol, col)) || (rc = c->upload(&dsm, sm)))
+        (rc = c->upload(&dcol, col)) || (rc = c->upload(&dsm, sm)) || (rc = c->upload(&dd3, d3)))
         return rc;
     c->PC.nn = N;
     c->PC.node_ptr = dnp;
@@ -918,6 +921,7 @@ int build_pair_c(mm_ctx *c, const HostOps &H, const std::vector<MatCoef> &mc, co
     c->PC.row_ptr = drp;
     c->PC.col = dcol;
     c->PC.sm = dsm;
+    c->PC.d3 = dd3;
     c->pc_entries = ent0[N];
     return 0;
 }
@@ -1210,15 +1214,14 @@ int mm_ctx_create(const mm_mesh_desc *d, const mm_options *opts, mm_ctx **out) {
             std::vector<int> fill(iptr.begin(), iptr.end() - 1);
             for (size_t s4 = 0; s4 < islot.size(); ++s4) islot[fill[H.mag_tets[s4]]++] = (int)s4;
             int *dp, *dsl;
-            double *dcy, *dcb;
+            double *dcy;
             if ((rc = c->upload(&dp, iptr)) || (rc = c->upload(&dsl, islot)) ||
-                (rc = c->alloc(&dcy, 12 * (size_t)c->NEm)) || (rc = c->alloc(&dcb, 4 * (size_t)c->NEm)))
+                (rc = c->alloc(&dcy, 3 * (size_t)c->NEm)))
                 return bail(rc);
             c->E.nn = c->N;
             c->E.inc_ptr = dp;
             c->E.inc_slot = dsl;
             c->E.cy = dcy;
-            c->E.cb = dcb;
             if ((rc = build_pair_c(c, H, material_table(d, c->V), iptr, islot))) return bail(rc);
         }
     }

@@ -63,18 +63,18 @@ struct ElemDev {
     const double *vol = nullptr;      // [n]
     const int *mat = nullptr;         // [n] material index
     // node-centric sums: nodes [nn]; per node its slots 4 e + k (element e, local node k)
-    // in ascending order, CSR (inc_ptr [nn + 1], inc_slot [4 n]); contribution buffers
-    // cy [12 n] (C v per slot; the gradient uses its first 3 n) and cb [4 n] (D v)
+    // in ascending order, CSR (inc_ptr [nn + 1], inc_slot [4 n]); contribution buffer
+    // cy [3 n] (the gradient's per-element vector)
     int64_t nn = 0;
     const int *inc_ptr = nullptr, *inc_slot = nullptr;
-    double *cy = nullptr, *cb = nullptr;
+    double *cy = nullptr;
 };
 
-// C of the general meshes assembled per node and material, for the applies that need no
-// D v (the truncated CG's C d): node i has one row per material of its magnet elements
-// (ascending), each row its couplings within that material (ascending column) as the
-// exchange value S and the anisotropy's consistent-mass value M;
-// (C v)_i = sum over the rows (S v)_i + u_m (u_m . (M v)_i), u_m the material's axis.
+// C and D of the general meshes assembled per node and material: node i has one row per
+// material of its magnet elements (ascending), each row its couplings within that material
+// (ascending column) as the exchange value S, the anisotropy's consistent-mass value M and
+// the divergence's vector d; (C v)_i = sum over the rows (S v)_i + u_m (u_m . (M v)_i),
+// u_m the material's axis, and (D v)_i = sum over the rows and entries d . v_j.
 struct PairCDev {
     int64_t nn = 0;
     const int *node_ptr = nullptr;   // [nn + 1] rows of each node
@@ -82,6 +82,9 @@ struct PairCDev {
     const int *row_ptr = nullptr;    // [rows + 1] entry offsets
     const int *col = nullptr;        // [entries] node index
     const double2 *sm = nullptr;     // [entries] (S, M)
+    // [3 entries]: the divergence's coupling d (b_i = sum over the rows d . v_j), read only
+    // when b = D v is asked for
+    const double *d3 = nullptr;
 };
 
 // Structured Kuhn grid: the magnet block of mn^3 uniform voxels (edge h) starting at
@@ -215,10 +218,8 @@ void launch_grid_ru_residual(const Launch &L, const PoissonGrid<double> &P, cons
 void launch_add_correction(const Launch &L, int64_t n, double *x, float *dx);
 
 // Element-by-element operators (general meshes), accumulated into zeroed outputs.
-void launch_ebe_exani(const Launch &L, const ElemDev &E, const double *v, double *y, double *b,
-                      const int *skip_flag);
-// y = C v for every node (zero rows: 0), from the assembled rows
-void launch_pair_c(const Launch &L, const PairCDev &P, const double *v, double *y, const int *skip);
+// y = C v and, when b is given, b = D v for every node (zero rows: 0), from the assembled rows
+void launch_pair_c(const Launch &L, const PairCDev &P, const double *v, double *y, double *b, const int *skip);
 void launch_ebe_grad(const Launch &L, const ElemDev &E, const double *u, double *g, double invV);
 
 // Structured magnet operators (node-centric, overwrite the magnet entries only).

@@ -518,9 +518,10 @@ void launch_add_correction(const Launch &L, int64_t n, double *x, float *dx) {
 //   anisotropy:  y_k   += (-2K/V) (vol/20) u (u.v_k + sum_l u.v_l)
 //   divergence:  b_k   += Js vol G_k . mean(v)
 //   gradient:    g_k   += Js vol/(4V) grad(u)        (same for the 4 nodes)
-// Each element writes its contributions (per element and local node; the gradient's once
-// per element) and a node-centric pass sums them in a fixed order, so the result is the
-// same in every run (atomic accumulation would depend on the order of arrival).
+// The first three are assembled per node on the host (k_pair_c below).  The gradient runs
+// element by element: each element writes its vector once and a node-centric pass sums
+// them in a fixed order, so the result is the same in every run (atomic accumulation
+// would depend on the order of arrival).
 
 struct ElemLocal {
     double G[4][3];
@@ -534,59 +535,6 @@ __device__ __forceinline__ void load_G(const ElemDev &E, int e, ElemLocal &L) {
         for (int a = 0; a < 3; ++a) L.G[k][a] = E.G[(3 * (k - 1) + a) * (int64_t)n + e];
 #pragma unroll
     for (int a = 0; a < 3; ++a) L.G[0][a] = -(L.G[1][a] + L.G[2][a] + L.G[3][a]);
-}
-
-__global__ __launch_bounds__(kBlock) void k_ebe_exani(ElemDev E, const MatCoef *__restrict__ mats,
-                                                      const double *__restrict__ v,
-                                                      double *__restrict__ cy,
-                                                      double *__restrict__ cb, const int *skip) {
-    if (skip && *skip) return;
-    const int e = blockIdx.x * kBlock + threadIdx.x;
-    if (e >= E.n) return;
-    const int4 t = E.tets[e];
-    const int tn[4] = {t.x, t.y, t.z, t.w};
-    ElemLocal L;
-    load_G(E, e, L);
-    const double vol = E.vol[e];
-    const MatCoef mc = mats[E.mat[e]];
-    double vl[4][3];
-#pragma unroll
-    for (int k = 0; k < 4; ++k)
-#pragma unroll
-        for (int a = 0; a < 3; ++a) vl[k][a] = v[3 * (int64_t)tn[k] + a];
-    double gv[3][3];
-#pragma unroll
-    for (int a = 0; a < 3; ++a)
-#pragma unroll
-        for (int c = 0; c < 3; ++c)
-            gv[a][c] = L.G[0][c] * vl[0][a] + L.G[1][c] * vl[1][a] + L.G[2][c] * vl[2][a] +
-                       L.G[3][c] * vl[3][a];
-    const double cex = mc.cex * vol;
-    const double ck = mc.cK * vol / 20.0;
-    double s[4], S = 0.0;
-#pragma unroll
-    for (int k = 0; k < 4; ++k) {
-        s[k] = mc.u[0] * vl[k][0] + mc.u[1] * vl[k][1] + mc.u[2] * vl[k][2];
-        S += s[k];
-    }
-#pragma unroll
-    for (int k = 0; k < 4; ++k) {
-#pragma unroll
-        for (int a = 0; a < 3; ++a) {
-            const double yk = cex * (L.G[k][0] * gv[a][0] + L.G[k][1] * gv[a][1] + L.G[k][2] * gv[a][2]) +
-                              ck * mc.u[a] * (s[k] + S);
-            cy[(4 * (int64_t)e + k) * 3 + a] = yk;
-        }
-    }
-    if (cb) {
-        const double js = mc.Js * vol;
-        double mb[3];
-#pragma unroll
-        for (int a = 0; a < 3; ++a) mb[a] = 0.25 * (vl[0][a] + vl[1][a] + vl[2][a] + vl[3][a]);
-#pragma unroll
-        for (int k = 0; k < 4; ++k)
-            cb[4 * (int64_t)e + k] = js * (L.G[k][0] * mb[0] + L.G[k][1] * mb[1] + L.G[k][2] * mb[2]);
-    }
 }
 
 __global__ __launch_bounds__(kBlock) void k_ebe_grad(ElemDev E, const MatCoef *__restrict__ mats,
@@ -650,20 +598,22 @@ __global__ __launch_bounds__(kBlock) void k_ebe_gather(int64_t n, const int *__r
 // Assembled C: kPairLanes lanes share a node; per row of the node the lanes stride over
 // its entries (S v and M v in registers), a butterfly adds the lanes' sums, and every lane
 // adds S v + u (u . M v) in row order (so lane 0 holds the node's sum in a fixed order).
+// With DIV, b = D v from the same gathered v.
 constexpr int kPairLanes = 8;
 
+template <bool DIV>
 __global__ __launch_bounds__(kBlock) void k_pair_c(PairCDev P, const MatCoef *__restrict__ mats,
                                                    const double *__restrict__ v, double *__restrict__ y,
-                                                   const int *skip) {
+                                                   double *__restrict__ b, const int *skip) {
     if (skip && *skip) return;
     const int lane = threadIdx.x & (kPairLanes - 1);
     const int64_t stride = (int64_t)gridDim.x * (kBlock / kPairLanes);
     for (int64_t i = blockIdx.x * (int64_t)(kBlock / kPairLanes) + threadIdx.x / kPairLanes; i < P.nn;
          i += stride) {
-        double y3[3] = {0.0, 0.0, 0.0};
+        double y3[3] = {0.0, 0.0, 0.0}, bi = 0.0;
         const int r1 = P.node_ptr[i + 1];
         for (int r = P.node_ptr[i]; r < r1; ++r) {   // the same rows for every lane of the group
-            double sv[3] = {0.0, 0.0, 0.0}, mv[3] = {0.0, 0.0, 0.0};
+            double sv[3] = {0.0, 0.0, 0.0}, mv[3] = {0.0, 0.0, 0.0}, dv = 0.0;
             const int q1 = P.row_ptr[r + 1];
             for (int q = P.row_ptr[r] + lane; q < q1; q += kPairLanes) {
                 const int64_t j = P.col[q];
@@ -673,23 +623,29 @@ __global__ __launch_bounds__(kBlock) void k_pair_c(PairCDev P, const MatCoef *__
                     const double vj = v[3 * j + a];
                     sv[a] += w.x * vj;
                     mv[a] += w.y * vj;
+                    if (DIV) dv += P.d3[3 * (int64_t)q + a] * vj;
                 }
             }
 #pragma unroll
-            for (int off = kPairLanes / 2; off > 0; off >>= 1)
+            for (int off = kPairLanes / 2; off > 0; off >>= 1) {
 #pragma unroll
                 for (int a = 0; a < 3; ++a) {
                     sv[a] += __shfl_xor(sv[a], off, kPairLanes);
                     mv[a] += __shfl_xor(mv[a], off, kPairLanes);
                 }
+                if (DIV) dv += __shfl_xor(dv, off, kPairLanes);
+            }
             const double *u = mats[P.row_mat[r]].u;
             const double um = u[0] * mv[0] + u[1] * mv[1] + u[2] * mv[2];
 #pragma unroll
             for (int a = 0; a < 3; ++a) y3[a] += sv[a] + u[a] * um;
+            bi += dv;
         }
-        if (lane == 0)
+        if (lane == 0) {
 #pragma unroll
             for (int a = 0; a < 3; ++a) y[3 * i + a] = y3[a];
+            if (DIV) b[i] = bi;
+        }
     }
 }
 
@@ -698,22 +654,13 @@ static inline int gather_grid(int64_t nodes) {
     return (int)std::max<int64_t>(1, std::min<int64_t>(b, 16384));
 }
 
-void launch_ebe_exani(const Launch &L, const ElemDev &E, const double *v, double *y, double *b,
-                      const int *skip_flag) {
-    if (E.n > 0)
-        hipLaunchKernelGGL(k_ebe_exani, dim3((E.n + kBlock - 1) / kBlock), dim3(kBlock), 0, L.s, E,
-                           L.mats, v, E.cy, b ? E.cb : nullptr, skip_flag);
-    const dim3 g(gather_grid(E.nn));
-    hipLaunchKernelGGL((k_ebe_gather<3, 1>), g, dim3(kBlock), 0, L.s, E.nn, E.inc_ptr, E.inc_slot, E.cy, y,
-                       skip_flag);
+void launch_pair_c(const Launch &L, const PairCDev &P, const double *v, double *y, double *b, const int *skip) {
+    const int64_t nb = (P.nn * kPairLanes + kBlock - 1) / kBlock;
+    const int g = (int)std::max<int64_t>(1, std::min<int64_t>(nb, 16384));
     if (b)
-        hipLaunchKernelGGL((k_ebe_gather<1, 1>), g, dim3(kBlock), 0, L.s, E.nn, E.inc_ptr, E.inc_slot, E.cb, b,
-                           skip_flag);
-}
-void launch_pair_c(const Launch &L, const PairCDev &P, const double *v, double *y, const int *skip) {
-    const int64_t b = (P.nn * kPairLanes + kBlock - 1) / kBlock;
-    const int g = (int)std::max<int64_t>(1, std::min<int64_t>(b, 16384));
-    hipLaunchKernelGGL(k_pair_c, dim3(g), dim3(kBlock), 0, L.s, P, L.mats, v, y, skip);
+        hipLaunchKernelGGL(k_pair_c<true>, dim3(g), dim3(kBlock), 0, L.s, P, L.mats, v, y, b, skip);
+    else
+        hipLaunchKernelGGL(k_pair_c<false>, dim3(g), dim3(kBlock), 0, L.s, P, L.mats, v, y, nullptr, skip);
 }
 void launch_ebe_grad(const Launch &L, const ElemDev &E, const double *u, double *g, double invV) {
     if (E.n > 0)
