@@ -142,10 +142,11 @@ def gpu_run(args, precision, traffic_probe=False, learned=None, multigrid=0, amg
         amg_setup_s = time.perf_counter() - t1
         amg_sizes = model.amg_levels()
         log(f"[bench] AMG set-up: levels {amg_sizes} ({amg_setup_s:.2f} s)")
-    if multigrid and not args.no_secant:
+    if (multigrid or amg) and not args.no_secant:
         model.secant_start(True)
     tag = (precision + (f" + multigrid ({len(levels)} levels)" if multigrid else "")
-           + (f" + AMG ({len(amg_sizes)} levels)" if amg else "") + (" + learned" if learned else ""))
+           + (f" + AMG ({len(amg_sizes)} levels)" if amg else "") + (" + learned" if learned else "")
+           + (" + secant" if (multigrid or amg) and not args.no_secant else ""))
     if learned:
         from mammos_mumag_amd.learned import Network
         net = Network(model, max_batch=32, seed=0)
@@ -820,7 +821,7 @@ def main():
                                + (f" (smoothed-aggregation AMG preconditioner, levels {res['amg_levels']}, set-up "
                                   f"{res['amg_setup_s']:.2f} s outside the timed steps)" if amg else "")
                                + " + FP64 outer" if args.precision == "mixed" else "FP64")
-                            + (" + secant start of the line search's later solves" if mgl and not args.no_secant
+                            + (" + secant start of the line search's later solves" if (mgl or amg) and not args.no_secant
                                else "")),
                    "learned": (None if not learned else dict(
                        network=os.path.relpath(learned, ROOT), policy=args.ws_policy,
