@@ -41,13 +41,16 @@ def _model(name):
     """The device context of ``<name>``; ``MAMMOS_MUMAG_PRECISION`` = ``fp64`` (default) or
     ``mixed`` selects the arithmetic.  The mixed form preconditions its FP32 potential solve
     with the smoothed-aggregation AMG when the mesh has more nodes than the AMG's coarsest
-    level (500); ``MAMMOS_MUMAG_AMG=0`` keeps Jacobi.  Either changes iteration counts, not
-    the solve's tolerance."""
+    level (500; ``MAMMOS_MUMAG_AMG=0`` keeps Jacobi) and starts the line search's later
+    potential solves from the secant step.  These change iteration counts, not the solve's
+    tolerance; ``fp64`` keeps the reference's plain method and counts."""
     from ..model import Model
     precision = os.environ.get("MAMMOS_MUMAG_PRECISION", "fp64")
     model = Model.from_files(name, precision=precision)
-    if precision == "mixed" and os.environ.get("MAMMOS_MUMAG_AMG", "1") != "0" and model.N > 500:
-        model.amg()
+    if precision == "mixed":
+        if os.environ.get("MAMMOS_MUMAG_AMG", "1") != "0" and model.N > 500:
+            model.amg()
+        model.secant_start(True)
     return model
 
 
