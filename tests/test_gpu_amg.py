@@ -88,3 +88,38 @@ def test_mixed_amg_loop_matches_fp64():
         assert np.max(np.abs(r1[:, 1] - r0[:, 1])) < 1e-6, form
         assert np.max(np.abs(r1[:, 0] / r0[:, 0] - 1)) < 1e-6, form
     assert out["mixed+amg"][1]["pcg_iters"] < out["mixed"][1]["pcg_iters"] / 4
+
+
+def test_amg_error_behaviour():
+    """mm_amg's contract: MM_E_STATE on structured or FP64 contexts, when K is already at
+    the coarsest size, and for another level bound once set up; MM_E_ARG for a bound
+    outside 2..10; 0 restores Jacobi and the same bound re-enables the built hierarchy."""
+    from mammos_mumag_amd import Model, _lib
+    small = generate.build(edge=2.0, n=2, layers=2, grains=1, cone=0.0, seed=3)   # 343 nodes
+    g = generate.build(edge=8.0, n=8, layers=3, grains=4, cone=15.0, seed=3)
+    print("[gpu] mm_amg error paths", flush=True)
+    for kw in (dict(structured=True, precision="mixed"), dict(structured=False, precision="fp64")):
+        model = Model.from_structured(g, **kw)
+        with pytest.raises(_lib.MumagError, match=r"\(-4\)"):
+            model.amg()
+        model.close()
+    model = Model.from_structured(small, structured=False, precision="mixed")
+    assert model.N <= 500
+    with pytest.raises(_lib.MumagError, match=r"\(-4\)"):
+        model.amg()
+    model.close()
+    model = Model.from_structured(g, structured=False, precision="mixed")
+    for bad in (1, 11, -1):
+        with pytest.raises(_lib.MumagError, match=r"\(-1\)"):
+            model.amg(bad)
+    assert model.amg_levels() == []
+    model.amg(4)
+    sizes = model.amg_levels()
+    assert len(sizes) >= 2 and sizes[0] == model.N
+    with pytest.raises(_lib.MumagError, match=r"\(-4\)"):
+        model.amg(5)
+    model.amg(0)
+    assert model.amg_levels() == []
+    model.amg(4)
+    assert model.amg_levels() == sizes
+    model.close()
