@@ -214,8 +214,9 @@ int mm_mg_apply(mm_ctx *ctx, const float *r, float *z);
  * solvers.py:222-231): smoothed aggregation on K (strong couplings
  * |a_ij| > 0.02 sqrt(a_ii a_jj) between free nodes, three-pass aggregation, one
  * damped-Jacobi smoothing of the piecewise-constant prolongation, Galerkin coarse
- * operators) down to at most 500 unknowns, solved by their dense inverse; degree-2
- * Chebyshev smoothing; up to max_levels levels (2..10; 0 restores Jacobi).  Set up on the
+ * operators) down to at most 500 unknowns, solved by their dense inverse; Chebyshev
+ * smoothing of degree 2 on the finest level and 1 on the coarser ones; up to max_levels
+ * levels (2..10; 0 restores Jacobi).  Set up on the
  * host once per context (a later call with the same bound re-enables it; another bound
  * returns MM_E_STATE).  MM_E_STATE on structured or MM_PREC_FP64 contexts and when K is
  * already at the coarsest size, MM_E_ARG for a bound outside 2..10.  The stop test on

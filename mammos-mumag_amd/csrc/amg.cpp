@@ -212,7 +212,7 @@ bool spd_inverse(const CsrHost &A, std::vector<double> &inv) {
 
 }  // namespace
 
-std::string amg_setup(const CsrHost &A0, int max_levels, int coarse, int deg, double alpha, double theta,
+std::string amg_setup(const CsrHost &A0, int max_levels, int coarse, int deg0, int deg, double alpha, double theta,
                       AmgHost &out) {
     out.levels.clear();
     CsrHost A = A0;
@@ -225,7 +225,7 @@ std::string amg_setup(const CsrHost &A0, int max_levels, int coarse, int deg, do
         lev.dinv.resize(A.n);
         for (int i = 0; i < A.n; ++i) lev.dinv[i] = 1.0 / d[i];
         lev.rho = gershgorin(A, d);
-        lev.om = omegas(deg, lev.rho / alpha, lev.rho);
+        lev.om = omegas(out.levels.empty() ? deg0 : deg, lev.rho / alpha, lev.rho);
         const int n = A.n;
         if (n <= coarse || (int)out.levels.size() + 1 == max_levels) {
             lev.A = std::move(A);

@@ -35,8 +35,9 @@ struct AmgHost {
 // The hierarchy of A (square CSR with Dirichlet rows as identity rows): strong couplings
 // |a_ij| > theta sqrt(a_ii a_jj) between free nodes, three-pass aggregation in index order,
 // P = (I - 4 / (3 rho) D^-1 A) T, A_c = P^T A P, down to at most `coarse` unknowns;
-// smoothing of degree `deg` on [rho / alpha, rho].  Returns "" or an error message.
-std::string amg_setup(const CsrHost &A, int max_levels, int coarse, int deg, double alpha, double theta,
+// smoothing of degree `deg0` on level 0 and `deg` on the coarser levels, on
+// [rho / alpha, rho].  Returns "" or an error message.
+std::string amg_setup(const CsrHost &A, int max_levels, int coarse, int deg0, int deg, double alpha, double theta,
                       AmgHost &out);
 
 // CSR on the device (FP32) for rows with many entries (the restrictions, the matrices of

@@ -1950,10 +1950,13 @@ int mm_amg(mm_ctx *c, int32_t max_levels) {
     // couplings (graded air boxes, the spherical shell's tensor); 0 (every coupling strong)
     // took 1.5x (graded box) to 3.9x (shell) the PCG iterations, 0.05 and above grew the
     // coarse operators faster than it cut iterations
-    constexpr int kCoarse = 500, kDeg = 2;
+    // smoothing degree 2 on level 0, 1 on the coarser levels: one PCG iteration more per
+    // solve than degree 2 everywhere (19 -> 20 on a graded air-box mesh, 29 -> 30 on a
+    // shell mesh) for ~19% less work per V-cycle (level 1 carries ~0.6 of level 0's entries)
+    constexpr int kCoarse = 500, kDeg0 = 2, kDeg = 1;
     constexpr double kAlpha = 30.0, kTheta = 0.02;
     AmgHost H;
-    const std::string err = amg_setup(c->hK, max_levels, kCoarse, kDeg, kAlpha, kTheta, H);
+    const std::string err = amg_setup(c->hK, max_levels, kCoarse, kDeg0, kDeg, kAlpha, kTheta, H);
     if (!err.empty()) return fail(MM_E_STATE, err);
     const int nl = (int)H.levels.size();
     if (nl < 2) return fail(MM_E_STATE, "amg: the matrix is already at the coarsest size");

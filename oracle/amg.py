@@ -26,7 +26,9 @@ scripts/solvers.py:222-231).  Set-up, per level (matrix A, diagonal D):
   coarsening stalls, fewer than 20% removed); the last level's matrix is inverted.
   Jacobi takes 313 and 550 iterations on the two meshes above.
 
-V-cycle: on each level but the last, `deg` Chebyshev-Richardson steps
+V-cycle: on each level but the last, `deg` (level 0; `deg_coarse` on the coarser levels,
+1 by default: one PCG iteration more per solve than 2 there, for ~19% less work per
+V-cycle) Chebyshev-Richardson steps
 x += w_k D^-1 (b - A x), w_k the reciprocal Chebyshev nodes of [rho / alpha, rho] (largest
 node first; the first step from x = 0), the residual restricted by P^T, the coarse
 correction P x_c added, the same steps in reverse order; the last level x = A^-1 b.  The
@@ -113,7 +115,7 @@ def omegas(deg, lo, hi):
 
 
 class AMG:
-    def __init__(self, A, theta=0.02, coarse=500, max_levels=10, deg=2, alpha=30.0):
+    def __init__(self, A, theta=0.02, coarse=500, max_levels=10, deg=2, alpha=30.0, deg_coarse=1):
         A = sp.csr_matrix(A, dtype=np.float64)
         A.sort_indices()
         free = ~dirichlet_rows(A)
@@ -121,7 +123,8 @@ class AMG:
         while True:
             n = A.shape[0]
             rho = gershgorin(A)
-            lev = dict(A=A, dinv=1.0 / A.diagonal(), rho=rho, om=omegas(deg, rho / alpha, rho))
+            d = deg if not self.levels else deg_coarse
+            lev = dict(A=A, dinv=1.0 / A.diagonal(), rho=rho, om=omegas(d, rho / alpha, rho))
             self.levels.append(lev)
             if n <= coarse or len(self.levels) == max_levels:
                 break
