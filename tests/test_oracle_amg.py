@@ -125,3 +125,30 @@ def test_galerkin_vcycle_and_cg(K):
     xm, itm = pcg(M.apply)
     assert itm < itj / 5
     assert np.linalg.norm(xm - xj) <= 1e-8 * np.linalg.norm(xj)
+
+
+def test_strength_threshold_follows_anisotropy():
+    """On a 2D Laplacian with x couplings -1 and y couplings -0.01 (s_ij = 0.01 / 2.02 below
+    the threshold 0.02 across rows, 1 / 2.02 along them) every aggregate stays within one
+    grid row (semi-coarsening along the strong direction); with theta = 0 every coupling
+    is strong and the aggregates span rows."""
+    nx, ny = 12, 6
+    idx = np.arange(nx * ny).reshape(ny, nx)
+    rows, cols, vals = [], [], []
+    for j in range(ny):
+        for i in range(nx):
+            p = idx[j, i]
+            rows.append(p), cols.append(p), vals.append(2.02)
+            for di, dj, w in ((1, 0, -1.0), (-1, 0, -1.0), (0, 1, -0.01), (0, -1, -0.01)):
+                if 0 <= i + di < nx and 0 <= j + dj < ny:
+                    rows.append(p), cols.append(idx[j + dj, i + di]), vals.append(w)
+    A = sp.csr_matrix((vals, (rows, cols)), shape=(nx * ny, nx * ny))
+    free = np.ones(nx * ny, dtype=bool)
+    row_of = np.repeat(np.arange(ny), nx)
+    agg, na = amg.aggregate(amg.strength(A, free, 0.02), free)
+    assert np.all(agg >= 0)
+    for a in range(na):
+        assert len(set(row_of[agg == a])) == 1
+    agg0, na0 = amg.aggregate(amg.strength(A, free, 0.0), free)
+    assert any(len(set(row_of[agg0 == a])) > 1 for a in range(na0))
+    assert na0 < na
